@@ -1,0 +1,101 @@
+"""ctypes binding of libhprmat_hip.so (the C ABI declared in include/hprmat_hip.h).
+
+The shared library is built in-tree by ``python -m hprmat_amd.build``.  There
+is no fallback: if the library is missing or fails to load, every entry point
+raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libhprmat_hip.so')
+
+# Methods and stopping rules (include/hprmat_hip.h).
+PLAIN_FP64 = 0
+MIXED_FP32 = 1
+MIXED_FP32_3M = 2
+SPLIT_BF16 = 3
+CRIT_ABS_MAX = 0
+CRIT_REL_FRO = 1
+
+REFERENCE_SYMBOLS = (
+    'gpu_solver_init_', 'gpu_solver_finalize_', 'gpu_host_unregister_',
+    'gpu_solve_mixed_', 'gpu_solve_hybrid_', 'gpu_solve_tf32_', 'gpu_solve_auto_',
+    'gpu_solve_double_', 'gpu_is_available_', 'gpu_get_device_count_', 'gpu_get_info_',
+    'gpu_multi_init_', 'gpu_multi_finalize_', 'gpu_solve_multi_', 'gpu_solve_multi_mixed_',
+)
+NATIVE_SYMBOLS = (
+    'hprmat_hip_lu_create', 'hprmat_hip_lu_destroy', 'hprmat_hip_lu_factor',
+    'hprmat_hip_lu_info', 'hprmat_hip_lu_pivots', 'hprmat_hip_lu_solve', 'hprmat_hip_solve',
+    'hprmat_hip_assemble', 'hprmat_hip_r_to_s', 'hprmat_hip_version',
+)
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [('info', ctypes.c_int), ('sweeps', ctypes.c_int), ('checks', ctypes.c_int),
+                ('converged', ctypes.c_int), ('res_max', ctypes.c_double),
+                ('res_rel', ctypes.c_double), ('b_fro', ctypes.c_double),
+                ('hist', ctypes.c_double * 32)]
+
+    def as_dict(self):
+        return dict(info=self.info, sweeps=self.sweeps, checks=self.checks,
+                    converged=bool(self.converged), res_max=self.res_max,
+                    res_rel=self.res_rel, b_fro=self.b_fro,
+                    hist=[self.hist[i] for i in range(min(self.checks, 32))])
+
+
+_lib = None
+
+
+def load():
+    """Load the library once; raises RuntimeError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError('libhprmat_hip.so is not built (run: python -m hprmat_amd.build)')
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    L = ctypes.c_int64
+    D = ctypes.c_double
+    IP = ctypes.POINTER(ctypes.c_int)
+    DP = ctypes.POINTER(ctypes.c_double)
+    for name in ('gpu_solve_mixed_', 'gpu_solve_hybrid_', 'gpu_solve_tf32_', 'gpu_solve_auto_',
+                 'gpu_solve_multi_mixed_'):
+        f = getattr(lib, name)
+        f.argtypes = [P, P, IP, IP, IP, DP, IP]
+        f.restype = I
+    for name in ('gpu_solve_double_', 'gpu_solve_multi_'):
+        f = getattr(lib, name)
+        f.argtypes = [P, P, IP, IP, IP]
+        f.restype = I
+    lib.gpu_solver_init_.argtypes = [IP]
+    lib.gpu_multi_init_.argtypes = [IP]
+    for name in ('gpu_solver_finalize_', 'gpu_host_unregister_', 'gpu_is_available_',
+                 'gpu_get_device_count_', 'gpu_multi_finalize_'):
+        getattr(lib, name).argtypes = []
+        getattr(lib, name).restype = I
+    lib.gpu_get_info_.argtypes = [ctypes.c_char_p, IP]
+    lib.gpu_get_info_.restype = None
+    lib.hprmat_hip_lu_create.argtypes = [ctypes.POINTER(P)]
+    lib.hprmat_hip_lu_destroy.argtypes = [P]
+    lib.hprmat_hip_lu_factor.argtypes = [P, I, P, L, I, P]
+    lib.hprmat_hip_lu_info.argtypes = [P, IP]
+    lib.hprmat_hip_lu_pivots.argtypes = [P, IP]
+    DPtr = ctypes.c_void_p
+    lib.hprmat_hip_assemble.argtypes = [I, I, DPtr, DPtr, DPtr, D, DPtr, DPtr, DPtr, DPtr, DPtr, L, P]
+    lib.hprmat_hip_r_to_s.argtypes = [I, I, DPtr, L, DPtr, D, D, DPtr, I, DPtr, DPtr, DPtr, DPtr, DPtr, P]
+    lib.hprmat_hip_lu_solve.argtypes = [P, P, L, I, I, D, I, I, P, ctypes.POINTER(Stats)]
+    lib.hprmat_hip_solve.argtypes = [I, P, L, P, L, I, I, I, D, I, I, P, ctypes.POINTER(Stats)]
+    lib.hprmat_hip_version.restype = ctypes.c_char_p
+    for name in REFERENCE_SYMBOLS + NATIVE_SYMBOLS:
+        if getattr(lib, name, None) is None:
+            raise RuntimeError('libhprmat_hip.so lacks symbol ' + name)
+    _lib = lib
+    return lib
+
+
+def check(rc, what):
+    if rc != 0:
+        raise RuntimeError('%s failed with status %d' % (what, rc))

@@ -1,0 +1,381 @@
+// C ABI of libhprmat_hip: the reference-compatible gpu_* entry points
+// (src/cusolver_interface.cu:193-1641) and the native device-pointer
+// interface used by the Python binding.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "hprmat_hip.h"
+#include "lu.hpp"
+
+using hpr::RefineState;
+
+struct hprmat_hip_lu {
+  int n = 0;
+  int method = -1;
+  int64_t ld = 0;
+  void* fac = nullptr;
+  size_t fac_cap = 0;
+  int* ipiv = nullptr;
+  size_t ipiv_cap = 0;
+  int* dinfo = nullptr;
+  void* scr = nullptr;
+  size_t scr_cap = 0;
+  hpr::Scratch sc;
+  const double2* A64 = nullptr;
+  int64_t lda = 0;
+  double2* Brm = nullptr;
+  double2* Xrm = nullptr;
+  double2* R = nullptr;
+  void* RF = nullptr;
+  size_t rhs_cap = 0;  // elements of each n x nrhs buffer
+  double* part = nullptr;
+  RefineState* state = nullptr;
+  hpr::LuPlan plan;
+};
+
+namespace {
+
+int grow(void** p, size_t* cap, size_t bytes) {
+  if (bytes <= *cap && *p) return 0;
+  if (*p) { hipFree(*p); *p = nullptr; *cap = 0; }
+  if (hipMalloc(p, bytes ? bytes : 256) != hipSuccess) { *p = nullptr; return -2; }
+  *cap = bytes;
+  return 0;
+}
+
+size_t factor_elem_bytes(int method) { return method == HPRMAT_PLAIN_FP64 ? 16 : 8; }
+
+int64_t padded_ld(int n) { return (((int64_t)n + 15) / 16) * 16 + 16; }
+
+void free_lu(hprmat_hip_lu* h) {
+  void* ps[] = {h->fac, h->ipiv, h->dinfo, h->scr, h->Brm, h->Xrm, h->R, h->RF, h->part, h->state};
+  for (void* p : ps)
+    if (p) hipFree(p);
+  *h = hprmat_hip_lu();
+}
+
+int lu_factor(hprmat_hip_lu* h, int method, const void* dA, int64_t lda, int n, hipStream_t st) {
+  if (n < 0 || lda < n) return -1;
+  if (method < HPRMAT_PLAIN_FP64 || method > HPRMAT_MIXED_FP32_3M) {
+    fprintf(stderr, "hprmat_hip: factorization method %d is not available\n", method);
+    return -1;
+  }
+  const int64_t ld = padded_ld(n);
+  if (grow(&h->fac, &h->fac_cap, (size_t)ld * n * factor_elem_bytes(method))) return -2;
+  size_t icap = h->ipiv_cap;
+  if (grow((void**)&h->ipiv, &icap, (size_t)(n + 1) * sizeof(int))) return -2;
+  h->ipiv_cap = icap;
+  if (!h->dinfo) {
+    if (hipMalloc((void**)&h->dinfo, 256) != hipSuccess) return -2;
+  }
+  if (!h->state) {
+    if (hipMalloc((void**)&h->state, sizeof(RefineState)) != hipSuccess) return -2;
+    if (hipMalloc((void**)&h->part, 8192 * sizeof(double)) != hipSuccess) return -2;
+  }
+  if (grow(&h->scr, &h->scr_cap, hpr::scratch_bytes(n))) return -2;
+  hpr::scratch_bind(h->sc, h->scr, n);
+  h->n = n;
+  h->method = method;
+  h->ld = ld;
+  h->A64 = (const double2*)dA;
+  h->lda = lda;
+  const double2* A = (const double2*)dA;
+  int rc;
+  if (method == HPRMAT_PLAIN_FP64) {
+    double2* F = (double2*)h->fac;
+    rc = hpr::transpose_cast<double, double>(n, n, A, lda, F, ld, st);
+    if (!rc) rc = hpr::getrf<double>(F, ld, n, h->ipiv, h->dinfo, h->sc, h->plan, false, st);
+  } else {
+    float2* F = (float2*)h->fac;
+    rc = hpr::transpose_cast<double, float>(n, n, A, lda, F, ld, st);
+    if (!rc)
+      rc = hpr::getrf<float>(F, ld, n, h->ipiv, h->dinfo, h->sc, h->plan,
+                             method == HPRMAT_MIXED_FP32_3M, st);
+  }
+  return rc;
+}
+
+int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, double tol,
+             int crit, int final_check, hipStream_t st, hprmat_hip_stats* stats) {
+  if (h->method < 0) return -1;
+  const int n = h->n;
+  if (nrhs < 0 || ldb < n) return -1;
+  const size_t cnt = (size_t)n * nrhs;
+  if (cnt > h->rhs_cap || !h->Brm) {
+    size_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    if (h->Brm) { hipFree(h->Brm); hipFree(h->Xrm); hipFree(h->R); hipFree(h->RF); }
+    h->Brm = h->Xrm = h->R = nullptr;
+    h->RF = nullptr;
+    h->rhs_cap = 0;
+    if (grow((void**)&h->Brm, &c0, cnt * 16) || grow((void**)&h->Xrm, &c1, cnt * 16) ||
+        grow((void**)&h->R, &c2, cnt * 16) || grow(&h->RF, &c3, cnt * 16))
+      return -2;
+    h->rhs_cap = cnt;
+  }
+  if (hpr::transpose_cast<double, double>(n, nrhs, (const double2*)dB, ldb, h->Brm, nrhs, st))
+    return -1;
+  int rc;
+  const bool g3m = h->method == HPRMAT_MIXED_FP32_3M;
+  if (h->method == HPRMAT_PLAIN_FP64)
+    rc = hpr::refine_solve<double>((const double2*)h->fac, h->ld, n, h->ipiv, h->A64, h->lda,
+                                   h->Brm, h->Xrm, nrhs, max_refine, tol, crit, h->state, h->R,
+                                   (double2*)h->RF, h->part, h->plan, false, st);
+  else
+    rc = hpr::refine_solve<float>((const float2*)h->fac, h->ld, n, h->ipiv, h->A64, h->lda,
+                                  h->Brm, h->Xrm, nrhs, max_refine, tol, crit, h->state, h->R,
+                                  (float2*)h->RF, h->part, h->plan, g3m, st);
+  if (rc) return rc;
+  if (final_check)
+    rc = hpr::residual_check(n, nrhs, h->A64, h->lda, h->Brm, h->Xrm, h->R, h->part, h->state,
+                             crit, tol, st);
+  if (rc) return rc;
+  if (hpr::transpose_cast<double, double>(nrhs, n, h->Xrm, nrhs, (double2*)dB, ldb, st)) return -1;
+  if (stats) {
+    RefineState s;
+    int info = 0;
+    HPR_CHECK(hipMemcpyAsync(&s, h->state, sizeof(s), hipMemcpyDeviceToHost, st));
+    HPR_CHECK(hipMemcpyAsync(&info, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, st));
+    HPR_CHECK(hipStreamSynchronize(st));
+    stats->info = info;
+    stats->sweeps = s.sweeps;
+    stats->checks = s.checks;
+    stats->converged = s.done;
+    stats->res_max = s.res_max;
+    stats->res_rel = s.res_fro / (s.b_fro != 0.0 ? s.b_fro : 1.0);
+    stats->b_fro = s.b_fro;
+    for (int i = 0; i < 32; ++i) stats->hist[i] = i < s.checks ? s.hist[i] : 0.0;
+  }
+  return 0;
+}
+
+// ---- process-global state of the reference ABI ---------------------------
+struct Global {
+  int init = 0;
+  int device = 0;
+  hprmat_hip_lu lu;
+  hprmat_hip_lu native;  // library-global handle of hprmat_hip_solve
+  void* dA = nullptr;
+  size_t dA_cap = 0;
+  void* dB = nullptr;
+  size_t dB_cap = 0;
+  hipStream_t stream = nullptr;
+} G;
+
+int ensure_init() {
+  if (G.init) return 0;
+  int d = 0;
+  return gpu_solver_init_(&d);
+}
+
+// Host-pointer solve shared by the gpu_solve_* entry points.
+int host_solve(int method, double* hA, double* hB, int n, int nrhs, int max_refine, double tol,
+               int crit, int guard, int* info) {
+  if (ensure_init() != 0) { *info = -1; return -1; }
+  const size_t abytes = (size_t)n * n * 16, bbytes = (size_t)n * nrhs * 16;
+  if (grow(&G.dA, &G.dA_cap, abytes) || grow(&G.dB, &G.dB_cap, bbytes)) { *info = -2; return -1; }
+  HPR_CHECK(hipMemcpyAsync(G.dA, hA, abytes, hipMemcpyHostToDevice, G.stream));
+  HPR_CHECK(hipMemcpyAsync(G.dB, hB, bbytes, hipMemcpyHostToDevice, G.stream));
+  int rc = lu_factor(&G.lu, method, G.dA, n, n, G.stream);
+  if (rc == -2) { *info = -2; return -1; }
+  if (rc) return -1;
+  int finfo = 0;
+  HPR_CHECK(hipMemcpyAsync(&finfo, G.lu.dinfo, sizeof(int), hipMemcpyDeviceToHost, G.stream));
+  HPR_CHECK(hipStreamSynchronize(G.stream));
+  int err = 0;
+  HPR_CHECK(hipMemcpy(&err, G.lu.sc.err, sizeof(int), hipMemcpyDeviceToHost));
+  if (err) { fprintf(stderr, "hprmat_hip: panel barrier timed out\n"); return -1; }
+  if (finfo != 0) {
+    fprintf(stderr, "hprmat_hip: getrf failed with info = %d\n", finfo);
+    *info = finfo;
+    return -1;
+  }
+  hprmat_hip_stats s;
+  rc = lu_solve(&G.lu, G.dB, n, nrhs, max_refine, tol, crit, guard, G.stream, &s);
+  if (rc == -2) { *info = -2; return -1; }
+  if (rc) return -1;
+  if (guard && max_refine > 0) {
+    RefineState st;
+    HPR_CHECK(hipMemcpy(&st, G.lu.state, sizeof(st), hipMemcpyDeviceToHost));
+    if (st.res_absmax > 1e-6 * (st.b_max + 1.0) && getenv("HPRMAT_MGPU_NO_FALLBACK") == NULL) {
+      fprintf(stderr, "hprmat_hip: mixed refinement did not converge (||r||_inf/||b||_inf = %.2e)\n",
+              st.res_absmax / (st.b_max + 1.0));
+      *info = 99;
+      return -1;
+    }
+  }
+  HPR_CHECK(hipMemcpyAsync(hB, G.dB, bbytes, hipMemcpyDeviceToHost, G.stream));
+  HPR_CHECK(hipStreamSynchronize(G.stream));
+  *info = 0;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpu_solver_init_(int* device_id) {
+  if (G.init) return 0;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return -1;
+  G.device = device_id ? *device_id : 0;
+  HPR_CHECK(hipSetDevice(G.device));
+  hipDeviceProp_t prop;
+  HPR_CHECK(hipGetDeviceProperties(&prop, G.device));
+  printf("GPU Solver initialized on: %s (%.1f GB)\n", prop.name, prop.totalGlobalMem / 1e9);
+  HPR_CHECK(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
+  G.init = 1;
+  return 0;
+}
+
+int gpu_solver_finalize_(void) {
+  if (!G.init) return 0;
+  hipDeviceSynchronize();
+  free_lu(&G.lu);
+  free_lu(&G.native);
+  if (G.dA) hipFree(G.dA);
+  if (G.dB) hipFree(G.dB);
+  G.dA = G.dB = nullptr;
+  G.dA_cap = G.dB_cap = 0;
+  if (G.stream) hipStreamDestroy(G.stream);
+  G.stream = nullptr;
+  G.init = 0;
+  return 0;
+}
+
+int gpu_host_unregister_(void) { return 0; }
+
+int gpu_solve_mixed_(double* A, double* B, int* n, int* nrhs, int* max_refine, double* tol,
+                     int* info) {
+  return host_solve(HPRMAT_MIXED_FP32, A, B, *n, *nrhs, *max_refine, *tol, HPRMAT_CRIT_ABS_MAX, 0,
+                    info);
+}
+
+int gpu_solve_hybrid_(double* A, double* B, int* n, int* nrhs, int* max_refine, double* tol,
+                      int* info) {
+  return host_solve(HPRMAT_MIXED_FP32, A, B, *n, *nrhs, *max_refine, *tol, HPRMAT_CRIT_ABS_MAX, 0,
+                    info);
+}
+
+int gpu_solve_tf32_(double* A, double* B, int* n, int* nrhs, int* max_refine, double* tol,
+                    int* info) {
+  int refine = *max_refine > 0 ? *max_refine : 5;
+  return host_solve(HPRMAT_MIXED_FP32_3M, A, B, *n, *nrhs, refine, *tol, HPRMAT_CRIT_ABS_MAX, 0,
+                    info);
+}
+
+int gpu_solve_auto_(double* A, double* B, int* n, int* nrhs, int* max_refine, double* tol,
+                    int* info) {
+  if (ensure_init() == 0) {
+    size_t need = (size_t)(*n) * (*n) * 32 + (size_t)(*n) * (*nrhs) * (16 * 3 + 8);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, G.device) == hipSuccess && need > prop.totalGlobalMem * 0.9)
+      fprintf(stderr, "Warning: Matrix too large for single GPU (%.2f GB required, %.2f GB available)\n",
+              need / 1e9, prop.totalGlobalMem / 1e9);
+  }
+  return gpu_solve_mixed_(A, B, n, nrhs, max_refine, tol, info);
+}
+
+int gpu_solve_double_(double* A, double* B, int* n, int* nrhs, int* info) {
+  return host_solve(HPRMAT_PLAIN_FP64, A, B, *n, *nrhs, 0, 0.0, HPRMAT_CRIT_ABS_MAX, 0, info);
+}
+
+int gpu_is_available_(void) {
+  int c = 0;
+  return (hipGetDeviceCount(&c) == hipSuccess && c > 0) ? 1 : 0;
+}
+
+int gpu_get_device_count_(void) {
+  int c = 0;
+  return hipGetDeviceCount(&c) == hipSuccess ? c : 0;
+}
+
+void gpu_get_info_(char* info_str, int* len) {
+  if (!gpu_is_available_()) {
+    snprintf(info_str, *len, "No GPU available");
+    return;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+    snprintf(info_str, *len, "No GPU available");
+    return;
+  }
+  snprintf(info_str, *len, "%s (%.1f GB)", prop.name, prop.totalGlobalMem / 1e9);
+}
+
+int gpu_multi_init_(int* ngpu) {
+  if (!gpu_is_available_()) { *ngpu = 0; return -1; }
+  if (ensure_init() != 0) { *ngpu = 0; return -1; }
+  *ngpu = 1;
+  return 0;
+}
+
+int gpu_multi_finalize_(void) { return 0; }
+
+int gpu_solve_multi_(double* A, double* B, int* n, int* nrhs, int* info) {
+  return gpu_solve_double_(A, B, n, nrhs, info);
+}
+
+int gpu_solve_multi_mixed_(double* A, double* B, int* n, int* nrhs, int* max_refine, double* tol,
+                           int* info) {
+  return host_solve(HPRMAT_MIXED_FP32, A, B, *n, *nrhs, *max_refine, *tol, HPRMAT_CRIT_ABS_MAX, 1,
+                    info);
+}
+
+int hprmat_hip_lu_create(hprmat_hip_lu_t* h) {
+  if (!h) return -1;
+  *h = new (std::nothrow) hprmat_hip_lu();
+  return *h ? 0 : -2;
+}
+
+int hprmat_hip_lu_destroy(hprmat_hip_lu_t h) {
+  if (!h) return 0;
+  hipDeviceSynchronize();
+  free_lu(h);
+  delete h;
+  return 0;
+}
+
+int hprmat_hip_lu_factor(hprmat_hip_lu_t h, int method, const void* dA, int64_t lda, int n,
+                         void* stream) {
+  if (!h) return -1;
+  return lu_factor(h, method, dA, lda, n, (hipStream_t)stream);
+}
+
+int hprmat_hip_lu_info(hprmat_hip_lu_t h, int* info) {
+  if (!h || !h->dinfo) return -1;
+  HPR_CHECK(hipDeviceSynchronize());
+  int err = 0;
+  HPR_CHECK(hipMemcpy(info, h->dinfo, sizeof(int), hipMemcpyDeviceToHost));
+  HPR_CHECK(hipMemcpy(&err, h->sc.err, sizeof(int), hipMemcpyDeviceToHost));
+  return err ? -3 : 0;
+}
+
+int hprmat_hip_lu_pivots(hprmat_hip_lu_t h, int* ipiv) {
+  if (!h || !h->ipiv || h->n <= 0) return -1;
+  HPR_CHECK(hipDeviceSynchronize());
+  HPR_CHECK(hipMemcpy(ipiv, h->ipiv, (size_t)h->n * sizeof(int), hipMemcpyDeviceToHost));
+  for (int i = 0; i < h->n; ++i) ipiv[i] += 1;
+  return 0;
+}
+
+int hprmat_hip_lu_solve(hprmat_hip_lu_t h, void* dB, int64_t ldb, int nrhs, int max_refine,
+                        double tol, int criterion, int final_check, void* stream,
+                        hprmat_hip_stats* stats) {
+  if (!h) return -1;
+  return lu_solve(h, dB, ldb, nrhs, max_refine, tol, criterion, final_check, (hipStream_t)stream,
+                  stats);
+}
+
+int hprmat_hip_solve(int method, const void* dA, int64_t lda, void* dB, int64_t ldb, int n,
+                     int nrhs, int max_refine, double tol, int criterion, int final_check,
+                     void* stream, hprmat_hip_stats* stats) {
+  int rc = lu_factor(&G.native, method, dA, lda, n, (hipStream_t)stream);
+  if (rc) return rc;
+  return lu_solve(&G.native, dB, ldb, nrhs, max_refine, tol, criterion, final_check,
+                  (hipStream_t)stream, stats);
+}
+
+const char* hprmat_hip_version(void) { return "hprmat_hip 0.1 (gfx950)"; }
+
+}  // extern "C"

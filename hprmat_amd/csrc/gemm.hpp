@@ -1,0 +1,173 @@
+// Complex GEMM on MFMA: C = beta * C + alpha * A * B, alpha = +-1, beta in {0, 1}.
+//
+// A is M x K (row-major, or column-major when A_COL), B is K x N row-major,
+// C is M x N row-major.  One workgroup of 256 threads (4 waves) computes a
+// BM x BN tile; each wave owns a WM x WN sub-tile made of 16 x 16 MFMA blocks
+// with separate real/imaginary accumulators.  The complex product is formed
+// from real 16x16x4 MFMAs, either as four products (Ar Br - Ai Bi,
+// Ar Bi + Ai Br) or, with GAUSS3M, as Gauss's three products
+// P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi) (Higham, SIAM J. Matrix
+// Anal. Appl. 13 (1992)), which saves a quarter of the matrix-core work.
+//
+// Tiles of A and B are staged through LDS split into real and imaginary
+// planes, k-major, so every MFMA operand read is a conflict-free ds_read of
+// 16 consecutive elements.  The next K-slice is prefetched into registers
+// while the current one is multiplied.  Workgroup ids are remapped so that
+// the workgroups of one XCD (ids congruent mod 8) sweep a compact group of
+// tiles and share A row-blocks and B column-blocks in that XCD's L2.
+#pragma once
+#include "common.hpp"
+
+template <class T, int BM, int BN, int BK, int WM, int WN, bool A_COL, bool GAUSS3M>
+__global__ void __launch_bounds__(256)
+zgemm_kernel(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
+             const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
+             int64_t ldc, T alpha, int beta, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  static_assert((BM / WM) * (BN / WN) == 4, "four waves per workgroup");
+  static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 4 == 0, "tile shape");
+  constexpr int NT = 256;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int PA = BM + 1, PB = BN + 1;  // padded LDS row lengths
+  constexpr int EA = BM * BK / NT, EB = BK * BN / NT;
+  static_assert(EA * NT == BM * BK && EB * NT == BK * BN, "load split");
+  typedef typename acc_t<T>::type acc;
+
+  __shared__ T As[2][BK][PA];  // [re/im][k][i]
+  __shared__ T Bs[2][BK][PB];  // [re/im][k][j]
+
+  // XCD-aware tile order: workgroups b with equal b % 8 run on one XCD.
+  const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+  const int total = tiles_m * tiles_n;
+  int t = blockIdx.x;
+  if ((total & 7) == 0) t = (t & 7) * (total >> 3) + (t >> 3);
+  constexpr int GROUP = 8;
+  const int group = t / (GROUP * tiles_n);
+  const int first_m = group * GROUP;
+  const int gsz = min(tiles_m - first_m, GROUP);
+  const int tm = first_m + (t % (GROUP * tiles_n)) % gsz;
+  const int tn = (t % (GROUP * tiles_n)) / gsz;
+  const int row0 = tm * BM, col0 = tn * BN;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave / (BN / WN), wc = wave % (BN / WN);
+  const int li = lane & 15, lk = lane >> 4;
+
+  acc acc_re[FM][FN], acc_im[FM][FN], acc_p3[GAUSS3M ? FM : 1][GAUSS3M ? FN : 1];
+  for (int a = 0; a < FM; ++a)
+    for (int b = 0; b < FN; ++b) {
+      acc_re[a][b] = acc{};
+      acc_im[a][b] = acc{};
+      if (GAUSS3M) acc_p3[a][b] = acc{};
+    }
+
+  cx<T> ra[EA], rb[EB];
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < EA; ++e) {
+      int idx = tid + e * NT, i, k;
+      if (A_COL) { i = idx % BM; k = idx / BM; } else { i = idx / BK; k = idx % BK; }
+      int gi = row0 + i, gk = k0 + k;
+      ra[e] = (gi < M && gk < K)
+                  ? (A_COL ? A[(int64_t)gk * lda + gi] : A[(int64_t)gi * lda + gk])
+                  : mkc<T>(0, 0);
+    }
+#pragma unroll
+    for (int e = 0; e < EB; ++e) {
+      int idx = tid + e * NT, k = idx / BN, j = idx % BN;
+      int gk = k0 + k, gj = col0 + j;
+      rb[e] = (gk < K && gj < N) ? B[(int64_t)gk * ldb + gj] : mkc<T>(0, 0);
+    }
+  };
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int e = 0; e < EA; ++e) {
+      int idx = tid + e * NT, i, k;
+      if (A_COL) { i = idx % BM; k = idx / BM; } else { i = idx / BK; k = idx % BK; }
+      As[0][k][i] = ra[e].x;
+      As[1][k][i] = ra[e].y;
+    }
+#pragma unroll
+    for (int e = 0; e < EB; ++e) {
+      int idx = tid + e * NT, k = idx / BN, j = idx % BN;
+      Bs[0][k][j] = rb[e].x;
+      Bs[1][k][j] = rb[e].y;
+    }
+  };
+
+  load_tiles(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
+    store_tiles();
+    __syncthreads();
+    if (k0 + BK < K) load_tiles(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T ar[FM], ai[FM], br[FN], bi[FN];
+#pragma unroll
+      for (int a = 0; a < FM; ++a) {
+        int i = wr * WM + a * 16 + li;
+        ar[a] = As[0][kk + lk][i];
+        ai[a] = As[1][kk + lk][i];
+      }
+#pragma unroll
+      for (int b = 0; b < FN; ++b) {
+        int j = wc * WN + b * 16 + li;
+        br[b] = Bs[0][kk + lk][j];
+        bi[b] = Bs[1][kk + lk][j];
+      }
+      if (GAUSS3M) {
+        T as[FM], bs[FN];
+#pragma unroll
+        for (int a = 0; a < FM; ++a) as[a] = ar[a] + ai[a];
+#pragma unroll
+        for (int b = 0; b < FN; ++b) bs[b] = br[b] + bi[b];
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+          for (int b = 0; b < FN; ++b) {
+            acc_re[a][b] = mfma16x4(ar[a], br[b], acc_re[a][b]);  // P1
+            acc_im[a][b] = mfma16x4(ai[a], bi[b], acc_im[a][b]);  // P2
+            acc_p3[a][b] = mfma16x4(as[a], bs[b], acc_p3[a][b]);  // P3
+          }
+      } else {
+        T nai[FM];
+#pragma unroll
+        for (int a = 0; a < FM; ++a) nai[a] = -ai[a];
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+          for (int b = 0; b < FN; ++b) {
+            acc_re[a][b] = mfma16x4(ar[a], br[b], acc_re[a][b]);
+            acc_re[a][b] = mfma16x4(nai[a], bi[b], acc_re[a][b]);
+            acc_im[a][b] = mfma16x4(ar[a], bi[b], acc_im[a][b]);
+            acc_im[a][b] = mfma16x4(ai[a], br[b], acc_im[a][b]);
+          }
+      }
+    }
+  }
+
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int gi = row0 + wr * WM + a * 16 + mfma_row<T>(lk, r);
+        int gj = col0 + wc * WN + b * 16 + li;
+        if (gi < M && gj < N) {
+          T pr, pi;
+          if (GAUSS3M) {
+            pr = acc_re[a][b][r] - acc_im[a][b][r];
+            pi = acc_p3[a][b][r] - acc_re[a][b][r] - acc_im[a][b][r];
+          } else {
+            pr = acc_re[a][b][r];
+            pi = acc_im[a][b][r];
+          }
+          cx<T>* pc = C + (int64_t)gi * ldc + gj;
+          cx<T> c0 = beta ? *pc : mkc<T>(0, 0);
+          *pc = mkc<T>(c0.x + alpha * pr, c0.y + alpha * pi);
+        }
+      }
+}

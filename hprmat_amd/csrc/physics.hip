@@ -1,0 +1,174 @@
+// Lagrange-mesh system assembly and the R -> S step on the device.
+//
+// Assembly restates src/rmatrix_hp.F90:246-293 for one interval (ns = 1):
+//   C[(c,i),(c',j)] = delta_cc' [ (T+L)_ij + delta_ij (l_c(l_c+1)/r_i^2 -+ k_c^2)
+//                                 - [closed c] xc1_c blo0_ij / a ]
+//                     + delta_ij V_cc'(r_i)
+// written straight into column-major device storage, so the N x N matrix
+// never crosses PCIe (the inputs are O(nr^2 + nr nch^2)).
+//
+// R and S restate src/rmat_solvers.F90:597-609 (R = normfac q2^T X) and
+// src/rmatrix_hp.F90:336-368 (Z_O, Z_I, S = Z_O^-1 Z_I, FP64), with the
+// nopen x nopen solve done by this library's own getrf/getrs.
+#include "hprmat_hip.h"
+#include "lu.hpp"
+
+namespace {
+
+__global__ void __launch_bounds__(256)
+assemble_kernel(int nch, int nr, const double* __restrict__ tc, const double* __restrict__ blo0,
+                const double* __restrict__ r, double rmax0, const int* __restrict__ lval,
+                const double* __restrict__ qk, const double* __restrict__ xc1,
+                const double2* __restrict__ cpot, double2* __restrict__ A, int64_t lda) {
+  const int n = nch * nr;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  const int col = blockIdx.y;
+  if (row >= n) return;
+  const int c = row / nr, i = row % nr;
+  const int c2 = col / nr, j = col % nr;
+  double vr = 0.0, vi = 0.0;
+  if (c == c2) {
+    vr = tc[i * nr + j];
+    const double q = qk[c];
+    if (i == j) {
+      const double l = (double)lval[c];
+      vr += l * (l + 1.0) / (r[i] * r[i]);
+      if (q > 0.0) vr -= q * q;
+      if (q < 0.0) vr += q * q;
+    }
+    if (q < 0.0) vr -= xc1[c] * blo0[i * nr + j] / rmax0;
+  }
+  if (i == j) {
+    double2 v = cpot[((int64_t)i * nch + c) * nch + c2];
+    vr += v.x;
+    vi += v.y;
+  }
+  A[(int64_t)col * lda + row] = make_double2(vr, vi);
+}
+
+// R(a, b) = normfac * sum_r q2[r] X[a*nr + r, b]  (X column-major, R column-major)
+__global__ void rmatrix_kernel(int nch, int nr, const double2* __restrict__ X, int64_t ldx,
+                               const double* __restrict__ q2, double normfac,
+                               double2* __restrict__ R) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nch * nch) return;
+  const int a = t % nch, b = t / nch;
+  double sr = 0.0, si = 0.0;
+  const double2* x = X + (int64_t)b * ldx + (int64_t)a * nr;
+  for (int r = 0; r < nr; ++r) {
+    sr += q2[r] * x[r].x;
+    si += q2[r] * x[r].y;
+  }
+  R[(int64_t)b * nch + a] = make_double2(sr * normfac, si * normfac);
+}
+
+__device__ inline double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Z_O (row-major, the matrix to factor) and Z_I (row-major RHS) over the
+// open channels: crma2(m1, m2) = R(o1, o2)/a * sqrt(k_o1 / k_o2),
+// Z_O(:, m) = -crma2(:, m) cop(m) + delta co(m), Z_I likewise with conjugates.
+__global__ void zmatrices_kernel(int nch, int nopen, const double2* __restrict__ R, double rmax,
+                                 const double* __restrict__ qk, const int* __restrict__ open,
+                                 const double2* __restrict__ co, const double2* __restrict__ cop,
+                                 double2* __restrict__ ZO, double2* __restrict__ ZI) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nopen * nopen) return;
+  const int m1 = t / nopen, m2 = t % nopen;  // row m1, column m2
+  const int o1 = open[m1], o2 = open[m2];
+  double2 rr = R[(int64_t)o2 * nch + o1];
+  const double s = sqrt(qk[o1] / qk[o2]) / rmax;
+  double2 c2 = make_double2(rr.x * s, rr.y * s);
+  double2 zo = zmul(c2, cop[m2]);
+  double2 cc = make_double2(cop[m2].x, -cop[m2].y);
+  double2 zi = zmul(c2, cc);
+  zo = make_double2(-zo.x, -zo.y);
+  zi = make_double2(-zi.x, -zi.y);
+  if (m1 == m2) {
+    zo.x += co[m1].x; zo.y += co[m1].y;
+    zi.x += co[m1].x; zi.y -= co[m1].y;
+  }
+  ZO[(int64_t)m1 * nopen + m2] = zo;
+  ZI[(int64_t)m1 * nopen + m2] = zi;
+}
+
+// S column-major from the row-major solution.
+__global__ void store_colmajor_kernel(int n, const double2* __restrict__ Xr, double2* __restrict__ S) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * n) return;
+  const int i = t / n, j = t % n;
+  S[(int64_t)j * n + i] = Xr[t];
+}
+
+struct SmallWork {
+  void* buf = nullptr;
+  size_t cap = 0;
+} g_small;
+
+}  // namespace
+
+extern "C" {
+
+/* Assemble the complex-symmetric Lagrange-mesh matrix of one interval into
+   column-major device storage dA (lda >= nch*nr).  Inputs are device arrays:
+   tc, blo0 (nr x nr, row-major, already scaled by the interval length as
+   rmat_ini does), r (nr mesh radii), lval, qk, xc1 (nch; qk < 0 marks a
+   closed channel, xc1 its Whittaker log-derivative a W'/W), cpot (nr x nch x
+   nch complex, index [i][c][c']).  Replaces the host loops at
+   src/rmatrix_hp.F90:246-293. */
+int hprmat_hip_assemble(int nch, int nr, const double* tc, const double* blo0, const double* r,
+                        double rmax0, const int* lval, const double* qk, const double* xc1,
+                        const void* cpot, void* dA, int64_t lda, void* stream) {
+  const int n = nch * nr;
+  if (nch <= 0 || nr <= 0 || lda < n || n > 65535 * 4) return -1;
+  dim3 grid((n + 255) / 256, n);
+  assemble_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(nch, nr, tc, blo0, r, rmax0, lval, qk, xc1,
+                                                        (const double2*)cpot, (double2*)dA, lda);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+/* R = normfac q2^T X (nch x nch, column-major) and, when nopen > 0,
+   S = Z_O^-1 Z_I (nopen x nopen, column-major) in FP64 on the device.
+   X is the column-major N x nch solution of the Lagrange-mesh system; open
+   lists the open channels (0-based), co = G + iF and cop = rho O' per open
+   channel.  Replaces src/rmat_solvers.F90:597-609 and
+   src/rmatrix_hp.F90:336-368 (ZGESV(nopen) there). */
+int hprmat_hip_r_to_s(int nch, int nr, const void* dX, int64_t ldx, const double* q2,
+                      double normfac, double rmax, const double* qk, int nopen, const int* open,
+                      const void* co, const void* cop, void* dR, void* dS, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (nch <= 0 || nr <= 0 || nopen < 0 || nopen > nch) return -1;
+  rmatrix_kernel<<<(nch * nch + 255) / 256, 256, 0, st>>>(nch, nr, (const double2*)dX, ldx, q2,
+                                                         normfac, (double2*)dR);
+  if (nopen == 0) return hipGetLastError() == hipSuccess ? 0 : -1;
+  const int64_t ld = nopen;
+  const size_t zbytes = (size_t)nopen * nopen * sizeof(double2);
+  const size_t need = 2 * zbytes + 256 + (size_t)(nopen + 64) * sizeof(int) + 256 +
+                      hpr::scratch_bytes(nopen) + 256;
+  if (need > g_small.cap) {
+    if (g_small.buf) (void)hipFree(g_small.buf);
+    g_small.buf = nullptr;
+    g_small.cap = 0;
+    if (hipMalloc(&g_small.buf, need) != hipSuccess) return -2;
+    g_small.cap = need;
+  }
+  char* p = (char*)g_small.buf;
+  double2* ZO = (double2*)p;
+  double2* ZI = (double2*)(p + zbytes);
+  int* ipiv = (int*)(p + ((2 * zbytes + 255) & ~(size_t)255));
+  int* info = ipiv + nopen + 32;
+  void* scr = (char*)info + 256;
+  scr = (void*)(((uintptr_t)scr + 255) & ~(uintptr_t)255);
+  hpr::Scratch sc;
+  hpr::scratch_bind(sc, scr, nopen);
+  zmatrices_kernel<<<(nopen * nopen + 255) / 256, 256, 0, st>>>(
+      nch, nopen, (const double2*)dR, rmax, qk, open, (const double2*)co, (const double2*)cop, ZO, ZI);
+  hpr::LuPlan plan;
+  if (hpr::getrf<double>(ZO, ld, nopen, ipiv, info, sc, plan, false, st)) return -1;
+  if (hpr::getrs<double>(ZO, ld, nopen, ipiv, ZI, ld, nopen, nullptr, plan, false, st)) return -1;
+  store_colmajor_kernel<<<(nopen * nopen + 255) / 256, 256, 0, st>>>(nopen, ZI, (double2*)dS);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // extern "C"

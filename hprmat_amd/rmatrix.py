@@ -1,0 +1,188 @@
+"""Lagrange-mesh R-matrix front end of the GPU solve path (one interval).
+
+Mirrors the reference's physics entry points for ns = 1:
+* ``rmat_ini``       - src/rmatrix_hp.F90:61-159 (mesh, T+L, Bloch, q2)
+* ``channel_data``   - src/rmatrix_hp.F90:224-244 at eta = 0 (closed forms)
+* ``assemble``       - src/rmatrix_hp.F90:246-293, on the device
+* ``solve_rmatrix``  - src/rmat_solvers.F90:49-90 solver_type dispatch,
+                       every type served by libhprmat_hip on the GPU
+* ``r_to_s``         - src/rmat_solvers.F90:597-609 + src/rmatrix_hp.F90:336-368
+* ``rmatrix``        - the whole ns = 1 pipeline returning (S, nopen)
+
+Host work is O(nr^2 + nr nch^2); the N x N matrix is built, factored and
+solved on the device.
+"""
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+from scipy import special
+
+from . import _lib, solver
+
+
+@dataclass
+class Mesh:
+    nr: int
+    a: float
+    x: np.ndarray      # Gauss-Legendre nodes on (0, 1)
+    w: np.ndarray      # weights on (0, 1)
+    r: np.ndarray      # radii a x
+    tc: np.ndarray     # T + L(0), nr x nr, scaled by 1/a^2
+    blo0: np.ndarray   # Bloch block, nr x nr, scaled by 1/a
+    q2: np.ndarray     # basis values at r = a, scaled by 1/sqrt(a)
+
+
+def rmat_ini(nr, a):
+    """Regularised Lagrange-Legendre mesh of one interval (0, a)."""
+    t, w = np.polynomial.legendre.leggauss(nr)
+    x = (1.0 + t) / 2.0
+    w = w / 2.0
+    x2 = x * (1.0 - x)
+    i = np.arange(nr)
+    sign = np.where((i[:, None] + i[None, :]) % 2 == 1, -1.0, 1.0)
+    xi, xj = x[:, None], x[None, :]
+    with np.errstate(divide='ignore', invalid='ignore'):
+        off = (nr * (nr + 1) + 1 + (xi + xj - 2 * xi * xj) / (xi - xj) ** 2
+               - 1 / (1 - xi) - 1 / (1 - xj)) / np.sqrt(x2[:, None] * x2[None, :])
+    tc = sign * off
+    np.fill_diagonal(tc, (4 * nr * (nr + 1) + 3 + (1 - 6 * x) / x2) / (3 * x2))
+    blo0 = sign / np.sqrt(x2[:, None] * x2[None, :])
+    q2 = np.where((nr + i + 1) % 2 == 1, -1.0, 1.0) / np.sqrt(x2)
+    return Mesh(nr, a, x, w, x * a, tc / a ** 2, blo0 / a, q2 / math.sqrt(a))
+
+
+def channel_data(lval, qk, a):
+    """Open-channel (co, cop) and closed-channel xc1 at eta = 0.
+
+    co = G + iF, cop = rho (G' + iF') with F = rho j_l, G = -rho y_l, and
+    xc1 = rho W'/W with W = (2/pi) rho k_l(rho), rho = |k| a.
+    """
+    lval = np.asarray(lval)
+    qk = np.asarray(qk, dtype=np.float64)
+    rho = np.abs(qk) * a
+    open_idx = np.nonzero(qk > 0)[0]
+    co = np.zeros(len(open_idx), dtype=np.complex128)
+    cop = np.zeros(len(open_idx), dtype=np.complex128)
+    xc1 = np.zeros(len(qk))
+    for m, c in enumerate(open_idx):
+        l, p = int(lval[c]), rho[c]
+        j, jd = special.spherical_jn(l, p), special.spherical_jn(l, p, derivative=True)
+        y, yd = special.spherical_yn(l, p), special.spherical_yn(l, p, derivative=True)
+        co[m] = complex(-p * y, p * j)
+        cop[m] = complex(-(y + p * yd), j + p * jd) * p
+    for c in np.nonzero(qk < 0)[0]:
+        l, p = int(lval[c]), rho[c]
+        k, kd = special.spherical_kn(l, p), special.spherical_kn(l, p, derivative=True)
+        xc1[c] = p * (k + p * kd) / (p * k)
+    return open_idx.astype(np.int32), co, cop, xc1
+
+
+def _dev(x, dtype):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype).cuda()
+
+
+class DeviceMesh:
+    """Mesh arrays resident on the device (shared by many assemblies)."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        self.tc = _dev(mesh.tc, torch.float64)
+        self.blo0 = _dev(mesh.blo0, torch.float64)
+        self.r = _dev(mesh.r, torch.float64)
+        self.q2 = _dev(mesh.q2, torch.float64)
+
+
+def assemble(dmesh, lval, qk, xc1, cpot, out=None):
+    """Column-major storage of C (returned as an N x N tensor, see solver.colmajor)."""
+    lib = _lib.load()
+    nr = dmesh.mesh.nr
+    nch = len(lval)
+    n = nch * nr
+    if out is None:
+        out = torch.empty((n, n), dtype=torch.complex128, device='cuda')
+    lv = _dev(np.asarray(lval, dtype=np.int32), torch.int32)
+    q = _dev(qk, torch.float64)
+    xc = _dev(xc1, torch.float64)
+    cp = cpot if isinstance(cpot, torch.Tensor) else _dev(cpot, torch.complex128)
+    rc = lib.hprmat_hip_assemble(
+        nch, nr, ctypes.c_void_p(dmesh.tc.data_ptr()), ctypes.c_void_p(dmesh.blo0.data_ptr()),
+        ctypes.c_void_p(dmesh.r.data_ptr()), ctypes.c_double(dmesh.mesh.a),
+        ctypes.c_void_p(lv.data_ptr()), ctypes.c_void_p(q.data_ptr()),
+        ctypes.c_void_p(xc.data_ptr()), ctypes.c_void_p(cp.data_ptr()),
+        ctypes.c_void_p(out.data_ptr()), ctypes.c_int64(n), solver._stream_ptr())
+    _lib.check(rc, 'hprmat_hip_assemble')
+    return out
+
+
+def rhs_colmajor(dmesh, nch):
+    """Column-major storage of B (nch x N tensor): column c holds q2 in block c."""
+    nr = dmesh.mesh.nr
+    b = torch.zeros((nch, nch * nr), dtype=torch.complex128, device='cuda')
+    for c in range(nch):
+        b[c, c * nr:(c + 1) * nr] = dmesh.q2.to(torch.complex128)
+    return b
+
+
+# solver_type -> (method, max_refine, tol, criterion).  Types 1/2 keep the
+# semantics of the reference's CPU types (plain FP64; FP32 + FP64 refinement
+# with ||r||_F/||b||_F < 1e-10, two sweeps) but run on the GPU; type 3
+# (Woodbury, CPU-only) is not provided.
+SOLVER_TYPES = {
+    1: ('fp64', 0, 0.0, 'rel_fro'),
+    2: ('fp32', 2, 1.0e-10, 'rel_fro'),
+    4: ('fp32', None, 1.0e-10, 'abs_max'),
+    5: ('fp32_3m', 5, 1.0e-12, 'abs_max'),
+    6: ('fp32', 2, 1.0e-12, 'abs_max'),
+}
+
+
+def solve_colmajor(a_cm, b_cm, solver_type=1, max_refine=0):
+    """Solve C X = B in place (column-major storages) for a solver_type."""
+    if solver_type not in SOLVER_TYPES:
+        raise NotImplementedError('solver_type %d is not provided by libhprmat_hip' % solver_type)
+    method, mr, tol, crit = SOLVER_TYPES[solver_type]
+    if mr is None:
+        mr = max_refine
+    lu = solver.LU().factor(a_cm, method)
+    st = lu.solve_(b_cm, mr, tol, crit, final_check=False)
+    if st['info'] != 0:
+        raise RuntimeError('zero pivot at column %d' % st['info'])
+    return st
+
+
+def r_to_s(dmesh, x_cm, nch, qk, open_idx, co, cop, normfac=1.0):
+    """(R, S) on the device from the column-major solution storage x_cm (nch x N)."""
+    lib = _lib.load()
+    nopen = len(open_idx)
+    r = torch.empty((nch, nch), dtype=torch.complex128, device='cuda')
+    s = torch.empty((max(nopen, 1), max(nopen, 1)), dtype=torch.complex128, device='cuda')
+    q = _dev(qk, torch.float64)
+    o = _dev(np.asarray(open_idx, dtype=np.int32) if nopen else np.zeros(1, np.int32), torch.int32)
+    c0 = _dev(co if nopen else np.zeros(1, complex), torch.complex128)
+    c1 = _dev(cop if nopen else np.zeros(1, complex), torch.complex128)
+    rc = lib.hprmat_hip_r_to_s(nch, dmesh.mesh.nr, ctypes.c_void_p(x_cm.data_ptr()),
+                               ctypes.c_int64(x_cm.shape[1]), ctypes.c_void_p(dmesh.q2.data_ptr()),
+                               ctypes.c_double(normfac), ctypes.c_double(dmesh.mesh.a),
+                               ctypes.c_void_p(q.data_ptr()), nopen, ctypes.c_void_p(o.data_ptr()),
+                               ctypes.c_void_p(c0.data_ptr()), ctypes.c_void_p(c1.data_ptr()),
+                               ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(s.data_ptr()),
+                               solver._stream_ptr())
+    _lib.check(rc, 'hprmat_hip_r_to_s')
+    # column-major storages -> matrices
+    return r.transpose(0, 1), s[:nopen, :nopen].transpose(0, 1)
+
+
+def rmatrix(lval, qk, a, nr, cpot, solver_type=1, max_refine=0, dmesh=None):
+    """ns = 1 pipeline on the GPU: returns dict(S, R, nopen, stats)."""
+    if dmesh is None:
+        dmesh = DeviceMesh(rmat_ini(nr, a))
+    nch = len(lval)
+    open_idx, co, cop, xc1 = channel_data(lval, qk, a)
+    a_cm = assemble(dmesh, lval, qk, xc1, cpot)
+    b_cm = rhs_colmajor(dmesh, nch)
+    st = solve_colmajor(a_cm, b_cm, solver_type, max_refine)
+    r, s = r_to_s(dmesh, b_cm, nch, qk, open_idx, co, cop)
+    return dict(S=s, R=r, nopen=len(open_idx), stats=st, open=open_idx)

@@ -1,0 +1,114 @@
+"""PyTorch-facing dense complex solver over libhprmat_hip's native C ABI.
+
+Tensors are complex128 on the GPU.  The library works on column-major
+(Fortran) storage, like the reference's gpu_* entry points
+(src/cusolver_interface.cu); a torch tensor is row-major, so ``colmajor(A)``
+returns the contiguous storage of A in column-major order (a tensor shaped
+like A^T).  For complex-symmetric matrices (A^T = A, the Lagrange-mesh case)
+the row-major storage already is the column-major one and no copy is made.
+All work is ordered on torch's current stream.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+
+METHODS = {'fp64': _lib.PLAIN_FP64, 'fp32': _lib.MIXED_FP32,
+           'fp32_3m': _lib.MIXED_FP32_3M, 'bf16x3': _lib.SPLIT_BF16}
+CRITERIA = {'abs_max': _lib.CRIT_ABS_MAX, 'rel_fro': _lib.CRIT_REL_FRO}
+
+
+def _stream_ptr():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _check_tensor(t, name):
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.complex128 or not t.is_cuda:
+        raise TypeError('%s must be a complex128 CUDA tensor' % name)
+    if not t.is_contiguous():
+        raise ValueError('%s must be contiguous' % name)
+
+
+def colmajor(a, symmetric=False):
+    """Column-major storage of matrix ``a`` (returned as a contiguous tensor)."""
+    return a if symmetric else a.transpose(-2, -1).contiguous()
+
+
+class LU:
+    """LU factors of one n x n complex matrix, resident on the device.
+
+    ``factor`` keeps a reference to the FP64 column-major matrix because the
+    refinement residual reads it; ``solve`` reuses the factors (also for the
+    adjoint solve of a complex-symmetric matrix, A^T = A).
+    """
+
+    def __init__(self):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.hprmat_hip_lu_create(ctypes.byref(h)), 'hprmat_hip_lu_create')
+        self._h = h
+        self.n = 0
+        self.method = None
+        self._a_cm = None
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value:
+            self._lib.hprmat_hip_lu_destroy(h)
+            self._h = None
+
+    def factor(self, a_cm, method='fp64'):
+        """Factor the matrix whose column-major storage is ``a_cm`` (n x n)."""
+        _check_tensor(a_cm, 'a_cm')
+        n = a_cm.shape[0]
+        if a_cm.shape != (n, n):
+            raise ValueError('matrix must be square')
+        rc = self._lib.hprmat_hip_lu_factor(self._h, METHODS[method], ctypes.c_void_p(a_cm.data_ptr()),
+                                            n, n, _stream_ptr())
+        _lib.check(rc, 'hprmat_hip_lu_factor')
+        self.n, self.method, self._a_cm = n, method, a_cm
+        return self
+
+    def info(self):
+        v = ctypes.c_int()
+        _lib.check(self._lib.hprmat_hip_lu_info(self._h, ctypes.byref(v)), 'hprmat_hip_lu_info')
+        return v.value
+
+    def pivots(self):
+        """Row interchanges as LAPACK returns them (1-based numpy int32 array)."""
+        import numpy as np
+        out = np.zeros(self.n, dtype=np.int32)
+        rc = self._lib.hprmat_hip_lu_pivots(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        _lib.check(rc, 'hprmat_hip_lu_pivots')
+        return out
+
+    def solve_(self, b_cm, max_refine=0, tol=0.0, criterion='rel_fro', final_check=False,
+               stats=True):
+        """Solve in place; ``b_cm`` is the column-major storage of B (nrhs x n tensor)."""
+        _check_tensor(b_cm, 'b_cm')
+        if b_cm.dim() != 2 or b_cm.shape[1] != self.n:
+            raise ValueError('b_cm must have shape (nrhs, n)')
+        st = _lib.Stats() if stats else None
+        rc = self._lib.hprmat_hip_lu_solve(self._h, ctypes.c_void_p(b_cm.data_ptr()), self.n,
+                                           b_cm.shape[0], max_refine, tol, CRITERIA[criterion],
+                                           int(final_check), _stream_ptr(),
+                                           ctypes.byref(st) if st is not None else None)
+        _lib.check(rc, 'hprmat_hip_lu_solve')
+        return st.as_dict() if st is not None else None
+
+
+def solve(a, b, method='fp64', max_refine=0, tol=1e-12, criterion='rel_fro', final_check=True,
+          symmetric=False):
+    """X = A^-1 B for complex128 CUDA tensors a (n x n) and b (n x nrhs).
+
+    Returns (X, stats) with X an (n x nrhs) tensor.
+    """
+    _check_tensor(a, 'a')
+    a_cm = colmajor(a, symmetric)
+    b_cm = b.transpose(0, 1).contiguous()
+    lu = LU().factor(a_cm, method)
+    st = lu.solve_(b_cm, max_refine, tol, criterion, final_check)
+    if st['info'] != 0:
+        raise RuntimeError('zero pivot at column %d' % st['info'])
+    return b_cm.transpose(0, 1), st

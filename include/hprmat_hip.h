@@ -1,0 +1,181 @@
+/*
+ * hprmat_hip.h - C ABI of libhprmat_hip, the gfx950 dense complex LU + solve
+ * library behind HPRMAT's GPU solver types.
+ *
+ * Part 1 reproduces, name for name and argument for argument, the 15
+ * extern "C" entry points that HPRMAT's Fortran module
+ * src/gpu_solver_interface.F90:42-162 binds with bind(C, name=...): a build
+ * of HPRMAT that links libhprmat_hip instead of its CUDA object needs no
+ * source change.  Conventions kept from the reference:
+ *   - arguments by reference (Fortran style), n and nrhs are 32-bit;
+ *   - A and B are complex(c_double_complex) = interleaved (re, im) doubles,
+ *     column-major, lda = ldb = n;
+ *   - A is read only; B holds the RHS on entry and the solution on success
+ *     and is left untouched on failure;
+ *   - return 0 on success, -1 on failure; *info = 0 ok, -1 init failed,
+ *     -2 allocation failed, > 0 zero pivot (1-based), 99 refinement did not
+ *     converge (multi_mixed guard).
+ * Internal indexing is 64-bit (the reference's int32 n*n overflows at
+ * N > 46340); device buffers are library-owned and grow-only.
+ *
+ * Part 2 is the native device-pointer interface used by the Python/PyTorch
+ * binding: stream-ordered, no hidden host synchronisation unless statistics
+ * are requested, LU handles whose factors are reused for adjoint solves.
+ */
+#ifndef HPRMAT_HIP_H
+#define HPRMAT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- Part 1: reference-compatible ABI --------------------- */
+
+/* src/cusolver_interface.cu:195-219 (gpu_solver_init_): select the device,
+   print its name; idempotent. */
+int gpu_solver_init_(int *device_id);
+/* src/cusolver_interface.cu:221-256: release all device buffers. */
+int gpu_solver_finalize_(void);
+/* src/cusolver_interface.cu:262-269: drop the cached page-locked host
+   registration.  This library never page-locks the caller's buffer, so the
+   call only returns 0 (kept for link compatibility). */
+int gpu_host_unregister_(void);
+/* src/cusolver_interface.cu:321-443: complex-FP32 LU on the GPU, FP32 solve,
+   then up to max_refine FP64 refinement sweeps stopping when the residual
+   entry of largest |Re|+|Im| has modulus < tol. */
+int gpu_solve_mixed_(double *A, double *B, int *n, int *nrhs, int *max_refine,
+                     double *tol, int *info);
+/* src/cusolver_interface.cu:456-592: same factorization; the reference forms
+   the FP64 residual on the host, here it is formed on the GPU (MI355X FP64 is
+   not throttled), with identical stopping rule and results. */
+int gpu_solve_hybrid_(double *A, double *B, int *n, int *nrhs, int *max_refine,
+                      double *tol, int *info);
+/* src/cusolver_interface.cu:600-734: reduced-precision factorization with
+   refinement (default 5 sweeps when max_refine <= 0).  gfx950 has no TF32
+   matrix path; this entry point uses the split-BF16 three-product
+   factorization (TF32-like accuracy). */
+int gpu_solve_tf32_(double *A, double *B, int *n, int *nrhs, int *max_refine,
+                    double *tol, int *info);
+/* src/cusolver_interface.cu:1600-1639: memory-fit warning, then mixed. */
+int gpu_solve_auto_(double *A, double *B, int *n, int *nrhs, int *max_refine,
+                    double *tol, int *info);
+/* src/cusolver_interface.cu:737-795: plain FP64 LU + solve, no refinement. */
+int gpu_solve_double_(double *A, double *B, int *n, int *nrhs, int *info);
+/* src/cusolver_interface.cu:797-804 */
+int gpu_is_available_(void);
+/* src/cusolver_interface.cu:807-814 */
+int gpu_get_device_count_(void);
+/* src/cusolver_interface.cu:816-825: "name (xx.x GB)", NUL-terminated. */
+void gpu_get_info_(char *info_str, int *len);
+/* src/cusolver_interface.cu:832-912: one-GPU build; *ngpu is set to the
+   number of devices used (1). */
+int gpu_multi_init_(int *ngpu);
+/* src/cusolver_interface.cu:915-1019 */
+int gpu_multi_finalize_(void);
+/* src/cusolver_interface.cu:1023-1331: FP64 LU + solve (one GPU). */
+int gpu_solve_multi_(double *A, double *B, int *n, int *nrhs, int *info);
+/* src/cusolver_interface.cu:1345-1597: FP32 LU + FP64 refinement, then the
+   reference's convergence guard ||r||_max > 1e-6 (||b||_max + 1) -> info 99
+   (disabled by HPRMAT_MGPU_NO_FALLBACK). */
+int gpu_solve_multi_mixed_(double *A, double *B, int *n, int *nrhs,
+                           int *max_refine, double *tol, int *info);
+
+/* ---------------- Part 2: native device-pointer interface -------------- */
+
+/* Factorization precision of the LU (all use FP64 residuals/updates). */
+enum {
+  HPRMAT_PLAIN_FP64 = 0,     /* complex FP64 LU on FP64 MFMA                */
+  HPRMAT_MIXED_FP32 = 1,     /* complex FP32 LU, 4 real MFMA products       */
+  HPRMAT_MIXED_FP32_3M = 2,  /* complex FP32 LU, Gauss 3-product MFMA       */
+  HPRMAT_SPLIT_BF16 = 3      /* FP32-stored LU, split-BF16 3-term products  */
+};
+/* Stopping rule of the refinement. */
+enum {
+  HPRMAT_CRIT_ABS_MAX = 0,   /* |r_ij| of the largest |Re|+|Im| entry < tol  */
+  HPRMAT_CRIT_REL_FRO = 1    /* ||B - A X||_F / ||B||_F <= tol               */
+};
+
+typedef struct hprmat_hip_stats {
+  int info;        /* 0 ok, > 0 zero pivot (1-based)                     */
+  int sweeps;      /* corrections applied                                */
+  int checks;      /* residual evaluations                               */
+  int converged;   /* stopping rule met at the last evaluation           */
+  double res_max;  /* modulus of the largest-|Re|+|Im| residual entry    */
+  double res_rel;  /* ||B - A X||_F / ||B||_F at the last evaluation     */
+  double b_fro;    /* ||B||_F                                            */
+  double hist[32]; /* relative residual after each evaluation            */
+} hprmat_hip_stats;
+
+typedef struct hprmat_hip_lu *hprmat_hip_lu_t;
+
+/* No reference counterpart (the reference keeps one process-global factor).
+   Create / destroy an LU handle owning device factors and work buffers. */
+int hprmat_hip_lu_create(hprmat_hip_lu_t *h);
+int hprmat_hip_lu_destroy(hprmat_hip_lu_t h);
+
+/* Factor the n x n complex FP64 matrix at device pointer dA (column-major,
+   lda) in the given precision.  dA is not modified and must stay valid while
+   the handle is used for refined solves (the FP64 residual reads it).
+   Stream-ordered, no host synchronisation; the zero-pivot flag is read with
+   hprmat_hip_lu_info.  Replaces cusolverDn?getrf at
+   src/cusolver_interface.cu:383 and :779. */
+int hprmat_hip_lu_factor(hprmat_hip_lu_t h, int method, const void *dA,
+                         int64_t lda, int n, void *stream);
+/* Synchronise and return the factorization's zero-pivot index (0 = none). */
+int hprmat_hip_lu_info(hprmat_hip_lu_t h, int *info);
+
+/* Synchronise and copy the row interchanges to host ipiv[n], 1-based as
+   LAPACK returns them (row i was interchanged with row ipiv[i]).  The
+   reference discards cusolverDn?getrf's d_ipiv (src/cusolver_interface.cu:111). */
+int hprmat_hip_lu_pivots(hprmat_hip_lu_t h, int *ipiv);
+
+/* Solve A X = B in place on the device (dB column-major, ldb, nrhs columns):
+   factor-precision solve, then up to max_refine FP64 refinement sweeps with
+   the given stopping rule; final_check adds one residual evaluation after
+   the last correction.  stats may be NULL (then nothing synchronises).
+   Replaces cusolverDn?getrs + the cublasZgemm / cublasIzamax refinement loop
+   at src/cusolver_interface.cu:393-434. */
+int hprmat_hip_lu_solve(hprmat_hip_lu_t h, void *dB, int64_t ldb, int nrhs,
+                        int max_refine, double tol, int criterion,
+                        int final_check, void *stream, hprmat_hip_stats *stats);
+
+/* One-shot factor + solve with a library-global handle (device pointers). */
+int hprmat_hip_solve(int method, const void *dA, int64_t lda, void *dB,
+                     int64_t ldb, int n, int nrhs, int max_refine, double tol,
+                     int criterion, int final_check, void *stream,
+                     hprmat_hip_stats *stats);
+
+/* Assemble the complex-symmetric Lagrange-mesh matrix of one interval
+   (ns = 1) into column-major device storage dA (lda >= nch*nr):
+     C[(c,i),(c',j)] = delta_cc' [(T+L)_ij + delta_ij (l_c(l_c+1)/r_i^2 -+ k_c^2)
+                                  - [qk_c < 0] xc1_c blo0_ij / rmax0]
+                       + delta_ij cpot[i][c][c'].
+   All pointers are device arrays: tc, blo0 (nr x nr row-major, scaled as
+   rmat_ini leaves them), r (nr radii), lval, qk, xc1 (nch), cpot (nr x nch x
+   nch complex).  Replaces the host assembly at src/rmatrix_hp.F90:246-293. */
+int hprmat_hip_assemble(int nch, int nr, const double *tc, const double *blo0,
+                        const double *r, double rmax0, const int *lval,
+                        const double *qk, const double *xc1, const void *cpot,
+                        void *dA, int64_t lda, void *stream);
+
+/* R = normfac q2^T X (nch x nch) and S = Z_O^-1 Z_I (nopen x nopen), both
+   column-major FP64 on the device.  dX is the column-major N x nch solution
+   (ldx), q2 the nr boundary values, qk the nch wave numbers, open the
+   0-based open-channel indices, co = G + iF and cop = rho O' per open
+   channel (device arrays).  Replaces src/rmat_solvers.F90:597-609 and
+   src/rmatrix_hp.F90:336-368. */
+int hprmat_hip_r_to_s(int nch, int nr, const void *dX, int64_t ldx,
+                      const double *q2, double normfac, double rmax,
+                      const double *qk, int nopen, const int *open,
+                      const void *co, const void *cop, void *dR, void *dS,
+                      void *stream);
+
+/* Library version string. */
+const char *hprmat_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HPRMAT_HIP_H */
