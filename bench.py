@@ -1,0 +1,173 @@
+#!/usr/bin/env python
+"""Benchmark of the dense complex LU + solve path on one MI355X.
+
+Workload: one Lagrange-mesh system of nch = 128 channels x nr = 200 mesh
+points (N = 25600 complex, complex-symmetric), 128 right-hand sides (one per
+channel), synthetic potentials from hprmat_amd.synth (seed 0).  A step is one
+full solve with the matrix already resident in HBM: factor (LU with partial
+pivoting) + triangular solves + FP64 iterative refinement until
+||B - A X||_F / ||B||_F <= 1e-12 (the last residual evaluation included).
+
+Methods timed back to back: 'fp64' (plain FP64 LU, FP64 refinement), and the
+mixed methods 'fp32_3m' / 'fp32' (complex-FP32 LU on MFMA, FP64 refinement).
+``value`` is the solves/s of the fastest mixed method that reaches the
+residual gate; per-method figures, the roofline of the dominant kernel (the
+trailing-update GEMM, timed with HIP events on the launch stream) and the
+CPU baseline (LAPACK zgesv + R, reference solver type 1) are in the line.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = "solves/s & TFLOP/s, N=25600 complex LU+solve: mixed vs FP64 at resid≤1e-12"
+PEAK = {'fp64': 78.6, 'fp32': 157.3, 'fp32_3m': 157.3}   # dense MFMA TFLOP/s, MI355X
+
+
+def flops_solve(n, nrhs):
+    return 8.0 / 3.0 * n ** 3 + 8.0 * n * n * nrhs
+
+
+def cpu_baseline(nch=64, nr=200, reps=3):
+    """Reference solver type 1 (ZGESV + R extraction) on host cores, bounded sample."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import lagrange, rmatrix as ormat, solve as osolve, synth as osynth
+    threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    lval, qk, cpot, _ = osynth.system(nch, nr, 15.0, 8.0, 5.0, 0)
+    mesh = lagrange.rmat_ini(nr, 1, 15.0)
+    from oracle import channels
+    _, _, _, xc1 = channels.channel_functions(lval, qk, 15.0)
+    c = ormat.assemble(mesh, lval, qk, cpot, xc1)
+    b = ormat.rhs(mesh['q2'][0], nch)
+    n = nch * nr
+    with threadpool_limits(limits=threads):
+        osolve.zgesv(c[:512, :512], b[:512, :4])          # warm-up of the BLAS pool
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            x, info = osolve.zgesv(c, b)
+            ormat.extract_r(x, mesh['q2'][0], nch)
+            ts.append(time.perf_counter() - t0)
+    ts.sort()
+    t = ts[len(ts) // 2]
+    tflops = flops_solve(n, nch) / t / 1e12
+    return {'value': tflops * 1e12 / flops_solve(25600, 128), 'unit': 'solves/s',
+            'cores': threads, 'kind': 'port',
+            'tflops': tflops, 'seconds': t,
+            'sample': ('LAPACK zgesv (scipy/OpenBLAS) + R extraction on N=%d (nch=%d x nr=%d, '
+                       'nrhs=%d) of the same recipe, median of %d; value = measured TFLOP/s '
+                       'divided by the N=25600/nrhs=128 flop count' % (n, nch, nr, nch, reps))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--nch', type=int, default=128)
+    ap.add_argument('--nr', type=int, default=200)
+    ap.add_argument('--methods', default='fp64,fp32_3m,fp32')
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: this build has no multi-GPU path '
+              '(--gpus %d requested); run with --gpus 1' % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from hprmat_amd import _lib, rmatrix, solver, synth
+    lib = _lib.load()
+    dev = torch.device('cuda:0')
+    torch.cuda.set_device(dev)
+
+    sysm = synth.System(args.nch, args.nr, 15.0, 8.0, 5.0, 0)
+    n, nrhs = sysm.n, sysm.nch
+    dmesh = rmatrix.DeviceMesh(sysm.mesh)
+    open_idx, co, cop, xc1 = rmatrix.channel_data(sysm.lval, sysm.qk, sysm.a)
+    a_cm = rmatrix.assemble(dmesh, sysm.lval, sysm.qk, xc1, sysm.cpot)
+    b0 = rmatrix.rhs_colmajor(dmesh, nrhs)
+    torch.cuda.synchronize()
+    fl = flops_solve(n, nrhs)
+
+    methods = {}
+    for m in args.methods.split(','):
+        lu = solver.LU()
+        max_refine = 10
+        b = b0.clone()
+
+        def step():
+            b.copy_(b0)
+            lu.factor(a_cm, m)
+            lu.solve_(b, max_refine, 1e-12, 'rel_fro', True, stats=False)
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        lib.hprmat_hip_prof_enable(1)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            step()
+        e1.record()
+        torch.cuda.synchronize()
+        lib.hprmat_hip_prof_enable(0)
+        import ctypes
+        gms, gfl, gcnt = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        lib.hprmat_hip_prof_read(ctypes.byref(gms), ctypes.byref(gfl), ctypes.byref(gcnt))
+        ms = e0.elapsed_time(e1) / args.steps
+        # verification pass (outside the timed region): same inputs, with statistics
+        b.copy_(b0)
+        lu.factor(a_cm, m)
+        st = lu.solve_(b, max_refine, 1e-12, 'rel_fro', True, stats=True)
+        methods[m] = {
+            'ms': ms, 'solves_per_s': 1000.0 / ms, 'tflops': fl / (ms * 1e-3) / 1e12,
+            'res_rel': st['res_rel'], 'sweeps': st['sweeps'], 'converged': st['converged'],
+            'info': st['info'],
+            'gemm': {'ms_per_launch': gms.value / max(gcnt.value, 1),
+                     'launches_per_step': gcnt.value // args.steps,
+                     'tflops': gfl.value / (gms.value * 1e-3) / 1e12 if gms.value > 0 else 0.0,
+                     'flops_per_launch': gfl.value / max(gcnt.value, 1)},
+        }
+        methods[m]['hist'] = st['hist']
+        print('# %s: %.1f ms/solve, %.2f solves/s, %.1f TFLOP/s, res_rel %.2e, sweeps %d, '
+              'trailing GEMM %.1f TFLOP/s, hist %s' % (
+                  m, ms, 1000.0 / ms, methods[m]['tflops'], st['res_rel'], st['sweeps'],
+                  methods[m]['gemm']['tflops'], ' '.join('%.1e' % h for h in st['hist'])),
+              file=sys.stderr, flush=True)
+        del lu
+
+    mixed = [m for m in methods if m != 'fp64' and methods[m]['res_rel'] <= 1e-12]
+    head = min(mixed, key=lambda m: methods[m]['ms']) if mixed else min(methods, key=lambda m: methods[m]['ms'])
+    hm = methods[head]
+    g = hm['gemm']
+    roof = {'bound': 'mfma', 'achieved': g['tflops'], 'peak': PEAK[head],
+            'unit': 'TFLOP/s', 'frac': g['tflops'] / PEAK[head], 'traffic': None,
+            'kernel': 'zgemm_kernel (trailing update A22 -= L21 U12, %s)' % head,
+            'flops_per_launch': g['flops_per_launch'], 'ms_per_launch': g['ms_per_launch']}
+    out = {
+        'metric': METRIC, 'value': hm['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': hm['ms'],
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'c64 LU + c128 refinement' if head != 'fp64' else 'c128',
+        'data': 'synthetic',
+        'config': {'workload': 'single Lagrange-mesh solve N=%d (nch=%d x nr=%d), nrhs=%d, '
+                               'gate ||r||_F/||b||_F<=1e-12' % (n, args.nch, args.nr, nrhs),
+                   'N': n, 'nrhs': nrhs, 'method': head, 'flops_per_solve': fl},
+        'tflops': hm['tflops'],
+        'methods': methods,
+        'speedup_mixed_vs_fp64': (methods['fp64']['ms'] / hm['ms']) if 'fp64' in methods else None,
+        'roofline': roof,
+    }
+    if not args.no_cpu:
+        out['cpu_baseline'] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

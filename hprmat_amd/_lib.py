@@ -27,7 +27,8 @@ REFERENCE_SYMBOLS = (
 NATIVE_SYMBOLS = (
     'hprmat_hip_lu_create', 'hprmat_hip_lu_destroy', 'hprmat_hip_lu_factor',
     'hprmat_hip_lu_info', 'hprmat_hip_lu_pivots', 'hprmat_hip_lu_solve', 'hprmat_hip_solve',
-    'hprmat_hip_assemble', 'hprmat_hip_r_to_s', 'hprmat_hip_version',
+    'hprmat_hip_assemble', 'hprmat_hip_r_to_s', 'hprmat_hip_prof_enable',
+    'hprmat_hip_prof_read', 'hprmat_hip_version',
 )
 
 
@@ -89,6 +90,8 @@ def load():
     lib.hprmat_hip_lu_solve.argtypes = [P, P, L, I, I, D, I, I, P, ctypes.POINTER(Stats)]
     lib.hprmat_hip_solve.argtypes = [I, P, L, P, L, I, I, I, D, I, I, P, ctypes.POINTER(Stats)]
     lib.hprmat_hip_version.restype = ctypes.c_char_p
+    lib.hprmat_hip_prof_enable.argtypes = [I]
+    lib.hprmat_hip_prof_read.argtypes = [DP, DP, IP]
     for name in REFERENCE_SYMBOLS + NATIVE_SYMBOLS:
         if getattr(lib, name, None) is None:
             raise RuntimeError('libhprmat_hip.so lacks symbol ' + name)

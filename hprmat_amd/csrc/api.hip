@@ -19,6 +19,10 @@ struct hprmat_hip_lu {
   int* ipiv = nullptr;
   size_t ipiv_cap = 0;
   int* dinfo = nullptr;
+  void* dinv = nullptr;       // inverses of the 64 x 64 diagonal blocks
+  size_t dinv_cap = 0;
+  int* perm = nullptr;        // final row permutation
+  size_t perm_cap = 0;
   void* scr = nullptr;
   size_t scr_cap = 0;
   hpr::Scratch sc;
@@ -49,7 +53,8 @@ size_t factor_elem_bytes(int method) { return method == HPRMAT_PLAIN_FP64 ? 16 :
 int64_t padded_ld(int n) { return (((int64_t)n + 15) / 16) * 16 + 16; }
 
 void free_lu(hprmat_hip_lu* h) {
-  void* ps[] = {h->fac, h->ipiv, h->dinfo, h->scr, h->Brm, h->Xrm, h->R, h->RF, h->part, h->state};
+  void* ps[] = {h->fac, h->ipiv, h->dinfo, h->dinv, h->perm, h->scr,  h->Brm,
+                h->Xrm, h->R,    h->RF,    h->part, h->state};
   for (void* p : ps)
     if (p) hipFree(p);
   *h = hprmat_hip_lu();
@@ -73,8 +78,10 @@ int lu_factor(hprmat_hip_lu* h, int method, const void* dA, int64_t lda, int n, 
     if (hipMalloc((void**)&h->state, sizeof(RefineState)) != hipSuccess) return -2;
     if (hipMalloc((void**)&h->part, 8192 * sizeof(double)) != hipSuccess) return -2;
   }
-  if (grow(&h->scr, &h->scr_cap, hpr::scratch_bytes(n))) return -2;
-  hpr::scratch_bind(h->sc, h->scr, n);
+  if (grow(&h->scr, &h->scr_cap, hpr::scratch_bytes())) return -2;
+  hpr::scratch_bind(h->sc, h->scr);
+  if (grow(&h->dinv, &h->dinv_cap, hpr::dinv_elems(n) * factor_elem_bytes(method))) return -2;
+  if (grow((void**)&h->perm, &h->perm_cap, (size_t)(n + 1) * sizeof(int))) return -2;
   h->n = n;
   h->method = method;
   h->ld = ld;
@@ -85,13 +92,15 @@ int lu_factor(hprmat_hip_lu* h, int method, const void* dA, int64_t lda, int n, 
   if (method == HPRMAT_PLAIN_FP64) {
     double2* F = (double2*)h->fac;
     rc = hpr::transpose_cast<double, double>(n, n, A, lda, F, ld, st);
-    if (!rc) rc = hpr::getrf<double>(F, ld, n, h->ipiv, h->dinfo, h->sc, h->plan, false, st);
+    if (!rc)
+      rc = hpr::getrf<double>(F, ld, n, h->ipiv, h->dinfo, (double2*)h->dinv, h->perm, h->sc,
+                              h->plan, false, st);
   } else {
     float2* F = (float2*)h->fac;
     rc = hpr::transpose_cast<double, float>(n, n, A, lda, F, ld, st);
     if (!rc)
-      rc = hpr::getrf<float>(F, ld, n, h->ipiv, h->dinfo, h->sc, h->plan,
-                             method == HPRMAT_MIXED_FP32_3M, st);
+      rc = hpr::getrf<float>(F, ld, n, h->ipiv, h->dinfo, (float2*)h->dinv, h->perm, h->sc,
+                             h->plan, method == HPRMAT_MIXED_FP32_3M, st);
   }
   return rc;
 }
@@ -118,11 +127,13 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
   int rc;
   const bool g3m = h->method == HPRMAT_MIXED_FP32_3M;
   if (h->method == HPRMAT_PLAIN_FP64)
-    rc = hpr::refine_solve<double>((const double2*)h->fac, h->ld, n, h->ipiv, h->A64, h->lda,
+    rc = hpr::refine_solve<double>((const double2*)h->fac, h->ld, n, (const double2*)h->dinv,
+                                   h->perm, h->A64, h->lda,
                                    h->Brm, h->Xrm, nrhs, max_refine, tol, crit, h->state, h->R,
                                    (double2*)h->RF, h->part, h->plan, false, st);
   else
-    rc = hpr::refine_solve<float>((const float2*)h->fac, h->ld, n, h->ipiv, h->A64, h->lda,
+    rc = hpr::refine_solve<float>((const float2*)h->fac, h->ld, n, (const float2*)h->dinv,
+                                  h->perm, h->A64, h->lda,
                                   h->Brm, h->Xrm, nrhs, max_refine, tol, crit, h->state, h->R,
                                   (float2*)h->RF, h->part, h->plan, g3m, st);
   if (rc) return rc;
@@ -374,6 +385,12 @@ int hprmat_hip_solve(int method, const void* dA, int64_t lda, void* dB, int64_t 
   if (rc) return rc;
   return lu_solve(&G.native, dB, ldb, nrhs, max_refine, tol, criterion, final_check,
                   (hipStream_t)stream, stats);
+}
+
+int hprmat_hip_prof_enable(int on) { return hpr::prof_enable(on); }
+
+int hprmat_hip_prof_read(double* ms, double* flops, int* count) {
+  return hpr::prof_read(ms, flops, count);
 }
 
 const char* hprmat_hip_version(void) { return "hprmat_hip 0.1 (gfx950)"; }

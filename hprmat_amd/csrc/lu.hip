@@ -10,9 +10,16 @@
 //     apply the panel's interchanges to the columns left and right of it;
 //     U12 = L11^-1 A12 (64-row diagonal solves + MFMA updates);
 //     A22 -= L21 U12 (MFMA complex GEMM, the O(N^3) hot loop).
-// getrs follows LAPACK zgetrs: row interchanges of B, forward substitution
-// with unit L, backward substitution with U, blocked by rs_nb rows with MFMA
-// GEMM updates (cusolverDn?getrs, src/cusolver_interface.cu:393, 787).
+//   after each panel, the inverses of its 64 x 64 diagonal blocks of L and U
+//   are formed (diag_inv_kernel) and kept for the triangular solves;
+//   the final row permutation is composed once at the end.
+// getrs follows LAPACK zgetrs: the permutation is applied while casting the
+// right-hand side, then forward (unit L) and backward (U) substitution by
+// 256-row block rows: one MFMA block-solve kernel per block row using the
+// stored 64 x 64 inverses, and one MFMA GEMM update of the remaining rows
+// (cusolverDn?getrs, src/cusolver_interface.cu:393, 787).
+#include <vector>
+
 #include "lu.hpp"
 #include "gemm.hpp"
 #include "lu_kernels.hpp"
@@ -22,15 +29,15 @@ namespace hpr {
 namespace {
 
 constexpr int LEAF_W = 32;
-constexpr int MAX_LEAF_G = 480;
-
-template <class T> constexpr int leaf_rpw_max() { return (64 * 1024) / (LEAF_W * (int)sizeof(cx<T>)); }
+constexpr int MAX_LEAF_G = 240;               // co-resident: one leaf workgroup per CU
+constexpr size_t LEAF_SLAB_TARGET = 96 * 1024;  // LDS slab per workgroup (preferred)
+constexpr size_t LEAF_SLAB_MAX = 150 * 1024;    // hard limit (160 KiB LDS per CU)
 
 // Dynamic LDS above 64 KiB needs an explicit opt-in per kernel; the return
 // code is checked so a refusal does not linger as the sticky last error.
 template <class K> void allow_big_lds(K kernel) {
   hipError_t e = hipFuncSetAttribute((const void*)kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 152 * 1024);
   if (e != hipSuccess) {
     fprintf(stderr, "hprmat_hip: hipFuncSetAttribute failed: %s\n", hipGetErrorString(e));
     (void)hipGetLastError();
@@ -181,24 +188,36 @@ norm_final_kernel(int nblocks, const double* __restrict__ part, RefineState* __r
   }
 }
 
+
+// Leaf layout: G workgroups of rpw rows each, preferring a 96 KiB LDS slab and
+// growing it (up to 150 KiB) when more than MAX_LEAF_G workgroups would be
+// needed, so that every leaf workgroup is co-resident (one per CU).
+template <class T> int leaf_layout(int m, int& G, int& rpw) {
+  const size_t row_bytes = LEAF_W * sizeof(cx<T>);
+  int rpw_t = (int)(LEAF_SLAB_TARGET / row_bytes);
+  G = (m + rpw_t - 1) / rpw_t;
+  if (G < 1) G = 1;
+  if (G > MAX_LEAF_G) G = MAX_LEAF_G;
+  rpw = (m + G - 1) / G;
+  return ((size_t)rpw * row_bytes <= LEAF_SLAB_MAX) ? 0 : -1;
+}
+
 template <class T> int launch_leaf(cx<T>* P, int64_t ld, int m, int w, int row_base, int* ipiv,
                                    int* info, const Scratch& s, int leaf, hipStream_t st) {
-  const int rpw_max = leaf_rpw_max<T>();
-  int G = (m + rpw_max - 1) / rpw_max;
-  if (G < 1) G = 1;
-  if (G > MAX_LEAF_G) return -1;
-  const int rpw = (m + G - 1) / G;
+  int G, rpw;
+  if (leaf_layout<T>(m, G, rpw)) return -1;
   LeafXchg<T> x;
   char* base = (char*)s.xchg;
   x.val = (T*)base;
   x.idx = (int*)(base + 2 * MAX_LEAF_G * sizeof(double));
   x.rowj = (cx<T>*)(base + 2 * MAX_LEAF_G * (sizeof(double) + sizeof(int)));
   x.row = x.rowj + 2 * LEAF_W;
+  x.flag = s.flags;
   size_t lds = (size_t)rpw * LEAF_W * sizeof(cx<T>);
   static bool attr = false;
   if (!attr) { allow_big_lds(leaf_getf2_kernel<T, LEAF_W>); attr = true; }
   leaf_getf2_kernel<T, LEAF_W><<<G, 256, lds, st>>>(P, ld, m, w, rpw, row_base, ipiv, info, x,
-                                                    s.counters + leaf, s.err);
+                                                    (unsigned)leaf * LEAF_W, s.err);
   return 0;
 }
 
@@ -206,71 +225,86 @@ template <class T> void launch_laswp(cx<T>* A, int64_t ld, int c0, int c1, int k
                                      const int* ipiv, const int* skip, hipStream_t st) {
   if (c1 <= c0 || nk <= 0) return;
   constexpr int CW = 128 / sizeof(cx<T>);
-  size_t lds = (size_t)2 * nk * 128 + (size_t)3 * nk * sizeof(int);
+  size_t lds = (size_t)2 * nk * 128 + (size_t)4 * nk * sizeof(int);
   static bool attr = false;
   if (!attr) { allow_big_lds(laswp_kernel<T>); attr = true; }
   int grid = (c1 - c0 + CW - 1) / CW;
   laswp_kernel<T><<<grid, 256, lds, st>>>(A, ld, c0, c1, k1, nk, ipiv, skip);
 }
 
-// Triangular solve with the kb x kb diagonal block at D (lower unit or upper
-// non-unit) of the kb x ncols block row S, by 64-row diagonal solves and MFMA
-// updates of the remaining rows of the block.
+// Block-row triangular solve (kb <= 256) with the stored diagonal inverses.
 template <class T, bool UPPER>
-void trsm_rows(const cx<T>* D, int64_t ldd, int kb, cx<T>* S, int64_t lds, int ncols, int tri,
-               bool g3m, const int* skip, hipStream_t st) {
+void trsm_blk(const cx<T>* D, int64_t ldd, int kb, const cx<T>* dinv, cx<T>* S, int64_t lds,
+              int ncols, const int* skip, hipStream_t st) {
   if (ncols <= 0 || kb <= 0) return;
-  const int grid = (ncols + 15) / 16;
-  if (!UPPER) {
-    for (int b = 0; b < kb; b += tri) {
-      int bs = min(tri, kb - b);
-      trsm_diag_kernel<T, false><<<grid, 256, 0, st>>>(D + (int64_t)b * ldd + b, ldd, bs,
-                                                       S + (int64_t)b * lds, lds, ncols, skip);
-      if (b + bs < kb)
-        gemm<T>(kb - b - bs, ncols, bs, D + (int64_t)(b + bs) * ldd + b, ldd, false,
-                S + (int64_t)b * lds, lds, S + (int64_t)(b + bs) * lds, lds, T(-1), 1, g3m, skip, st);
-    }
-  } else {
-    for (int b = ((kb - 1) / tri) * tri; b >= 0; b -= tri) {
-      int bs = min(tri, kb - b);
-      trsm_diag_kernel<T, true><<<grid, 256, 0, st>>>(D + (int64_t)b * ldd + b, ldd, bs,
-                                                      S + (int64_t)b * lds, lds, ncols, skip);
-      if (b > 0)
-        gemm<T>(b, ncols, bs, D + b, ldd, false, S + (int64_t)b * lds, lds, S, lds, T(-1), 1, g3m,
-                skip, st);
-    }
+  trsm_blk_kernel<T, UPPER><<<(ncols + 15) / 16, 256, 0, st>>>(D, ldd, kb, dinv, S, lds, ncols, skip);
+}
+
+// ---- optional HIP-event timing of the trailing-update GEMM ------------------
+// When enabled, each trailing update of getrf is bracketed by two events on
+// the launch stream; prof_read sums the elapsed times and the algorithmic
+// flops (8 M N K per complex update).
+struct ProfRec {
+  hipEvent_t a, b;
+  double flops;
+};
+bool g_prof_on = false;
+std::vector<ProfRec> g_prof;
+size_t g_prof_used = 0;
+
+ProfRec* prof_slot() {
+  if (g_prof_used == g_prof.size()) {
+    ProfRec r;
+    if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return nullptr;
+    g_prof.push_back(r);
   }
+  return &g_prof[g_prof_used++];
+}
+
+size_t xchg_bytes() {
+  size_t x = 2 * MAX_LEAF_G * (sizeof(double) + sizeof(int)) +
+             (2 * LEAF_W + (size_t)2 * MAX_LEAF_G * LEAF_W) * sizeof(double2);
+  return (x + 255) & ~(size_t)255;
 }
 
 }  // namespace
 
-size_t scratch_bytes(int n) {
-  size_t leaves = (size_t)n / LEAF_W + 8;
-  size_t x = 2 * MAX_LEAF_G * (sizeof(double) + sizeof(int)) +
-             (2 * LEAF_W + (size_t)2 * MAX_LEAF_G * LEAF_W) * sizeof(double2);
-  x = (x + 255) & ~(size_t)255;
-  return x + leaves * sizeof(unsigned) + 256;
+int prof_enable(int on) {
+  if (on) g_prof_used = 0;
+  g_prof_on = on != 0;
+  return 0;
 }
 
-int scratch_bind(Scratch& s, void* base, int n) {
-  size_t leaves = (size_t)n / LEAF_W + 8;
-  size_t x = 2 * MAX_LEAF_G * (sizeof(double) + sizeof(int)) +
-             (2 * LEAF_W + (size_t)2 * MAX_LEAF_G * LEAF_W) * sizeof(double2);
-  x = (x + 255) & ~(size_t)255;
-  s.xchg = base;
-  s.xchg_bytes = x;
-  s.counters = (unsigned*)((char*)base + x);
-  s.ncounters = leaves;
-  s.err = (int*)((char*)base + x + leaves * sizeof(unsigned));
+int prof_read(double* ms, double* flops, int* count) {
+  double t = 0.0, f = 0.0;
+  for (size_t i = 0; i < g_prof_used; ++i) {
+    float e = 0.f;
+    HPR_CHECK(hipEventSynchronize(g_prof[i].b));
+    HPR_CHECK(hipEventElapsedTime(&e, g_prof[i].a, g_prof[i].b));
+    t += e;
+    f += g_prof[i].flops;
+  }
+  *ms = t;
+  *flops = f;
+  *count = (int)g_prof_used;
+  g_prof_used = 0;
   return 0;
+}
+
+size_t scratch_bytes() { return xchg_bytes() + 2 * MAX_LEAF_G * sizeof(unsigned) + 256; }
+
+void scratch_bind(Scratch& s, void* base) {
+  s.xchg = base;
+  s.xchg_bytes = xchg_bytes();
+  s.flags = (unsigned*)((char*)base + s.xchg_bytes);
+  s.err = (int*)(s.flags + 2 * MAX_LEAF_G);
 }
 
 template <class T>
 int gemm(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<T>* B,
          int64_t ldb, cx<T>* C, int64_t ldc, T alpha, int beta, bool g3m, const int* skip,
          hipStream_t st) {
-  if (M <= 0 || N <= 0) return 0;
-  if (K <= 0) return 0;
+  if (M <= 0 || N <= 0 || K <= 0) return 0;
   constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32;
   int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
   if (a_col) {
@@ -284,19 +318,18 @@ int gemm(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<
 }
 
 template <class T>
-int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const Scratch& s,
-          const LuPlan& plan, bool g3m, hipStream_t st) {
+int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, cx<T>* dinv, int* perm,
+          const Scratch& s, const LuPlan& plan, bool g3m, hipStream_t st) {
   if (n <= 0) return 0;
   const int nb = plan.nb;
-  size_t nleaves = 0;
-  for (int k = 0; k < n; k += nb) nleaves += (min(nb, n - k) + LEAF_W - 1) / LEAF_W;
-  if (nleaves > s.ncounters) return -1;
-  fill_u32_kernel<<<64, 256, 0, st>>>(s.counters, s.ncounters, 0u);
+  if (nb <= 0 || nb > 256 || nb % 64 != 0) return -1;
+  fill_u32_kernel<<<4, 256, 0, st>>>(s.flags, 2 * MAX_LEAF_G, 0u);
   fill_u32_kernel<<<1, 64, 0, st>>>((unsigned*)s.err, 1, 0u);
   fill_u32_kernel<<<1, 64, 0, st>>>((unsigned*)info, 1, 0u);
   int leaf = 0;
   for (int k = 0; k < n; k += nb) {
     const int kb = min(nb, n - k), m = n - k;
+    // -- panel: 32-column cooperative leaves, right-looking inside the panel
     for (int t = 0; t < kb; t += LEAF_W) {
       const int wt = min(LEAF_W, kb - t);
       cx<T>* P = A + (int64_t)(k + t) * ld + (k + t);
@@ -304,47 +337,71 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const Scratch& s,
       if (t > 0) launch_laswp<T>(A, ld, k, k + t, k + t, wt, ipiv, nullptr, st);
       if (t + wt < kb) {
         launch_laswp<T>(A, ld, k + t + wt, k + kb, k + t, wt, ipiv, nullptr, st);
-        trsm_rows<T, false>(P, ld, wt, P + wt, ld, kb - t - wt, plan.tri, g3m, nullptr, st);
+        trsm_diag_kernel<T, false><<<(kb - t - wt + 15) / 16, 256, 0, st>>>(
+            P, ld, wt, P + wt, ld, kb - t - wt, nullptr);
         gemm<T>(m - t - wt, kb - t - wt, wt, P + (int64_t)wt * ld, ld, false, P + wt, ld,
                 P + (int64_t)wt * ld + wt, ld, T(-1), 1, g3m, nullptr, st);
       }
     }
+    // -- inverses of the panel's diagonal blocks (for U12 here and getrs later)
+    diag_inv_kernel<T><<<dim3((kb + 63) / 64, 2), 256, 0, st>>>(A, ld, n, k / 64, dinv);
     if (k > 0) launch_laswp<T>(A, ld, 0, k, k, kb, ipiv, nullptr, st);
     if (k + kb < n) {
       cx<T>* Akk = A + (int64_t)k * ld + k;
       launch_laswp<T>(A, ld, k + kb, n, k, kb, ipiv, nullptr, st);
-      trsm_rows<T, false>(Akk, ld, kb, Akk + kb, ld, n - k - kb, plan.tri, g3m, nullptr, st);
+      trsm_blk<T, false>(Akk, ld, kb, dinv + (size_t)(k / 64) * 2 * 4096, Akk + kb, ld, n - k - kb,
+                         nullptr, st);
+      ProfRec* pr = g_prof_on ? prof_slot() : nullptr;
+      if (pr) (void)hipEventRecord(pr->a, st);
       gemm<T>(n - k - kb, n - k - kb, kb, Akk + (int64_t)kb * ld, ld, false, Akk + kb, ld,
               Akk + (int64_t)kb * ld + kb, ld, T(-1), 1, g3m, nullptr, st);
+      if (pr) {
+        (void)hipEventRecord(pr->b, st);
+        pr->flops = 8.0 * (double)(n - k - kb) * (double)(n - k - kb) * (double)kb;
+      }
     }
   }
+  const size_t perm_lds = (size_t)n * sizeof(int);
+  const int in_lds = perm_lds <= 140 * 1024;
+  static bool attr = false;
+  if (!attr) { allow_big_lds(perm_from_ipiv_kernel); attr = true; }
+  perm_from_ipiv_kernel<<<1, 256, in_lds ? perm_lds : 0, st>>>(ipiv, n, perm, in_lds);
   HPR_CHECK(hipGetLastError());
   return 0;
 }
 
 template <class T>
-int getrs(const cx<T>* LU, int64_t ld, int n, const int* ipiv, cx<T>* B, int64_t ldb, int nrhs,
+int getrs(const cx<T>* LU, int64_t ld, int n, const cx<T>* dinv, cx<T>* B, int64_t ldb, int nrhs,
           const int* skip, const LuPlan& plan, bool g3m, hipStream_t st) {
   if (n <= 0 || nrhs <= 0) return 0;
-  const int rs = plan.rs_nb;
-  for (int k = 0; k < n; k += 256) launch_laswp<T>(B, ldb, 0, nrhs, k, min(256, n - k), ipiv, skip, st);
+  const int rs = plan.nb;
   for (int b = 0; b < n; b += rs) {
-    int bs = min(rs, n - b);
-    trsm_rows<T, false>(LU + (int64_t)b * ld + b, ld, bs, B + (int64_t)b * ldb, ldb, nrhs, plan.tri,
-                        g3m, skip, st);
+    const int bs = min(rs, n - b);
+    trsm_blk<T, false>(LU + (int64_t)b * ld + b, ld, bs, dinv + (size_t)(b / 64) * 2 * 4096,
+                       B + (int64_t)b * ldb, ldb, nrhs, skip, st);
     if (b + bs < n)
       gemm<T>(n - b - bs, nrhs, bs, LU + (int64_t)(b + bs) * ld + b, ld, false, B + (int64_t)b * ldb,
               ldb, B + (int64_t)(b + bs) * ldb, ldb, T(-1), 1, g3m, skip, st);
   }
   for (int b = ((n - 1) / rs) * rs; b >= 0; b -= rs) {
-    int bs = min(rs, n - b);
-    trsm_rows<T, true>(LU + (int64_t)b * ld + b, ld, bs, B + (int64_t)b * ldb, ldb, nrhs, plan.tri,
-                       g3m, skip, st);
+    const int bs = min(rs, n - b);
+    trsm_blk<T, true>(LU + (int64_t)b * ld + b, ld, bs, dinv + (size_t)(b / 64) * 2 * 4096,
+                      B + (int64_t)b * ldb, ldb, nrhs, skip, st);
     if (b > 0)
       gemm<T>(b, nrhs, bs, LU + b, ld, false, B + (int64_t)b * ldb, ldb, B, ldb, T(-1), 1, g3m,
               skip, st);
   }
   HPR_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <class Tin, class Tout>
+int permute_cast(int n, int nrhs, const int* perm, const cx<Tin>* in, cx<Tout>* out,
+                 const int* skip, hipStream_t st) {
+  const size_t cnt = (size_t)n * nrhs;
+  if (cnt == 0) return 0;
+  const int grid = (int)min((size_t)4096, (cnt + 255) / 256);
+  permute_cast_kernel<Tin, Tout><<<grid, 256, 0, st>>>(n, nrhs, perm, in, out, skip);
   return 0;
 }
 
@@ -369,23 +426,23 @@ static int launch_norms(int n, int nrhs, const double2* R, double* part, RefineS
 }
 
 template <class TF>
-int refine_solve(const cx<TF>* LU, int64_t ld, int n, const int* ipiv, const double2* A64,
-                 int64_t lda, const double2* B, double2* X, int nrhs, int max_refine, double tol,
-                 int crit, RefineState* state, double2* R, cx<TF>* RF, double* part,
-                 const LuPlan& plan, bool g3m, hipStream_t st) {
+int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv, const int* perm,
+                 const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
+                 int max_refine, double tol, int crit, RefineState* state, double2* R, cx<TF>* RF,
+                 double* part, const LuPlan& plan, bool g3m, hipStream_t st) {
   const size_t cnt = (size_t)n * nrhs;
   const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
   const int* done = &state->done;
   launch_norms(n, nrhs, B, part, state, 0, crit, tol, nullptr, st);
-  cast_kernel<double, TF><<<eg, 256, 0, st>>>(cnt, B, RF, nullptr);
-  if (getrs<TF>(LU, ld, n, ipiv, RF, nrhs, nrhs, nullptr, plan, g3m, st)) return -1;
+  permute_cast<double, TF>(n, nrhs, perm, B, RF, nullptr, st);
+  if (getrs<TF>(LU, ld, n, dinv, RF, nrhs, nrhs, nullptr, plan, g3m, st)) return -1;
   cast_kernel<TF, double><<<eg, 256, 0, st>>>(cnt, RF, X, nullptr);
   for (int it = 0; it < max_refine; ++it) {
     cast_kernel<double, double><<<eg, 256, 0, st>>>(cnt, B, R, done);
     gemm<double>(n, nrhs, n, A64, lda, true, X, nrhs, R, nrhs, -1.0, 1, false, done, st);
     launch_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
-    cast_kernel<double, TF><<<eg, 256, 0, st>>>(cnt, R, RF, done);
-    if (getrs<TF>(LU, ld, n, ipiv, RF, nrhs, nrhs, done, plan, g3m, st)) return -1;
+    permute_cast<double, TF>(n, nrhs, perm, R, RF, done, st);
+    if (getrs<TF>(LU, ld, n, dinv, RF, nrhs, nrhs, done, plan, g3m, st)) return -1;
     add_correction_kernel<TF><<<eg, 256, 0, st>>>(cnt, X, RF, state);
   }
   HPR_CHECK(hipGetLastError());
@@ -407,19 +464,22 @@ int residual_check(int n, int nrhs, const double2* A64, int64_t lda, const doubl
 }
 
 #define INST(T)                                                                               \
-  template int getrf<T>(cx<T>*, int64_t, int, int*, int*, const Scratch&, const LuPlan&, bool, \
-                        hipStream_t);                                                          \
-  template int getrs<T>(const cx<T>*, int64_t, int, const int*, cx<T>*, int64_t, int,          \
+  template int getrf<T>(cx<T>*, int64_t, int, int*, int*, cx<T>*, int*, const Scratch&,        \
+                        const LuPlan&, bool, hipStream_t);                                     \
+  template int getrs<T>(const cx<T>*, int64_t, int, const cx<T>*, cx<T>*, int64_t, int,        \
                         const int*, const LuPlan&, bool, hipStream_t);                         \
   template int gemm<T>(int, int, int, const cx<T>*, int64_t, bool, const cx<T>*, int64_t,      \
                        cx<T>*, int64_t, T, int, bool, const int*, hipStream_t);                \
-  template int refine_solve<T>(const cx<T>*, int64_t, int, const int*, const double2*, int64_t, \
-                               const double2*, double2*, int, int, double, int, RefineState*,   \
-                               double2*, cx<T>*, double*, const LuPlan&, bool, hipStream_t);
+  template int refine_solve<T>(const cx<T>*, int64_t, int, const cx<T>*, const int*,          \
+                               const double2*, int64_t, const double2*, double2*, int, int,    \
+                               double, int, RefineState*, double2*, cx<T>*, double*,           \
+                               const LuPlan&, bool, hipStream_t);
 INST(double)
 INST(float)
 template int transpose_cast<double, double>(int, int, const cx<double>*, int64_t, cx<double>*, int64_t, hipStream_t);
 template int transpose_cast<double, float>(int, int, const cx<double>*, int64_t, cx<float>*, int64_t, hipStream_t);
 template int transpose_cast<float, double>(int, int, const cx<float>*, int64_t, cx<double>*, int64_t, hipStream_t);
+template int permute_cast<double, double>(int, int, const int*, const cx<double>*, cx<double>*, const int*, hipStream_t);
+template int permute_cast<double, float>(int, int, const int*, const cx<double>*, cx<float>*, const int*, hipStream_t);
 
 }  // namespace hpr

@@ -4,20 +4,17 @@
 
 namespace hpr {
 
-// Tunables of the blocked factorization (chosen per precision by the tuner).
+// Tunables of the blocked factorization.
 struct LuPlan {
-  int nb = 256;     // outer panel width
-  int tri = 64;     // diagonal block of the triangular solves (<= 64)
-  int rs_nb = 256;  // block row height of the getrs sweeps
+  int nb = 256;     // outer panel width and getrs block height (<= 256, multiple of 64)
 };
 
-// Device scratch used by getrf (exchange area of the cooperative panel
-// leaves, grid-barrier counters and the barrier error flag).
+// Device scratch used by getrf: exchange area of the cooperative panel
+// leaves (with their arrival flags) and the barrier error flag.
 struct Scratch {
   void* xchg = nullptr;
   size_t xchg_bytes = 0;
-  unsigned* counters = nullptr;
-  size_t ncounters = 0;
+  unsigned* flags = nullptr;
   int* err = nullptr;
 };
 
@@ -38,19 +35,29 @@ struct RefineState {
 
 enum Criterion { CRIT_ABS_MAX = 0, CRIT_REL_FRO = 1 };
 
-// Factor precision / trailing-update arithmetic.
-enum Method { M_FP64 = 0, M_FP32 = 1, M_FP32_3M = 2, M_BF16X3 = 3 };
+size_t scratch_bytes();
+void scratch_bind(Scratch& s, void* base);
 
-size_t scratch_bytes(int n);
-int scratch_bind(Scratch& s, void* base, int n);
+// Elements of the diagonal-block inverse area of an n x n factor.
+inline size_t dinv_elems(int n) { return (size_t)((n + 63) / 64) * 2 * 4096; }
 
+// Device factorization P A = L U (row-major A, in place).  Also produces the
+// inverses of the 64 x 64 diagonal blocks of L and U (dinv) and the final row
+// permutation perm (perm[i] = source row of row i of P A) used by getrs.
 template <class T>
-int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const Scratch& s,
-          const LuPlan& plan, bool gauss3m, hipStream_t st);
+int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, cx<T>* dinv, int* perm,
+          const Scratch& s, const LuPlan& plan, bool gauss3m, hipStream_t st);
 
+// Solve L U X = B in place for a right-hand side already permuted by perm
+// (row-major n x nrhs, leading dimension ldb).
 template <class T>
-int getrs(const cx<T>* LU, int64_t ld, int n, const int* ipiv, cx<T>* B, int64_t ldb,
-          int nrhs, const int* skip, const LuPlan& plan, bool gauss3m, hipStream_t st);
+int getrs(const cx<T>* LU, int64_t ld, int n, const cx<T>* dinv, cx<T>* B, int64_t ldb, int nrhs,
+          const int* skip, const LuPlan& plan, bool gauss3m, hipStream_t st);
+
+// out[i][:] = (Tout) in[perm[i]][:] (row-major n x nrhs).
+template <class Tin, class Tout>
+int permute_cast(int n, int nrhs, const int* perm, const cx<Tin>* in, cx<Tout>* out,
+                 const int* skip, hipStream_t st);
 
 // out[r * ldo + c] = (Tout) in[c * ldi + r], r < rows, c < cols
 template <class Tin, class Tout>
@@ -66,17 +73,22 @@ int gemm(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<
 // Mixed/plain refinement around resident factors.  A64 is the FP64 matrix in
 // column-major order (lda); B and X are row-major n x nrhs (ld = nrhs).
 // Work buffers: R (n x nrhs FP64), RF (n x nrhs in factor precision),
-// part (>= 4096 doubles).
+// part (>= 5120 doubles).
 template <class TF>
-int refine_solve(const cx<TF>* LU, int64_t ld, int n, const int* ipiv, const double2* A64,
-                 int64_t lda, const double2* B, double2* X, int nrhs, int max_refine,
-                 double tol, int crit, RefineState* state, double2* R, cx<TF>* RF,
-                 double* part, const LuPlan& plan, bool gauss3m, hipStream_t st);
+int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv, const int* perm,
+                 const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
+                 int max_refine, double tol, int crit, RefineState* state, double2* R,
+                 cx<TF>* RF, double* part, const LuPlan& plan, bool gauss3m, hipStream_t st);
 
 // One residual evaluation of a refined X (skipped when state->done is set):
 // R = B - A64 X, norms and the stopping rule recorded in state.
 int residual_check(int n, int nrhs, const double2* A64, int64_t lda, const double2* B,
                    const double2* X, double2* R, double* part, RefineState* state, int crit,
                    double tol, hipStream_t st);
+
+// HIP-event timing of getrf's trailing-update GEMM launches (for roofline
+// reporting): enable, run, then read total ms / algorithmic flops / count.
+int prof_enable(int on);
+int prof_read(double* ms, double* flops, int* count);
 
 }  // namespace hpr

@@ -9,61 +9,78 @@
 #include "common.hpp"
 
 // ---------------------------------------------------------------------------
-// Grid-wide barrier for the cooperative panel kernel.  The counter is zeroed
-// before the launch and counts arrivals monotonically; the spin is bounded so
-// that a barrier that can never complete (workgroups not co-resident) ends
-// with *err set instead of hanging.  Every waiting workgroup also watches
-// *err so one timeout releases all of them quickly.
+// Exchange area of the cooperative panel leaves (device memory).  Slots are
+// double-buffered by column parity: a workgroup can only overwrite parity p
+// for column j+2 after every workgroup has published column j+1, which each
+// does only after it finished reading column j.
 // ---------------------------------------------------------------------------
-__device__ inline void grid_barrier(unsigned* counter, unsigned target, int* err) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    atomicAdd(counter, 1u);
+template <class T> struct LeafXchg {
+  T* val;          // [2][Gmax]   candidate |Re|+|Im| (negative: no candidate)
+  int* idx;        // [2][Gmax]   candidate panel-local row
+  cx<T>* row;      // [2][Gmax][W] candidate row (all W leaf columns)
+  cx<T>* rowj;     // [2][W]      current row j (published by its owner)
+  unsigned* flag;  // [2][Gmax]   arrival sequence numbers, zeroed per factorization
+};
+
+// Wait until flag[q] >= seq for q < G (threads q, q + 256, ... each spin on
+// one flag).  Bounded: a wait that cannot complete sets *err and returns, and
+// every waiter watches *err so one timeout releases them all.
+__device__ inline void wait_flags(const unsigned* flag, int G, unsigned seq, int* err) {
+  for (int q = threadIdx.x; q < G; q += blockDim.x) {
     unsigned spins = 0;
-    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(flag + q, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < seq) {
       __builtin_amdgcn_s_sleep(1);
-      if ((++spins & 1023u) == 0) {
+      if ((++spins & 255u) == 0) {
         if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
         if (spins > (1u << 22)) { atomicExch(err, 1); break; }
       }
     }
-    __threadfence();
   }
   __syncthreads();
 }
 
-// Exchange area of one panel-leaf launch (device memory, per leaf).
-template <class T> struct LeafXchg {
-  T* val;        // [2][G]   candidate |Re|+|Im|  (negative: no candidate)
-  int* idx;      // [2][G]   candidate panel-local row
-  cx<T>* row;    // [2][G][W] candidate row (all W leaf columns)
-  cx<T>* rowj;   // [2][W]   current row j (published by its owner)
-};
+template <class T>
+__device__ inline void best_of(T& bv, int& bi, T ov, int oi) {
+  if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+}
+
+// Workgroup argmax of (value, index): largest value, lowest index on ties.
+template <class T>
+__device__ inline void wg_argmax(T& bv, int& bi, T* red_v, int* red_i) {
+  for (int off = 32; off > 0; off >>= 1) {
+    T ov = __shfl_xor(bv, off);
+    int oi = __shfl_xor(bi, off);
+    best_of(bv, bi, ov, oi);
+  }
+  if ((threadIdx.x & 63) == 0) { red_v[threadIdx.x >> 6] = bv; red_i[threadIdx.x >> 6] = bi; }
+  __syncthreads();
+  bv = red_v[0]; bi = red_i[0];
+  for (int q = 1; q < (int)(blockDim.x >> 6); ++q) best_of(bv, bi, red_v[q], red_i[q]);
+  __syncthreads();
+}
 
 // ---------------------------------------------------------------------------
 // Cooperative GEPP of an m x w leaf (w <= W) at P (row-major, ld).  G
 // workgroups each keep a slab of rpw consecutive rows in LDS for the whole
-// leaf.  Per column j: every workgroup publishes its best candidate (value,
-// row and the full row) and the owner of row j publishes row j; one grid
-// barrier; then every workgroup picks the same winner (largest value, lowest
-// row on ties), performs the interchange on the rows it owns, scales the
-// multipliers and applies the rank-1 update to its slab.  One grid barrier
-// per column.  ipiv receives global 0-based row indices (row_base + local).
+// leaf.  Per column j each workgroup finds its best candidate (wavefront
+// shuffles + LDS), publishes value, row index and the candidate row (and the
+// owner of row j publishes row j), raises its arrival flag, and waits for the
+// G flags; then every workgroup picks the same winner (largest |Re|+|Im|,
+// lowest row on ties), performs the interchange on the rows it owns, scales
+// the multipliers and applies the rank-1 update to its slab.  With G = 1 the
+// whole column step stays in LDS.  ipiv receives global 0-based rows.
 // ---------------------------------------------------------------------------
 template <class T, int W>
 __global__ void __launch_bounds__(256)
 leaf_getf2_kernel(cx<T>* __restrict__ P, int64_t ld, int m, int w, int rpw, int row_base,
                   int* __restrict__ ipiv, int* __restrict__ info, LeafXchg<T> x,
-                  unsigned* __restrict__ counter, int* __restrict__ err) {
+                  unsigned seq_base, int* __restrict__ err) {
   extern __shared__ unsigned char smem_raw[];
   cx<T>* slab = reinterpret_cast<cx<T>*>(smem_raw);  // [rpw][W]
   __shared__ cx<T> prow[W];
   __shared__ cx<T> oldj[W];
   __shared__ T red_v[4];
   __shared__ int red_i[4];
-  __shared__ int s_win, s_p;
-  __shared__ T s_best;
 
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
   const int lo = g * rpw, hi = min(m, lo + rpw), nrows = max(0, hi - lo);
@@ -83,57 +100,45 @@ leaf_getf2_kernel(cx<T>* __restrict__ P, int64_t ld, int m, int w, int rpw, int 
       T v = cabs1<T>(slab[(i - lo) * W + j]);
       if (v > bv) { bv = v; bi = i; }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      T ov = __shfl_xor(bv, off);
-      int oi = __shfl_xor(bi, off);
-      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    wg_argmax(bv, bi, red_v, red_i);
+    int p;
+    T best;
+    if (G == 1) {
+      best = bv;
+      p = (bv >= T(0)) ? bi : j;
+      if (tid < W) {
+        prow[tid] = slab[(p - lo) * W + tid];
+        oldj[tid] = slab[(j - lo) * W + tid];
+      }
+      __syncthreads();
+    } else {
+      if (tid == 0) {
+        x.val[par * G + g] = bv;
+        x.idx[par * G + g] = bi;
+      }
+      if (bv >= T(0) && tid < W) x.row[((int64_t)par * G + g) * W + tid] = slab[(bi - lo) * W + tid];
+      if (j >= lo && j < hi && tid < W) x.rowj[par * W + tid] = slab[(j - lo) * W + tid];
+      __syncthreads();
+      if (tid == 0) {
+        __threadfence();
+        __hip_atomic_store(x.flag + par * G + g, seq_base + (unsigned)j + 1u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+      wait_flags(x.flag + par * G, G, seq_base + (unsigned)j + 1u, err);
+      // -- global winner: every workgroup reduces the same G candidates; the
+      //    winning row's slab owner is row / rpw.
+      bv = T(-1); bi = 0x7fffffff;
+      for (int q = tid; q < G; q += 256) best_of(bv, bi, x.val[par * G + q], x.idx[par * G + q]);
+      wg_argmax(bv, bi, red_v, red_i);
+      best = bv;
+      p = (bv >= T(0)) ? bi : j;
+      const int win = (bv >= T(0)) ? (bi / rpw) : 0;
+      if (tid < W) {
+        prow[tid] = (bv >= T(0)) ? x.row[((int64_t)par * G + win) * W + tid] : mkc<T>(0, 0);
+        oldj[tid] = x.rowj[par * W + tid];
+      }
+      __syncthreads();
     }
-    if ((tid & 63) == 0) { red_v[tid >> 6] = bv; red_i[tid >> 6] = bi; }
-    __syncthreads();
-    if (tid == 0) {
-      for (int q = 1; q < 4; ++q)
-        if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < bi)) { bv = red_v[q]; bi = red_i[q]; }
-      x.val[par * G + g] = bv;
-      x.idx[par * G + g] = bi;
-      s_best = bv; s_p = bi;
-    }
-    __syncthreads();
-    if (s_best >= T(0) && tid < W)
-      x.row[((int64_t)par * G + g) * W + tid] = slab[(s_p - lo) * W + tid];
-    if (j >= lo && j < hi && tid < W) x.rowj[par * W + tid] = slab[(j - lo) * W + tid];
-
-    grid_barrier(counter, (unsigned)(G * (j + 1)), err);
-
-    // -- global winner: every workgroup reduces the same G candidates; the
-    //    winning row's slab owner is row / rpw.
-    bv = T(-1); bi = 0x7fffffff;
-    for (int q = tid; q < G; q += 256) {
-      T v = x.val[par * G + q];
-      int i = x.idx[par * G + q];
-      if (v > bv || (v == bv && i < bi)) { bv = v; bi = i; }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      T ov = __shfl_xor(bv, off);
-      int oi = __shfl_xor(bi, off);
-      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-    }
-    if ((tid & 63) == 0) { red_v[tid >> 6] = bv; red_i[tid >> 6] = bi; }
-    __syncthreads();
-    if (tid == 0) {
-      for (int q = 1; q < 4; ++q)
-        if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < bi)) { bv = red_v[q]; bi = red_i[q]; }
-      s_best = bv;
-      s_p = (bv >= T(0)) ? bi : j;
-      s_win = (bv >= T(0)) ? (bi / rpw) : 0;
-    }
-    __syncthreads();
-    const int p = s_p;
-    const T best = s_best;
-    if (tid < W) {
-      prow[tid] = (best >= T(0)) ? x.row[((int64_t)par * G + s_win) * W + tid] : mkc<T>(0, 0);
-      oldj[tid] = x.rowj[par * W + tid];
-    }
-    __syncthreads();
     if (g == 0 && tid == 0) {
       ipiv[j] = row_base + p;
       if (best == T(0) && *info == 0) *info = row_base + j + 1;
@@ -187,20 +192,23 @@ laswp_kernel(cx<T>* __restrict__ A, int64_t ld, int c0, int c1, int k1, int nk,
   cx<T>* buf = reinterpret_cast<cx<T>*>(smem_raw);  // [2 * nk][CW]
   int* slot = reinterpret_cast<int*>(buf + 2 * nk * CW);  // [nk]
   int* srcrow = slot + nk;                                 // [2 * nk]
+  int* piv = srcrow + 2 * nk;                              // [nk]
   const int tid = threadIdx.x;
   const int cb = c0 + blockIdx.x * CW;
   const int ncol = min(CW, c1 - cb);
   if (ncol <= 0) return;
 
+  for (int t = tid; t < nk; t += 256) piv[t] = ipiv[k1 + t];
+  __syncthreads();
   for (int t = tid; t < nk; t += 256) {
-    int p = ipiv[k1 + t];
+    int p = piv[t];
     int s;
     if (p < k1 + nk) {
       s = p - k1;
     } else {
       int first = t;
       for (int q = 0; q < t; ++q)
-        if (ipiv[k1 + q] == p) { first = q; break; }
+        if (piv[q] == p) { first = q; break; }
       s = nk + first;
     }
     slot[t] = s;
@@ -285,5 +293,200 @@ trsm_diag_kernel(const cx<T>* __restrict__ D, int64_t ldd, int nb, cx<T>* __rest
   for (int e = tid; e < nb * CW; e += 256) {
     int i = e / CW, c = e % CW;
     if (c < ncol) S[(int64_t)i * lds + cb + c] = s[i][c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Final row permutation of a factorization: perm[i] = source row of row i of
+// P A, from the interchange sequence ipiv (LAPACK order).  One workgroup;
+// the array lives in LDS when n fits, ipiv is staged in chunks so the single
+// sequential swap loop never waits on global memory.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+perm_from_ipiv_kernel(const int* __restrict__ ipiv, int n, int* __restrict__ perm, int in_lds) {
+  extern __shared__ int sh_perm[];
+  __shared__ int chunk[1024];
+  int* p = in_lds ? sh_perm : perm;
+  for (int i = threadIdx.x; i < n; i += 256) p[i] = i;
+  __syncthreads();
+  for (int c0 = 0; c0 < n; c0 += 1024) {
+    const int cn = min(1024, n - c0);
+    for (int i = threadIdx.x; i < cn; i += 256) chunk[i] = ipiv[c0 + i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 0; i < cn; ++i) {
+        int k = chunk[i], a = c0 + i;
+        if (k != a) { int t = p[a]; p[a] = p[k]; p[k] = t; }
+      }
+    }
+    __syncthreads();
+  }
+  if (in_lds)
+    for (int i = threadIdx.x; i < n; i += 256) perm[i] = sh_perm[i];
+}
+
+// out[i][:] = (Tout) in[perm[i]][:] for row-major n x nrhs arrays.
+template <class Tin, class Tout>
+__global__ void __launch_bounds__(256)
+permute_cast_kernel(int n, int nrhs, const int* __restrict__ perm, const cx<Tin>* __restrict__ in,
+                    cx<Tout>* __restrict__ out, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  const size_t total = (size_t)n * nrhs;
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+    const int i = (int)(e / nrhs), c = (int)(e % nrhs);
+    cx<Tin> v = in[(size_t)perm[i] * nrhs + c];
+    out[e] = mkc<Tout>((Tout)v.x, (Tout)v.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Inverses of the 64 x 64 diagonal blocks of a row-major LU factor, in place
+// in LDS (LAPACK ztrti2 order): blockIdx.y = 0 -> inverse of the unit lower
+// triangle, 1 -> inverse of the upper triangle.  Output: dinv[(b*2+y)*4096 +
+// i*64 + k], zero outside the triangle (and unit diagonal for L).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256)
+diag_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int blk0,
+                cx<T>* __restrict__ dinv) {
+  __shared__ cx<T> a[64][65];
+  __shared__ cx<T> tmp[64];
+  const int bi = blk0 + blockIdx.x, upper = blockIdx.y, tid = threadIdx.x;
+  const int r0 = bi * 64, bs = min(64, n - r0);
+  for (int e = tid; e < 64 * 64; e += 256) {
+    int i = e / 64, k = e % 64;
+    bool in = i < bs && k < bs && (upper ? (k >= i) : (k < i));
+    a[i][k] = in ? LU[(int64_t)(r0 + i) * ld + r0 + k] : mkc<T>(0, 0);
+  }
+  __syncthreads();
+  if (upper) {
+    for (int j = 0; j < bs; ++j) {
+      if (tid == 0) a[j][j] = crecip<T>(a[j][j]);
+      if (tid < j) tmp[tid] = a[tid][j];
+      __syncthreads();
+      cx<T> s = mkc<T>(0, 0);
+      if (tid < j)
+        for (int k = tid; k < j; ++k) s = mkc<T>(s.x + a[tid][k].x * tmp[k].x - a[tid][k].y * tmp[k].y,
+                                                 s.y + a[tid][k].x * tmp[k].y + a[tid][k].y * tmp[k].x);
+      __syncthreads();
+      if (tid < j) {
+        cx<T> ajj = a[j][j];
+        a[tid][j] = mkc<T>(-(ajj.x * s.x - ajj.y * s.y), -(ajj.x * s.y + ajj.y * s.x));
+      }
+      __syncthreads();
+    }
+  } else {
+    for (int j = bs - 1; j >= 0; --j) {
+      if (tid > j && tid < bs) tmp[tid] = a[tid][j];
+      __syncthreads();
+      cx<T> s = mkc<T>(0, 0);
+      if (tid > j && tid < bs) {
+        s = tmp[tid];
+        for (int k = j + 1; k < tid; ++k) s = mkc<T>(s.x + a[tid][k].x * tmp[k].x - a[tid][k].y * tmp[k].y,
+                                                     s.y + a[tid][k].x * tmp[k].y + a[tid][k].y * tmp[k].x);
+      }
+      __syncthreads();
+      if (tid > j && tid < bs) a[tid][j] = mkc<T>(-s.x, -s.y);
+      __syncthreads();
+    }
+    if (tid < bs) a[tid][tid] = mkc<T>(1, 0);
+    __syncthreads();
+  }
+  cx<T>* out = dinv + ((int64_t)bi * 2 + upper) * 4096;
+  for (int e = tid; e < 64 * 64; e += 256) out[e] = a[e / 64][e % 64];
+}
+
+// One wave: acc(16 x 16) += A(16 x K) B(K x 16), complex, operands in LDS as
+// real/imaginary planes (A row-major with row stride sa, B row-major with
+// row stride sb), K a multiple of 4.
+template <class T>
+__device__ inline void wave_cmma(const T* ar, const T* ai, int sa, const T* br, const T* bi, int sb,
+                                 int K, typename acc_t<T>::type& re, typename acc_t<T>::type& im) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const T a_r = ar[li * sa + k0 + lk], a_i = ai[li * sa + k0 + lk];
+    const T b_r = br[(k0 + lk) * sb + li], b_i = bi[(k0 + lk) * sb + li];
+    re = mfma16x4(a_r, b_r, re);
+    re = mfma16x4(-a_i, b_i, re);
+    im = mfma16x4(a_r, b_i, im);
+    im = mfma16x4(a_i, b_r, im);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solve of one block row (kb <= 256 rows) with the diagonal block
+// D of a row-major LU factor: D X = S in place, D unit lower (forward) or
+// upper (backward).  Each workgroup stages a 16-column strip of S in LDS and
+// walks the 64-row sub-blocks: X_sb = inv(D_sb) S_sb with the precomputed
+// inverse, then S_rc -= D_rc,sb X_sb for the remaining sub-blocks of the
+// block, every 64 x 64 x 16 product on MFMA (4 waves x one 16 x 16 tile).
+// dinv points at the inverse pair of this block's first 64-row sub-block.
+// ---------------------------------------------------------------------------
+template <class T, bool UPPER>
+__global__ void __launch_bounds__(256)
+trsm_blk_kernel(const cx<T>* __restrict__ D, int64_t ldd, int kb, const cx<T>* __restrict__ dinv,
+                cx<T>* __restrict__ S, int64_t lds, int ncols, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  constexpr int CW = 16, MAXKB = 256, PB = 65;
+  typedef typename acc_t<T>::type acc;
+  __shared__ T sr[MAXKB][CW], si[MAXKB][CW];
+  __shared__ T br_[64][PB], bi_[64][PB];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+  const int cb = blockIdx.x * CW;
+  const int ncol = min(CW, ncols - cb);
+  const int nsb = (kb + 63) / 64;
+  for (int e = tid; e < nsb * 64 * CW; e += 256) {
+    int i = e / CW, c = e % CW;
+    cx<T> v = (i < kb && c < ncol) ? S[(int64_t)i * lds + cb + c] : mkc<T>(0, 0);
+    sr[i][c] = v.x;
+    si[i][c] = v.y;
+  }
+  auto load_blk = [&](const cx<T>* src, int64_t lds_src, int rows, int cols) {
+    for (int e = tid; e < 64 * 64; e += 256) {
+      int i = e / 64, k = e % 64;
+      cx<T> v = (i < rows && k < cols) ? src[(int64_t)i * lds_src + k] : mkc<T>(0, 0);
+      br_[i][k] = v.x;
+      bi_[i][k] = v.y;
+    }
+  };
+  // acc = buf[wave rows] x s[sb rows]
+  auto product = [&](int sb, acc& re, acc& im) {
+    re = acc{};
+    im = acc{};
+    wave_cmma<T>(&br_[wave * 16][0], &bi_[wave * 16][0], PB, &sr[sb * 64][0], &si[sb * 64][0], CW, 64,
+                 re, im);
+  };
+  for (int step = 0; step < nsb; ++step) {
+    const int sb = UPPER ? (nsb - 1 - step) : step;
+    const int bs = min(64, kb - sb * 64);
+    __syncthreads();
+    load_blk(dinv + (int64_t)sb * 2 * 4096 + (UPPER ? 4096 : 0), 64, 64, 64);
+    __syncthreads();
+    acc re, im;
+    product(sb, re, im);
+    __syncthreads();
+    for (int r = 0; r < 4; ++r) {
+      int row = sb * 64 + wave * 16 + mfma_row<T>(lk, r);
+      bool ok = (row - sb * 64) < bs;
+      sr[row][li] = ok ? re[r] : T(0);
+      si[row][li] = ok ? im[r] : T(0);
+    }
+    const int rc0 = UPPER ? 0 : sb + 1, rc1 = UPPER ? sb : nsb;
+    for (int rc = rc0; rc < rc1; ++rc) {
+      __syncthreads();
+      load_blk(D + (int64_t)(rc * 64) * ldd + sb * 64, ldd, min(64, kb - rc * 64), bs);
+      __syncthreads();
+      product(sb, re, im);
+      for (int r = 0; r < 4; ++r) {
+        int row = rc * 64 + wave * 16 + mfma_row<T>(lk, r);
+        sr[row][li] -= re[r];
+        si[row][li] -= im[r];
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < kb * CW; e += 256) {
+    int i = e / CW, c = e % CW;
+    if (c < ncol) S[(int64_t)i * lds + cb + c] = mkc<T>(sr[i][c], si[i][c]);
   }
 }

@@ -144,8 +144,9 @@ int hprmat_hip_r_to_s(int nch, int nr, const void* dX, int64_t ldx, const double
   if (nopen == 0) return hipGetLastError() == hipSuccess ? 0 : -1;
   const int64_t ld = nopen;
   const size_t zbytes = (size_t)nopen * nopen * sizeof(double2);
-  const size_t need = 2 * zbytes + 256 + (size_t)(nopen + 64) * sizeof(int) + 256 +
-                      hpr::scratch_bytes(nopen) + 256;
+  const size_t ibytes = ((size_t)(nopen + 64) * sizeof(int) + 255) & ~(size_t)255;
+  const size_t dbytes = hpr::dinv_elems(nopen) * sizeof(double2);
+  const size_t need = 3 * zbytes + 3 * ibytes + dbytes + hpr::scratch_bytes() + 1024;
   if (need > g_small.cap) {
     if (g_small.buf) (void)hipFree(g_small.buf);
     g_small.buf = nullptr;
@@ -153,21 +154,26 @@ int hprmat_hip_r_to_s(int nch, int nr, const void* dX, int64_t ldx, const double
     if (hipMalloc(&g_small.buf, need) != hipSuccess) return -2;
     g_small.cap = need;
   }
-  char* p = (char*)g_small.buf;
+  char* p = (char*)(((uintptr_t)g_small.buf + 255) & ~(uintptr_t)255);
   double2* ZO = (double2*)p;
   double2* ZI = (double2*)(p + zbytes);
-  int* ipiv = (int*)(p + ((2 * zbytes + 255) & ~(size_t)255));
-  int* info = ipiv + nopen + 32;
-  void* scr = (char*)info + 256;
-  scr = (void*)(((uintptr_t)scr + 255) & ~(uintptr_t)255);
+  double2* ZP = (double2*)(p + 2 * zbytes);
+  p += ((3 * zbytes + 255) & ~(size_t)255);
+  int* ipiv = (int*)p;
+  int* perm = (int*)(p + ibytes);
+  int* info = (int*)(p + 2 * ibytes);
+  p += 3 * ibytes;
+  double2* dinv = (double2*)p;
+  p += (dbytes + 255) & ~(size_t)255;
   hpr::Scratch sc;
-  hpr::scratch_bind(sc, scr, nopen);
+  hpr::scratch_bind(sc, p);
   zmatrices_kernel<<<(nopen * nopen + 255) / 256, 256, 0, st>>>(
       nch, nopen, (const double2*)dR, rmax, qk, open, (const double2*)co, (const double2*)cop, ZO, ZI);
   hpr::LuPlan plan;
-  if (hpr::getrf<double>(ZO, ld, nopen, ipiv, info, sc, plan, false, st)) return -1;
-  if (hpr::getrs<double>(ZO, ld, nopen, ipiv, ZI, ld, nopen, nullptr, plan, false, st)) return -1;
-  store_colmajor_kernel<<<(nopen * nopen + 255) / 256, 256, 0, st>>>(nopen, ZI, (double2*)dS);
+  if (hpr::getrf<double>(ZO, ld, nopen, ipiv, info, dinv, perm, sc, plan, false, st)) return -1;
+  hpr::permute_cast<double, double>(nopen, nopen, perm, ZI, ZP, nullptr, st);
+  if (hpr::getrs<double>(ZO, ld, nopen, dinv, ZP, ld, nopen, nullptr, plan, false, st)) return -1;
+  store_colmajor_kernel<<<(nopen * nopen + 255) / 256, 256, 0, st>>>(nopen, ZP, (double2*)dS);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

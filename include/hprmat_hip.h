@@ -172,6 +172,14 @@ int hprmat_hip_r_to_s(int nch, int nr, const void *dX, int64_t ldx,
                       const void *co, const void *cop, void *dR, void *dS,
                       void *stream);
 
+/* No reference counterpart: HIP-event timing of the LU's trailing-update
+   GEMM (the dominant kernel) for roofline reports.  Enable, factor, then
+   read the summed elapsed milliseconds, the algorithmic flops (8 M N K per
+   complex update) and the number of timed launches (read synchronises and
+   resets). */
+int hprmat_hip_prof_enable(int on);
+int hprmat_hip_prof_read(double *ms, double *flops, int *count);
+
 /* Library version string. */
 const char *hprmat_hip_version(void);
 
