@@ -19,10 +19,17 @@ struct hprmat_hip_lu {
   int* ipiv = nullptr;
   size_t ipiv_cap = 0;
   int* dinfo = nullptr;
-  void* dinv = nullptr;       // inverses of the 64 x 64 diagonal blocks
-  size_t dinv_cap = 0;
+  void* dinv64 = nullptr;     // inverses of the 64 x 64 diagonal blocks
+  size_t dinv64_cap = 0;
+  void* dinv256 = nullptr;    // inverses of the 256 x 256 diagonal blocks
+  size_t dinv256_cap = 0;
+  void* ustage = nullptr;     // U12 staging block (256 x n)
+  size_t ustage_cap = 0;
   int* perm = nullptr;        // final row permutation
   size_t perm_cap = 0;
+  double* scale = nullptr;    // equilibration factors (when equil)
+  size_t scale_cap = 0;
+  int equil = 0;
   void* scr = nullptr;
   size_t scr_cap = 0;
   hpr::Scratch sc;
@@ -32,10 +39,13 @@ struct hprmat_hip_lu {
   double2* Xrm = nullptr;
   double2* R = nullptr;
   void* RF = nullptr;
+  void* RY = nullptr;
   size_t rhs_cap = 0;  // elements of each n x nrhs buffer
   double* part = nullptr;
   RefineState* state = nullptr;
   hpr::LuPlan plan;
+  hpr::SideStream side;       // look-ahead stream and its two events
+  int lookahead = 1;
 };
 
 namespace {
@@ -52,16 +62,33 @@ size_t factor_elem_bytes(int method) { return method == HPRMAT_PLAIN_FP64 ? 16 :
 
 int64_t padded_ld(int n) { return (((int64_t)n + 15) / 16) * 16 + 16; }
 
+int ensure_side(hprmat_hip_lu* h) {
+  if (h->side.stream) return 0;
+  if (hipStreamCreateWithFlags(&h->side.stream, hipStreamNonBlocking) != hipSuccess) return -1;
+  if (hipEventCreateWithFlags(&h->side.ready, hipEventDisableTiming) != hipSuccess) return -1;
+  if (hipEventCreateWithFlags(&h->side.rest, hipEventDisableTiming) != hipSuccess) return -1;
+  return 0;
+}
+
 void free_lu(hprmat_hip_lu* h) {
-  void* ps[] = {h->fac, h->ipiv, h->dinfo, h->dinv, h->perm, h->scr,  h->Brm,
-                h->Xrm, h->R,    h->RF,    h->part, h->state};
+  if (h->side.stream) {
+    (void)hipStreamSynchronize(h->side.stream);
+    (void)hipEventDestroy(h->side.ready);
+    (void)hipEventDestroy(h->side.rest);
+    (void)hipStreamDestroy(h->side.stream);
+  }
+  void* ps[] = {h->fac, h->ipiv, h->dinfo, h->dinv64, h->dinv256, h->ustage, h->perm, h->scale,
+                h->scr, h->Brm,  h->Xrm,   h->R,      h->RF,      h->RY,     h->part, h->state};
   for (void* p : ps)
     if (p) hipFree(p);
   *h = hprmat_hip_lu();
 }
 
-int lu_factor(hprmat_hip_lu* h, int method, const void* dA, int64_t lda, int n, hipStream_t st) {
+int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, int n,
+              hipStream_t st) {
   if (n < 0 || lda < n) return -1;
+  const int method = method_flags & 0xff;
+  const int equil = (method_flags & HPRMAT_EQUILIBRATE) ? 1 : 0;
   if (method < HPRMAT_PLAIN_FP64 || method > HPRMAT_MIXED_FP32_3M) {
     fprintf(stderr, "hprmat_hip: factorization method %d is not available\n", method);
     return -1;
@@ -80,27 +107,51 @@ int lu_factor(hprmat_hip_lu* h, int method, const void* dA, int64_t lda, int n, 
   }
   if (grow(&h->scr, &h->scr_cap, hpr::scratch_bytes())) return -2;
   hpr::scratch_bind(h->sc, h->scr);
-  if (grow(&h->dinv, &h->dinv_cap, hpr::dinv_elems(n) * factor_elem_bytes(method))) return -2;
+  const size_t eb = factor_elem_bytes(method);
+  if (grow(&h->dinv64, &h->dinv64_cap, hpr::dinv64_elems(n) * eb)) return -2;
+  if (grow(&h->dinv256, &h->dinv256_cap, hpr::dinv256_elems(n) * eb)) return -2;
+  if (grow(&h->ustage, &h->ustage_cap, hpr::ustage_elems(n) * eb)) return -2;
   if (grow((void**)&h->perm, &h->perm_cap, (size_t)(n + 1) * sizeof(int))) return -2;
+  if (grow((void**)&h->scale, &h->scale_cap, (size_t)(n + 1) * sizeof(double))) return -2;
+  h->equil = equil;
+  const double* sc = nullptr;
+  if (equil) {
+    if (hpr::diag_scale(n, (const double2*)dA, lda, h->scale, st)) return -1;
+    sc = h->scale;
+  }
   h->n = n;
   h->method = method;
   h->ld = ld;
   h->A64 = (const double2*)dA;
   h->lda = lda;
   const double2* A = (const double2*)dA;
+  const hpr::SideStream* side = nullptr;
+  if (h->lookahead && n > 2 * h->plan.nb) {
+    if (ensure_side(h)) return -1;
+    side = &h->side;
+  }
   int rc;
   if (method == HPRMAT_PLAIN_FP64) {
     double2* F = (double2*)h->fac;
-    rc = hpr::transpose_cast<double, double>(n, n, A, lda, F, ld, st);
+    rc = hpr::transpose_cast<double, double>(n, n, A, lda, F, ld, st, sc, sc);
+    hpr::FactorWork<double> w;
+    w.dinv64 = (double2*)h->dinv64;
+    w.dinv256 = (double2*)h->dinv256;
+    w.ustage = (double2*)h->ustage;
+    w.perm = h->perm;
     if (!rc)
-      rc = hpr::getrf<double>(F, ld, n, h->ipiv, h->dinfo, (double2*)h->dinv, h->perm, h->sc,
-                              h->plan, false, st);
+      rc = hpr::getrf<double>(F, ld, n, h->ipiv, h->dinfo, w, h->sc, h->plan, false, st, side);
   } else {
     float2* F = (float2*)h->fac;
-    rc = hpr::transpose_cast<double, float>(n, n, A, lda, F, ld, st);
+    rc = hpr::transpose_cast<double, float>(n, n, A, lda, F, ld, st, sc, sc);
+    hpr::FactorWork<float> w;
+    w.dinv64 = (float2*)h->dinv64;
+    w.dinv256 = (float2*)h->dinv256;
+    w.ustage = (float2*)h->ustage;
+    w.perm = h->perm;
     if (!rc)
-      rc = hpr::getrf<float>(F, ld, n, h->ipiv, h->dinfo, (float2*)h->dinv, h->perm, h->sc,
-                             h->plan, method == HPRMAT_MIXED_FP32_3M, st);
+      rc = hpr::getrf<float>(F, ld, n, h->ipiv, h->dinfo, w, h->sc, h->plan,
+                             method == HPRMAT_MIXED_FP32_3M, st, side);
   }
   return rc;
 }
@@ -112,13 +163,14 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
   if (nrhs < 0 || ldb < n) return -1;
   const size_t cnt = (size_t)n * nrhs;
   if (cnt > h->rhs_cap || !h->Brm) {
-    size_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-    if (h->Brm) { hipFree(h->Brm); hipFree(h->Xrm); hipFree(h->R); hipFree(h->RF); }
+    size_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+    if (h->Brm) { hipFree(h->Brm); hipFree(h->Xrm); hipFree(h->R); hipFree(h->RF); hipFree(h->RY); }
     h->Brm = h->Xrm = h->R = nullptr;
-    h->RF = nullptr;
+    h->RF = h->RY = nullptr;
     h->rhs_cap = 0;
     if (grow((void**)&h->Brm, &c0, cnt * 16) || grow((void**)&h->Xrm, &c1, cnt * 16) ||
-        grow((void**)&h->R, &c2, cnt * 16) || grow(&h->RF, &c3, cnt * 16))
+        grow((void**)&h->R, &c2, cnt * 16) || grow(&h->RF, &c3, cnt * 16) ||
+        grow(&h->RY, &c4, cnt * 16))
       return -2;
     h->rhs_cap = cnt;
   }
@@ -127,15 +179,15 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
   int rc;
   const bool g3m = h->method == HPRMAT_MIXED_FP32_3M;
   if (h->method == HPRMAT_PLAIN_FP64)
-    rc = hpr::refine_solve<double>((const double2*)h->fac, h->ld, n, (const double2*)h->dinv,
-                                   h->perm, h->A64, h->lda,
-                                   h->Brm, h->Xrm, nrhs, max_refine, tol, crit, h->state, h->R,
-                                   (double2*)h->RF, h->part, h->plan, false, st);
+    rc = hpr::refine_solve<double>((const double2*)h->fac, h->ld, n, (const double2*)h->dinv256,
+                                   h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, tol,
+                                   crit, h->state, h->R, (double2*)h->RF, (double2*)h->RY,
+                                   h->part, h->plan, false, st, h->equil ? h->scale : nullptr);
   else
-    rc = hpr::refine_solve<float>((const float2*)h->fac, h->ld, n, (const float2*)h->dinv,
-                                  h->perm, h->A64, h->lda,
-                                  h->Brm, h->Xrm, nrhs, max_refine, tol, crit, h->state, h->R,
-                                  (float2*)h->RF, h->part, h->plan, g3m, st);
+    rc = hpr::refine_solve<float>((const float2*)h->fac, h->ld, n, (const float2*)h->dinv256,
+                                  h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, tol,
+                                  crit, h->state, h->R, (float2*)h->RF, (float2*)h->RY, h->part,
+                                  h->plan, g3m, st, h->equil ? h->scale : nullptr);
   if (rc) return rc;
   if (final_check)
     rc = hpr::residual_check(n, nrhs, h->A64, h->lda, h->Brm, h->Xrm, h->R, h->part, h->state,

@@ -52,7 +52,8 @@ __global__ void fill_u32_kernel(unsigned* p, size_t n, unsigned v) {
 template <class Tin, class Tout>
 __global__ void __launch_bounds__(256)
 transpose_cast_kernel(int rows, int cols, const cx<Tin>* __restrict__ in, int64_t ldi,
-                      cx<Tout>* __restrict__ out, int64_t ldo) {
+                      cx<Tout>* __restrict__ out, int64_t ldo, const double* __restrict__ rs,
+                      const double* __restrict__ cs) {
   __shared__ cx<Tin> tile[32][33];
   const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -65,7 +66,12 @@ transpose_cast_kernel(int rows, int cols, const cx<Tin>* __restrict__ in, int64_
     int r = r0 + q, c = c0 + tx;
     if (r < rows && c < cols) {
       cx<Tin> v = tile[tx][q];
-      out[(int64_t)r * ldo + c] = mkc<Tout>((Tout)v.x, (Tout)v.y);
+      if (rs) {
+        const double f = rs[r] * cs[c];
+        out[(int64_t)r * ldo + c] = mkc<Tout>((Tout)(v.x * f), (Tout)(v.y * f));
+      } else {
+        out[(int64_t)r * ldo + c] = mkc<Tout>((Tout)v.x, (Tout)v.y);
+      }
     }
   }
 }
@@ -81,19 +87,34 @@ __global__ void cast_kernel(size_t n, const cx<Tin>* __restrict__ in, cx<Tout>* 
   }
 }
 
+// X (+)= diag(s) D for row-major n x nrhs arrays (s = null: no scaling);
+// accumulate = 0 stores the first solution, 1 adds a correction (and counts
+// the sweep); both are skipped once the refinement has converged.
 template <class TF>
-__global__ void add_correction_kernel(size_t n, double2* __restrict__ X, const cx<TF>* __restrict__ D,
-                                      RefineState* __restrict__ st) {
-  if (st->done) return;
+__global__ void add_correction_kernel(int nrhs, size_t cnt, double2* __restrict__ X,
+                                      const cx<TF>* __restrict__ D, const double* __restrict__ s,
+                                      int accumulate, RefineState* __restrict__ st) {
+  if (accumulate && st->done) return;
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) st->sweeps += 1;
-  for (; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  if (i == 0 && accumulate) st->sweeps += 1;
+  for (; i < cnt; i += (size_t)gridDim.x * blockDim.x) {
     cx<TF> d = D[i];
-    double2 x = X[i];
-    x.x += (double)d.x;
-    x.y += (double)d.y;
+    const double f = s ? s[i / nrhs] : 1.0;
+    double2 x = accumulate ? X[i] : make_double2(0.0, 0.0);
+    x.x += (double)d.x * f;
+    x.y += (double)d.y * f;
     X[i] = x;
   }
+}
+
+// s[i] = 1 / sqrt(|a_ii|) of a column-major matrix (1 where a_ii = 0).
+__global__ void diag_scale_kernel(int n, const double2* __restrict__ A, int64_t lda,
+                                  double* __restrict__ s) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double2 a = A[(int64_t)i * lda + i];
+  const double m = sqrt(a.x * a.x + a.y * a.y);
+  s[i] = m > 0.0 ? 1.0 / sqrt(m) : 1.0;
 }
 
 // Residual statistics, stage 1: per-block (max |Re|+|Im|, its column-major
@@ -214,10 +235,13 @@ template <class T> int launch_leaf(cx<T>* P, int64_t ld, int m, int w, int row_b
   x.row = x.rowj + 2 * LEAF_W;
   x.flag = s.flags;
   size_t lds = (size_t)rpw * LEAF_W * sizeof(cx<T>);
+  const size_t stage = (size_t)G * LEAF_W * sizeof(cx<T>);
+  const int stage_all = (G > 1 && G <= 32 && lds + stage <= LEAF_SLAB_MAX) ? 1 : 0;
+  if (stage_all) lds += stage;
   static bool attr = false;
   if (!attr) { allow_big_lds(leaf_getf2_kernel<T, LEAF_W>); attr = true; }
   leaf_getf2_kernel<T, LEAF_W><<<G, 256, lds, st>>>(P, ld, m, w, rpw, row_base, ipiv, info, x,
-                                                    (unsigned)leaf * LEAF_W, s.err);
+                                                    (unsigned)leaf * LEAF_W, stage_all, s.err);
   return 0;
 }
 
@@ -230,14 +254,6 @@ template <class T> void launch_laswp(cx<T>* A, int64_t ld, int c0, int c1, int k
   if (!attr) { allow_big_lds(laswp_kernel<T>); attr = true; }
   int grid = (c1 - c0 + CW - 1) / CW;
   laswp_kernel<T><<<grid, 256, lds, st>>>(A, ld, c0, c1, k1, nk, ipiv, skip);
-}
-
-// Block-row triangular solve (kb <= 256) with the stored diagonal inverses.
-template <class T, bool UPPER>
-void trsm_blk(const cx<T>* D, int64_t ldd, int kb, const cx<T>* dinv, cx<T>* S, int64_t lds,
-              int ncols, const int* skip, hipStream_t st) {
-  if (ncols <= 0 || kb <= 0) return;
-  trsm_blk_kernel<T, UPPER><<<(ncols + 15) / 16, 256, 0, st>>>(D, ldd, kb, dinv, S, lds, ncols, skip);
 }
 
 // ---- optional HIP-event timing of the trailing-update GEMM ------------------
@@ -318,18 +334,22 @@ int gemm(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<
 }
 
 template <class T>
-int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, cx<T>* dinv, int* perm,
-          const Scratch& s, const LuPlan& plan, bool g3m, hipStream_t st) {
+int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>& w,
+          const Scratch& s, const LuPlan& plan, bool g3m, hipStream_t st, const SideStream* side) {
   if (n <= 0) return 0;
   const int nb = plan.nb;
-  if (nb <= 0 || nb > 256 || nb % 64 != 0) return -1;
+  if (nb != 256) return -1;
+  const bool la = side && side->stream && n > 2 * nb;  // look-ahead
   fill_u32_kernel<<<4, 256, 0, st>>>(s.flags, 2 * MAX_LEAF_G, 0u);
   fill_u32_kernel<<<1, 64, 0, st>>>((unsigned*)s.err, 1, 0u);
   fill_u32_kernel<<<1, 64, 0, st>>>((unsigned*)info, 1, 0u);
   int leaf = 0;
+  bool rest_pending = false;
   for (int k = 0; k < n; k += nb) {
     const int kb = min(nb, n - k), m = n - k;
-    // -- panel: 32-column cooperative leaves, right-looking inside the panel
+    // -- panel: 32-column cooperative leaves, right-looking inside the panel.
+    //    With look-ahead this overlaps the side stream's update of the
+    //    columns beyond the panel from the previous step.
     for (int t = 0; t < kb; t += LEAF_W) {
       const int wt = min(LEAF_W, kb - t);
       cx<T>* P = A + (int64_t)(k + t) * ld + (k + t);
@@ -343,52 +363,87 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, cx<T>* dinv, int* p
                 P + (int64_t)wt * ld + wt, ld, T(-1), 1, g3m, nullptr, st);
       }
     }
-    // -- inverses of the panel's diagonal blocks (for U12 here and getrs later)
-    diag_inv_kernel<T><<<dim3((kb + 63) / 64, 2), 256, 0, st>>>(A, ld, n, k / 64, dinv);
+    // -- inverses of the panel's diagonal blocks: 64 x 64, then 256 x 256
+    diag_inv_kernel<T><<<dim3((kb + 63) / 64, 2), 256, 0, st>>>(A, ld, n, k / 64, w.dinv64);
+    blk_inv_kernel<T><<<dim3(1, 2, 4), 256, 0, st>>>(A, ld, n, k / 256, w.dinv64, w.dinv256);
+    // columns outside the panel may still be updated (and read) on the side
+    // stream: wait before interchanging rows there
+    if (rest_pending) {
+      HPR_CHECK(hipStreamWaitEvent(st, side->rest, 0));
+      rest_pending = false;
+    }
     if (k > 0) launch_laswp<T>(A, ld, 0, k, k, kb, ipiv, nullptr, st);
     if (k + kb < n) {
+      const int nc = n - k - kb;
       cx<T>* Akk = A + (int64_t)k * ld + k;
       launch_laswp<T>(A, ld, k + kb, n, k, kb, ipiv, nullptr, st);
-      trsm_blk<T, false>(Akk, ld, kb, dinv + (size_t)(k / 64) * 2 * 4096, Akk + kb, ld, n - k - kb,
-                         nullptr, st);
+      // U12 = inv(L11) A12, staged (row stride nc), then A22 -= L21 U12
+      gemm<T>(kb, nc, kb, w.dinv256 + (size_t)(k / 256) * 2 * 65536, 256, false, Akk + kb, ld,
+              w.ustage, nc, T(1), 0, g3m, nullptr, st);
+      // next panel's columns first (on the main stream), the rest on the side
+      const int c_next = (la && nc > nb) ? nb : nc;
       ProfRec* pr = g_prof_on ? prof_slot() : nullptr;
       if (pr) (void)hipEventRecord(pr->a, st);
-      gemm<T>(n - k - kb, n - k - kb, kb, Akk + (int64_t)kb * ld, ld, false, Akk + kb, ld,
+      gemm<T>(nc, c_next, kb, Akk + (int64_t)kb * ld, ld, false, w.ustage, nc,
               Akk + (int64_t)kb * ld + kb, ld, T(-1), 1, g3m, nullptr, st);
       if (pr) {
         (void)hipEventRecord(pr->b, st);
-        pr->flops = 8.0 * (double)(n - k - kb) * (double)(n - k - kb) * (double)kb;
+        pr->flops = 8.0 * (double)nc * (double)c_next * (double)kb;
+      }
+      hipStream_t cs = st;
+      if (c_next < nc) {
+        HPR_CHECK(hipEventRecord(side->ready, st));
+        HPR_CHECK(hipStreamWaitEvent(side->stream, side->ready, 0));
+        cs = side->stream;
+        ProfRec* pr2 = g_prof_on ? prof_slot() : nullptr;
+        if (pr2) (void)hipEventRecord(pr2->a, cs);
+        gemm<T>(nc, nc - c_next, kb, Akk + (int64_t)kb * ld, ld, false, w.ustage + c_next, nc,
+                Akk + (int64_t)kb * ld + kb + c_next, ld, T(-1), 1, g3m, nullptr, cs);
+        if (pr2) {
+          (void)hipEventRecord(pr2->b, cs);
+          pr2->flops = 8.0 * (double)nc * (double)(nc - c_next) * (double)kb;
+        }
+      }
+      const size_t cnt = (size_t)kb * nc;
+      copy2d_kernel<T><<<(int)min((size_t)4096, (cnt + 255) / 256), 256, 0, cs>>>(
+          kb, nc, w.ustage, nc, Akk + kb, ld);
+      if (cs != st) {
+        HPR_CHECK(hipEventRecord(side->rest, cs));
+        rest_pending = true;
       }
     }
   }
+  if (rest_pending) HPR_CHECK(hipStreamWaitEvent(st, side->rest, 0));
   const size_t perm_lds = (size_t)n * sizeof(int);
   const int in_lds = perm_lds <= 140 * 1024;
   static bool attr = false;
   if (!attr) { allow_big_lds(perm_from_ipiv_kernel); attr = true; }
-  perm_from_ipiv_kernel<<<1, 256, in_lds ? perm_lds : 0, st>>>(ipiv, n, perm, in_lds);
+  perm_from_ipiv_kernel<<<1, 256, in_lds ? perm_lds : 0, st>>>(ipiv, n, w.perm, in_lds);
   HPR_CHECK(hipGetLastError());
   return 0;
 }
 
 template <class T>
-int getrs(const cx<T>* LU, int64_t ld, int n, const cx<T>* dinv, cx<T>* B, int64_t ldb, int nrhs,
-          const int* skip, const LuPlan& plan, bool g3m, hipStream_t st) {
+int getrs(const cx<T>* LU, int64_t ld, int n, const cx<T>* dinv256, cx<T>* B, int64_t ldb,
+          int nrhs, cx<T>* Y, const int* skip, const LuPlan& plan, bool g3m, hipStream_t st) {
   if (n <= 0 || nrhs <= 0) return 0;
-  const int rs = plan.nb;
+  const int rs = 256;
+  // forward: Y_b = inv(L_bb) B_b, B_rest -= L_rest,b Y_b
   for (int b = 0; b < n; b += rs) {
     const int bs = min(rs, n - b);
-    trsm_blk<T, false>(LU + (int64_t)b * ld + b, ld, bs, dinv + (size_t)(b / 64) * 2 * 4096,
-                       B + (int64_t)b * ldb, ldb, nrhs, skip, st);
+    gemm<T>(bs, nrhs, bs, dinv256 + (size_t)(b / 256) * 2 * 65536, 256, false, B + (int64_t)b * ldb,
+            ldb, Y + (int64_t)b * ldb, ldb, T(1), 0, g3m, skip, st);
     if (b + bs < n)
-      gemm<T>(n - b - bs, nrhs, bs, LU + (int64_t)(b + bs) * ld + b, ld, false, B + (int64_t)b * ldb,
+      gemm<T>(n - b - bs, nrhs, bs, LU + (int64_t)(b + bs) * ld + b, ld, false, Y + (int64_t)b * ldb,
               ldb, B + (int64_t)(b + bs) * ldb, ldb, T(-1), 1, g3m, skip, st);
   }
+  // backward: B_b = inv(U_bb) Y_b, Y_above -= U_above,b B_b
   for (int b = ((n - 1) / rs) * rs; b >= 0; b -= rs) {
     const int bs = min(rs, n - b);
-    trsm_blk<T, true>(LU + (int64_t)b * ld + b, ld, bs, dinv + (size_t)(b / 64) * 2 * 4096,
-                      B + (int64_t)b * ldb, ldb, nrhs, skip, st);
+    gemm<T>(bs, nrhs, bs, dinv256 + ((size_t)(b / 256) * 2 + 1) * 65536, 256, false,
+            Y + (int64_t)b * ldb, ldb, B + (int64_t)b * ldb, ldb, T(1), 0, g3m, skip, st);
     if (b > 0)
-      gemm<T>(b, nrhs, bs, LU + b, ld, false, B + (int64_t)b * ldb, ldb, B, ldb, T(-1), 1, g3m,
+      gemm<T>(b, nrhs, bs, LU + b, ld, false, B + (int64_t)b * ldb, ldb, Y, ldb, T(-1), 1, g3m,
               skip, st);
   }
   HPR_CHECK(hipGetLastError());
@@ -397,20 +452,27 @@ int getrs(const cx<T>* LU, int64_t ld, int n, const cx<T>* dinv, cx<T>* B, int64
 
 template <class Tin, class Tout>
 int permute_cast(int n, int nrhs, const int* perm, const cx<Tin>* in, cx<Tout>* out,
-                 const int* skip, hipStream_t st) {
+                 const int* skip, hipStream_t st, const double* scale) {
   const size_t cnt = (size_t)n * nrhs;
   if (cnt == 0) return 0;
   const int grid = (int)min((size_t)4096, (cnt + 255) / 256);
-  permute_cast_kernel<Tin, Tout><<<grid, 256, 0, st>>>(n, nrhs, perm, in, out, skip);
+  permute_cast_kernel<Tin, Tout><<<grid, 256, 0, st>>>(n, nrhs, perm, in, out, scale, skip);
+  return 0;
+}
+
+int diag_scale(int n, const double2* A, int64_t lda, double* s, hipStream_t st) {
+  if (n <= 0) return 0;
+  diag_scale_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, A, lda, s);
+  HPR_CHECK(hipGetLastError());
   return 0;
 }
 
 template <class Tin, class Tout>
 int transpose_cast(int rows, int cols, const cx<Tin>* in, int64_t ldi, cx<Tout>* out, int64_t ldo,
-                   hipStream_t st) {
+                   hipStream_t st, const double* rs, const double* cs) {
   if (rows <= 0 || cols <= 0) return 0;
   dim3 grid((rows + 31) / 32, (cols + 31) / 32);
-  transpose_cast_kernel<Tin, Tout><<<grid, 256, 0, st>>>(rows, cols, in, ldi, out, ldo);
+  transpose_cast_kernel<Tin, Tout><<<grid, 256, 0, st>>>(rows, cols, in, ldi, out, ldo, rs, cs);
   HPR_CHECK(hipGetLastError());
   return 0;
 }
@@ -426,24 +488,25 @@ static int launch_norms(int n, int nrhs, const double2* R, double* part, RefineS
 }
 
 template <class TF>
-int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv, const int* perm,
+int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, const int* perm,
                  const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
                  int max_refine, double tol, int crit, RefineState* state, double2* R, cx<TF>* RF,
-                 double* part, const LuPlan& plan, bool g3m, hipStream_t st) {
+                 cx<TF>* RY, double* part, const LuPlan& plan, bool g3m, hipStream_t st,
+                 const double* scale) {
   const size_t cnt = (size_t)n * nrhs;
   const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
   const int* done = &state->done;
   launch_norms(n, nrhs, B, part, state, 0, crit, tol, nullptr, st);
-  permute_cast<double, TF>(n, nrhs, perm, B, RF, nullptr, st);
-  if (getrs<TF>(LU, ld, n, dinv, RF, nrhs, nrhs, nullptr, plan, g3m, st)) return -1;
-  cast_kernel<TF, double><<<eg, 256, 0, st>>>(cnt, RF, X, nullptr);
+  permute_cast<double, TF>(n, nrhs, perm, B, RF, nullptr, st, scale);
+  if (getrs<TF>(LU, ld, n, dinv256, RF, nrhs, nrhs, RY, nullptr, plan, g3m, st)) return -1;
+  add_correction_kernel<TF><<<eg, 256, 0, st>>>(nrhs, cnt, X, RF, scale, 0, state);
   for (int it = 0; it < max_refine; ++it) {
     cast_kernel<double, double><<<eg, 256, 0, st>>>(cnt, B, R, done);
     gemm<double>(n, nrhs, n, A64, lda, true, X, nrhs, R, nrhs, -1.0, 1, false, done, st);
     launch_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
-    permute_cast<double, TF>(n, nrhs, perm, R, RF, done, st);
-    if (getrs<TF>(LU, ld, n, dinv, RF, nrhs, nrhs, done, plan, g3m, st)) return -1;
-    add_correction_kernel<TF><<<eg, 256, 0, st>>>(cnt, X, RF, state);
+    permute_cast<double, TF>(n, nrhs, perm, R, RF, done, st, scale);
+    if (getrs<TF>(LU, ld, n, dinv256, RF, nrhs, nrhs, RY, done, plan, g3m, st)) return -1;
+    add_correction_kernel<TF><<<eg, 256, 0, st>>>(nrhs, cnt, X, RF, scale, 1, state);
   }
   HPR_CHECK(hipGetLastError());
   return 0;
@@ -464,22 +527,22 @@ int residual_check(int n, int nrhs, const double2* A64, int64_t lda, const doubl
 }
 
 #define INST(T)                                                                               \
-  template int getrf<T>(cx<T>*, int64_t, int, int*, int*, cx<T>*, int*, const Scratch&,        \
-                        const LuPlan&, bool, hipStream_t);                                     \
-  template int getrs<T>(const cx<T>*, int64_t, int, const cx<T>*, cx<T>*, int64_t, int,        \
+  template int getrf<T>(cx<T>*, int64_t, int, int*, int*, const FactorWork<T>&, const Scratch&, \
+                        const LuPlan&, bool, hipStream_t, const SideStream*);                  \
+  template int getrs<T>(const cx<T>*, int64_t, int, const cx<T>*, cx<T>*, int64_t, int, cx<T>*, \
                         const int*, const LuPlan&, bool, hipStream_t);                         \
   template int gemm<T>(int, int, int, const cx<T>*, int64_t, bool, const cx<T>*, int64_t,      \
                        cx<T>*, int64_t, T, int, bool, const int*, hipStream_t);                \
   template int refine_solve<T>(const cx<T>*, int64_t, int, const cx<T>*, const int*,          \
                                const double2*, int64_t, const double2*, double2*, int, int,    \
-                               double, int, RefineState*, double2*, cx<T>*, double*,           \
-                               const LuPlan&, bool, hipStream_t);
+                               double, int, RefineState*, double2*, cx<T>*, cx<T>*, double*,   \
+                               const LuPlan&, bool, hipStream_t, const double*);
 INST(double)
 INST(float)
-template int transpose_cast<double, double>(int, int, const cx<double>*, int64_t, cx<double>*, int64_t, hipStream_t);
-template int transpose_cast<double, float>(int, int, const cx<double>*, int64_t, cx<float>*, int64_t, hipStream_t);
-template int transpose_cast<float, double>(int, int, const cx<float>*, int64_t, cx<double>*, int64_t, hipStream_t);
-template int permute_cast<double, double>(int, int, const int*, const cx<double>*, cx<double>*, const int*, hipStream_t);
-template int permute_cast<double, float>(int, int, const int*, const cx<double>*, cx<float>*, const int*, hipStream_t);
+template int transpose_cast<double, double>(int, int, const cx<double>*, int64_t, cx<double>*, int64_t, hipStream_t, const double*, const double*);
+template int transpose_cast<double, float>(int, int, const cx<double>*, int64_t, cx<float>*, int64_t, hipStream_t, const double*, const double*);
+template int transpose_cast<float, double>(int, int, const cx<float>*, int64_t, cx<double>*, int64_t, hipStream_t, const double*, const double*);
+template int permute_cast<double, double>(int, int, const int*, const cx<double>*, cx<double>*, const int*, hipStream_t, const double*);
+template int permute_cast<double, float>(int, int, const int*, const cx<double>*, cx<float>*, const int*, hipStream_t, const double*);
 
 }  // namespace hpr

@@ -38,31 +38,61 @@ enum Criterion { CRIT_ABS_MAX = 0, CRIT_REL_FRO = 1 };
 size_t scratch_bytes();
 void scratch_bind(Scratch& s, void* base);
 
-// Elements of the diagonal-block inverse area of an n x n factor.
-inline size_t dinv_elems(int n) { return (size_t)((n + 63) / 64) * 2 * 4096; }
+// Device work areas of one factorization (element counts for an n x n
+// factor): inverses of the 64 x 64 and 256 x 256 diagonal blocks of L and U,
+// the 256-row staging block of U12 and the final row permutation.
+inline size_t dinv64_elems(int n) { return (size_t)((n + 63) / 64) * 2 * 4096; }
+inline size_t dinv256_elems(int n) { return (size_t)((n + 255) / 256) * 2 * 65536; }
+inline size_t ustage_elems(int n) { return (size_t)256 * (n > 0 ? n : 1); }
+
+template <class T> struct FactorWork {
+  cx<T>* dinv64 = nullptr;
+  cx<T>* dinv256 = nullptr;
+  cx<T>* ustage = nullptr;
+  int* perm = nullptr;
+};
+
+// Second stream of the look-ahead: the trailing update of all but the next
+// panel's columns runs there while the main stream factors the next panel.
+// Both streams keep the default priority.
+struct SideStream {
+  hipStream_t stream = nullptr;
+  hipEvent_t ready = nullptr;   // main -> side: U12 staged, next panel updated
+  hipEvent_t rest = nullptr;    // side -> main: rest of the trailing update done
+};
 
 // Device factorization P A = L U (row-major A, in place).  Also produces the
-// inverses of the 64 x 64 diagonal blocks of L and U (dinv) and the final row
-// permutation perm (perm[i] = source row of row i of P A) used by getrs.
+// inverses of the 256 x 256 diagonal blocks of L and U (w.dinv256, used as
+// GEMM operands by the triangular solves) and the final row permutation
+// w.perm (perm[i] = source row of row i of P A) used by getrs.
 template <class T>
-int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, cx<T>* dinv, int* perm,
-          const Scratch& s, const LuPlan& plan, bool gauss3m, hipStream_t st);
+int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>& w,
+          const Scratch& s, const LuPlan& plan, bool gauss3m, hipStream_t st,
+          const SideStream* side = nullptr);
 
 // Solve L U X = B in place for a right-hand side already permuted by perm
-// (row-major n x nrhs, leading dimension ldb).
+// (row-major n x nrhs, leading dimension ldb); Y is an n x nrhs work array
+// with the same leading dimension.
 template <class T>
-int getrs(const cx<T>* LU, int64_t ld, int n, const cx<T>* dinv, cx<T>* B, int64_t ldb, int nrhs,
-          const int* skip, const LuPlan& plan, bool gauss3m, hipStream_t st);
+int getrs(const cx<T>* LU, int64_t ld, int n, const cx<T>* dinv256, cx<T>* B, int64_t ldb,
+          int nrhs, cx<T>* Y, const int* skip, const LuPlan& plan, bool gauss3m, hipStream_t st);
 
-// out[i][:] = (Tout) in[perm[i]][:] (row-major n x nrhs).
+// out[i][:] = (Tout) (scale[perm[i]] in[perm[i]][:]) (row-major n x nrhs;
+// scale = null: no scaling).
 template <class Tin, class Tout>
 int permute_cast(int n, int nrhs, const int* perm, const cx<Tin>* in, cx<Tout>* out,
-                 const int* skip, hipStream_t st);
+                 const int* skip, hipStream_t st, const double* scale = nullptr);
 
-// out[r * ldo + c] = (Tout) in[c * ldi + r], r < rows, c < cols
+// out[r * ldo + c] = (Tout) (rs[r] cs[c] in[c * ldi + r]), r < rows, c < cols
+// (rs = null: no scaling).
 template <class Tin, class Tout>
 int transpose_cast(int rows, int cols, const cx<Tin>* in, int64_t ldi, cx<Tout>* out,
-                   int64_t ldo, hipStream_t st);
+                   int64_t ldo, hipStream_t st, const double* rs = nullptr,
+                   const double* cs = nullptr);
+
+// Symmetric equilibration factors s[i] = 1 / sqrt(|a_ii|) (1 where a_ii = 0)
+// of a column-major FP64 matrix: the factored matrix is diag(s) A diag(s).
+int diag_scale(int n, const double2* A, int64_t lda, double* s, hipStream_t st);
 
 // Complex GEMM launcher: C = beta C + alpha A B (row-major; A column-major when a_col).
 template <class T>
@@ -72,13 +102,16 @@ int gemm(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<
 
 // Mixed/plain refinement around resident factors.  A64 is the FP64 matrix in
 // column-major order (lda); B and X are row-major n x nrhs (ld = nrhs).
-// Work buffers: R (n x nrhs FP64), RF (n x nrhs in factor precision),
-// part (>= 5120 doubles).
+// Work buffers: R (n x nrhs FP64), RF and RY (n x nrhs in factor precision),
+// part (>= 5120 doubles).  With scale non-null the factors are those of
+// diag(s) A diag(s): right-hand sides are scaled by s before the solves and
+// the solves' results by s after them.
 template <class TF>
-int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv, const int* perm,
+int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, const int* perm,
                  const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
                  int max_refine, double tol, int crit, RefineState* state, double2* R,
-                 cx<TF>* RF, double* part, const LuPlan& plan, bool gauss3m, hipStream_t st);
+                 cx<TF>* RF, cx<TF>* RY, double* part, const LuPlan& plan, bool gauss3m,
+                 hipStream_t st, const double* scale);
 
 // One residual evaluation of a refined X (skipped when state->done is set):
 // R = B - A64 X, norms and the stopping rule recorded in state.

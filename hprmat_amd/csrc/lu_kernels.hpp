@@ -74,9 +74,10 @@ template <class T, int W>
 __global__ void __launch_bounds__(256)
 leaf_getf2_kernel(cx<T>* __restrict__ P, int64_t ld, int m, int w, int rpw, int row_base,
                   int* __restrict__ ipiv, int* __restrict__ info, LeafXchg<T> x,
-                  unsigned seq_base, int* __restrict__ err) {
+                  unsigned seq_base, int stage_all, int* __restrict__ err) {
   extern __shared__ unsigned char smem_raw[];
   cx<T>* slab = reinterpret_cast<cx<T>*>(smem_raw);  // [rpw][W]
+  cx<T>* stage = slab + (size_t)rpw * W;              // [G][W] when stage_all
   __shared__ cx<T> prow[W];
   __shared__ cx<T> oldj[W];
   __shared__ T red_v[4];
@@ -126,17 +127,21 @@ leaf_getf2_kernel(cx<T>* __restrict__ P, int64_t ld, int m, int w, int rpw, int 
       }
       wait_flags(x.flag + par * G, G, seq_base + (unsigned)j + 1u, err);
       // -- global winner: every workgroup reduces the same G candidates; the
-      //    winning row's slab owner is row / rpw.
+      //    winning row's slab owner is row / rpw.  With stage_all the G
+      //    candidate rows are fetched in the same round trip as the values.
+      if (stage_all)
+        for (int e = tid; e < G * W; e += 256) stage[e] = x.row[(int64_t)par * G * W + e];
+      if (tid < W) oldj[tid] = x.rowj[par * W + tid];
       bv = T(-1); bi = 0x7fffffff;
       for (int q = tid; q < G; q += 256) best_of(bv, bi, x.val[par * G + q], x.idx[par * G + q]);
       wg_argmax(bv, bi, red_v, red_i);
       best = bv;
       p = (bv >= T(0)) ? bi : j;
       const int win = (bv >= T(0)) ? (bi / rpw) : 0;
-      if (tid < W) {
-        prow[tid] = (bv >= T(0)) ? x.row[((int64_t)par * G + win) * W + tid] : mkc<T>(0, 0);
-        oldj[tid] = x.rowj[par * W + tid];
-      }
+      if (tid < W)
+        prow[tid] = (bv < T(0)) ? mkc<T>(0, 0)
+                    : stage_all ? stage[win * W + tid]
+                                : x.row[((int64_t)par * G + win) * W + tid];
       __syncthreads();
     }
     if (g == 0 && tid == 0) {
@@ -325,17 +330,21 @@ perm_from_ipiv_kernel(const int* __restrict__ ipiv, int n, int* __restrict__ per
     for (int i = threadIdx.x; i < n; i += 256) perm[i] = sh_perm[i];
 }
 
-// out[i][:] = (Tout) in[perm[i]][:] for row-major n x nrhs arrays.
+// out[i][:] = (Tout) (s[perm[i]] in[perm[i]][:]) for row-major n x nrhs arrays
+// (s = null: no scaling).
 template <class Tin, class Tout>
 __global__ void __launch_bounds__(256)
 permute_cast_kernel(int n, int nrhs, const int* __restrict__ perm, const cx<Tin>* __restrict__ in,
-                    cx<Tout>* __restrict__ out, const int* __restrict__ skip) {
+                    cx<Tout>* __restrict__ out, const double* __restrict__ scale,
+                    const int* __restrict__ skip) {
   if (skip && *skip) return;
   const size_t total = (size_t)n * nrhs;
   for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
     const int i = (int)(e / nrhs), c = (int)(e % nrhs);
-    cx<Tin> v = in[(size_t)perm[i] * nrhs + c];
-    out[e] = mkc<Tout>((Tout)v.x, (Tout)v.y);
+    const int src = perm[i];
+    cx<Tin> v = in[(size_t)src * nrhs + c];
+    const double f = scale ? scale[src] : 1.0;
+    out[e] = mkc<Tout>((Tout)(v.x * f), (Tout)(v.y * f));
   }
 }
 
@@ -396,97 +405,115 @@ diag_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int blk0,
   for (int e = tid; e < 64 * 64; e += 256) out[e] = a[e / 64][e % 64];
 }
 
-// One wave: acc(16 x 16) += A(16 x K) B(K x 16), complex, operands in LDS as
-// real/imaginary planes (A row-major with row stride sa, B row-major with
-// row stride sb), K a multiple of 4.
+// ---------------------------------------------------------------------------
+// 64 x 64 x 64 complex tile product on MFMA for one workgroup of 4 waves:
+// acc[t] += A(rows 16w .. 16w+15, :) x B(:, cols 16t .. 16t+15), with A and B
+// held in LDS as real/imaginary planes of row stride 65 (conflict-free
+// operand reads).  Accumulator element r of lane l is row mfma_row(l/16, r),
+// column l % 16 of its 16 x 16 tile.
+// ---------------------------------------------------------------------------
 template <class T>
-__device__ inline void wave_cmma(const T* ar, const T* ai, int sa, const T* br, const T* bi, int sb,
-                                 int K, typename acc_t<T>::type& re, typename acc_t<T>::type& im) {
-  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-  for (int k0 = 0; k0 < K; k0 += 4) {
-    const T a_r = ar[li * sa + k0 + lk], a_i = ai[li * sa + k0 + lk];
-    const T b_r = br[(k0 + lk) * sb + li], b_i = bi[(k0 + lk) * sb + li];
-    re = mfma16x4(a_r, b_r, re);
-    re = mfma16x4(-a_i, b_i, re);
-    im = mfma16x4(a_r, b_i, im);
-    im = mfma16x4(a_i, b_r, im);
+__device__ inline void prod64(const T (*ar)[65], const T (*ai)[65], const T (*br)[65],
+                              const T (*bi)[65], typename acc_t<T>::type* re,
+                              typename acc_t<T>::type* im) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+  for (int k0 = 0; k0 < 64; k0 += 4) {
+    const T a_r = ar[16 * w + li][k0 + lk], a_i = ai[16 * w + li][k0 + lk];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const T b_r = br[k0 + lk][16 * t + li], b_i = bi[k0 + lk][16 * t + li];
+      re[t] = mfma16x4(a_r, b_r, re[t]);
+      re[t] = mfma16x4(-a_i, b_i, re[t]);
+      im[t] = mfma16x4(a_r, b_i, im[t]);
+      im[t] = mfma16x4(a_i, b_r, im[t]);
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Triangular solve of one block row (kb <= 256 rows) with the diagonal block
-// D of a row-major LU factor: D X = S in place, D unit lower (forward) or
-// upper (backward).  Each workgroup stages a 16-column strip of S in LDS and
-// walks the 64-row sub-blocks: X_sb = inv(D_sb) S_sb with the precomputed
-// inverse, then S_rc -= D_rc,sb X_sb for the remaining sub-blocks of the
-// block, every 64 x 64 x 16 product on MFMA (4 waves x one 16 x 16 tile).
-// dinv points at the inverse pair of this block's first 64-row sub-block.
+// Inverse of one 256 x 256 diagonal block of a row-major LU factor, one 64-
+// column block column per workgroup (grid: blocks x {L, U} x 4), from the
+// 64 x 64 inverses of its diagonal sub-blocks (block forward / backward
+// substitution on MFMA):
+//   L: X_jj = inv(L_jj),  X_ij = -inv(L_ii) sum_{k=j}^{i-1} L_ik X_kj  (i > j)
+//   U: X_jj = inv(U_jj),  X_ij = -inv(U_ii) sum_{k=i+1}^{j} U_ik X_kj  (i < j)
+// Output dinv256[(b*2+u)*65536 + r*256 + c], zero outside the triangle and
+// outside a trailing partial block.
 // ---------------------------------------------------------------------------
-template <class T, bool UPPER>
+template <class T>
 __global__ void __launch_bounds__(256)
-trsm_blk_kernel(const cx<T>* __restrict__ D, int64_t ldd, int kb, const cx<T>* __restrict__ dinv,
-                cx<T>* __restrict__ S, int64_t lds, int ncols, const int* __restrict__ skip) {
-  if (skip && *skip) return;
-  constexpr int CW = 16, MAXKB = 256, PB = 65;
+blk_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int bb0,
+               const cx<T>* __restrict__ dinv64, cx<T>* __restrict__ dinv256) {
   typedef typename acc_t<T>::type acc;
-  __shared__ T sr[MAXKB][CW], si[MAXKB][CW];
-  __shared__ T br_[64][PB], bi_[64][PB];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
-  const int cb = blockIdx.x * CW;
-  const int ncol = min(CW, ncols - cb);
-  const int nsb = (kb + 63) / 64;
-  for (int e = tid; e < nsb * 64 * CW; e += 256) {
-    int i = e / CW, c = e % CW;
-    cx<T> v = (i < kb && c < ncol) ? S[(int64_t)i * lds + cb + c] : mkc<T>(0, 0);
-    sr[i][c] = v.x;
-    si[i][c] = v.y;
+  __shared__ T ar[64][65], ai[64][65], br[64][65], bi[64][65];
+  const int bb = bb0 + blockIdx.x, upper = blockIdx.y, jb = blockIdx.z;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+  const int r0 = bb * 256, kb = min(256, n - r0), nsb = (kb + 63) / 64;
+  cx<T>* X = dinv256 + ((int64_t)bb * 2 + upper) * 65536;
+  for (int ib = 0; ib < 4; ++ib) {
+    const bool zero = jb >= nsb || ib >= nsb || (upper ? ib > jb : ib < jb);
+    if (!zero) continue;
+    for (int e = tid; e < 4096; e += 256) X[(ib * 64 + e / 64) * 256 + jb * 64 + e % 64] = mkc<T>(0, 0);
   }
-  auto load_blk = [&](const cx<T>* src, int64_t lds_src, int rows, int cols) {
-    for (int e = tid; e < 64 * 64; e += 256) {
-      int i = e / 64, k = e % 64;
-      cx<T> v = (i < rows && k < cols) ? src[(int64_t)i * lds_src + k] : mkc<T>(0, 0);
-      br_[i][k] = v.x;
-      bi_[i][k] = v.y;
-    }
-  };
-  // acc = buf[wave rows] x s[sb rows]
-  auto product = [&](int sb, acc& re, acc& im) {
-    re = acc{};
-    im = acc{};
-    wave_cmma<T>(&br_[wave * 16][0], &bi_[wave * 16][0], PB, &sr[sb * 64][0], &si[sb * 64][0], CW, 64,
-                 re, im);
-  };
-  for (int step = 0; step < nsb; ++step) {
-    const int sb = UPPER ? (nsb - 1 - step) : step;
-    const int bs = min(64, kb - sb * 64);
-    __syncthreads();
-    load_blk(dinv + (int64_t)sb * 2 * 4096 + (UPPER ? 4096 : 0), 64, 64, 64);
-    __syncthreads();
-    acc re, im;
-    product(sb, re, im);
-    __syncthreads();
-    for (int r = 0; r < 4; ++r) {
-      int row = sb * 64 + wave * 16 + mfma_row<T>(lk, r);
-      bool ok = (row - sb * 64) < bs;
-      sr[row][li] = ok ? re[r] : T(0);
-      si[row][li] = ok ? im[r] : T(0);
-    }
-    const int rc0 = UPPER ? 0 : sb + 1, rc1 = UPPER ? sb : nsb;
-    for (int rc = rc0; rc < rc1; ++rc) {
-      __syncthreads();
-      load_blk(D + (int64_t)(rc * 64) * ldd + sb * 64, ldd, min(64, kb - rc * 64), bs);
-      __syncthreads();
-      product(sb, re, im);
-      for (int r = 0; r < 4; ++r) {
-        int row = rc * 64 + wave * 16 + mfma_row<T>(lk, r);
-        sr[row][li] -= re[r];
-        si[row][li] -= im[r];
-      }
-    }
+  if (jb >= nsb) return;
+  auto inv64 = [&](int sb) { return dinv64 + ((int64_t)(bb * 4 + sb) * 2 + upper) * 4096; };
+  {
+    const cx<T>* src = inv64(jb);
+    for (int e = tid; e < 4096; e += 256) X[(jb * 64 + e / 64) * 256 + jb * 64 + e % 64] = src[e];
   }
+  __threadfence();
   __syncthreads();
-  for (int e = tid; e < kb * CW; e += 256) {
-    int i = e / CW, c = e % CW;
-    if (c < ncol) S[(int64_t)i * lds + cb + c] = mkc<T>(sr[i][c], si[i][c]);
+  auto load = [&](const cx<T>* src, int64_t lds, int rows, int cols, T (*dr)[65], T (*di)[65]) {
+    for (int e = tid; e < 4096; e += 256) {
+      const int i = e / 64, k = e % 64;
+      cx<T> v = (i < rows && k < cols) ? src[(int64_t)i * lds + k] : mkc<T>(0, 0);
+      dr[i][k] = v.x;
+      di[i][k] = v.y;
+    }
+  };
+  const int step = upper ? -1 : 1;
+  for (int ib = jb + step; ib >= 0 && ib < nsb; ib += step) {
+    acc re[4], im[4];
+    for (int t = 0; t < 4; ++t) { re[t] = acc{}; im[t] = acc{}; }
+    const int k_lo = upper ? ib + 1 : jb, k_hi = upper ? jb : ib - 1;
+    for (int kk = k_lo; kk <= k_hi; ++kk) {
+      __syncthreads();
+      load(LU + (int64_t)(r0 + 64 * ib) * ld + r0 + 64 * kk, ld, min(64, kb - 64 * ib),
+           min(64, kb - 64 * kk), ar, ai);
+      load(X + (int64_t)(64 * kk) * 256 + 64 * jb, 256, 64, 64, br, bi);
+      __syncthreads();
+      prod64<T>(ar, ai, br, bi, re, im);
+    }
+    __syncthreads();
+    for (int t = 0; t < 4; ++t)
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * w + mfma_row<T>(lk, r), col = 16 * t + li;
+        br[row][col] = re[t][r];
+        bi[row][col] = im[t][r];
+      }
+    load(inv64(ib), 64, 64, 64, ar, ai);
+    __syncthreads();
+    for (int t = 0; t < 4; ++t) { re[t] = acc{}; im[t] = acc{}; }
+    prod64<T>(ar, ai, br, bi, re, im);
+    for (int t = 0; t < 4; ++t)
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * w + mfma_row<T>(lk, r), col = 16 * t + li;
+        X[(int64_t)(64 * ib + row) * 256 + 64 * jb + col] = mkc<T>(-re[t][r], -im[t][r]);
+      }
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+// dst[r * ldd + c] = src[r * lds + c] for r < rows, c < cols.
+template <class T>
+__global__ void __launch_bounds__(256)
+copy2d_kernel(int rows, int cols, const cx<T>* __restrict__ src, int64_t lds, cx<T>* __restrict__ dst,
+              int64_t ldd) {
+  const size_t total = (size_t)rows * cols;
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+    const int r = (int)(e / cols), c = (int)(e % cols);
+    dst[(int64_t)r * ldd + c] = src[(int64_t)r * lds + c];
   }
 }

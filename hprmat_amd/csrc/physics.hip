@@ -145,8 +145,10 @@ int hprmat_hip_r_to_s(int nch, int nr, const void* dX, int64_t ldx, const double
   const int64_t ld = nopen;
   const size_t zbytes = (size_t)nopen * nopen * sizeof(double2);
   const size_t ibytes = ((size_t)(nopen + 64) * sizeof(int) + 255) & ~(size_t)255;
-  const size_t dbytes = hpr::dinv_elems(nopen) * sizeof(double2);
-  const size_t need = 3 * zbytes + 3 * ibytes + dbytes + hpr::scratch_bytes() + 1024;
+  const size_t d64 = (hpr::dinv64_elems(nopen) * sizeof(double2) + 255) & ~(size_t)255;
+  const size_t d256 = (hpr::dinv256_elems(nopen) * sizeof(double2) + 255) & ~(size_t)255;
+  const size_t dst = (hpr::ustage_elems(nopen) * sizeof(double2) + 255) & ~(size_t)255;
+  const size_t need = 4 * zbytes + 3 * ibytes + d64 + d256 + dst + hpr::scratch_bytes() + 2048;
   if (need > g_small.cap) {
     if (g_small.buf) (void)hipFree(g_small.buf);
     g_small.buf = nullptr;
@@ -158,21 +160,27 @@ int hprmat_hip_r_to_s(int nch, int nr, const void* dX, int64_t ldx, const double
   double2* ZO = (double2*)p;
   double2* ZI = (double2*)(p + zbytes);
   double2* ZP = (double2*)(p + 2 * zbytes);
-  p += ((3 * zbytes + 255) & ~(size_t)255);
+  double2* ZY = (double2*)(p + 3 * zbytes);
+  p += ((4 * zbytes + 255) & ~(size_t)255);
   int* ipiv = (int*)p;
   int* perm = (int*)(p + ibytes);
   int* info = (int*)(p + 2 * ibytes);
   p += 3 * ibytes;
-  double2* dinv = (double2*)p;
-  p += (dbytes + 255) & ~(size_t)255;
+  hpr::FactorWork<double> w;
+  w.dinv64 = (double2*)p;
+  w.dinv256 = (double2*)(p + d64);
+  w.ustage = (double2*)(p + d64 + d256);
+  w.perm = perm;
+  p += d64 + d256 + dst;
   hpr::Scratch sc;
   hpr::scratch_bind(sc, p);
   zmatrices_kernel<<<(nopen * nopen + 255) / 256, 256, 0, st>>>(
       nch, nopen, (const double2*)dR, rmax, qk, open, (const double2*)co, (const double2*)cop, ZO, ZI);
   hpr::LuPlan plan;
-  if (hpr::getrf<double>(ZO, ld, nopen, ipiv, info, dinv, perm, sc, plan, false, st)) return -1;
+  if (hpr::getrf<double>(ZO, ld, nopen, ipiv, info, w, sc, plan, false, st)) return -1;
   hpr::permute_cast<double, double>(nopen, nopen, perm, ZI, ZP, nullptr, st);
-  if (hpr::getrs<double>(ZO, ld, nopen, dinv, ZP, ld, nopen, nullptr, plan, false, st)) return -1;
+  if (hpr::getrs<double>(ZO, ld, nopen, w.dinv256, ZP, ld, nopen, ZY, nullptr, plan, false, st))
+    return -1;
   store_colmajor_kernel<<<(nopen * nopen + 255) / 256, 256, 0, st>>>(nopen, ZP, (double2*)dS);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

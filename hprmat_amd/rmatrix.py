@@ -49,7 +49,9 @@ def rmat_ini(nr, a):
                - 1 / (1 - xi) - 1 / (1 - xj)) / np.sqrt(x2[:, None] * x2[None, :])
     tc = sign * off
     np.fill_diagonal(tc, (4 * nr * (nr + 1) + 3 + (1 - 6 * x) / x2) / (3 * x2))
+    tc = np.tril(tc) + np.tril(tc, -1).T           # exactly symmetric, lower triangle mirrored
     blo0 = sign / np.sqrt(x2[:, None] * x2[None, :])
+    blo0 = np.tril(blo0) + np.tril(blo0, -1).T
     q2 = np.where((nr + i + 1) % 2 == 1, -1.0, 1.0) / np.sqrt(x2)
     return Mesh(nr, a, x, w, x * a, tc / a ** 2, blo0 / a, q2 / math.sqrt(a))
 
@@ -126,28 +128,32 @@ def rhs_colmajor(dmesh, nch):
     return b
 
 
-# solver_type -> (method, max_refine, tol, criterion).  Types 1/2 keep the
-# semantics of the reference's CPU types (plain FP64; FP32 + FP64 refinement
-# with ||r||_F/||b||_F < 1e-10, two sweeps) but run on the GPU; type 3
-# (Woodbury, CPU-only) is not provided.
+# solver_type -> (method, max_refine, tol, criterion, equilibrate).  Types 1/2
+# keep the semantics of the reference's CPU types (plain FP64; FP32 + FP64
+# refinement with ||r||_F/||b||_F < 1e-10, two sweeps) but run on the GPU;
+# type 3 (Woodbury, CPU-only) is not provided.  'plain' and 'mixed' are this
+# library's refined methods: FP64 or FP32 factors of the equilibrated matrix,
+# FP64 refinement until ||r||_F/||b||_F <= 1e-12.
 SOLVER_TYPES = {
-    1: ('fp64', 0, 0.0, 'rel_fro'),
-    2: ('fp32', 2, 1.0e-10, 'rel_fro'),
-    4: ('fp32', None, 1.0e-10, 'abs_max'),
-    5: ('fp32_3m', 5, 1.0e-12, 'abs_max'),
-    6: ('fp32', 2, 1.0e-12, 'abs_max'),
+    1: ('fp64', 0, 0.0, 'rel_fro', False),
+    2: ('fp32', 2, 1.0e-10, 'rel_fro', False),
+    4: ('fp32', None, 1.0e-10, 'abs_max', False),
+    5: ('fp32_3m', 5, 1.0e-12, 'abs_max', False),
+    6: ('fp32', 2, 1.0e-12, 'abs_max', False),
+    'plain': ('fp64', 5, 1.0e-12, 'rel_fro', True),
+    'mixed': ('fp32', 30, 1.0e-12, 'rel_fro', True),
 }
 
 
 def solve_colmajor(a_cm, b_cm, solver_type=1, max_refine=0):
     """Solve C X = B in place (column-major storages) for a solver_type."""
     if solver_type not in SOLVER_TYPES:
-        raise NotImplementedError('solver_type %d is not provided by libhprmat_hip' % solver_type)
-    method, mr, tol, crit = SOLVER_TYPES[solver_type]
+        raise NotImplementedError('solver_type %r is not provided by libhprmat_hip' % (solver_type,))
+    method, mr, tol, crit, eq = SOLVER_TYPES[solver_type]
     if mr is None:
         mr = max_refine
-    lu = solver.LU().factor(a_cm, method)
-    st = lu.solve_(b_cm, mr, tol, crit, final_check=False)
+    lu = solver.LU().factor(a_cm, method, eq)
+    st = lu.solve_(b_cm, mr, tol, crit, final_check=eq)
     if st['info'] != 0:
         raise RuntimeError('zero pivot at column %d' % st['info'])
     return st

@@ -58,13 +58,19 @@ class LU:
             self._lib.hprmat_hip_lu_destroy(h)
             self._h = None
 
-    def factor(self, a_cm, method='fp64'):
-        """Factor the matrix whose column-major storage is ``a_cm`` (n x n)."""
+    def factor(self, a_cm, method='fp64', equilibrate=False):
+        """Factor the matrix whose column-major storage is ``a_cm`` (n x n).
+
+        ``equilibrate`` factors diag(s) A diag(s) with s_i = |a_ii|^-1/2, which
+        keeps the diagonal blocks of U well conditioned on Lagrange-mesh
+        matrices (the refinement residual still uses A itself).
+        """
         _check_tensor(a_cm, 'a_cm')
         n = a_cm.shape[0]
         if a_cm.shape != (n, n):
             raise ValueError('matrix must be square')
-        rc = self._lib.hprmat_hip_lu_factor(self._h, METHODS[method], ctypes.c_void_p(a_cm.data_ptr()),
+        code = METHODS[method] | (_lib.EQUILIBRATE if equilibrate else 0)
+        rc = self._lib.hprmat_hip_lu_factor(self._h, code, ctypes.c_void_p(a_cm.data_ptr()),
                                             n, n, _stream_ptr())
         _lib.check(rc, 'hprmat_hip_lu_factor')
         self.n, self.method, self._a_cm = n, method, a_cm
@@ -99,7 +105,7 @@ class LU:
 
 
 def solve(a, b, method='fp64', max_refine=0, tol=1e-12, criterion='rel_fro', final_check=True,
-          symmetric=False):
+          symmetric=False, equilibrate=False):
     """X = A^-1 B for complex128 CUDA tensors a (n x n) and b (n x nrhs).
 
     Returns (X, stats) with X an (n x nrhs) tensor.
@@ -107,7 +113,7 @@ def solve(a, b, method='fp64', max_refine=0, tol=1e-12, criterion='rel_fro', fin
     _check_tensor(a, 'a')
     a_cm = colmajor(a, symmetric)
     b_cm = b.transpose(0, 1).contiguous()
-    lu = LU().factor(a_cm, method)
+    lu = LU().factor(a_cm, method, equilibrate)
     st = lu.solve_(b_cm, max_refine, tol, criterion, final_check)
     if st['info'] != 0:
         raise RuntimeError('zero pivot at column %d' % st['info'])

@@ -1,0 +1,114 @@
+"""GPU parity of the dense complex LU + solve path (libhprmat_hip) against the
+FP64 oracle (LAPACK).  Needs an MI355X; the library must be built in-tree."""
+import numpy as np
+import pytest
+import torch
+from scipy.linalg import lapack
+
+from hprmat_amd import rmatrix, solver, synth
+from oracle import solve as osolve
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_system(n, nrhs, seed, dd=0.02):
+    rng = np.random.default_rng(seed)
+    a = (rng.random((n, n)) - 0.5) + 1j * (rng.random((n, n)) - 0.5) + 2 * n * dd * np.eye(n)
+    b = rng.random((n, nrhs)) + 1j * rng.random((n, nrhs))
+    return a, b
+
+
+def gpu_solve(a, b, method, max_refine=0, tol=1e-13, equilibrate=False):
+    x, st = solver.solve(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), method=method,
+                         max_refine=max_refine, tol=tol, equilibrate=equilibrate)
+    return x.cpu().numpy(), st
+
+
+@pytest.mark.parametrize('n', [1, 7, 32, 33, 64, 255, 256, 257, 500, 1100])
+@pytest.mark.parametrize('method', ['fp64', 'fp32', 'fp32_3m'])
+def test_solve_matches_lapack(n, method):
+    a, b = rand_system(n, 5, n)
+    x_ref, info = osolve.zgesv(a, b)
+    assert info == 0
+    x, st = gpu_solve(a, b, method, max_refine=0 if method == 'fp64' else 10)
+    err = np.abs(x - x_ref).max() / np.abs(x_ref).max()
+    assert st['info'] == 0
+    assert err <= 1e-12, (n, method, err, st)
+
+
+@pytest.mark.parametrize('n', [8, 40, 100, 300, 700, 1300])
+def test_pivots_match_lapack_zgetrf(n):
+    # index output: bit-exact
+    a, _ = rand_system(n, 1, 1000 + n, dd=0.0)
+    lu = solver.LU().factor(solver.colmajor(torch.from_numpy(a).cuda()), 'fp64')
+    piv = lu.pivots()
+    _, piv_ref, info = lapack.zgetrf(a)
+    assert info == 0
+    assert np.array_equal(piv, piv_ref + 1)
+
+
+@pytest.mark.parametrize('col', [0, 5, 63])
+def test_zero_pivot_info_matches_lapack(col):
+    n = 96
+    a, _ = rand_system(n, 1, 7, dd=0.0)
+    a[:, col] = 0
+    lu = solver.LU().factor(solver.colmajor(torch.from_numpy(a).cuda()), 'fp64')
+    _, _, info_ref = lapack.zgetrf(a)
+    assert lu.info() == info_ref == col + 1
+
+
+@pytest.mark.parametrize('method', ['fp64', 'fp32'])
+def test_equilibrated_factor_same_solution(method):
+    s = synth.config('small')
+    dm = rmatrix.DeviceMesh(s.mesh)
+    _, _, _, xc1 = rmatrix.channel_data(s.lval, s.qk, s.a)
+    a_cm = rmatrix.assemble(dm, s.lval, s.qk, xc1, s.cpot)
+    b0 = rmatrix.rhs_colmajor(dm, s.nch)
+    xs = []
+    for eq in (False, True):
+        b = b0.clone()
+        st = solver.LU().factor(a_cm, method, eq).solve_(b, 10, 1e-13, 'rel_fro', True)
+        assert st['res_rel'] <= 1e-12
+        xs.append(b.cpu().numpy())
+    assert np.abs(xs[0] - xs[1]).max() <= 1e-11 * np.abs(xs[0]).max()
+
+
+def test_manufactured_solution_large():
+    # X = 1 (examples/Ex0/benchmark_mgpu.f90:71-81), diagonally dominant, N = 6000
+    n = 6000
+    g = torch.Generator(device='cuda').manual_seed(3)
+    a = torch.complex(torch.rand(n, n, generator=g, device='cuda', dtype=torch.float64),
+                      torch.rand(n, n, generator=g, device='cuda', dtype=torch.float64))
+    a += 2 * n * torch.eye(n, device='cuda', dtype=torch.complex128)
+    b = a @ torch.ones(n, 3, device='cuda', dtype=torch.complex128)
+    for m, r in [('fp64', 0), ('fp32', 5)]:
+        x, st = solver.solve(a, b, method=m, max_refine=r, tol=1e-15)
+        assert (x - 1).abs().max().item() < 1e-12, (m, st)
+
+
+def test_refinement_history_is_monotone_until_floor():
+    s = synth.System(16, 200, 15.0, 8.0, 5.0, 0)
+    dm = rmatrix.DeviceMesh(s.mesh)
+    _, _, _, xc1 = rmatrix.channel_data(s.lval, s.qk, s.a)
+    a_cm = rmatrix.assemble(dm, s.lval, s.qk, xc1, s.cpot)
+    b = rmatrix.rhs_colmajor(dm, s.nch)
+    st = solver.LU().factor(a_cm, 'fp32', True).solve_(b, 12, 1e-12, 'rel_fro', True)
+    h = st['hist']
+    assert st['converged'] and h[-1] <= 1e-12
+    assert all(h[i + 1] < h[i] for i in range(len(h) - 1))
+
+
+def test_full_size_mixed_solve_gate():
+    # N = 25600 complex, 128 right-hand sides: the benchmark system.  Checked by
+    # properties that do not need a CPU solve of that size: the FP64 relative
+    # residual gate and the complex symmetry of R = q2^T X.
+    s = synth.config('single')
+    dm = rmatrix.DeviceMesh(s.mesh)
+    o, co, cop, xc1 = rmatrix.channel_data(s.lval, s.qk, s.a)
+    a_cm = rmatrix.assemble(dm, s.lval, s.qk, xc1, s.cpot)
+    b = rmatrix.rhs_colmajor(dm, s.nch)
+    st = solver.LU().factor(a_cm, 'fp32', True).solve_(b, 16, 1e-12, 'rel_fro', True)
+    assert st['converged'] and st['res_rel'] <= 1e-12, st
+    r, _ = rmatrix.r_to_s(dm, b, s.nch, s.qk, o, co, cop)
+    r = r.cpu().numpy()
+    assert np.abs(r - r.T).max() <= 1e-9 * np.abs(r).max()

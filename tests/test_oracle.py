@@ -1,0 +1,189 @@
+"""CPU checks that pin the oracle (test infrastructure) independently of its own formulas.
+
+* LEGZO restatement vs numpy's Gauss-Legendre rule;
+* eta = 0 channel functions vs mpmath Coulomb / Whittaker functions and the
+  Wronskian;
+* the whole ns = 1 pipeline vs the golden phase shifts printed by the
+  reference's examples (rmat_pierre/Ex2 Reid NN, rmat_pierre/Ex5 Yamaguchi);
+* free motion (V = 0) gives S = 1; a real potential with all channels open
+  gives a unitary, symmetric S;
+* the unblocked GEPP restatement reproduces LAPACK zgetrf (pivots and
+  factors), and the type-2 / GPU refinement rules behave as the reference's.
+"""
+import json
+import math
+import os
+
+import mpmath
+import numpy as np
+import pytest
+from scipy.linalg import lapack
+
+from oracle import channels, lagrange, rmatrix, solve, synth
+
+GOLDEN = os.path.join(os.path.dirname(__file__), 'golden', 'rmat_pierre.json')
+
+
+@pytest.mark.parametrize('n', [1, 2, 3, 10, 15, 60, 200])
+def test_legzo_matches_numpy(n):
+    x, w = lagrange.legzo(n)
+    t, ww = np.polynomial.legendre.leggauss(n)
+    assert np.allclose(x, (1 + t) / 2, atol=1e-14, rtol=0)
+    assert np.allclose(w, ww / 2, atol=1e-14, rtol=0)
+
+
+def test_mesh_symmetry_and_rank_one_bloch():
+    m = lagrange.rmat_ini(40, 1, 10.0)
+    assert np.allclose(m['tc'][0], m['tc'][0].T)
+    # the Bloch block of the first interval is the rank-one q2 q2^T
+    q2 = m['q2'][0]
+    assert np.allclose(m['blo0'], np.outer(q2, q2), rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize('l', [0, 1, 2, 3])
+@pytest.mark.parametrize('rho', [0.3, 2.0, 7.5, 31.0])
+def test_open_channel_functions_match_mpmath(l, rho):
+    co, cop = channels.open_channel(l, rho, 1.0)
+    f = float(mpmath.coulombf(l, 0, rho))
+    g = float(mpmath.coulombg(l, 0, rho))
+    fd = float(mpmath.diff(lambda r: mpmath.coulombf(l, 0, r), rho))
+    gd = float(mpmath.diff(lambda r: mpmath.coulombg(l, 0, r), rho))
+    assert abs(co - complex(g, f)) <= 1e-12 * max(1.0, abs(co))
+    assert abs(cop - complex(gd, fd) * rho) <= 1e-11 * max(1.0, abs(cop))
+    assert abs(fd * g - f * gd - 1.0) < 1e-12          # Wronskian
+
+
+@pytest.mark.parametrize('l', [0, 1, 2, 3])
+@pytest.mark.parametrize('rho', [0.2, 1.5, 6.0, 25.0])
+def test_closed_channel_log_derivative_matches_whittaker(l, rho):
+    xc1, w = channels.closed_channel(l, rho, 1.0)
+    wm = lambda z: mpmath.whitw(0, l + 0.5, 2 * z)
+    ld = float(mpmath.diff(wm, rho) / wm(rho)) * rho
+    assert abs(xc1 - ld) <= 1e-10 * max(1.0, abs(ld))
+    assert abs(w / float(wm(rho)) - 1.0) < 1e-10
+
+
+def _reid(j, nr, a):
+    hm = 20.736 / 0.5
+    m = lagrange.rmat_ini(nr, 1, a)
+    cp = np.zeros((nr, 2, 2), complex)
+    for i, r in enumerate(m['zrma']):
+        xx = r * 0.7
+        zz = math.exp(-xx)
+        vcen = (-10.463 * zz + 105.468 * zz ** 2 - 3187.8 * zz ** 4 + 9924.3 * zz ** 6) / xx
+        vtens = (-10.463 * ((1 + 3 / xx + 3 / xx ** 2) * zz - (12 / xx + 3 / xx ** 2) * zz ** 4) / xx
+                 + 351.77 * zz ** 4 / xx - 1673.5 * zz ** 6 / xx)
+        vls = 708.91 * zz ** 4 / xx - 2713.1 * zz ** 6 / xx
+        cp[i, 0, 0] = vcen - 2 * (j - 1) * vtens / (2 * j + 1) + (j - 1) * vls
+        cp[i, 0, 1] = cp[i, 1, 0] = 6 * vtens * math.sqrt(j * (j + 1.0)) / (2 * j + 1)
+        cp[i, 1, 1] = vcen - 2 * (j + 2) * vtens / (2 * j + 1) - (j + 2) * vls
+    return m, cp / hm, hm
+
+
+def test_golden_ex2_reid():
+    gold = json.load(open(GOLDEN))['ex2_reid_nn']
+    for g in gold:
+        m, cp, hm = _reid(g['j'], g['nr'], g['rmax'])
+        qk = np.array([math.sqrt(g['e'] / hm)] * 2)
+        s = rmatrix.rmatrix([g['j'] - 1, g['j'] + 1], qk, g['rmax'], g['nr'], cp, mesh=m)['S']
+        d1 = math.atan2(s[0, 0].imag, s[0, 0].real) / 2
+        d2 = math.atan2(s[1, 1].imag, s[1, 1].real) / 2
+        for got, ref in [(d1, g['delta1_rad']), (d2, g['delta2_rad']), (abs(s[0, 1]), g['eta12'])]:
+            assert abs(got - ref) <= 5e-5 * abs(ref) + 1e-9, (g, got, ref)
+
+
+def _yamaguchi(nr, a):
+    al, be = 0.2316053, 1.3918324
+    m = lagrange.rmat_ini(nr, 1, a)
+    z = m['zrma']
+    cpnl = np.zeros((nr, nr, 1, 1), complex)
+    cpnl[:, :, 0, 0] = -2 * be * (al + be) ** 2 * np.exp(-be * (z[:, None] + z[None, :]))
+    return m, cpnl
+
+
+def test_golden_ex5_yamaguchi():
+    hm = 41.472
+    for g in json.load(open(GOLDEN))['ex5_yamaguchi']:
+        m, cpnl = _yamaguchi(g['nr'], g['rmax'])
+        qk = np.array([math.sqrt(g['e'] / hm)])
+        s = rmatrix.rmatrix([0], qk, g['rmax'], g['nr'], np.zeros((g['nr'], 1, 1), complex),
+                            cpnl=cpnl, mesh=m)['S']
+        delta = math.atan2(s[0, 0].imag, s[0, 0].real) / 2 * 180 / math.pi
+        assert abs(delta - g['delta_deg']) <= 5e-5 * abs(g['delta_deg'])
+        assert abs(abs(s[0, 0]) - g['modulus']) < 1e-4
+
+
+@pytest.mark.parametrize('l', [0, 1, 2, 3])
+def test_free_motion_gives_unit_s(l):
+    nr, a = 60, 15.0
+    m = lagrange.rmat_ini(nr, 1, a)
+    s = rmatrix.rmatrix([l], np.array([1.3]), a, nr, np.zeros((nr, 1, 1), complex), mesh=m)['S']
+    assert abs(s[0, 0] - 1.0) < 1e-8
+
+
+def test_real_potential_all_open_unitary_symmetric():
+    lval, qk, cpot, _ = synth.system(4, 60, 15.0, 3.0, 6.0, 3, w0_zero=True)
+    out = rmatrix.rmatrix(lval, qk, 15.0, 60, cpot)
+    s = out['S']
+    assert out['nopen'] == 4
+    assert np.allclose(s, s.T, atol=1e-11)
+    assert np.allclose(s.conj().T @ s, np.eye(4), atol=1e-10)
+
+
+def test_absorption_bounds_s():
+    lval, qk, cpot, _ = synth.system(4, 60, 15.0, 3.0, 2.0, 0)
+    s = rmatrix.rmatrix(lval, qk, 15.0, 60, cpot)['S']
+    assert np.linalg.norm(s, 2) <= 1.0 + 1e-10
+    assert np.allclose(s, s.T, atol=1e-11)
+
+
+@pytest.mark.parametrize('n', [1, 5, 17, 40])
+def test_getf2_matches_lapack_zgetrf(n):
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    lu, piv, info = solve.getf2(a)
+    lu2, piv2, info2 = lapack.zgetrf(a)
+    assert info == info2 == 0
+    assert np.array_equal(piv, piv2)
+    assert np.allclose(lu, lu2, rtol=1e-12, atol=1e-12)
+
+
+def test_getf2_zero_pivot_info_matches_lapack():
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((6, 6)) + 0j
+    a[:, 2] = 0
+    _, _, info = solve.getf2(a)
+    _, _, info2 = lapack.zgetrf(a)
+    assert info == info2 == 3
+
+
+def test_manufactured_solution_and_exact_condition():
+    # X = 1 (examples/Ex0/benchmark_mgpu.f90:71-81) on a diagonally dominant matrix
+    rng = np.random.default_rng(1)
+    n = 300
+    a = rng.random((n, n)) + 1j * rng.random((n, n)) + 2 * n * np.eye(n)
+    x, info = solve.zgesv(a, a @ np.ones((n, 2)))
+    assert info == 0 and np.abs(x - 1).max() < 1e-12
+    # Householder H D H with cond = kappa exactly (examples/Ex0/benchmark_mgpu_illcond.f90:41-78)
+    v = rng.standard_normal(n)
+    h = np.eye(n) - 2 * np.outer(v, v) / (v @ v)
+    kappa = 1e6
+    d = np.geomspace(1.0, 1.0 / kappa, n)
+    a = h @ np.diag(d) @ h
+    assert abs(np.linalg.cond(a) / kappa - 1) < 1e-6
+    x, info = solve.zgesv(a, a @ np.ones(n))
+    assert np.abs(x - 1).max() < 1e-16 * kappa * 100
+
+
+def test_refinement_rules():
+    lval, qk, cpot, _ = synth.system(4, 60, 15.0, 3.0, 2.0, 0)
+    out = rmatrix.rmatrix(lval, qk, 15.0, 60, cpot)
+    a, b = out['C'], out['B']
+    x, used, hist = solve.mixed_cpu(a, b)
+    assert used <= 2 and len(hist) >= 1
+    # the type-2 rule stops once ||r||_F / ||b||_F < 1e-10
+    assert hist[-1] < 1e-10 or used == 2
+    x2, used2 = solve.mixed_gpu_semantics(a, b, 5, 1e-10)
+    assert solve.izamax_modulus(b - a @ x2) < 1e-10 or used2 == 5
+    xf, _ = solve.fp64_refined(a, b)
+    assert solve.rel_residual(a, xf, b) <= 1e-12
