@@ -381,8 +381,10 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       gemm<T>(kb, nc, kb, w.dinv256 + (size_t)(k / 256) * 2 * 65536, 256, false, Akk + kb, ld,
               w.ustage, nc, T(1), 0, g3m, nullptr, st);
       // next panel's columns first (on the main stream), the rest on the side
+      // (the timed "trailing update" is the launch over the columns beyond the
+      // next panel, or the whole update when there is no look-ahead)
       const int c_next = (la && nc > nb) ? nb : nc;
-      ProfRec* pr = g_prof_on ? prof_slot() : nullptr;
+      ProfRec* pr = (g_prof_on && c_next == nc) ? prof_slot() : nullptr;
       if (pr) (void)hipEventRecord(pr->a, st);
       gemm<T>(nc, c_next, kb, Akk + (int64_t)kb * ld, ld, false, w.ustage, nc,
               Akk + (int64_t)kb * ld + kb, ld, T(-1), 1, g3m, nullptr, st);
