@@ -84,6 +84,14 @@ def main():
     lib = _lib.load()
     dev = torch.device('cuda:0')
     torch.cuda.set_device(dev)
+    # all work on one non-default stream (the library's look-ahead stream must
+    # not be implicitly ordered against the legacy default stream)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        run(args, torch, lib, _lib, rmatrix, solver, synth)
+
+
+def run(args, torch, lib, _lib, rmatrix, solver, synth):
 
     sysm = synth.System(args.nch, args.nr, 15.0, 8.0, 5.0, 0)
     n, nrhs = sysm.n, sysm.nch

@@ -62,11 +62,48 @@ size_t factor_elem_bytes(int method) { return method == HPRMAT_PLAIN_FP64 ? 16 :
 
 int64_t padded_ld(int n) { return (((int64_t)n + 15) / 16) * 16 + 16; }
 
+// The look-ahead stream runs the bulk of every trailing update.  It is
+// restricted (CU mask) to all but a reserved set of CUs (every 8th, 32 of 256
+// by default; HPRMAT_HIP_RESERVE_CU overrides, 0 disables), so the
+// latency-bound panel kernels on the main stream always find free CUs
+// instead of waiting for GEMM workgroups to drain.  Default priority.
 int ensure_side(hprmat_hip_lu* h) {
   if (h->side.stream) return 0;
-  if (hipStreamCreateWithFlags(&h->side.stream, hipStreamNonBlocking) != hipSuccess) return -1;
-  if (hipEventCreateWithFlags(&h->side.ready, hipEventDisableTiming) != hipSuccess) return -1;
-  if (hipEventCreateWithFlags(&h->side.rest, hipEventDisableTiming) != hipSuccess) return -1;
+  int dev = 0, ncu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  int reserve = 32;
+  const char* env = getenv("HPRMAT_HIP_RESERVE_CU");
+  if (env) reserve = atoi(env);
+  bool masked = false;
+  if (reserve > 0 && ncu > 2 * reserve) {
+    const int stride = ncu / reserve;
+    const int words = (ncu + 31) / 32;
+    uint32_t mask[64] = {0};
+    if (words <= 64) {
+      int kept = 0;
+      for (int c = 0; c < ncu; ++c)
+        if (c % stride != 0 || c / stride >= reserve) { mask[c / 32] |= 1u << (c % 32); ++kept; }
+      hipError_t e = hipExtStreamCreateWithCUMask(&h->side.stream, (uint32_t)words, mask);
+      masked = e == hipSuccess;
+      if (!masked) {
+        fprintf(stderr, "hprmat_hip: CU-masked stream unavailable (%s), using all CUs\n",
+                hipGetErrorString(e));
+        (void)hipGetLastError();
+        h->side.stream = nullptr;
+      }
+    }
+  }
+  hipError_t e;
+  if (!masked && (e = hipStreamCreateWithFlags(&h->side.stream, hipStreamNonBlocking)) != hipSuccess) {
+    fprintf(stderr, "hprmat_hip: side stream creation failed: %s\n", hipGetErrorString(e));
+    return -1;
+  }
+  if ((e = hipEventCreateWithFlags(&h->side.ready, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&h->side.rest, hipEventDisableTiming)) != hipSuccess) {
+    fprintf(stderr, "hprmat_hip: event creation failed: %s\n", hipGetErrorString(e));
+    return -1;
+  }
   return 0;
 }
 
@@ -388,7 +425,11 @@ int gpu_solve_multi_mixed_(double* A, double* B, int* n, int* nrhs, int* max_ref
 int hprmat_hip_lu_create(hprmat_hip_lu_t* h) {
   if (!h) return -1;
   *h = new (std::nothrow) hprmat_hip_lu();
-  return *h ? 0 : -2;
+  if (!*h) return -2;
+  // HPRMAT_HIP_LOOKAHEAD=0 factors without the look-ahead stream (diagnostics)
+  const char* la = getenv("HPRMAT_HIP_LOOKAHEAD");
+  if (la && la[0] == '0') (*h)->lookahead = 0;
+  return 0;
 }
 
 int hprmat_hip_lu_destroy(hprmat_hip_lu_t h) {

@@ -30,8 +30,6 @@ namespace {
 
 constexpr int LEAF_W = 32;
 constexpr int MAX_LEAF_G = 240;               // co-resident: one leaf workgroup per CU
-constexpr size_t LEAF_SLAB_TARGET = 96 * 1024;  // LDS slab per workgroup (preferred)
-constexpr size_t LEAF_SLAB_MAX = 150 * 1024;    // hard limit (160 KiB LDS per CU)
 
 // Dynamic LDS above 64 KiB needs an explicit opt-in per kernel; the return
 // code is checked so a refusal does not linger as the sticky last error.
@@ -210,38 +208,32 @@ norm_final_kernel(int nblocks, const double* __restrict__ part, RefineState* __r
 }
 
 
-// Leaf layout: G workgroups of rpw rows each, preferring a 96 KiB LDS slab and
-// growing it (up to 150 KiB) when more than MAX_LEAF_G workgroups would be
-// needed, so that every leaf workgroup is co-resident (one per CU).
-template <class T> int leaf_layout(int m, int& G, int& rpw) {
-  const size_t row_bytes = LEAF_W * sizeof(cx<T>);
-  int rpw_t = (int)(LEAF_SLAB_TARGET / row_bytes);
-  G = (m + rpw_t - 1) / rpw_t;
-  if (G < 1) G = 1;
-  if (G > MAX_LEAF_G) G = MAX_LEAF_G;
-  rpw = (m + G - 1) / G;
-  return ((size_t)rpw * row_bytes <= LEAF_SLAB_MAX) ? 0 : -1;
-}
+// Leaf geometry: threads per workgroup and rows per thread (rows live in
+// registers: FP32 2 rows x 32 complex = 128 VGPRs, FP64 1 row = 128 VGPRs).
+template <class T> struct LeafCfg;
+template <> struct LeafCfg<float> { static constexpr int NT = 512, RPT = 2; };
+template <> struct LeafCfg<double> { static constexpr int NT = 512, RPT = 1; };
+
+// Exchange area layout (u64 words): val [2][MAX_LEAF_G][2], idx [2][MAX_LEAF_G],
+// row [2][MAX_LEAF_G][LEAF_W][4], rowj [2][LEAF_W][4] (sized for FP64).
+constexpr size_t XW_VAL = 2 * MAX_LEAF_G * 2, XW_IDX = 2 * MAX_LEAF_G,
+                 XW_ROW = (size_t)2 * MAX_LEAF_G * LEAF_W * 4, XW_ROWJ = 2 * LEAF_W * 4;
+constexpr size_t XW_TOTAL = XW_VAL + XW_IDX + XW_ROW + XW_ROWJ;
 
 template <class T> int launch_leaf(cx<T>* P, int64_t ld, int m, int w, int row_base, int* ipiv,
                                    int* info, const Scratch& s, int leaf, hipStream_t st) {
-  int G, rpw;
-  if (leaf_layout<T>(m, G, rpw)) return -1;
-  LeafXchg<T> x;
-  char* base = (char*)s.xchg;
-  x.val = (T*)base;
-  x.idx = (int*)(base + 2 * MAX_LEAF_G * sizeof(double));
-  x.rowj = (cx<T>*)(base + 2 * MAX_LEAF_G * (sizeof(double) + sizeof(int)));
-  x.row = x.rowj + 2 * LEAF_W;
-  x.flag = s.flags;
-  size_t lds = (size_t)rpw * LEAF_W * sizeof(cx<T>);
-  const size_t stage = (size_t)G * LEAF_W * sizeof(cx<T>);
-  const int stage_all = (G > 1 && G <= 32 && lds + stage <= LEAF_SLAB_MAX) ? 1 : 0;
-  if (stage_all) lds += stage;
-  static bool attr = false;
-  if (!attr) { allow_big_lds(leaf_getf2_kernel<T, LEAF_W>); attr = true; }
-  leaf_getf2_kernel<T, LEAF_W><<<G, 256, lds, st>>>(P, ld, m, w, rpw, row_base, ipiv, info, x,
-                                                    (unsigned)leaf * LEAF_W, stage_all, s.err);
+  constexpr int NT = LeafCfg<T>::NT, RPT = LeafCfg<T>::RPT;
+  const int G = (m + NT * RPT - 1) / (NT * RPT);
+  if (G < 1 || G > MAX_LEAF_G) return -1;   // all leaf workgroups must be co-resident
+  LeafXchg x;
+  u64* base = (u64*)s.xchg;
+  x.val = base;
+  x.idx = x.val + XW_VAL;
+  x.row = x.idx + XW_IDX;
+  x.rowj = x.row + XW_ROW;
+  x.gstride = MAX_LEAF_G;
+  leaf_getf2_kernel<T, LEAF_W, NT, RPT><<<G, NT, 0, st>>>(P, ld, m, w, row_base, ipiv, info, x,
+                                                          (unsigned)leaf * LEAF_W, s.err);
   return 0;
 }
 
@@ -277,11 +269,7 @@ ProfRec* prof_slot() {
   return &g_prof[g_prof_used++];
 }
 
-size_t xchg_bytes() {
-  size_t x = 2 * MAX_LEAF_G * (sizeof(double) + sizeof(int)) +
-             (2 * LEAF_W + (size_t)2 * MAX_LEAF_G * LEAF_W) * sizeof(double2);
-  return (x + 255) & ~(size_t)255;
-}
+size_t xchg_bytes() { return (XW_TOTAL * sizeof(u64) + 255) & ~(size_t)255; }
 
 }  // namespace
 
@@ -307,13 +295,12 @@ int prof_read(double* ms, double* flops, int* count) {
   return 0;
 }
 
-size_t scratch_bytes() { return xchg_bytes() + 2 * MAX_LEAF_G * sizeof(unsigned) + 256; }
+size_t scratch_bytes() { return xchg_bytes() + 256; }
 
 void scratch_bind(Scratch& s, void* base) {
   s.xchg = base;
   s.xchg_bytes = xchg_bytes();
-  s.flags = (unsigned*)((char*)base + s.xchg_bytes);
-  s.err = (int*)(s.flags + 2 * MAX_LEAF_G);
+  s.err = (int*)((char*)base + s.xchg_bytes);
 }
 
 template <class T>
@@ -340,7 +327,8 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
   const int nb = plan.nb;
   if (nb != 256) return -1;
   const bool la = side && side->stream && n > 2 * nb;  // look-ahead
-  fill_u32_kernel<<<4, 256, 0, st>>>(s.flags, 2 * MAX_LEAF_G, 0u);
+  (void)hipGetLastError();  // start from a clean error state (checked at the end)
+  fill_u32_kernel<<<64, 256, 0, st>>>((unsigned*)s.xchg, s.xchg_bytes / sizeof(unsigned), 0u);
   fill_u32_kernel<<<1, 64, 0, st>>>((unsigned*)s.err, 1, 0u);
   fill_u32_kernel<<<1, 64, 0, st>>>((unsigned*)info, 1, 0u);
   int leaf = 0;

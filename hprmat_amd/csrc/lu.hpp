@@ -10,11 +10,11 @@ struct LuPlan {
 };
 
 // Device scratch used by getrf: exchange area of the cooperative panel
-// leaves (with their arrival flags) and the barrier error flag.
+// leaves (tagged words, zeroed at the start of each factorization) and the
+// error flag of their bounded waits.
 struct Scratch {
   void* xchg = nullptr;
   size_t xchg_bytes = 0;
-  unsigned* flags = nullptr;
   int* err = nullptr;
 };
 

@@ -9,34 +9,88 @@
 #include "common.hpp"
 
 // ---------------------------------------------------------------------------
-// Exchange area of the cooperative panel leaves (device memory).  Slots are
-// double-buffered by column parity: a workgroup can only overwrite parity p
-// for column j+2 after every workgroup has published column j+1, which each
-// does only after it finished reading column j.
+// Exchange area of the cooperative panel leaves (device memory), made of
+// tagged 64-bit words: high 32 bits = column tag (unique per column within a
+// factorization, the area is zeroed at its start), low 32 bits = payload.  A
+// reader accepts a word only when its tag is the current column's, so data
+// and "arrival" travel together: no flag, no fence, no store-completion wait.
+// Words are written and read with relaxed agent-scope atomics, which compile
+// to single coherent (sc1) global loads/stores; an agent-scope fence would
+// instead emit buffer_wbl2/buffer_inv and stall on (and evict) the dirty
+// tiles of the trailing-update GEMM that runs concurrently on the side stream.
+// Slots are double-buffered by column parity: a workgroup can only write
+// parity p for column j+2 after all workgroups published column j+1, which
+// each does only after it finished reading column j.
 // ---------------------------------------------------------------------------
-template <class T> struct LeafXchg {
-  T* val;          // [2][Gmax]   candidate |Re|+|Im| (negative: no candidate)
-  int* idx;        // [2][Gmax]   candidate panel-local row
-  cx<T>* row;      // [2][Gmax][W] candidate row (all W leaf columns)
-  cx<T>* rowj;     // [2][W]      current row j (published by its owner)
-  unsigned* flag;  // [2][Gmax]   arrival sequence numbers, zeroed per factorization
+typedef unsigned long long u64;
+template <class T> struct words_per;                 // 32-bit payload words per real
+template <> struct words_per<float> { static constexpr int n = 1; };
+template <> struct words_per<double> { static constexpr int n = 2; };
+
+struct LeafXchg {
+  u64* val;    // [2][Gmax]            candidate |Re|+|Im| (payload words: words_per)
+  u64* idx;    // [2][Gmax]            candidate panel-local row
+  u64* row;    // [2][Gmax][W*2*wpr]   candidate row (all W leaf columns)
+  u64* rowj;   // [2][W*2*wpr]         current row j (published by its owner)
+  int gstride; // Gmax
 };
 
-// Wait until flag[q] >= seq for q < G (threads q, q + 256, ... each spin on
-// one flag).  Bounded: a wait that cannot complete sets *err and returns, and
-// every waiter watches *err so one timeout releases them all.
-__device__ inline void wait_flags(const unsigned* flag, int G, unsigned seq, int* err) {
-  for (int q = threadIdx.x; q < G; q += blockDim.x) {
-    unsigned spins = 0;
-    while (__hip_atomic_load(flag + q, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < seq) {
-      __builtin_amdgcn_s_sleep(1);
-      if ((++spins & 255u) == 0) {
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-        if (spins > (1u << 22)) { atomicExch(err, 1); break; }
-      }
+__device__ inline void put_word(u64* p, unsigned tag, unsigned payload) {
+  __hip_atomic_store(p, ((u64)tag << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline u64 get_word(const u64* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline bool tag_ok(u64 w, unsigned tag) { return (unsigned)(w >> 32) == tag; }
+
+// Real value <-> tagged words.
+__device__ inline void put_real(u64* p, unsigned tag, float v) { put_word(p, tag, __float_as_uint(v)); }
+__device__ inline void put_real(u64* p, unsigned tag, double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  put_word(p, tag, (unsigned)b);
+  put_word(p + 1, tag, (unsigned)(b >> 32));
+}
+// Spin (bounded, watching *err) until every word of a real carries the tag.
+template <class T> __device__ inline T get_real(const u64* p, unsigned tag, int* err);
+template <> __device__ inline float get_real<float>(const u64* p, unsigned tag, int* err) {
+  unsigned spins = 0;
+  u64 w = get_word(p);
+  while (!tag_ok(w, tag)) {
+    __builtin_amdgcn_s_sleep(1);
+    if ((++spins & 255u) == 0) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (spins > (1u << 22)) { atomicExch(err, 1); break; }
     }
+    w = get_word(p);
   }
-  __syncthreads();
+  return __uint_as_float((unsigned)w);
+}
+template <> __device__ inline double get_real<double>(const u64* p, unsigned tag, int* err) {
+  unsigned spins = 0;
+  u64 lo = get_word(p), hi = get_word(p + 1);
+  while (!tag_ok(lo, tag) || !tag_ok(hi, tag)) {
+    __builtin_amdgcn_s_sleep(1);
+    if ((++spins & 255u) == 0) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (spins > (1u << 22)) { atomicExch(err, 1); break; }
+    }
+    lo = get_word(p);
+    hi = get_word(p + 1);
+  }
+  return __longlong_as_double((long long)((lo & 0xffffffffull) | ((hi & 0xffffffffull) << 32)));
+}
+__device__ inline int get_int(const u64* p, unsigned tag, int* err) {
+  unsigned spins = 0;
+  u64 w = get_word(p);
+  while (!tag_ok(w, tag)) {
+    __builtin_amdgcn_s_sleep(1);
+    if ((++spins & 255u) == 0) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (spins > (1u << 22)) { atomicExch(err, 1); break; }
+    }
+    w = get_word(p);
+  }
+  return (int)(unsigned)w;
 }
 
 template <class T>
@@ -44,7 +98,8 @@ __device__ inline void best_of(T& bv, int& bi, T ov, int oi) {
   if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
 }
 
-// Workgroup argmax of (value, index): largest value, lowest index on ties.
+// Workgroup argmax of (value, index): largest value, lowest index on ties
+// (red_v / red_i hold one slot per wavefront).
 template <class T>
 __device__ inline void wg_argmax(T& bv, int& bi, T* red_v, int* red_i) {
   for (int off = 32; off > 0; off >>= 1) {
@@ -61,121 +116,190 @@ __device__ inline void wg_argmax(T& bv, int& bi, T* red_v, int* red_i) {
 
 // ---------------------------------------------------------------------------
 // Cooperative GEPP of an m x w leaf (w <= W) at P (row-major, ld).  G
-// workgroups each keep a slab of rpw consecutive rows in LDS for the whole
-// leaf.  Per column j each workgroup finds its best candidate (wavefront
-// shuffles + LDS), publishes value, row index and the candidate row (and the
-// owner of row j publishes row j), raises its arrival flag, and waits for the
-// G flags; then every workgroup picks the same winner (largest |Re|+|Im|,
-// lowest row on ties), performs the interchange on the rows it owns, scales
-// the multipliers and applies the rank-1 update to its slab.  With G = 1 the
-// whole column step stays in LDS.  ipiv receives global 0-based rows.
+// workgroups of NT threads own NT * RPT consecutive rows each; every thread
+// keeps its RPT rows of the leaf in registers for the whole leaf (the column
+// loop is fully unrolled, so row updates are register FMAs against the pivot
+// row broadcast from LDS).  Per column j: each workgroup finds its best
+// candidate (wavefront shuffles + LDS), publishes value, row index and the
+// candidate row as tagged words (and the owner of row j publishes row j);
+// every workgroup polls the G (value, index) pairs until all carry column j's
+// tag, picks the same winner (largest |Re|+|Im|, lowest row on ties),
+// fetches the winning row, performs the interchange on the rows it owns and
+// applies the scaling and rank-1 update: two global round trips per column,
+// none when G = 1.  ipiv receives global 0-based rows.
 // ---------------------------------------------------------------------------
-template <class T, int W>
-__global__ void __launch_bounds__(256)
-leaf_getf2_kernel(cx<T>* __restrict__ P, int64_t ld, int m, int w, int rpw, int row_base,
-                  int* __restrict__ ipiv, int* __restrict__ info, LeafXchg<T> x,
-                  unsigned seq_base, int stage_all, int* __restrict__ err) {
-  extern __shared__ unsigned char smem_raw[];
-  cx<T>* slab = reinterpret_cast<cx<T>*>(smem_raw);  // [rpw][W]
-  cx<T>* stage = slab + (size_t)rpw * W;              // [G][W] when stage_all
-  __shared__ cx<T> prow[W];
-  __shared__ cx<T> oldj[W];
-  __shared__ T red_v[4];
-  __shared__ int red_i[4];
+template <class T, int W, int NT, int RPT>
+__global__ void __launch_bounds__(NT)
+leaf_getf2_kernel(cx<T>* __restrict__ P, int64_t ld, int m, int w, int row_base,
+                  int* __restrict__ ipiv, int* __restrict__ info, LeafXchg x, unsigned seq_base,
+                  int* __restrict__ err) {
+  constexpr int RW = 2 * words_per<T>::n;   // words per complex element
+  constexpr int RPW = NT * RPT;             // rows per workgroup
+  constexpr int STAGE_ROWS = 128;           // rows per LDS staging chunk
+  __shared__ cx<T> prow[W];                 // pivot row (rotated order, see below)
+  __shared__ cx<T> oldj[W];                 // row j before the interchange
+  __shared__ T red_v[NT / 64];
+  __shared__ int red_i[NT / 64];
+  const int tid = threadIdx.x;
+  const int G = gridDim.x, g = blockIdx.x;
+  const int lo = g * RPW, hi = min(m, lo + RPW);
+  const int gs = x.gstride;
 
-  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
-  const int lo = g * rpw, hi = min(m, lo + rpw), nrows = max(0, hi - lo);
-
-  for (int e = tid; e < nrows * W; e += 256) {
-    int i = e / W, c = e % W;
-    slab[e] = (c < w) ? P[(int64_t)(lo + i) * ld + c] : mkc<T>(0, 0);
+  // Rows live in registers in ROTATED order: before step j, register c holds
+  // column (c + j) mod W, so the step always works on register 0 and every
+  // index is a compile-time constant; each step rotates the row left by one
+  // and appends the finished multiplier.  Registers W-j .. W-1 hold finished
+  // L entries, which the update must not touch (the pivot row is zeroed there).
+  // coalesced load through LDS, NT rows at a time (a thread's row is W
+  // consecutive elements, so direct per-thread loads would touch one cache
+  // line per lane)
+  __shared__ cx<T> stage[STAGE_ROWS][W + 1];
+  cx<T> a[RPT][W];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    for (int s0 = 0; s0 < NT; s0 += STAGE_ROWS) {
+      __syncthreads();
+      for (int e = tid; e < STAGE_ROWS * W; e += NT) {
+        const int i = e / W, c = e % W, row = lo + r * NT + s0 + i;
+        stage[i][c] = (row < hi && c < w) ? P[(int64_t)row * ld + c] : mkc<T>(0, 0);
+      }
+      __syncthreads();
+      if (tid >= s0 && tid < s0 + STAGE_ROWS) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) a[r][c] = stage[tid - s0][c];
+      }
+    }
   }
-  __syncthreads();
 
   for (int j = 0; j < w; ++j) {
     const int par = j & 1;
-    // -- local candidate among owned rows >= j
+    const unsigned tag = seq_base + (unsigned)j + 1u;
+    // -- local candidate among owned rows >= j (rows ascend with r)
     T bv = T(-1);
     int bi = 0x7fffffff;
-    for (int i = max(lo, j) + tid; i < hi; i += 256) {
-      T v = cabs1<T>(slab[(i - lo) * W + j]);
-      if (v > bv) { bv = v; bi = i; }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int row = lo + tid + r * NT;
+      if (row < hi && row >= j) {
+        const T v = cabs1<T>(a[r][0]);
+        if (v > bv) { bv = v; bi = row; }
+      }
     }
     wg_argmax(bv, bi, red_v, red_i);
+    // -- stage this workgroup's candidate row and row j in LDS
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int row = lo + tid + r * NT;
+      if (row == bi && bv >= T(0)) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) prow[c] = a[r][c];
+      }
+      if (row == j && row < hi) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) oldj[c] = a[r][c];
+      }
+    }
+    __syncthreads();
     int p;
     T best;
-    if (G == 1) {
+    if (G > 1) {
+      // -- publish (fire and forget): value, index, candidate row, row j
+      if (tid == 0) {
+        put_real(x.val + ((size_t)par * gs + g) * words_per<T>::n, tag, bv);
+        put_word(x.idx + (size_t)par * gs + g, tag, (unsigned)bi);
+      }
+      if (tid < W * RW) {
+        const int c = tid / RW, part = tid % RW;
+        const int comp = part / words_per<T>::n;
+        if (words_per<T>::n == 1 || (part % 2) == 0) {
+          u64* dst = x.row + (((size_t)par * gs + g) * W + c) * RW + comp * words_per<T>::n;
+          put_real(dst, tag, comp ? prow[c].y : prow[c].x);
+          if (j >= lo && j < hi) {
+            u64* dj = x.rowj + ((size_t)par * W + c) * RW + comp * words_per<T>::n;
+            put_real(dj, tag, comp ? oldj[c].y : oldj[c].x);
+          }
+        }
+      }
+      // -- gather the G candidates (polling their tags) and reduce
+      bv = T(-1); bi = 0x7fffffff;
+      for (int q = tid; q < G; q += NT)
+        best_of(bv, bi, get_real<T>(x.val + ((size_t)par * gs + q) * words_per<T>::n, tag, err),
+                get_int(x.idx + (size_t)par * gs + q, tag, err));
+      wg_argmax(bv, bi, red_v, red_i);   // its barriers also order the prow/oldj reads above
       best = bv;
       p = (bv >= T(0)) ? bi : j;
-      if (tid < W) {
-        prow[tid] = slab[(p - lo) * W + tid];
-        oldj[tid] = slab[(j - lo) * W + tid];
+      const int win = (bv >= T(0)) ? (bi / RPW) : 0;
+      // -- winning row and row j (each word's tag checked)
+      if (tid < 2 * W) {
+        const int c = tid % W;
+        const bool want_j = tid >= W;
+        const u64* src = want_j ? x.rowj + ((size_t)par * W + c) * RW
+                                : x.row + (((size_t)par * gs + win) * W + c) * RW;
+        cx<T> v = mkc<T>(0, 0);
+        if (want_j || bv >= T(0))
+          v = mkc<T>(get_real<T>(src, tag, err), get_real<T>(src + words_per<T>::n, tag, err));
+        if (want_j) oldj[c] = v; else prow[c] = v;
       }
       __syncthreads();
     } else {
-      if (tid == 0) {
-        x.val[par * G + g] = bv;
-        x.idx[par * G + g] = bi;
-      }
-      if (bv >= T(0) && tid < W) x.row[((int64_t)par * G + g) * W + tid] = slab[(bi - lo) * W + tid];
-      if (j >= lo && j < hi && tid < W) x.rowj[par * W + tid] = slab[(j - lo) * W + tid];
-      __syncthreads();
-      if (tid == 0) {
-        __threadfence();
-        __hip_atomic_store(x.flag + par * G + g, seq_base + (unsigned)j + 1u, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      }
-      wait_flags(x.flag + par * G, G, seq_base + (unsigned)j + 1u, err);
-      // -- global winner: every workgroup reduces the same G candidates; the
-      //    winning row's slab owner is row / rpw.  With stage_all the G
-      //    candidate rows are fetched in the same round trip as the values.
-      if (stage_all)
-        for (int e = tid; e < G * W; e += 256) stage[e] = x.row[(int64_t)par * G * W + e];
-      if (tid < W) oldj[tid] = x.rowj[par * W + tid];
-      bv = T(-1); bi = 0x7fffffff;
-      for (int q = tid; q < G; q += 256) best_of(bv, bi, x.val[par * G + q], x.idx[par * G + q]);
-      wg_argmax(bv, bi, red_v, red_i);
       best = bv;
       p = (bv >= T(0)) ? bi : j;
-      const int win = (bv >= T(0)) ? (bi / rpw) : 0;
-      if (tid < W)
-        prow[tid] = (bv < T(0)) ? mkc<T>(0, 0)
-                    : stage_all ? stage[win * W + tid]
-                                : x.row[((int64_t)par * G + win) * W + tid];
-      __syncthreads();
     }
     if (g == 0 && tid == 0) {
       ipiv[j] = row_base + p;
       if (best == T(0) && *info == 0) *info = row_base + j + 1;
     }
-    // -- interchange rows j and p inside the owned slabs
-    if (p != j && tid < w) {
-      if (j >= lo && j < hi) slab[(j - lo) * W + tid] = prow[tid];
-      if (p >= lo && p < hi) slab[(p - lo) * W + tid] = oldj[tid];
-    }
-    __syncthreads();
-    // -- multipliers and rank-1 update of the rows below j
-    if (best > T(0)) {
-      const cx<T> rp = crecip<T>(prow[j]);
-      const int r0 = max(lo, j + 1);
-      const int cols = w - j - 1;
-      if (cols > 0) {
-        for (int e = tid; e < (hi - r0) * cols; e += 256) {
-          int i = r0 + e / cols, c = j + 1 + e % cols;
-          cx<T> l = cmul<T>(slab[(i - lo) * W + j], rp);
-          slab[(i - lo) * W + c] = cfms<T>(slab[(i - lo) * W + c], l, prow[c]);
+    // -- interchange, multipliers, rank-1 update and rotation, in registers
+    const bool piv_ok = best > T(0);
+    const cx<T> rp = piv_ok ? crecip<T>(prow[0]) : mkc<T>(0, 0);
+    const int live = W - j;   // registers 1 .. live-1 still hold unfinished columns
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int row = lo + tid + r * NT;
+      if (p != j) {
+        if (row == j) {
+#pragma unroll
+          for (int c = 0; c < W; ++c) a[r][c] = prow[c];
+        } else if (row == p) {
+#pragma unroll
+          for (int c = 0; c < W; ++c) a[r][c] = oldj[c];
         }
-        __syncthreads();
       }
-      for (int i = r0 + tid; i < hi; i += 256)
-        slab[(i - lo) * W + j] = cmul<T>(slab[(i - lo) * W + j], rp);
+      cx<T> head = a[r][0];
+      if (piv_ok && row > j && row < hi) {
+        const cx<T> l = cmul<T>(head, rp);
+        head = l;
+#pragma unroll
+        for (int c = 1; c < W; ++c)
+          if (c < live) a[r][c] = cfms<T>(a[r][c], l, prow[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < W - 1; ++c) a[r][c] = a[r][c + 1];
+      a[r][W - 1] = head;
     }
-    __syncthreads();
+    __syncthreads();   // prow / oldj are rewritten by the next column
   }
 
-  for (int e = tid; e < nrows * W; e += 256) {
-    int i = e / W, c = e % W;
-    if (c < w) P[(int64_t)(lo + i) * ld + c] = slab[e];
+  // undo the rotation (register c now holds column (c + w) mod W) and store
+  // coalesced through LDS
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    for (int s0 = 0; s0 < NT; s0 += STAGE_ROWS) {
+      __syncthreads();
+      if (tid >= s0 && tid < s0 + STAGE_ROWS) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+          int col = c + w;
+          if (col >= W) col -= W;
+          stage[tid - s0][col] = a[r][c];
+        }
+      }
+      __syncthreads();
+      for (int e = tid; e < STAGE_ROWS * W; e += NT) {
+        const int i = e / W, c = e % W, row = lo + r * NT + s0 + i;
+        if (row < hi && c < w) P[(int64_t)row * ld + c] = stage[i][c];
+      }
+    }
   }
 }
 
