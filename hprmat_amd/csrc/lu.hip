@@ -208,12 +208,6 @@ norm_final_kernel(int nblocks, const double* __restrict__ part, RefineState* __r
 }
 
 
-// Leaf geometry: threads per workgroup and rows per thread (rows live in
-// registers: FP32 2 rows x 32 complex = 128 VGPRs, FP64 1 row = 128 VGPRs).
-template <class T> struct LeafCfg;
-template <> struct LeafCfg<float> { static constexpr int NT = 512, RPT = 2; };
-template <> struct LeafCfg<double> { static constexpr int NT = 512, RPT = 1; };
-
 // Exchange area layout (u64 words): val [2][MAX_LEAF_G][2], idx [2][MAX_LEAF_G],
 // row [2][MAX_LEAF_G][LEAF_W][4], rowj [2][LEAF_W][4] (sized for FP64).
 constexpr size_t XW_VAL = 2 * MAX_LEAF_G * 2, XW_IDX = 2 * MAX_LEAF_G,
@@ -222,8 +216,8 @@ constexpr size_t XW_TOTAL = XW_VAL + XW_IDX + XW_ROW + XW_ROWJ;
 
 template <class T> int launch_leaf(cx<T>* P, int64_t ld, int m, int w, int row_base, int* ipiv,
                                    int* info, const Scratch& s, int leaf, hipStream_t st) {
-  constexpr int NT = LeafCfg<T>::NT, RPT = LeafCfg<T>::RPT;
-  const int G = (m + NT * RPT - 1) / (NT * RPT);
+  constexpr int NT = 256;   // one row per thread
+  const int G = (m + NT - 1) / NT;
   if (G < 1 || G > MAX_LEAF_G) return -1;   // all leaf workgroups must be co-resident
   LeafXchg x;
   u64* base = (u64*)s.xchg;
@@ -232,8 +226,8 @@ template <class T> int launch_leaf(cx<T>* P, int64_t ld, int m, int w, int row_b
   x.row = x.idx + XW_IDX;
   x.rowj = x.row + XW_ROW;
   x.gstride = MAX_LEAF_G;
-  leaf_getf2_kernel<T, LEAF_W, NT, RPT><<<G, NT, 0, st>>>(P, ld, m, w, row_base, ipiv, info, x,
-                                                          (unsigned)leaf * LEAF_W, s.err);
+  leaf_getf2_kernel<T, LEAF_W><<<G, NT, 0, st>>>(P, ld, m, w, row_base, ipiv, info, x,
+                                                  (unsigned)leaf * LEAF_W, s.err);
   return 0;
 }
 

@@ -93,212 +93,271 @@ __device__ inline int get_int(const u64* p, unsigned tag, int* err) {
   return (int)(unsigned)w;
 }
 
-template <class T>
-__device__ inline void best_of(T& bv, int& bi, T ov, int oi) {
-  if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+// ---------------------------------------------------------------------------
+// Wavefront reductions on DPP.  Four DPP steps (swap neighbours, swap pairs,
+// mirror halves of 8, mirror the 16-lane row) give every lane the result over
+// its row; the four row results are combined through v_readlane, so the value
+// returned is uniform across the wavefront.  All 64 lanes must be active.
+// ---------------------------------------------------------------------------
+constexpr int DPP_QUAD_1032 = 0xB1, DPP_QUAD_2301 = 0x4E, DPP_ROW_HALF_MIRROR = 0x141,
+              DPP_ROW_MIRROR = 0x140;
+
+template <int CTRL> __device__ inline int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL> __device__ inline float dpp_r(float v) {
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+template <int CTRL> __device__ inline double dpp_r(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)dpp_i<CTRL>((int)b), hi = (unsigned)dpp_i<CTRL>((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline int lane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ inline float lane_r(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ inline double lane_r(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Workgroup argmax of (value, index): largest value, lowest index on ties
-// (red_v / red_i hold one slot per wavefront).
-template <class T>
-__device__ inline void wg_argmax(T& bv, int& bi, T* red_v, int* red_i) {
-  for (int off = 32; off > 0; off >>= 1) {
-    T ov = __shfl_xor(bv, off);
-    int oi = __shfl_xor(bi, off);
-    best_of(bv, bi, ov, oi);
+template <class T> __device__ inline T wave_max(T v) {
+  v = fmax(v, dpp_r<DPP_QUAD_1032>(v));
+  v = fmax(v, dpp_r<DPP_QUAD_2301>(v));
+  v = fmax(v, dpp_r<DPP_ROW_HALF_MIRROR>(v));
+  v = fmax(v, dpp_r<DPP_ROW_MIRROR>(v));
+  return fmax(fmax(lane_r(v, 0), lane_r(v, 16)), fmax(lane_r(v, 32), lane_r(v, 48)));
+}
+__device__ inline int wave_min(int v) {
+  v = min(v, dpp_i<DPP_QUAD_1032>(v));
+  v = min(v, dpp_i<DPP_QUAD_2301>(v));
+  v = min(v, dpp_i<DPP_ROW_HALF_MIRROR>(v));
+  v = min(v, dpp_i<DPP_ROW_MIRROR>(v));
+  return min(min(lane_i(v, 0), lane_i(v, 16)), min(lane_i(v, 32), lane_i(v, 48)));
+}
+// (largest value, lowest index among the lanes holding it), uniform.
+template <class T> __device__ inline void wave_argmax(T& v, int& i) {
+  const T mx = wave_max(v);
+  i = wave_min(v == mx ? i : 0x7fffffff);
+  v = mx;
+}
+
+// Spin (bounded, watching *err) until the word carries the tag; payload.
+__device__ inline unsigned get_u32(const u64* p, unsigned tag, int* err) {
+  unsigned spins = 0;
+  u64 w = get_word(p);
+  while (!tag_ok(w, tag)) {
+    __builtin_amdgcn_s_sleep(1);
+    if ((++spins & 255u) == 0) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (spins > (1u << 22)) { atomicExch(err, 1); break; }
+    }
+    w = get_word(p);
   }
-  if ((threadIdx.x & 63) == 0) { red_v[threadIdx.x >> 6] = bv; red_i[threadIdx.x >> 6] = bi; }
-  __syncthreads();
-  bv = red_v[0]; bi = red_i[0];
-  for (int q = 1; q < (int)(blockDim.x >> 6); ++q) best_of(bv, bi, red_v[q], red_i[q]);
-  __syncthreads();
+  return (unsigned)w;
 }
 
 // ---------------------------------------------------------------------------
 // Cooperative GEPP of an m x w leaf (w <= W) at P (row-major, ld).  G
-// workgroups of NT threads own NT * RPT consecutive rows each; every thread
-// keeps its RPT rows of the leaf in registers for the whole leaf (the column
-// loop is fully unrolled, so row updates are register FMAs against the pivot
-// row broadcast from LDS).  Per column j: each workgroup finds its best
-// candidate (wavefront shuffles + LDS), publishes value, row index and the
-// candidate row as tagged words (and the owner of row j publishes row j);
-// every workgroup polls the G (value, index) pairs until all carry column j's
-// tag, picks the same winner (largest |Re|+|Im|, lowest row on ties),
-// fetches the winning row, performs the interchange on the rows it owns and
-// applies the scaling and rank-1 update: two global round trips per column,
-// none when G = 1.  ipiv receives global 0-based rows.
+// workgroups of 256 threads own 256 consecutive rows each, one row per thread;
+// a thread keeps its row's W leaf columns in registers for the whole leaf.
+// The footprint (about 100 VGPRs for FP32, 170 for FP64, under 30 KB of LDS)
+// leaves room on each CU for the trailing-update GEMM of the previous panel,
+// which runs concurrently on another stream.
+//
+// Rows live in registers in ROTATED order: before step j, register c holds
+// column (c + j) mod W, so step j always works on register 0 and every
+// register index is a compile-time constant; the update writes register c
+// from register c + 1 (the rotation costs nothing) and appends the finished
+// multiplier at W - 1.  Registers W - j .. W - 1 hold finished L entries, so
+// the copy of the pivot row used by the update ("masked") is zero there.
+//
+// Per column j:
+//  1. each wavefront reduces |Re| + |Im| over its rows >= j on DPP (largest
+//     value, lowest row on ties, the i?amax rule); the winning lane stages
+//     its row (full and masked) in LDS, the owner of row j stages row j; one
+//     barrier; every wave combines the four wave results the same way.
+//  2. G > 1: wavefront 0 publishes the workgroup's candidate row, row j (from
+//     its owner), then the candidate value and row index as tagged words;
+//     polls the G candidates and reduces them on DPP; fetches the winning row
+//     and, in the workgroup that owns the pivot row, row j; second barrier.
+//  3. interchange, multiplier and rank-1 update of the thread's row.
+// LDS is double-buffered by column parity, so no barrier closes a column.
+// ipiv receives global 0-based rows.
 // ---------------------------------------------------------------------------
-template <class T, int W, int NT, int RPT>
-__global__ void __launch_bounds__(NT)
+template <class T, int W>
+__global__ void __launch_bounds__(256)
 leaf_getf2_kernel(cx<T>* __restrict__ P, int64_t ld, int m, int w, int row_base,
                   int* __restrict__ ipiv, int* __restrict__ info, LeafXchg x, unsigned seq_base,
                   int* __restrict__ err) {
-  constexpr int RW = 2 * words_per<T>::n;   // words per complex element
-  constexpr int RPW = NT * RPT;             // rows per workgroup
-  constexpr int STAGE_ROWS = 128;           // rows per LDS staging chunk
-  __shared__ cx<T> prow[W];                 // pivot row (rotated order, see below)
-  __shared__ cx<T> oldj[W];                 // row j before the interchange
-  __shared__ T red_v[NT / 64];
-  __shared__ int red_i[NT / 64];
-  const int tid = threadIdx.x;
+  constexpr int NT = 256, NW = NT / 64, STAGE = 32;
+  constexpr int WPR = words_per<T>::n, RW = 2 * WPR;   // 32-bit words per real / complex
+  // column buffers, aliased with the staging tile used before and after the loop
+  constexpr int COLBUF = 2 * (2 * NW + 3) * W, STAGEBUF = STAGE * (W + 1);
+  __shared__ cx<T> lds[COLBUF > STAGEBUF ? COLBUF : STAGEBUF];
+  __shared__ T red_v[2][NW];
+  __shared__ int red_i[2][NW];
+  __shared__ T sel_v[2];
+  __shared__ int sel_i[2];
+  cx<T>(*cand)[NW][W] = reinterpret_cast<cx<T>(*)[NW][W]>(lds);                // [2][NW][W]
+  cx<T>(*candm)[NW][W] = reinterpret_cast<cx<T>(*)[NW][W]>(lds + 2 * NW * W);  // [2][NW][W]
+  cx<T>(*prow)[W] = reinterpret_cast<cx<T>(*)[W]>(lds + 4 * NW * W);           // [2][W]
+  cx<T>(*pupd)[W] = reinterpret_cast<cx<T>(*)[W]>(lds + 4 * NW * W + 2 * W);   // [2][W]
+  cx<T>(*oldj)[W] = reinterpret_cast<cx<T>(*)[W]>(lds + 4 * NW * W + 4 * W);   // [2][W]
+  cx<T>(*stage)[W + 1] = reinterpret_cast<cx<T>(*)[W + 1]>(lds);               // [STAGE][W+1]
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int G = gridDim.x, g = blockIdx.x;
-  const int lo = g * RPW, hi = min(m, lo + RPW);
+  const int lo = g * NT, hi = min(m, lo + NT);
+  const int row = lo + tid;
+  const bool mine = row < hi;
   const int gs = x.gstride;
 
-  // Rows live in registers in ROTATED order: before step j, register c holds
-  // column (c + j) mod W, so the step always works on register 0 and every
-  // index is a compile-time constant; each step rotates the row left by one
-  // and appends the finished multiplier.  Registers W-j .. W-1 hold finished
-  // L entries, which the update must not touch (the pivot row is zeroed there).
-  // coalesced load through LDS, NT rows at a time (a thread's row is W
-  // consecutive elements, so direct per-thread loads would touch one cache
-  // line per lane)
-  __shared__ cx<T> stage[STAGE_ROWS][W + 1];
-  cx<T> a[RPT][W];
+  // coalesced load through LDS, STAGE rows at a time
+  cx<T> a[W];
+  for (int s0 = 0; s0 < NT; s0 += STAGE) {
+    __syncthreads();
+    for (int e = tid; e < STAGE * W; e += NT) {
+      const int i = e / W, c = e % W, r = lo + s0 + i;
+      stage[i][c] = (r < hi && c < w) ? P[(int64_t)r * ld + c] : mkc<T>(0, 0);
+    }
+    __syncthreads();
+    if (tid >= s0 && tid < s0 + STAGE) {
 #pragma unroll
-  for (int r = 0; r < RPT; ++r) {
-    for (int s0 = 0; s0 < NT; s0 += STAGE_ROWS) {
-      __syncthreads();
-      for (int e = tid; e < STAGE_ROWS * W; e += NT) {
-        const int i = e / W, c = e % W, row = lo + r * NT + s0 + i;
-        stage[i][c] = (row < hi && c < w) ? P[(int64_t)row * ld + c] : mkc<T>(0, 0);
-      }
-      __syncthreads();
-      if (tid >= s0 && tid < s0 + STAGE_ROWS) {
-#pragma unroll
-        for (int c = 0; c < W; ++c) a[r][c] = stage[tid - s0][c];
-      }
+      for (int c = 0; c < W; ++c) a[c] = stage[tid - s0][c];
     }
   }
+  __syncthreads();   // the staging tile is reused as column buffers
 
   for (int j = 0; j < w; ++j) {
     const int par = j & 1;
     const unsigned tag = seq_base + (unsigned)j + 1u;
-    // -- local candidate among owned rows >= j (rows ascend with r)
-    T bv = T(-1);
-    int bi = 0x7fffffff;
+    const int live = W - j;   // registers 0 .. live-1 hold unfinished columns
+    // -- 1. pivot search inside the workgroup
+    T v = (mine && row >= j) ? cabs1<T>(a[0]) : T(-1);
+    int vi = row;
+    wave_argmax(v, vi);
+    if (row == vi) {
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-      const int row = lo + tid + r * NT;
-      if (row < hi && row >= j) {
-        const T v = cabs1<T>(a[r][0]);
-        if (v > bv) { bv = v; bi = row; }
+      for (int c = 0; c < W; ++c) {
+        cand[par][wave][c] = a[c];
+        candm[par][wave][c] = (c < live) ? a[c] : mkc<T>(0, 0);
       }
     }
-    wg_argmax(bv, bi, red_v, red_i);
-    // -- stage this workgroup's candidate row and row j in LDS
+    if (row == j) {
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-      const int row = lo + tid + r * NT;
-      if (row == bi && bv >= T(0)) {
-#pragma unroll
-        for (int c = 0; c < W; ++c) prow[c] = a[r][c];
-      }
-      if (row == j && row < hi) {
-#pragma unroll
-        for (int c = 0; c < W; ++c) oldj[c] = a[r][c];
-      }
+      for (int c = 0; c < W; ++c) oldj[par][c] = a[c];
     }
+    if (lane == 0) { red_v[par][wave] = v; red_i[par][wave] = vi; }
     __syncthreads();
-    int p;
+    T bv = red_v[par][0];
+    int bi = red_i[par][0], bw = 0;
+#pragma unroll
+    for (int q = 1; q < NW; ++q) {
+      const T ov = red_v[par][q];
+      if (ov > bv) { bv = ov; bi = red_i[par][q]; bw = q; }   // waves ascend in rows
+    }
     T best;
-    if (G > 1) {
-      // -- publish (fire and forget): value, index, candidate row, row j
-      if (tid == 0) {
-        put_real(x.val + ((size_t)par * gs + g) * words_per<T>::n, tag, bv);
-        put_word(x.idx + (size_t)par * gs + g, tag, (unsigned)bi);
-      }
-      if (tid < W * RW) {
-        const int c = tid / RW, part = tid % RW;
-        const int comp = part / words_per<T>::n;
-        if (words_per<T>::n == 1 || (part % 2) == 0) {
-          u64* dst = x.row + (((size_t)par * gs + g) * W + c) * RW + comp * words_per<T>::n;
-          put_real(dst, tag, comp ? prow[c].y : prow[c].x);
-          if (j >= lo && j < hi) {
-            u64* dj = x.rowj + ((size_t)par * W + c) * RW + comp * words_per<T>::n;
-            put_real(dj, tag, comp ? oldj[c].y : oldj[c].x);
-          }
+    int p;
+    const cx<T>* F;   // pivot row (full)
+    const cx<T>* U;   // pivot row, finished columns zeroed
+    if (G == 1) {
+      best = bv;
+      p = (bv >= T(0)) ? bi : j;
+      F = cand[par][bw];
+      U = candm[par][bw];
+    } else {
+      if (wave == 0) {
+        const bool own_j = (j >= lo && j < hi);
+        // publish: candidate row, row j, then value and index
+        const unsigned* cw = reinterpret_cast<const unsigned*>(cand[par][bw]);
+        u64* dst = x.row + ((size_t)par * gs + g) * (W * RW);
+        for (int e = lane; e < W * RW; e += 64) put_word(dst + e, tag, cw[e]);
+        if (own_j) {
+          const unsigned* jw = reinterpret_cast<const unsigned*>(oldj[par]);
+          u64* dj = x.rowj + (size_t)par * (W * RW);
+          for (int e = lane; e < W * RW; e += 64) put_word(dj + e, tag, jw[e]);
+        }
+        if (lane == 0) {
+          put_real(x.val + ((size_t)par * gs + g) * WPR, tag, bv);
+          put_word(x.idx + (size_t)par * gs + g, tag, (unsigned)bi);
+        }
+        // gather the G candidates and reduce
+        T qv = T(-1);
+        int qi = 0x7fffffff;
+        for (int q = lane; q < G; q += 64) {
+          const T ov = get_real<T>(x.val + ((size_t)par * gs + q) * WPR, tag, err);
+          const int oi = get_int(x.idx + (size_t)par * gs + q, tag, err);
+          if (ov > qv || (ov == qv && oi < qi)) { qv = ov; qi = oi; }
+        }
+        wave_argmax(qv, qi);
+        const bool have = qv >= T(0);
+        const int pp = have ? qi : j;
+        if (lane == 0) { sel_v[par] = qv; sel_i[par] = pp; }
+        // winning row (full and masked)
+        const int win = have ? qi / NT : 0;
+        const u64* src = x.row + ((size_t)par * gs + win) * (W * RW);
+        unsigned* fw = reinterpret_cast<unsigned*>(prow[par]);
+        unsigned* uw = reinterpret_cast<unsigned*>(pupd[par]);
+        for (int e = lane; e < W * RW; e += 64) {
+          const unsigned u = have ? get_u32(src + e, tag, err) : 0u;
+          fw[e] = u;
+          uw[e] = (e / RW < live) ? u : 0u;
+        }
+        // row j, needed only by the owner of the pivot row
+        if (pp != j && pp >= lo && pp < hi && !own_j) {
+          const u64* sj = x.rowj + (size_t)par * (W * RW);
+          unsigned* ow = reinterpret_cast<unsigned*>(oldj[par]);
+          for (int e = lane; e < W * RW; e += 64) ow[e] = get_u32(sj + e, tag, err);
         }
       }
-      // -- gather the G candidates (polling their tags) and reduce
-      bv = T(-1); bi = 0x7fffffff;
-      for (int q = tid; q < G; q += NT)
-        best_of(bv, bi, get_real<T>(x.val + ((size_t)par * gs + q) * words_per<T>::n, tag, err),
-                get_int(x.idx + (size_t)par * gs + q, tag, err));
-      wg_argmax(bv, bi, red_v, red_i);   // its barriers also order the prow/oldj reads above
-      best = bv;
-      p = (bv >= T(0)) ? bi : j;
-      const int win = (bv >= T(0)) ? (bi / RPW) : 0;
-      // -- winning row and row j (each word's tag checked)
-      if (tid < 2 * W) {
-        const int c = tid % W;
-        const bool want_j = tid >= W;
-        const u64* src = want_j ? x.rowj + ((size_t)par * W + c) * RW
-                                : x.row + (((size_t)par * gs + win) * W + c) * RW;
-        cx<T> v = mkc<T>(0, 0);
-        if (want_j || bv >= T(0))
-          v = mkc<T>(get_real<T>(src, tag, err), get_real<T>(src + words_per<T>::n, tag, err));
-        if (want_j) oldj[c] = v; else prow[c] = v;
-      }
       __syncthreads();
-    } else {
-      best = bv;
-      p = (bv >= T(0)) ? bi : j;
+      best = sel_v[par];
+      p = sel_i[par];
+      F = prow[par];
+      U = pupd[par];
     }
     if (g == 0 && tid == 0) {
       ipiv[j] = row_base + p;
       if (best == T(0) && *info == 0) *info = row_base + j + 1;
     }
-    // -- interchange, multipliers, rank-1 update and rotation, in registers
-    const bool piv_ok = best > T(0);
-    const cx<T> rp = piv_ok ? crecip<T>(prow[0]) : mkc<T>(0, 0);
-    const int live = W - j;   // registers 1 .. live-1 still hold unfinished columns
+    // -- 3. interchange, multiplier, rank-1 update (rotating by one register)
+    if (p != j) {
+      if (row == j) {
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-      const int row = lo + tid + r * NT;
-      if (p != j) {
-        if (row == j) {
+        for (int c = 0; c < W; ++c) a[c] = F[c];
+      } else if (row == p) {
 #pragma unroll
-          for (int c = 0; c < W; ++c) a[r][c] = prow[c];
-        } else if (row == p) {
-#pragma unroll
-          for (int c = 0; c < W; ++c) a[r][c] = oldj[c];
-        }
+        for (int c = 0; c < W; ++c) a[c] = oldj[par][c];
       }
-      cx<T> head = a[r][0];
-      if (piv_ok && row > j && row < hi) {
-        const cx<T> l = cmul<T>(head, rp);
-        head = l;
-#pragma unroll
-        for (int c = 1; c < W; ++c)
-          if (c < live) a[r][c] = cfms<T>(a[r][c], l, prow[c]);
-      }
-#pragma unroll
-      for (int c = 0; c < W - 1; ++c) a[r][c] = a[r][c + 1];
-      a[r][W - 1] = head;
     }
-    __syncthreads();   // prow / oldj are rewritten by the next column
+    const bool upd = best > T(0) && mine && row > j;
+    const cx<T> l = upd ? cmul<T>(a[0], crecip<T>(F[0])) : mkc<T>(0, 0);
+    const cx<T> head = upd ? l : a[0];
+#pragma unroll
+    for (int c = 0; c < W - 1; ++c) a[c] = cfms<T>(a[c + 1], l, U[c + 1]);
+    a[W - 1] = head;
   }
 
   // undo the rotation (register c now holds column (c + w) mod W) and store
   // coalesced through LDS
+  for (int s0 = 0; s0 < NT; s0 += STAGE) {
+    __syncthreads();
+    if (tid >= s0 && tid < s0 + STAGE) {
 #pragma unroll
-  for (int r = 0; r < RPT; ++r) {
-    for (int s0 = 0; s0 < NT; s0 += STAGE_ROWS) {
-      __syncthreads();
-      if (tid >= s0 && tid < s0 + STAGE_ROWS) {
-#pragma unroll
-        for (int c = 0; c < W; ++c) {
-          int col = c + w;
-          if (col >= W) col -= W;
-          stage[tid - s0][col] = a[r][c];
-        }
+      for (int c = 0; c < W; ++c) {
+        int col = c + w;
+        if (col >= W) col -= W;
+        stage[tid - s0][col] = a[c];
       }
-      __syncthreads();
-      for (int e = tid; e < STAGE_ROWS * W; e += NT) {
-        const int i = e / W, c = e % W, row = lo + r * NT + s0 + i;
-        if (row < hi && c < w) P[(int64_t)row * ld + c] = stage[i][c];
-      }
+    }
+    __syncthreads();
+    for (int e = tid; e < STAGE * W; e += NT) {
+      const int i = e / W, c = e % W, r = lo + s0 + i;
+      if (r < hi && c < w) P[(int64_t)r * ld + c] = stage[i][c];
     }
   }
 }
