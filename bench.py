@@ -8,9 +8,16 @@ full solve with the matrix already resident in HBM: factor (LU with partial
 pivoting) + triangular solves + FP64 iterative refinement until
 ||B - A X||_F / ||B||_F <= 1e-12 (the last residual evaluation included).
 
-Methods timed back to back: 'fp64' (plain FP64 LU, FP64 refinement), and the
-mixed methods 'fp32_3m' / 'fp32' (complex-FP32 LU on MFMA, FP64 refinement).
-``value`` is the solves/s of the fastest mixed method that reaches the
+Methods timed back to back (name: factorization, refinement):
+  fp64      plain FP64 LU, FP64 residual check / refinement to the gate;
+  fp32      complex-FP32 LU on MFMA, FP64 refinement to the gate (<= 30 sweeps);
+  fp32_ir5  the same factorization with at most 5 FP64 sweeps;
+  bf16x3    FP32 LU whose trailing update uses split-BF16 three-product
+            MFMA arithmetic, FP64 refinement to the gate (<= 30 sweeps);
+  fp32_3m   complex-FP32 LU with Gauss's three-product complex GEMM.
+Each method's solution is compared with the fp64 method's (max relative
+error of X and max |dS| of the S-matrix formed from it on the device).
+``value`` is the solves/s of the fastest non-FP64 method that reaches the
 residual gate; per-method figures, the roofline of the dominant kernel (the
 trailing-update GEMM, timed with HIP events on the launch stream) and the
 CPU baseline (LAPACK zgesv + R, reference solver type 1) are in the line.
@@ -24,7 +31,11 @@ import sys
 import time
 
 METRIC = "solves/s & TFLOP/s, N=25600 complex LU+solve: mixed vs FP64 at resid≤1e-12"
-PEAK = {'fp64': 78.6, 'fp32': 157.3, 'fp32_3m': 157.3}   # dense MFMA TFLOP/s, MI355X
+PEAK = {'fp64': 78.6, 'fp32': 157.3, 'fp32_ir5': 157.3, 'fp32_3m': 157.3,
+        'bf16x3': 2500.0}   # dense MFMA TFLOP/s, MI355X (bf16x3: BF16 matrix peak)
+# name -> (factorization method, max refinement sweeps)
+SPECS = {'fp64': ('fp64', 10), 'fp32': ('fp32', 30), 'fp32_ir5': ('fp32', 5),
+         'bf16x3': ('bf16x3', 30), 'fp32_3m': ('fp32_3m', 30)}
 
 
 def flops_solve(n, nrhs):
@@ -71,7 +82,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--nch', type=int, default=128)
     ap.add_argument('--nr', type=int, default=200)
-    ap.add_argument('--methods', default='fp64,fp32_3m,fp32')
+    ap.add_argument('--methods', default='fp64,fp32,fp32_ir5,bf16x3')
     ap.add_argument('--no-cpu', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1:
@@ -103,14 +114,15 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
     fl = flops_solve(n, nrhs)
 
     methods = {}
+    x_ref = s_ref = None
     for m in args.methods.split(','):
         lu = solver.LU()
-        max_refine = 10
+        fm, max_refine = SPECS[m]
         b = b0.clone()
 
         def step():
             b.copy_(b0)
-            lu.factor(a_cm, m, equilibrate=True)
+            lu.factor(a_cm, fm, equilibrate=True)
             lu.solve_(b, max_refine, 1e-12, 'rel_fro', True, stats=False)
 
         for _ in range(args.warmup):
@@ -131,8 +143,11 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
         ms = e0.elapsed_time(e1) / args.steps
         # verification pass (outside the timed region): same inputs, with statistics
         b.copy_(b0)
-        lu.factor(a_cm, m, equilibrate=True)
+        lu.factor(a_cm, fm, equilibrate=True)
         st = lu.solve_(b, max_refine, 1e-12, 'rel_fro', True, stats=True)
+        _, s_m = rmatrix.r_to_s(dmesh, b, nrhs, sysm.qk, open_idx, co, cop)
+        if m == 'fp64':
+            x_ref, s_ref = b.clone(), s_m.clone()
         methods[m] = {
             'ms': ms, 'solves_per_s': 1000.0 / ms, 'tflops': fl / (ms * 1e-3) / 1e12,
             'res_rel': st['res_rel'], 'sweeps': st['sweeps'], 'converged': st['converged'],
@@ -143,6 +158,9 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
                      'flops_per_launch': gfl.value / max(gcnt.value, 1)},
         }
         methods[m]['hist'] = st['hist']
+        if x_ref is not None:
+            methods[m]['err_x_vs_fp64'] = float((b - x_ref).abs().max() / x_ref.abs().max())
+            methods[m]['dS_vs_fp64'] = float((s_m - s_ref).abs().max())
         print('# %s: %.1f ms/solve, %.2f solves/s, %.1f TFLOP/s, res_rel %.2e, sweeps %d, '
               'trailing GEMM %.1f TFLOP/s, hist %s' % (
                   m, ms, 1000.0 / ms, methods[m]['tflops'], st['res_rel'], st['sweeps'],
@@ -151,12 +169,15 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
         del lu
 
     mixed = [m for m in methods if m != 'fp64' and methods[m]['res_rel'] <= 1e-12]
+    # nopen actually drawn (the workload text quotes about 100)
+    nopen = len(open_idx)
     head = min(mixed, key=lambda m: methods[m]['ms']) if mixed else min(methods, key=lambda m: methods[m]['ms'])
     hm = methods[head]
     g = hm['gemm']
     roof = {'bound': 'mfma', 'achieved': g['tflops'], 'peak': PEAK[head],
             'unit': 'TFLOP/s', 'frac': g['tflops'] / PEAK[head], 'traffic': None,
-            'kernel': 'zgemm_kernel (trailing update A22 -= L21 U12, %s)' % head,
+            'kernel': '%s (trailing update A22 -= L21 U12, %s)' % (
+                'zgemm_bf16x3_kernel' if head == 'bf16x3' else 'zgemm_kernel', head),
             'flops_per_launch': g['flops_per_launch'], 'ms_per_launch': g['ms_per_launch']}
     out = {
         'metric': METRIC, 'value': hm['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1,
@@ -166,7 +187,7 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
         'data': 'synthetic',
         'config': {'workload': 'single Lagrange-mesh solve N=%d (nch=%d x nr=%d), nrhs=%d, '
                                'gate ||r||_F/||b||_F<=1e-12' % (n, args.nch, args.nr, nrhs),
-                   'N': n, 'nrhs': nrhs, 'method': head, 'flops_per_solve': fl},
+                   'N': n, 'nrhs': nrhs, 'nopen': nopen, 'method': head, 'flops_per_solve': fl},
         'tflops': hm['tflops'],
         'methods': methods,
         'speedup_mixed_vs_fp64': (methods['fp64']['ms'] / hm['ms']) if 'fp64' in methods else None,

@@ -25,6 +25,8 @@ struct hprmat_hip_lu {
   size_t dinv256_cap = 0;
   void* ustage = nullptr;     // U12 staging block (256 x n)
   size_t ustage_cap = 0;
+  void* split = nullptr;      // split-BF16 operand planes (HPRMAT_SPLIT_BF16)
+  size_t split_cap = 0;
   int* perm = nullptr;        // final row permutation
   size_t perm_cap = 0;
   double* scale = nullptr;    // equilibration factors (when equil)
@@ -69,33 +71,8 @@ int64_t padded_ld(int n) { return (((int64_t)n + 15) / 16) * 16 + 16; }
 // instead of waiting for GEMM workgroups to drain.  Default priority.
 int ensure_side(hprmat_hip_lu* h) {
   if (h->side.stream) return 0;
-  int dev = 0, ncu = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  int reserve = 32;
-  const char* env = getenv("HPRMAT_HIP_RESERVE_CU");
-  if (env) reserve = atoi(env);
-  bool masked = false;
-  if (reserve > 0 && ncu > 2 * reserve) {
-    const int stride = ncu / reserve;
-    const int words = (ncu + 31) / 32;
-    uint32_t mask[64] = {0};
-    if (words <= 64) {
-      int kept = 0;
-      for (int c = 0; c < ncu; ++c)
-        if (c % stride != 0 || c / stride >= reserve) { mask[c / 32] |= 1u << (c % 32); ++kept; }
-      hipError_t e = hipExtStreamCreateWithCUMask(&h->side.stream, (uint32_t)words, mask);
-      masked = e == hipSuccess;
-      if (!masked) {
-        fprintf(stderr, "hprmat_hip: CU-masked stream unavailable (%s), using all CUs\n",
-                hipGetErrorString(e));
-        (void)hipGetLastError();
-        h->side.stream = nullptr;
-      }
-    }
-  }
   hipError_t e;
-  if (!masked && (e = hipStreamCreateWithFlags(&h->side.stream, hipStreamNonBlocking)) != hipSuccess) {
+  if ((e = hipStreamCreateWithFlags(&h->side.stream, hipStreamNonBlocking)) != hipSuccess) {
     fprintf(stderr, "hprmat_hip: side stream creation failed: %s\n", hipGetErrorString(e));
     return -1;
   }
@@ -114,8 +91,9 @@ void free_lu(hprmat_hip_lu* h) {
     (void)hipEventDestroy(h->side.rest);
     (void)hipStreamDestroy(h->side.stream);
   }
-  void* ps[] = {h->fac, h->ipiv, h->dinfo, h->dinv64, h->dinv256, h->ustage, h->perm, h->scale,
-                h->scr, h->Brm,  h->Xrm,   h->R,      h->RF,      h->RY,     h->part, h->state};
+  void* ps[] = {h->fac, h->ipiv, h->dinfo, h->dinv64, h->dinv256, h->ustage, h->split,
+                h->perm, h->scale, h->scr, h->Brm, h->Xrm, h->R, h->RF, h->RY, h->part,
+                h->state};
   for (void* p : ps)
     if (p) hipFree(p);
   *h = hprmat_hip_lu();
@@ -126,7 +104,7 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
   if (n < 0 || lda < n) return -1;
   const int method = method_flags & 0xff;
   const int equil = (method_flags & HPRMAT_EQUILIBRATE) ? 1 : 0;
-  if (method < HPRMAT_PLAIN_FP64 || method > HPRMAT_MIXED_FP32_3M) {
+  if (method < HPRMAT_PLAIN_FP64 || method > HPRMAT_SPLIT_BF16) {
     fprintf(stderr, "hprmat_hip: factorization method %d is not available\n", method);
     return -1;
   }
@@ -150,6 +128,9 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
   if (grow(&h->ustage, &h->ustage_cap, hpr::ustage_elems(n) * eb)) return -2;
   if (grow((void**)&h->perm, &h->perm_cap, (size_t)(n + 1) * sizeof(int))) return -2;
   if (grow((void**)&h->scale, &h->scale_cap, (size_t)(n + 1) * sizeof(double))) return -2;
+  if (method == HPRMAT_SPLIT_BF16 &&
+      grow(&h->split, &h->split_cap, hpr::split_elems(n) * sizeof(unsigned short)))
+    return -2;
   h->equil = equil;
   const double* sc = nullptr;
   if (equil) {
@@ -177,7 +158,8 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
     w.ustage = (double2*)h->ustage;
     w.perm = h->perm;
     if (!rc)
-      rc = hpr::getrf<double>(F, ld, n, h->ipiv, h->dinfo, w, h->sc, h->plan, false, st, side);
+      rc = hpr::getrf<double>(F, ld, n, h->ipiv, h->dinfo, w, h->sc, h->plan, hpr::GEMM_4M, st,
+                              side);
   } else {
     float2* F = (float2*)h->fac;
     rc = hpr::transpose_cast<double, float>(n, n, A, lda, F, ld, st, sc, sc);
@@ -186,9 +168,12 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
     w.dinv256 = (float2*)h->dinv256;
     w.ustage = (float2*)h->ustage;
     w.perm = h->perm;
+    w.split = (unsigned short*)h->split;
+    const int gmode = method == HPRMAT_MIXED_FP32_3M ? hpr::GEMM_3M
+                      : method == HPRMAT_SPLIT_BF16   ? hpr::GEMM_BF16X3
+                                                      : hpr::GEMM_4M;
     if (!rc)
-      rc = hpr::getrf<float>(F, ld, n, h->ipiv, h->dinfo, w, h->sc, h->plan,
-                             method == HPRMAT_MIXED_FP32_3M, st, side);
+      rc = hpr::getrf<float>(F, ld, n, h->ipiv, h->dinfo, w, h->sc, h->plan, gmode, st, side);
   }
   return rc;
 }
@@ -360,7 +345,7 @@ int gpu_solve_hybrid_(double* A, double* B, int* n, int* nrhs, int* max_refine, 
 int gpu_solve_tf32_(double* A, double* B, int* n, int* nrhs, int* max_refine, double* tol,
                     int* info) {
   int refine = *max_refine > 0 ? *max_refine : 5;
-  return host_solve(HPRMAT_MIXED_FP32_3M, A, B, *n, *nrhs, refine, *tol, HPRMAT_CRIT_ABS_MAX, 0,
+  return host_solve(HPRMAT_SPLIT_BF16, A, B, *n, *nrhs, refine, *tol, HPRMAT_CRIT_ABS_MAX, 0,
                     info);
 }
 

@@ -171,3 +171,201 @@ zgemm_kernel(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
         }
       }
 }
+
+// ---------------------------------------------------------------------------
+// Second-generation complex GEMM (same contract as zgemm_kernel).
+//
+// * LDS is double-buffered: the next K-slice is loaded into registers while
+//   the current one is multiplied, then stored into the other buffer, so one
+//   barrier per K-slice suffices.
+// * Bank-conflict-free LDS: MFMA operand reads take 16 consecutive elements
+//   from each of two k-rows per half-wavefront.  Planes filled along k from a
+//   row-major A use an XOR swizzle of the column index (i ^ swz(k), swz(k) =
+//   (2k mod 32) ^ 16 (k mod 2)) that separates the two k-rows of a read into
+//   opposite 16-bank halves and spreads the k-strided stores over all banks;
+//   planes filled along i or j are padded to a row length = 16 mod 32.
+// * Dynamic LDS: 2 buffers x 2 planes x BK x (row) for A and B.
+// ---------------------------------------------------------------------------
+template <int BK, int BM, int BN, bool A_COL, class T>
+constexpr size_t zgemm2_lds_bytes() {
+  return (size_t)2 * 2 * BK * ((A_COL ? BM + 16 : BM) + (BN + 16)) * sizeof(T);
+}
+
+__device__ inline int zg2_swz(int k) { return ((2 * k) & 31) ^ ((k & 1) << 4); }
+
+template <class T, int BM, int BN, int BK, int WM, int WN, bool A_COL, bool GAUSS3M>
+__global__ void __launch_bounds__(256)
+zgemm2_kernel(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
+              const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
+              int64_t ldc, T alpha, int beta, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  static_assert((BM / WM) * (BN / WN) == 4, "four waves per workgroup");
+  static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 4 == 0 && BM % 32 == 0, "tile shape");
+  constexpr int NT = 256;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int LA = A_COL ? BM + 16 : BM, LB = BN + 16;
+  constexpr int EA = BM * BK / NT, EB = BK * BN / NT;
+  static_assert(EA * NT == BM * BK && EB * NT == BK * BN, "load split");
+  typedef typename acc_t<T>::type acc;
+
+  extern __shared__ __align__(16) unsigned char zg2_smem[];
+  T* As = reinterpret_cast<T*>(zg2_smem);          // [buf][plane][BK][LA]
+  T* Bs = As + 2 * 2 * BK * LA;                    // [buf][plane][BK][LB]
+
+  const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+  const int total = tiles_m * tiles_n;
+  int t = blockIdx.x;
+  if ((total & 7) == 0) t = (t & 7) * (total >> 3) + (t >> 3);
+  constexpr int GROUP = 8;
+  const int group = t / (GROUP * tiles_n);
+  const int first_m = group * GROUP;
+  const int gsz = min(tiles_m - first_m, GROUP);
+  const int tm = first_m + (t % (GROUP * tiles_n)) % gsz;
+  const int tn = (t % (GROUP * tiles_n)) / gsz;
+  const int row0 = tm * BM, col0 = tn * BN;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave / (BN / WN), wc = wave % (BN / WN);
+  const int li = lane & 15, lk = lane >> 4;
+
+  acc acc_re[FM][FN], acc_im[FM][FN], acc_p3[GAUSS3M ? FM : 1][GAUSS3M ? FN : 1];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b) {
+      acc_re[a][b] = acc{};
+      acc_im[a][b] = acc{};
+      if (GAUSS3M) acc_p3[a][b] = acc{};
+    }
+
+  cx<T> ra[EA], rb[EB];
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < EA; ++e) {
+      const int idx = tid + e * NT;
+      int i, k;
+      if (A_COL) { i = idx % BM; k = idx / BM; } else { i = idx / BK; k = idx % BK; }
+      const int gi = row0 + i, gk = k0 + k;
+      ra[e] = (gi < M && gk < K)
+                  ? (A_COL ? A[(int64_t)gk * lda + gi] : A[(int64_t)gi * lda + gk])
+                  : mkc<T>(0, 0);
+    }
+#pragma unroll
+    for (int e = 0; e < EB; ++e) {
+      const int idx = tid + e * NT, k = idx / BN, j = idx % BN;
+      const int gk = k0 + k, gj = col0 + j;
+      rb[e] = (gk < K && gj < N) ? B[(int64_t)gk * ldb + gj] : mkc<T>(0, 0);
+    }
+  };
+  auto store_tiles = [&](int buf) {
+    T* a0 = As + (size_t)(buf * 2) * BK * LA;
+    T* a1 = a0 + BK * LA;
+    T* b0 = Bs + (size_t)(buf * 2) * BK * LB;
+    T* b1 = b0 + BK * LB;
+#pragma unroll
+    for (int e = 0; e < EA; ++e) {
+      const int idx = tid + e * NT;
+      int i, k;
+      if (A_COL) { i = idx % BM; k = idx / BM; } else { i = idx / BK; k = idx % BK; i ^= zg2_swz(k); }
+      a0[k * LA + i] = ra[e].x;
+      a1[k * LA + i] = ra[e].y;
+    }
+#pragma unroll
+    for (int e = 0; e < EB; ++e) {
+      const int idx = tid + e * NT, k = idx / BN, j = idx % BN;
+      b0[k * LB + j] = rb[e].x;
+      b1[k * LB + j] = rb[e].y;
+    }
+  };
+
+  load_tiles(0);
+  store_tiles(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    const bool more = k0 + BK < K;
+    if (more) load_tiles(k0 + BK);
+    const T* a0 = As + (size_t)(buf * 2) * BK * LA;
+    const T* a1 = a0 + BK * LA;
+    const T* b0 = Bs + (size_t)(buf * 2) * BK * LB;
+    const T* b1 = b0 + BK * LB;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int k = kk + lk;
+      T ar[FM], ai[FM], br[FN], bi[FN];
+#pragma unroll
+      for (int a = 0; a < FM; ++a) {
+        int i = wr * WM + a * 16 + li;
+        if (!A_COL) i ^= zg2_swz(k);
+        ar[a] = a0[k * LA + i];
+        ai[a] = a1[k * LA + i];
+      }
+#pragma unroll
+      for (int b = 0; b < FN; ++b) {
+        const int j = wc * WN + b * 16 + li;
+        br[b] = b0[k * LB + j];
+        bi[b] = b1[k * LB + j];
+      }
+      if (GAUSS3M) {
+        T as[FM], bs[FN];
+#pragma unroll
+        for (int a = 0; a < FM; ++a) as[a] = ar[a] + ai[a];
+#pragma unroll
+        for (int b = 0; b < FN; ++b) bs[b] = br[b] + bi[b];
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+          for (int b = 0; b < FN; ++b) {
+            acc_re[a][b] = mfma16x4(ar[a], br[b], acc_re[a][b]);
+            acc_im[a][b] = mfma16x4(ai[a], bi[b], acc_im[a][b]);
+            acc_p3[a][b] = mfma16x4(as[a], bs[b], acc_p3[a][b]);
+          }
+      } else {
+        T nai[FM];
+#pragma unroll
+        for (int a = 0; a < FM; ++a) nai[a] = -ai[a];
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+          for (int b = 0; b < FN; ++b) {
+            acc_re[a][b] = mfma16x4(ar[a], br[b], acc_re[a][b]);
+            acc_im[a][b] = mfma16x4(ar[a], bi[b], acc_im[a][b]);
+          }
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+          for (int b = 0; b < FN; ++b) {
+            acc_re[a][b] = mfma16x4(nai[a], bi[b], acc_re[a][b]);
+            acc_im[a][b] = mfma16x4(ai[a], br[b], acc_im[a][b]);
+          }
+      }
+    }
+    if (more) store_tiles(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = row0 + wr * WM + a * 16 + mfma_row<T>(lk, r);
+        const int gj = col0 + wc * WN + b * 16 + li;
+        if (gi < M && gj < N) {
+          T pr, pi;
+          if (GAUSS3M) {
+            pr = acc_re[a][b][r] - acc_im[a][b][r];
+            pi = acc_p3[a][b][r] - acc_re[a][b][r] - acc_im[a][b][r];
+          } else {
+            pr = acc_re[a][b][r];
+            pi = acc_im[a][b][r];
+          }
+          cx<T>* pc = C + (int64_t)gi * ldc + gj;
+          const cx<T> c0 = beta ? *pc : mkc<T>(0, 0);
+          *pc = mkc<T>(c0.x + alpha * pr, c0.y + alpha * pi);
+        }
+      }
+}

@@ -20,7 +20,10 @@
 // (cusolverDn?getrs, src/cusolver_interface.cu:393, 787).
 #include <vector>
 
+#include <type_traits>
+
 #include "lu.hpp"
+#include "bf16x3.hpp"
 #include "gemm.hpp"
 #include "lu_kernels.hpp"
 
@@ -300,25 +303,78 @@ void scratch_bind(Scratch& s, void* base) {
 template <class T>
 int gemm(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<T>* B,
          int64_t ldb, cx<T>* C, int64_t ldc, T alpha, int beta, bool g3m, const int* skip,
-         hipStream_t st) {
+         hipStream_t st, size_t lds_extra) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
   constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32;
   int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+  // lds_extra reserves unused LDS per workgroup, which caps the number of
+  // workgroups per CU and so leaves room for the concurrently running panel
   if (a_col) {
-    if (g3m) zgemm_kernel<T, BM, BN, BK, WM, WN, true, true><<<tiles, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip);
-    else zgemm_kernel<T, BM, BN, BK, WM, WN, true, false><<<tiles, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip);
+    if (g3m) zgemm_kernel<T, BM, BN, BK, WM, WN, true, true><<<tiles, 256, lds_extra, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip);
+    else zgemm_kernel<T, BM, BN, BK, WM, WN, true, false><<<tiles, 256, lds_extra, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip);
   } else {
-    if (g3m) zgemm_kernel<T, BM, BN, BK, WM, WN, false, true><<<tiles, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip);
-    else zgemm_kernel<T, BM, BN, BK, WM, WN, false, false><<<tiles, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip);
+    if (g3m) zgemm_kernel<T, BM, BN, BK, WM, WN, false, true><<<tiles, 256, lds_extra, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip);
+    else zgemm_kernel<T, BM, BN, BK, WM, WN, false, false><<<tiles, 256, lds_extra, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip);
   }
   return 0;
 }
 
+// Extra LDS of the look-ahead trailing update (bytes, HPRMAT_HIP_TRAIL_LDS;
+// default: enough to hold FP32 updates at three workgroups per CU).
+size_t trail_lds_extra(bool fp32) {
+  static long v = [] {
+    const char* e = getenv("HPRMAT_HIP_TRAIL_LDS");
+    return e ? atol(e) : -1L;
+  }();
+  if (v >= 0) return (size_t)v;
+  return fp32 ? 32768 : 0;
+}
+
+// Split-BF16 planes of L21 (nc x kb at L, row stride ld) and of U12^T (U12:
+// kb x nc at U, row stride ldu) into `split` (see bf16x3.hpp).
+template <class T>
+void split_operands(int nc, int kb, const cx<T>* L, int64_t ld, const cx<T>* U, int64_t ldu,
+                    unsigned short* split, hipStream_t st) {
+  if constexpr (std::is_same<T, float>::value) {
+    const size_t plane = (size_t)nc * 256;
+    const dim3 grid(256 / 32, (nc + 31) / 32);
+    split_bf16_kernel<false><<<grid, 256, 0, st>>>(nc, kb, L, ld, split, plane, 256);
+    split_bf16_kernel<true><<<grid, 256, 0, st>>>(nc, kb, U, ldu, split + 4 * plane, plane, 256);
+  }
+}
+
+// Trailing update C[:, 0:ncols] -= L21 U12[:, c0:c0+ncols] (C is the block
+// at column c0 of A22), with the FP32/FP64 complex GEMM or the split-BF16
+// three-product GEMM over the planes made by split_operands.
+template <class T>
+void trailing(bool bfx, int nc, int ncols, int c0, int kb, const cx<T>* L, int64_t ld,
+              const cx<T>* U, int64_t ldu, cx<T>* C, int64_t ldc, bool g3m,
+              const unsigned short* split, hipStream_t st, size_t lds_extra) {
+  if (nc <= 0 || ncols <= 0) return;
+  if constexpr (std::is_same<T, float>::value) {
+    if (bfx) {
+      constexpr int BM = 64, BN = 64;
+      const size_t plane = (size_t)nc * 256, lds = zgemm_bf16x3_lds_bytes<BM, BN>();
+      static bool attr = false;
+      if (!attr) { allow_big_lds(zgemm_bf16x3_kernel<BM, BN, 32, 32>); attr = true; }
+      const int tiles = ((nc + BM - 1) / BM) * ((ncols + BN - 1) / BN);
+      zgemm_bf16x3_kernel<BM, BN, 32, 32><<<tiles, 256, lds, st>>>(
+          nc, ncols, 256, split, plane, split + 4 * plane + (size_t)c0 * 256, plane, C, ldc,
+          -1.f, 1);
+      return;
+    }
+  }
+  gemm<T>(nc, ncols, kb, L, ld, false, U + c0, ldu, C, ldc, T(-1), 1, g3m, nullptr, st, lds_extra);
+}
+
 template <class T>
 int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>& w,
-          const Scratch& s, const LuPlan& plan, bool g3m, hipStream_t st, const SideStream* side) {
+          const Scratch& s, const LuPlan& plan, int gmode, hipStream_t st, const SideStream* side) {
   if (n <= 0) return 0;
   const int nb = plan.nb;
+  const bool g3m = gmode == GEMM_3M;
+  const bool bfx = gmode == GEMM_BF16X3 && std::is_same<T, float>::value;
+  if (bfx && !w.split) return -1;
   if (nb != 256) return -1;
   const bool la = side && side->stream && n > 2 * nb;  // look-ahead
   (void)hipGetLastError();  // start from a clean error state (checked at the end)
@@ -367,9 +423,10 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       // next panel, or the whole update when there is no look-ahead)
       const int c_next = (la && nc > nb) ? nb : nc;
       ProfRec* pr = (g_prof_on && c_next == nc) ? prof_slot() : nullptr;
+      if (bfx) split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, st);
       if (pr) (void)hipEventRecord(pr->a, st);
-      gemm<T>(nc, c_next, kb, Akk + (int64_t)kb * ld, ld, false, w.ustage, nc,
-              Akk + (int64_t)kb * ld + kb, ld, T(-1), 1, g3m, nullptr, st);
+      trailing<T>(bfx, nc, c_next, 0, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
+                  Akk + (int64_t)kb * ld + kb, ld, g3m, w.split, st, 0);
       if (pr) {
         (void)hipEventRecord(pr->b, st);
         pr->flops = 8.0 * (double)nc * (double)c_next * (double)kb;
@@ -381,8 +438,9 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
         cs = side->stream;
         ProfRec* pr2 = g_prof_on ? prof_slot() : nullptr;
         if (pr2) (void)hipEventRecord(pr2->a, cs);
-        gemm<T>(nc, nc - c_next, kb, Akk + (int64_t)kb * ld, ld, false, w.ustage + c_next, nc,
-                Akk + (int64_t)kb * ld + kb + c_next, ld, T(-1), 1, g3m, nullptr, cs);
+        trailing<T>(bfx, nc, nc - c_next, c_next, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
+                    Akk + (int64_t)kb * ld + kb + c_next, ld, g3m, w.split, cs,
+                    trail_lds_extra(sizeof(T) == 4));
         if (pr2) {
           (void)hipEventRecord(pr2->b, cs);
           pr2->flops = 8.0 * (double)nc * (double)(nc - c_next) * (double)kb;
@@ -512,11 +570,11 @@ int residual_check(int n, int nrhs, const double2* A64, int64_t lda, const doubl
 
 #define INST(T)                                                                               \
   template int getrf<T>(cx<T>*, int64_t, int, int*, int*, const FactorWork<T>&, const Scratch&, \
-                        const LuPlan&, bool, hipStream_t, const SideStream*);                  \
+                        const LuPlan&, int, hipStream_t, const SideStream*);                   \
   template int getrs<T>(const cx<T>*, int64_t, int, const cx<T>*, cx<T>*, int64_t, int, cx<T>*, \
                         const int*, const LuPlan&, bool, hipStream_t);                         \
   template int gemm<T>(int, int, int, const cx<T>*, int64_t, bool, const cx<T>*, int64_t,      \
-                       cx<T>*, int64_t, T, int, bool, const int*, hipStream_t);                \
+                       cx<T>*, int64_t, T, int, bool, const int*, hipStream_t, size_t);        \
   template int refine_solve<T>(const cx<T>*, int64_t, int, const cx<T>*, const int*,          \
                                const double2*, int64_t, const double2*, double2*, int, int,    \
                                double, int, RefineState*, double2*, cx<T>*, cx<T>*, double*,   \

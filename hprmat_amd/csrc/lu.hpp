@@ -45,11 +45,19 @@ inline size_t dinv64_elems(int n) { return (size_t)((n + 63) / 64) * 2 * 4096; }
 inline size_t dinv256_elems(int n) { return (size_t)((n + 255) / 256) * 2 * 65536; }
 inline size_t ustage_elems(int n) { return (size_t)256 * (n > 0 ? n : 1); }
 
+// Split-BF16 operand planes of one trailing update: L21 and U12^T, four
+// BF16 planes each (n rows of 256 elements).
+inline size_t split_elems(int n) { return (size_t)2 * 4 * 256 * (n > 0 ? n : 1); }
+
+// Arithmetic of the FP32 trailing update A22 -= L21 U12.
+enum GemmMode { GEMM_4M = 0, GEMM_3M = 1, GEMM_BF16X3 = 2 };
+
 template <class T> struct FactorWork {
   cx<T>* dinv64 = nullptr;
   cx<T>* dinv256 = nullptr;
   cx<T>* ustage = nullptr;
   int* perm = nullptr;
+  unsigned short* split = nullptr;   // GEMM_BF16X3 only
 };
 
 // Second stream of the look-ahead: the trailing update of all but the next
@@ -64,10 +72,13 @@ struct SideStream {
 // Device factorization P A = L U (row-major A, in place).  Also produces the
 // inverses of the 256 x 256 diagonal blocks of L and U (w.dinv256, used as
 // GEMM operands by the triangular solves) and the final row permutation
-// w.perm (perm[i] = source row of row i of P A) used by getrs.
+// w.perm (perm[i] = source row of row i of P A) used by getrs.  gmode
+// (GemmMode) selects the arithmetic of the trailing update; the panel and
+// U12 = L11^-1 A12 always use the plain complex product of T (or Gauss's
+// three products for GEMM_3M).
 template <class T>
 int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>& w,
-          const Scratch& s, const LuPlan& plan, bool gauss3m, hipStream_t st,
+          const Scratch& s, const LuPlan& plan, int gmode, hipStream_t st,
           const SideStream* side = nullptr);
 
 // Solve L U X = B in place for a right-hand side already permuted by perm
@@ -98,7 +109,7 @@ int diag_scale(int n, const double2* A, int64_t lda, double* s, hipStream_t st);
 template <class T>
 int gemm(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<T>* B,
          int64_t ldb, cx<T>* C, int64_t ldc, T alpha, int beta, bool gauss3m,
-         const int* skip, hipStream_t st);
+         const int* skip, hipStream_t st, size_t lds_extra = 0);
 
 // Mixed/plain refinement around resident factors.  A64 is the FP64 matrix in
 // column-major order (lda); B and X are row-major n x nrhs (ld = nrhs).

@@ -177,7 +177,7 @@ int hprmat_hip_r_to_s(int nch, int nr, const void* dX, int64_t ldx, const double
   zmatrices_kernel<<<(nopen * nopen + 255) / 256, 256, 0, st>>>(
       nch, nopen, (const double2*)dR, rmax, qk, open, (const double2*)co, (const double2*)cop, ZO, ZI);
   hpr::LuPlan plan;
-  if (hpr::getrf<double>(ZO, ld, nopen, ipiv, info, w, sc, plan, false, st)) return -1;
+  if (hpr::getrf<double>(ZO, ld, nopen, ipiv, info, w, sc, plan, hpr::GEMM_4M, st)) return -1;
   hpr::permute_cast<double, double>(nopen, nopen, perm, ZI, ZP, nullptr, st);
   if (hpr::getrs<double>(ZO, ld, nopen, w.dinv256, ZP, ld, nopen, ZY, nullptr, plan, false, st))
     return -1;

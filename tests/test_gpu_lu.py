@@ -25,12 +25,12 @@ def gpu_solve(a, b, method, max_refine=0, tol=1e-13, equilibrate=False):
 
 
 @pytest.mark.parametrize('n', [1, 7, 32, 33, 64, 255, 256, 257, 500, 1100])
-@pytest.mark.parametrize('method', ['fp64', 'fp32', 'fp32_3m'])
+@pytest.mark.parametrize('method', ['fp64', 'fp32', 'fp32_3m', 'bf16x3'])
 def test_solve_matches_lapack(n, method):
     a, b = rand_system(n, 5, n)
     x_ref, info = osolve.zgesv(a, b)
     assert info == 0
-    x, st = gpu_solve(a, b, method, max_refine=0 if method == 'fp64' else 10)
+    x, st = gpu_solve(a, b, method, max_refine=0 if method == 'fp64' else 12)
     err = np.abs(x - x_ref).max() / np.abs(x_ref).max()
     assert st['info'] == 0
     assert err <= 1e-12, (n, method, err, st)
@@ -45,6 +45,45 @@ def test_pivots_match_lapack_zgetrf(n):
     _, piv_ref, info = lapack.zgetrf(a)
     assert info == 0
     assert np.array_equal(piv, piv_ref + 1)
+
+
+@pytest.mark.parametrize('n', [700, 1500])
+def test_split_bf16_trailing_update_precision(n):
+    # A split-BF16 operand keeps about 17 significant bits (hi and lo carry 8
+    # each; the dropped lo*lo term is below 2^-17 relative), FP32 24: with
+    # the panel in FP32 and only the trailing update in BF16 products, the
+    # unrefined bf16x3 error lies clearly above the FP32 one but within two
+    # orders of magnitude, which also shows that the BF16 kernel ran.  No
+    # diagonal shift: the multipliers are O(1), so the update is as large as A22.
+    a, b = rand_system(n, 4, 31 + n, dd=0.0)
+    x_ref, _ = osolve.zgesv(a, b)
+    errs = {}
+    for method in ('fp32', 'bf16x3'):
+        x, st = gpu_solve(a, b, method, max_refine=0)
+        assert st['info'] == 0
+        errs[method] = np.abs(x - x_ref).max() / np.abs(x_ref).max()
+    assert 2 * errs['fp32'] < errs['bf16x3'] < 200 * errs['fp32'], errs
+
+
+@pytest.mark.parametrize('n', [16500, 20000])
+@pytest.mark.parametrize('method', ['fp64', 'fp32'])
+def test_first_panel_pivots_tall(n, method):
+    # The pivots of the first 256-column panel depend only on A[:, :256], so a
+    # tall-skinny LAPACK getrf pins them at sizes where the panel leaf runs on
+    # 65 .. 79 cooperating workgroups (index output: bit-exact).
+    g = torch.Generator(device='cuda').manual_seed(n)
+    a = torch.complex(torch.rand(n, n, generator=g, device='cuda', dtype=torch.float64) - 0.5,
+                      torch.rand(n, n, generator=g, device='cuda', dtype=torch.float64) - 0.5)
+    lu = solver.LU().factor(solver.colmajor(a), method)
+    piv = lu.pivots()[:256]
+    panel = a[:, :256].cpu().numpy()
+    if method == 'fp64':
+        _, piv_ref, info = lapack.zgetrf(panel)
+    else:
+        _, piv_ref, info = lapack.cgetrf(panel.astype(np.complex64))
+    assert info == 0
+    mism = np.nonzero(piv != piv_ref[:256] + 1)[0]
+    assert mism.size == 0, (n, method, mism[:10])
 
 
 @pytest.mark.parametrize('col', [0, 5, 63])
