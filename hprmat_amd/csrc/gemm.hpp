@@ -19,11 +19,10 @@
 #include "common.hpp"
 
 template <class T, int BM, int BN, int BK, int WM, int WN, bool A_COL, bool GAUSS3M>
-__global__ void __launch_bounds__(256)
-zgemm_kernel(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
-             const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
-             int64_t ldc, T alpha, int beta, const int* __restrict__ skip) {
-  if (skip && *skip) return;
+__device__ __forceinline__ void
+zgemm_tile(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
+           const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
+           int64_t ldc, T alpha, int beta, int t_in) {
   static_assert((BM / WM) * (BN / WN) == 4, "four waves per workgroup");
   static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 4 == 0, "tile shape");
   constexpr int NT = 256;
@@ -39,7 +38,7 @@ zgemm_kernel(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
   // XCD-aware tile order: workgroups b with equal b % 8 run on one XCD.
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   const int total = tiles_m * tiles_n;
-  int t = blockIdx.x;
+  int t = t_in;
   if ((total & 7) == 0) t = (t & 7) * (total >> 3) + (t >> 3);
   constexpr int GROUP = 8;
   const int group = t / (GROUP * tiles_n);
@@ -170,6 +169,45 @@ zgemm_kernel(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
           *pc = mkc<T>(c0.x + alpha * pr, c0.y + alpha * pi);
         }
       }
+}
+
+
+template <class T, int BM, int BN, int BK, int WM, int WN, bool A_COL, bool GAUSS3M>
+__global__ void __launch_bounds__(256)
+zgemm_kernel(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
+             const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
+             int64_t ldc, T alpha, int beta, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  zgemm_tile<T, BM, BN, BK, WM, WN, A_COL, GAUSS3M>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta,
+                                                   blockIdx.x);
+}
+
+// Two independent GEMM problems (row-major A) in one launch: workgroups
+// [0, tiles1) take problem 1, the rest problem 2.  Used where two products
+// of one block step read the same operand and write disjoint outputs.
+template <class T> struct GemmProb {
+  int M, N, K;
+  const cx<T>* A;
+  int64_t lda;
+  const cx<T>* B;
+  int64_t ldb;
+  cx<T>* C;
+  int64_t ldc;
+  T alpha;
+  int beta;
+};
+
+template <class T, int BM, int BN, int BK, int WM, int WN>
+__global__ void __launch_bounds__(256)
+zgemm_dual_kernel(GemmProb<T> p1, GemmProb<T> p2, int tiles1, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  const int t = blockIdx.x;
+  if (t < tiles1)
+    zgemm_tile<T, BM, BN, BK, WM, WN, false, false>(p1.M, p1.N, p1.K, p1.A, p1.lda, p1.B, p1.ldb,
+                                                   p1.C, p1.ldc, p1.alpha, p1.beta, t);
+  else
+    zgemm_tile<T, BM, BN, BK, WM, WN, false, false>(p2.M, p2.N, p2.K, p2.A, p2.lda, p2.B, p2.ldb,
+                                                   p2.C, p2.ldc, p2.alpha, p2.beta, t - tiles1);
 }
 
 // ---------------------------------------------------------------------------

@@ -368,6 +368,48 @@ void trailing(bool bfx, int nc, int ncols, int c0, int kb, const cx<T>* L, int64
 }
 
 template <class T>
+void gemm_dual(const GemmProb<T>& p1, const GemmProb<T>& p2, const int* skip, hipStream_t st) {
+  constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32;
+  auto tiles = [](const GemmProb<T>& p) {
+    return (p.M > 0 && p.N > 0 && p.K > 0) ? ((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN) : 0;
+  };
+  const int t1 = tiles(p1), t2 = tiles(p2);
+  if (t1 + t2 == 0) return;
+  zgemm_dual_kernel<T, BM, BN, BK, WM, WN><<<t1 + t2, 256, 0, st>>>(p1, p2, t1, skip);
+}
+
+template <class T>
+GemmProb<T> prob(int M, int N, int K, const cx<T>* A, int64_t lda, const cx<T>* B, int64_t ldb,
+                 cx<T>* C, int64_t ldc, T alpha, int beta) {
+  GemmProb<T> p;
+  p.M = M; p.N = N; p.K = K; p.A = A; p.lda = lda; p.B = B; p.ldb = ldb; p.C = C; p.ldc = ldc;
+  p.alpha = alpha; p.beta = beta;
+  return p;
+}
+
+// Fold the inverses of the unit-lower diagonal blocks into L once per
+// factorization: for every 256-column block b, L[>b, b] <- L[>b, b] inv(L_bb),
+// so that each block step of the forward substitution is one two-problem
+// launch (inv(L_bb) B_b and B_>b -= W B_b read the same B_b).  L_bb has a
+// unit diagonal and |l_ij| <= 1, so inv(L_bb) is well conditioned; U is not
+// folded (its diagonal spans many orders of magnitude on Lagrange-mesh
+// matrices, and U[<b, b] inv(U_bb) in FP32 would cost refinement sweeps).
+// Row-wise transform, staged through `tmp` (256 x n elements), copied back.
+template <class T>
+void fold_inverses(cx<T>* A, int64_t ld, int n, const cx<T>* dinv256, cx<T>* tmp, hipStream_t st) {
+  for (int b0 = 0; b0 < n; b0 += 256) {
+    const int bs = min(256, n - b0), ml = n - b0 - bs;
+    if (ml <= 0) break;
+    const cx<T>* iL = dinv256 + (size_t)(b0 / 256) * 2 * 65536;
+    cx<T>* Lb = A + (int64_t)(b0 + bs) * ld + b0;
+    gemm<T>(ml, bs, bs, Lb, ld, false, iL, 256, tmp, bs, T(1), 0, false, nullptr, st);
+    const size_t cnt = (size_t)ml * bs;
+    copy2d_kernel<T><<<(int)min((size_t)4096, (cnt + 255) / 256), 256, 0, st>>>(ml, bs, tmp, bs,
+                                                                                 Lb, ld);
+  }
+}
+
+template <class T>
 int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>& w,
           const Scratch& s, const LuPlan& plan, int gmode, hipStream_t st, const SideStream* side) {
   if (n <= 0) return 0;
@@ -456,6 +498,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
     }
   }
   if (rest_pending) HPR_CHECK(hipStreamWaitEvent(st, side->rest, 0));
+  fold_inverses<T>(A, ld, n, w.dinv256, w.ustage, st);
   const size_t perm_lds = (size_t)n * sizeof(int);
   const int in_lds = perm_lds <= 140 * 1024;
   static bool attr = false;
@@ -470,16 +513,26 @@ int getrs(const cx<T>* LU, int64_t ld, int n, const cx<T>* dinv256, cx<T>* B, in
           int nrhs, cx<T>* Y, const int* skip, const LuPlan& plan, bool g3m, hipStream_t st) {
   if (n <= 0 || nrhs <= 0) return 0;
   const int rs = 256;
-  // forward: Y_b = inv(L_bb) B_b, B_rest -= L_rest,b Y_b
+  // L holds W = L[>b, b] inv(L_bb) (see fold_inverses).  Forward:
+  // Y_b = inv(L_bb) B_b and B_>b -= W B_b, one launch per block; backward:
+  // B_b = inv(U_bb) Y_b, then Y_<b -= U[<b, b] B_b.
   for (int b = 0; b < n; b += rs) {
     const int bs = min(rs, n - b);
-    gemm<T>(bs, nrhs, bs, dinv256 + (size_t)(b / 256) * 2 * 65536, 256, false, B + (int64_t)b * ldb,
-            ldb, Y + (int64_t)b * ldb, ldb, T(1), 0, g3m, skip, st);
-    if (b + bs < n)
-      gemm<T>(n - b - bs, nrhs, bs, LU + (int64_t)(b + bs) * ld + b, ld, false, Y + (int64_t)b * ldb,
-              ldb, B + (int64_t)(b + bs) * ldb, ldb, T(-1), 1, g3m, skip, st);
+    const cx<T>* iL = dinv256 + (size_t)(b / 256) * 2 * 65536;
+    const GemmProb<T> p1 = prob<T>(bs, nrhs, bs, iL, 256, B + (int64_t)b * ldb, ldb,
+                                   Y + (int64_t)b * ldb, ldb, T(1), 0);
+    const GemmProb<T> p2 = prob<T>(n - b - bs, nrhs, bs, LU + (int64_t)(b + bs) * ld + b, ld,
+                                   B + (int64_t)b * ldb, ldb, B + (int64_t)(b + bs) * ldb, ldb,
+                                   T(-1), 1);
+    if (g3m) {
+      gemm<T>(p1.M, p1.N, p1.K, p1.A, p1.lda, false, p1.B, p1.ldb, p1.C, p1.ldc, p1.alpha,
+              p1.beta, true, skip, st);
+      gemm<T>(p2.M, p2.N, p2.K, p2.A, p2.lda, false, p2.B, p2.ldb, p2.C, p2.ldc, p2.alpha,
+              p2.beta, true, skip, st);
+    } else {
+      gemm_dual<T>(p1, p2, skip, st);
+    }
   }
-  // backward: B_b = inv(U_bb) Y_b, Y_above -= U_above,b B_b
   for (int b = ((n - 1) / rs) * rs; b >= 0; b -= rs) {
     const int bs = min(rs, n - b);
     gemm<T>(bs, nrhs, bs, dinv256 + ((size_t)(b / 256) * 2 + 1) * 65536, 256, false,
