@@ -14,6 +14,9 @@ Methods timed back to back (name: factorization, refinement):
   fp32_ir5  the same factorization with at most 5 FP64 sweeps;
   bf16x3    FP32 LU whose trailing update uses split-BF16 three-product
             MFMA arithmetic, FP64 refinement to the gate (<= 30 sweeps);
+  fp32_gmres  the fp32 factorization with flexible-GMRES refinement (FP64
+            Arnoldi, the FP32 factors as preconditioner, <= 24 products);
+  bf16x3_gmres  the bf16x3 factorization with flexible-GMRES refinement;
   fp32_3m   complex-FP32 LU with Gauss's three-product complex GEMM.
 Each method's solution is compared with the fp64 method's (max relative
 error of X and max |dS| of the S-matrix formed from it on the device).
@@ -31,11 +34,12 @@ import sys
 import time
 
 METRIC = "solves/s & TFLOP/s, N=25600 complex LU+solve: mixed vs FP64 at resid≤1e-12"
-PEAK = {'fp64': 78.6, 'fp32': 157.3, 'fp32_ir5': 157.3, 'fp32_3m': 157.3,
-        'bf16x3': 2500.0}   # dense MFMA TFLOP/s, MI355X (bf16x3: BF16 matrix peak)
+PEAK = {'fp64': 78.6, 'fp32': 157.3, 'fp32_ir5': 157.3, 'fp32_gmres': 157.3, 'fp32_3m': 157.3,
+        'bf16x3': 2500.0, 'bf16x3_gmres': 2500.0}   # dense MFMA TFLOP/s, MI355X (bf16x3: BF16 matrix peak)
 # name -> (factorization method, max refinement sweeps)
-SPECS = {'fp64': ('fp64', 10), 'fp32': ('fp32', 30), 'fp32_ir5': ('fp32', 5),
-         'bf16x3': ('bf16x3', 30), 'fp32_3m': ('fp32_3m', 30)}
+SPECS = {'fp64': ('fp64', 10, 'ir'), 'fp32': ('fp32', 30, 'ir'), 'fp32_ir5': ('fp32', 5, 'ir'),
+         'fp32_gmres': ('fp32', 24, 'fgmres'), 'bf16x3': ('bf16x3', 30, 'ir'),
+         'bf16x3_gmres': ('bf16x3', 32, 'fgmres'), 'fp32_3m': ('fp32_3m', 30, 'ir')}
 
 
 def flops_solve(n, nrhs):
@@ -82,7 +86,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--nch', type=int, default=128)
     ap.add_argument('--nr', type=int, default=200)
-    ap.add_argument('--methods', default='fp64,fp32,fp32_ir5,bf16x3')
+    ap.add_argument('--methods', default='fp64,fp32_gmres,fp32,fp32_ir5,bf16x3,bf16x3_gmres')
     ap.add_argument('--no-cpu', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1:
@@ -117,13 +121,13 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
     x_ref = s_ref = None
     for m in args.methods.split(','):
         lu = solver.LU()
-        fm, max_refine = SPECS[m]
+        fm, max_refine, refinement = SPECS[m]
         b = b0.clone()
 
         def step():
             b.copy_(b0)
             lu.factor(a_cm, fm, equilibrate=True)
-            lu.solve_(b, max_refine, 1e-12, 'rel_fro', True, stats=False)
+            lu.solve_(b, max_refine, 1e-12, 'rel_fro', True, stats=False, refinement=refinement)
 
         for _ in range(args.warmup):
             step()
@@ -144,7 +148,7 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
         # verification pass (outside the timed region): same inputs, with statistics
         b.copy_(b0)
         lu.factor(a_cm, fm, equilibrate=True)
-        st = lu.solve_(b, max_refine, 1e-12, 'rel_fro', True, stats=True)
+        st = lu.solve_(b, max_refine, 1e-12, 'rel_fro', True, stats=True, refinement=refinement)
         _, s_m = rmatrix.r_to_s(dmesh, b, nrhs, sysm.qk, open_idx, co, cop)
         if m == 'fp64':
             x_ref, s_ref = b.clone(), s_m.clone()
@@ -177,7 +181,7 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
     roof = {'bound': 'mfma', 'achieved': g['tflops'], 'peak': PEAK[head],
             'unit': 'TFLOP/s', 'frac': g['tflops'] / PEAK[head], 'traffic': None,
             'kernel': '%s (trailing update A22 -= L21 U12, %s)' % (
-                'zgemm_bf16x3_kernel' if head == 'bf16x3' else 'zgemm_kernel', head),
+                'zgemm_bf16x3_kernel' if head.startswith('bf16x3') else 'zgemm_kernel', head),
             'flops_per_launch': g['flops_per_launch'], 'ms_per_launch': g['ms_per_launch']}
     out = {
         'metric': METRIC, 'value': hm['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1,

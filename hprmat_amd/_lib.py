@@ -18,6 +18,7 @@ SPLIT_BF16 = 3
 EQUILIBRATE = 0x100
 CRIT_ABS_MAX = 0
 CRIT_REL_FRO = 1
+FGMRES = 0x10            # flag OR-ed into the criterion: flexible GMRES refinement
 
 REFERENCE_SYMBOLS = (
     'gpu_solver_init_', 'gpu_solver_finalize_', 'gpu_host_unregister_',

@@ -27,6 +27,8 @@ struct hprmat_hip_lu {
   size_t ustage_cap = 0;
   void* split = nullptr;      // split-BF16 operand planes (HPRMAT_SPLIT_BF16)
   size_t split_cap = 0;
+  void* gm = nullptr;         // FGMRES workspace (HPRMAT_FGMRES solves)
+  size_t gm_cap = 0;
   int* perm = nullptr;        // final row permutation
   size_t perm_cap = 0;
   double* scale = nullptr;    // equilibration factors (when equil)
@@ -91,7 +93,7 @@ void free_lu(hprmat_hip_lu* h) {
     (void)hipEventDestroy(h->side.rest);
     (void)hipStreamDestroy(h->side.stream);
   }
-  void* ps[] = {h->fac, h->ipiv, h->dinfo, h->dinv64, h->dinv256, h->ustage, h->split,
+  void* ps[] = {h->fac, h->ipiv, h->dinfo, h->dinv64, h->dinv256, h->ustage, h->split, h->gm,
                 h->perm, h->scale, h->scr, h->Brm, h->Xrm, h->R, h->RF, h->RY, h->part,
                 h->state};
   for (void* p : ps)
@@ -200,7 +202,26 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
     return -1;
   int rc;
   const bool g3m = h->method == HPRMAT_MIXED_FP32_3M;
-  if (h->method == HPRMAT_PLAIN_FP64)
+  const bool fgmres = (crit & HPRMAT_FGMRES) != 0;
+  crit &= ~HPRMAT_FGMRES;
+  if (fgmres) {
+    const int m = max_refine < 1 ? 1 : (max_refine > hpr::GMRES_MAX_M ? hpr::GMRES_MAX_M : max_refine);
+    if (nrhs > 1024) return -1;
+    if (grow(&h->gm, &h->gm_cap, hpr::gmres_work_bytes(n, nrhs, m))) return -2;
+    hpr::GmresWork gw;
+    hpr::gmres_work_bind(gw, h->gm, n, nrhs, m);
+    if (h->method == HPRMAT_PLAIN_FP64)
+      rc = hpr::gmres_solve<double>((const double2*)h->fac, h->ld, n, (const double2*)h->dinv256,
+                                    h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, m,
+                                    tol, crit, h->state, h->R, (double2*)h->RF, (double2*)h->RY,
+                                    h->part, gw, h->plan, false, st, h->equil ? h->scale : nullptr);
+    else
+      rc = hpr::gmres_solve<float>((const float2*)h->fac, h->ld, n, (const float2*)h->dinv256,
+                                   h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, m,
+                                   tol, crit, h->state, h->R, (float2*)h->RF, (float2*)h->RY,
+                                   h->part, gw, h->plan, g3m, st, h->equil ? h->scale : nullptr);
+    final_check = 0;   // the FGMRES solve ends with a true residual evaluation
+  } else if (h->method == HPRMAT_PLAIN_FP64)
     rc = hpr::refine_solve<double>((const double2*)h->fac, h->ld, n, (const double2*)h->dinv256,
                                    h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, tol,
                                    crit, h->state, h->R, (double2*)h->RF, (double2*)h->RY,

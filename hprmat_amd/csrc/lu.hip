@@ -196,6 +196,7 @@ norm_final_kernel(int nblocks, const double* __restrict__ part, RefineState* __r
       st->b_fro = fro;
       st->b_max = sqrt(sa[0]);
       st->done = 0; st->sweeps = 0; st->checks = 0;
+      st->gm_halt = 0; st->gm_steps = 0;
       st->res_max = 0.0; st->res_fro = 0.0; st->res_absmax = 0.0;
     } else {
       st->res_max = sm[0];
@@ -572,8 +573,8 @@ int transpose_cast(int rows, int cols, const cx<Tin>* in, int64_t ldi, cx<Tout>*
   return 0;
 }
 
-static int launch_norms(int n, int nrhs, const double2* R, double* part, RefineState* state,
-                        int mode, int crit, double tol, const int* skip, hipStream_t st) {
+int residual_norms(int n, int nrhs, const double2* R, double* part, RefineState* state, int mode,
+                   int crit, double tol, const int* skip, hipStream_t st) {
   size_t total = (size_t)n * nrhs;
   int blocks = (int)min((size_t)NORM_BLOCKS, (total + 255) / 256);
   if (blocks < 1) blocks = 1;
@@ -591,14 +592,14 @@ int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, con
   const size_t cnt = (size_t)n * nrhs;
   const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
   const int* done = &state->done;
-  launch_norms(n, nrhs, B, part, state, 0, crit, tol, nullptr, st);
+  residual_norms(n, nrhs, B, part, state, 0, crit, tol, nullptr, st);
   permute_cast<double, TF>(n, nrhs, perm, B, RF, nullptr, st, scale);
   if (getrs<TF>(LU, ld, n, dinv256, RF, nrhs, nrhs, RY, nullptr, plan, g3m, st)) return -1;
   add_correction_kernel<TF><<<eg, 256, 0, st>>>(nrhs, cnt, X, RF, scale, 0, state);
   for (int it = 0; it < max_refine; ++it) {
     cast_kernel<double, double><<<eg, 256, 0, st>>>(cnt, B, R, done);
     gemm<double>(n, nrhs, n, A64, lda, true, X, nrhs, R, nrhs, -1.0, 1, false, done, st);
-    launch_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
+    residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
     permute_cast<double, TF>(n, nrhs, perm, R, RF, done, st, scale);
     if (getrs<TF>(LU, ld, n, dinv256, RF, nrhs, nrhs, RY, done, plan, g3m, st)) return -1;
     add_correction_kernel<TF><<<eg, 256, 0, st>>>(nrhs, cnt, X, RF, scale, 1, state);
@@ -616,7 +617,7 @@ int residual_check(int n, int nrhs, const double2* A64, int64_t lda, const doubl
   const int* done = &state->done;
   cast_kernel<double, double><<<eg, 256, 0, st>>>(cnt, B, R, done);
   gemm<double>(n, nrhs, n, A64, lda, true, X, nrhs, R, nrhs, -1.0, 1, false, done, st);
-  launch_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
+  residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
   HPR_CHECK(hipGetLastError());
   return 0;
 }

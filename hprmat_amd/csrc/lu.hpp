@@ -31,6 +31,8 @@ struct RefineState {
   double b_max;      // largest |b_ij|
   double res_absmax; // largest |r_ij|
   double hist[32];   // ||r||_F / ||b||_F after each residual evaluation
+  int gm_halt;       // FGMRES: no further Arnoldi step in this cycle (or done)
+  int gm_steps;      // FGMRES: Arnoldi steps taken in this cycle
 };
 
 enum Criterion { CRIT_ABS_MAX = 0, CRIT_REL_FRO = 1 };
@@ -129,6 +131,44 @@ int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, con
 int residual_check(int n, int nrhs, const double2* A64, int64_t lda, const double2* B,
                    const double2* X, double2* R, double* part, RefineState* state, int crit,
                    double tol, hipStream_t st);
+
+// Norms of an n x nrhs FP64 block R (row-major): mode 0 records ||B|| and
+// resets the state, mode 1 records a residual evaluation and the stopping
+// rule (skipped when *skip is set; mode 1 also when state->done).
+int residual_norms(int n, int nrhs, const double2* R, double* part, RefineState* state, int mode,
+                   int crit, double tol, const int* skip, hipStream_t st);
+
+// Workspace of gmres_solve for n x nrhs blocks and cycles of m <= 16 steps.
+struct GmresWork {
+  double2* V = nullptr;     // (m + 1) Krylov blocks, n x nrhs each
+  double2* Z = nullptr;     // m preconditioned blocks
+  double2* W = nullptr;     // n x nrhs
+  double2* H = nullptr;     // (m + 1) x m x nrhs rotated Hessenberg columns
+  double2* g = nullptr;     // (m + 1) x nrhs
+  double2* sn = nullptr;    // m x nrhs
+  double* cs = nullptr;     // m x nrhs
+  double2* h = nullptr;     // 2 x (m + 1) x nrhs projections (two CGS passes)
+  double2* y = nullptr;     // m x nrhs
+  double* inv = nullptr;    // nrhs
+  double2* part = nullptr;  // GMRES_DOT_BLOCKS x (m + 1) x nrhs partial dots
+};
+constexpr int GMRES_MAX_M = 16;
+constexpr int GMRES_DOT_BLOCKS = 256;
+size_t gmres_work_bytes(int n, int nrhs, int m);
+void gmres_work_bind(GmresWork& w, void* base, int n, int nrhs, int m);
+
+// Flexible GMRES iterative refinement (FGMRES-IR, right preconditioning by
+// the resident LU factors in their own precision, FP64 matrix-vector
+// products and orthogonalization).  Arguments as refine_solve; max_steps
+// bounds the preconditioned matrix-vector products (cycles of m steps, each
+// cycle starting from the true residual B - A X).  X ends with a true
+// residual evaluation recorded in state.  Nothing synchronises with the host.
+template <class TF>
+int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, const int* perm,
+                const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
+                int max_steps, int m, double tol, int crit, RefineState* state, double2* R,
+                cx<TF>* RF, cx<TF>* RY, double* part, const GmresWork& gw, const LuPlan& plan,
+                bool gauss3m, hipStream_t st, const double* scale);
 
 // HIP-event timing of getrf's trailing-update GEMM launches (for roofline
 // reporting): enable, run, then read total ms / algorithmic flops / count.

@@ -128,20 +128,21 @@ def rhs_colmajor(dmesh, nch):
     return b
 
 
-# solver_type -> (method, max_refine, tol, criterion, equilibrate).  Types 1/2
-# keep the semantics of the reference's CPU types (plain FP64; FP32 + FP64
-# refinement with ||r||_F/||b||_F < 1e-10, two sweeps) but run on the GPU;
-# type 3 (Woodbury, CPU-only) is not provided.  'plain' and 'mixed' are this
-# library's refined methods: FP64 or FP32 factors of the equilibrated matrix,
-# FP64 refinement until ||r||_F/||b||_F <= 1e-12.
+# solver_type -> (method, max_refine, tol, criterion, equilibrate, refinement).
+# Types 1/2 keep the semantics of the reference's CPU types (plain FP64; FP32
+# + FP64 refinement with ||r||_F/||b||_F < 1e-10, two sweeps) but run on the
+# GPU; type 3 (Woodbury, CPU-only) is not provided.  'plain' and 'mixed' are
+# this library's refined methods: FP64 factors with classic refinement, or
+# FP32 factors with flexible-GMRES refinement, both of the equilibrated
+# matrix, until ||r||_F/||b||_F <= 1e-12.
 SOLVER_TYPES = {
-    1: ('fp64', 0, 0.0, 'rel_fro', False),
-    2: ('fp32', 2, 1.0e-10, 'rel_fro', False),
-    4: ('fp32', None, 1.0e-10, 'abs_max', False),
-    5: ('fp32_3m', 5, 1.0e-12, 'abs_max', False),
-    6: ('fp32', 2, 1.0e-12, 'abs_max', False),
-    'plain': ('fp64', 5, 1.0e-12, 'rel_fro', True),
-    'mixed': ('fp32', 30, 1.0e-12, 'rel_fro', True),
+    1: ('fp64', 0, 0.0, 'rel_fro', False, 'ir'),
+    2: ('fp32', 2, 1.0e-10, 'rel_fro', False, 'ir'),
+    4: ('fp32', None, 1.0e-10, 'abs_max', False, 'ir'),
+    5: ('bf16x3', 5, 1.0e-12, 'abs_max', False, 'ir'),
+    6: ('fp32', 2, 1.0e-12, 'abs_max', False, 'ir'),
+    'plain': ('fp64', 5, 1.0e-12, 'rel_fro', True, 'ir'),
+    'mixed': ('fp32', 24, 1.0e-12, 'rel_fro', True, 'fgmres'),
 }
 
 
@@ -149,11 +150,11 @@ def solve_colmajor(a_cm, b_cm, solver_type=1, max_refine=0):
     """Solve C X = B in place (column-major storages) for a solver_type."""
     if solver_type not in SOLVER_TYPES:
         raise NotImplementedError('solver_type %r is not provided by libhprmat_hip' % (solver_type,))
-    method, mr, tol, crit, eq = SOLVER_TYPES[solver_type]
+    method, mr, tol, crit, eq, refinement = SOLVER_TYPES[solver_type]
     if mr is None:
         mr = max_refine
     lu = solver.LU().factor(a_cm, method, eq)
-    st = lu.solve_(b_cm, mr, tol, crit, final_check=eq)
+    st = lu.solve_(b_cm, mr, tol, crit, final_check=eq, refinement=refinement)
     if st['info'] != 0:
         raise RuntimeError('zero pivot at column %d' % st['info'])
     return st

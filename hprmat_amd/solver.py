@@ -90,14 +90,22 @@ class LU:
         return out
 
     def solve_(self, b_cm, max_refine=0, tol=0.0, criterion='rel_fro', final_check=False,
-               stats=True):
-        """Solve in place; ``b_cm`` is the column-major storage of B (nrhs x n tensor)."""
+               stats=True, refinement='ir'):
+        """Solve in place; ``b_cm`` is the column-major storage of B (nrhs x n tensor).
+
+        ``refinement``: 'ir' (classic iterative refinement, up to max_refine
+        corrections) or 'fgmres' (flexible GMRES with the factors as
+        preconditioner, up to max_refine preconditioned products).
+        """
         _check_tensor(b_cm, 'b_cm')
         if b_cm.dim() != 2 or b_cm.shape[1] != self.n:
             raise ValueError('b_cm must have shape (nrhs, n)')
+        if refinement not in ('ir', 'fgmres'):
+            raise ValueError('refinement must be ir or fgmres')
+        crit = CRITERIA[criterion] | (_lib.FGMRES if refinement == 'fgmres' else 0)
         st = _lib.Stats() if stats else None
         rc = self._lib.hprmat_hip_lu_solve(self._h, ctypes.c_void_p(b_cm.data_ptr()), self.n,
-                                           b_cm.shape[0], max_refine, tol, CRITERIA[criterion],
+                                           b_cm.shape[0], max_refine, tol, crit,
                                            int(final_check), _stream_ptr(),
                                            ctypes.byref(st) if st is not None else None)
         _lib.check(rc, 'hprmat_hip_lu_solve')
@@ -105,7 +113,7 @@ class LU:
 
 
 def solve(a, b, method='fp64', max_refine=0, tol=1e-12, criterion='rel_fro', final_check=True,
-          symmetric=False, equilibrate=False):
+          symmetric=False, equilibrate=False, refinement='ir'):
     """X = A^-1 B for complex128 CUDA tensors a (n x n) and b (n x nrhs).
 
     Returns (X, stats) with X an (n x nrhs) tensor.
@@ -114,7 +122,7 @@ def solve(a, b, method='fp64', max_refine=0, tol=1e-12, criterion='rel_fro', fin
     a_cm = colmajor(a, symmetric)
     b_cm = b.transpose(0, 1).contiguous()
     lu = LU().factor(a_cm, method, equilibrate)
-    st = lu.solve_(b_cm, max_refine, tol, criterion, final_check)
+    st = lu.solve_(b_cm, max_refine, tol, criterion, final_check, refinement=refinement)
     if st['info'] != 0:
         raise RuntimeError('zero pivot at column %d' % st['info'])
     return b_cm.transpose(0, 1), st

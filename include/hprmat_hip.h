@@ -97,7 +97,12 @@ enum {
 /* Stopping rule of the refinement. */
 enum {
   HPRMAT_CRIT_ABS_MAX = 0,   /* |r_ij| of the largest |Re|+|Im| entry < tol  */
-  HPRMAT_CRIT_REL_FRO = 1    /* ||B - A X||_F / ||B||_F <= tol               */
+  HPRMAT_CRIT_REL_FRO = 1,   /* ||B - A X||_F / ||B||_F <= tol               */
+  /* flag OR-ed into the rule: flexible GMRES refinement (FP64 Arnoldi per
+     right-hand side, the resident factors as right preconditioner, cycles of
+     min(max_refine, 16) steps); max_refine then bounds the preconditioned
+     matrix-vector products, `sweeps` counts them */
+  HPRMAT_FGMRES = 0x10
 };
 
 typedef struct hprmat_hip_stats {
