@@ -29,8 +29,8 @@ REFERENCE_SYMBOLS = (
 NATIVE_SYMBOLS = (
     'hprmat_hip_lu_create', 'hprmat_hip_lu_destroy', 'hprmat_hip_lu_factor',
     'hprmat_hip_lu_info', 'hprmat_hip_lu_pivots', 'hprmat_hip_lu_solve', 'hprmat_hip_solve',
-    'hprmat_hip_assemble', 'hprmat_hip_r_to_s', 'hprmat_hip_prof_enable',
-    'hprmat_hip_prof_read', 'hprmat_hip_version',
+    'hprmat_hip_assemble', 'hprmat_hip_r_to_s', 'hprmat_hip_adjoint_contract',
+    'hprmat_hip_prof_enable', 'hprmat_hip_prof_read', 'hprmat_hip_version',
 )
 
 
@@ -89,6 +89,7 @@ def load():
     DPtr = ctypes.c_void_p
     lib.hprmat_hip_assemble.argtypes = [I, I, DPtr, DPtr, DPtr, D, DPtr, DPtr, DPtr, DPtr, DPtr, L, P]
     lib.hprmat_hip_r_to_s.argtypes = [I, I, DPtr, L, DPtr, D, D, DPtr, I, DPtr, DPtr, DPtr, DPtr, DPtr, P]
+    lib.hprmat_hip_adjoint_contract.argtypes = [I, I, I, DPtr, L, DPtr, L, DPtr, P]
     lib.hprmat_hip_lu_solve.argtypes = [P, P, L, I, I, D, I, I, P, ctypes.POINTER(Stats)]
     lib.hprmat_hip_solve.argtypes = [I, P, L, P, L, I, I, I, D, I, I, P, ctypes.POINTER(Stats)]
     lib.hprmat_hip_version.restype = ctypes.c_char_p

@@ -108,6 +108,27 @@ struct SmallWork {
 
 }  // namespace
 
+// K[i][c][c'] = sum_k M[(c nr + i) + k ldm] X[(c' nr + i) + k ldx]: the
+// contraction of the adjoint and forward solutions over the right-hand sides
+// at every mesh point, one workgroup per mesh point i.
+__global__ void __launch_bounds__(256)
+adjoint_contract_kernel(int nch, int nr, int nrhs, const double2* __restrict__ M, int64_t ldm,
+                        const double2* __restrict__ X, int64_t ldx, double2* __restrict__ K) {
+  const int i = blockIdx.x;
+  for (int q = threadIdx.x; q < nch * nch; q += blockDim.x) {
+    const int c = q / nch, d = q % nch;
+    const double2* m = M + (int64_t)c * nr + i;
+    const double2* x = X + (int64_t)d * nr + i;
+    double sr = 0.0, si = 0.0;
+    for (int k = 0; k < nrhs; ++k) {
+      const double2 a = m[(int64_t)k * ldm], b = x[(int64_t)k * ldx];
+      sr += a.x * b.x - a.y * b.y;
+      si += a.x * b.y + a.y * b.x;
+    }
+    K[((int64_t)i * nch + c) * nch + d] = make_double2(sr, si);
+  }
+}
+
 extern "C" {
 
 /* Assemble the complex-symmetric Lagrange-mesh matrix of one interval into
@@ -182,6 +203,16 @@ int hprmat_hip_r_to_s(int nch, int nr, const void* dX, int64_t ldx, const double
   if (hpr::getrs<double>(ZO, ld, nopen, w.dinv256, ZP, ld, nopen, ZY, nullptr, plan, false, st))
     return -1;
   store_colmajor_kernel<<<(nopen * nopen + 255) / 256, 256, 0, st>>>(nopen, ZP, (double2*)dS);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int hprmat_hip_adjoint_contract(int nch, int nr, int nrhs, const void* dM, int64_t ldm,
+                                const void* dX, int64_t ldx, void* dK, void* stream) {
+  if (nch <= 0 || nr <= 0 || nrhs < 0 || ldm < (int64_t)nch * nr || ldx < (int64_t)nch * nr)
+    return -1;
+  hipStream_t st = (hipStream_t)stream;
+  adjoint_contract_kernel<<<nr, 256, 0, st>>>(nch, nr, nrhs, (const double2*)dM, ldm,
+                                              (const double2*)dX, ldx, (double2*)dK);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -180,6 +180,17 @@ int hprmat_hip_r_to_s(int nch, int nr, const void *dX, int64_t ldx,
                       const void *co, const void *cop, void *dR, void *dS,
                       void *stream);
 
+/* No reference counterpart (the reference has no derivatives): adjoint
+   contraction for gradients with respect to a local coupling potential.
+   With X = A^-1 B and M = A^-T G (for complex-symmetric A the same factors
+   solve it), a real loss with torch-convention gradient G = dL/dconj(X)
+   has dL/dconj(V_cc'(r_i)) = -conj(K[i][c][c']) where
+     K[i][c][c'] = sum_k M[(c nr + i), k] X[(c' nr + i), k].
+   dM, dX: column-major N x nrhs (ldm, ldx), N = nch nr; dK: nr x nch x nch
+   complex (row-major), all on the device. */
+int hprmat_hip_adjoint_contract(int nch, int nr, int nrhs, const void *dM, int64_t ldm,
+                                const void *dX, int64_t ldx, void *dK, void *stream);
+
 /* No reference counterpart: HIP-event timing of the LU's trailing-update
    GEMM (the dominant kernel) for roofline reports.  Enable, factor, then
    read the summed elapsed milliseconds, the algorithmic flops (8 M N K per

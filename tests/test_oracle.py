@@ -187,3 +187,37 @@ def test_refinement_rules():
     assert solve.izamax_modulus(b - a @ x2) < 1e-10 or used2 == 5
     xf, _ = solve.fp64_refined(a, b)
     assert solve.rel_residual(a, xf, b) <= 1e-12
+
+
+# ---- adjoint gradient of S-matrix losses (oracle/adjoint.py) ----------------
+
+def _fit_problem_small(nch=4, nr=16, energies=(1.5, 3.0, 6.0), seed=1):
+    from oracle import adjoint, synth
+    prob = adjoint.problem(nch, nr, 15.0, 3.0, seed, energies)
+    _, u = synth.draw(nch, 3.0, seed, 'term')
+    th_true = np.array(synth.params_from_u(u)).T
+    return prob, th_true, adjoint.s_matrices(prob, th_true)
+
+
+def test_adjoint_gradient_matches_central_differences():
+    # pins every factor and sign of the hand-derived adjoint chain
+    from oracle import adjoint
+    prob, th_true, tg = _fit_problem_small()
+    th = th_true * 1.05
+    _, g = adjoint.loss_and_grad(prob, th, tg)
+    fd = np.zeros((4, 4))
+    for t in range(4):
+        for p in range(4):
+            h = 1e-6 * max(1.0, abs(th[t, p]))
+            tp, tm = th.copy(), th.copy()
+            tp[t, p] += h
+            tm[t, p] -= h
+            fd[t, p] = (adjoint.loss(prob, tp, tg) - adjoint.loss(prob, tm, tg)) / (2 * h)
+    assert np.abs(g - fd).max() <= 1e-7 * np.abs(fd).max()
+
+
+def test_adjoint_loss_vanishes_at_target():
+    from oracle import adjoint
+    prob, th_true, tg = _fit_problem_small()
+    loss, g = adjoint.loss_and_grad(prob, th_true, tg)
+    assert loss == 0.0 and np.abs(g).max() == 0.0
