@@ -18,6 +18,8 @@ SPLIT_BF16 = 3
 EQUILIBRATE = 0x100
 CRIT_ABS_MAX = 0
 CRIT_REL_FRO = 1
+PLAN_TRAIL_TILE = 1      # hprmat_hip_lu_set_plan keys
+PLAN_LOOKAHEAD = 2
 FGMRES = 0x10            # flag OR-ed into the criterion: flexible GMRES refinement
 
 REFERENCE_SYMBOLS = (
@@ -27,9 +29,11 @@ REFERENCE_SYMBOLS = (
     'gpu_multi_init_', 'gpu_multi_finalize_', 'gpu_solve_multi_', 'gpu_solve_multi_mixed_',
 )
 NATIVE_SYMBOLS = (
-    'hprmat_hip_lu_create', 'hprmat_hip_lu_destroy', 'hprmat_hip_lu_factor',
+    'hprmat_hip_lu_create', 'hprmat_hip_lu_destroy', 'hprmat_hip_lu_set_plan',
+    'hprmat_hip_lu_factor',
     'hprmat_hip_lu_info', 'hprmat_hip_lu_pivots', 'hprmat_hip_lu_solve', 'hprmat_hip_solve',
-    'hprmat_hip_assemble', 'hprmat_hip_r_to_s', 'hprmat_hip_adjoint_contract',
+    'hprmat_hip_assemble', 'hprmat_hip_r_to_s', 'hprmat_hip_r_to_s_work_bytes',
+    'hprmat_hip_r_to_s_ws', 'hprmat_hip_adjoint_contract',
     'hprmat_hip_prof_enable', 'hprmat_hip_prof_read', 'hprmat_hip_version',
 )
 
@@ -83,6 +87,7 @@ def load():
     lib.gpu_get_info_.restype = None
     lib.hprmat_hip_lu_create.argtypes = [ctypes.POINTER(P)]
     lib.hprmat_hip_lu_destroy.argtypes = [P]
+    lib.hprmat_hip_lu_set_plan.argtypes = [P, I, I]
     lib.hprmat_hip_lu_factor.argtypes = [P, I, P, L, I, P]
     lib.hprmat_hip_lu_info.argtypes = [P, IP]
     lib.hprmat_hip_lu_pivots.argtypes = [P, IP]
@@ -90,6 +95,10 @@ def load():
     lib.hprmat_hip_assemble.argtypes = [I, I, DPtr, DPtr, DPtr, D, DPtr, DPtr, DPtr, DPtr, DPtr, L, P]
     lib.hprmat_hip_r_to_s.argtypes = [I, I, DPtr, L, DPtr, D, D, DPtr, I, DPtr, DPtr, DPtr, DPtr, DPtr, P]
     lib.hprmat_hip_adjoint_contract.argtypes = [I, I, I, DPtr, L, DPtr, L, DPtr, P]
+    lib.hprmat_hip_r_to_s_work_bytes.argtypes = [I]
+    lib.hprmat_hip_r_to_s_work_bytes.restype = ctypes.c_size_t
+    lib.hprmat_hip_r_to_s_ws.argtypes = [I, I, DPtr, L, DPtr, D, D, DPtr, I, DPtr, DPtr, DPtr, DPtr,
+                                         DPtr, DPtr, ctypes.c_size_t, P]
     lib.hprmat_hip_lu_solve.argtypes = [P, P, L, I, I, D, I, I, P, ctypes.POINTER(Stats)]
     lib.hprmat_hip_solve.argtypes = [I, P, L, P, L, I, I, I, D, I, I, P, ctypes.POINTER(Stats)]
     lib.hprmat_hip_version.restype = ctypes.c_char_p

@@ -145,11 +145,11 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
   h->A64 = (const double2*)dA;
   h->lda = lda;
   const double2* A = (const double2*)dA;
-  const hpr::SideStream* side = nullptr;
-  if (h->lookahead && n > 2 * h->plan.nb) {
-    if (ensure_side(h)) return -1;
-    side = &h->side;
-  }
+  // the look-ahead stream is created on the first factorization of a handle
+  // whatever its size, so later factorizations (possibly inside a stream
+  // capture) never create one
+  if (ensure_side(h)) return -1;
+  const hpr::SideStream* side = (h->lookahead && n > 2 * h->plan.nb) ? &h->side : nullptr;
   int rc;
   if (method == HPRMAT_PLAIN_FP64) {
     double2* F = (double2*)h->fac;
@@ -436,6 +436,22 @@ int hprmat_hip_lu_create(hprmat_hip_lu_t* h) {
   const char* la = getenv("HPRMAT_HIP_LOOKAHEAD");
   if (la && la[0] == '0') (*h)->lookahead = 0;
   return 0;
+}
+
+int hprmat_hip_lu_set_plan(hprmat_hip_lu_t h, int key, int value) {
+  if (!h) return -1;
+  switch (key) {
+    case HPRMAT_PLAN_TRAIL_TILE:
+      if (value < 0 || value > 2) return -1;
+      h->plan.trail = value;
+      return 0;
+    case HPRMAT_PLAN_LOOKAHEAD:
+      if (value < 0 || value > 1) return -1;
+      h->lookahead = value;
+      return 0;
+    default:
+      return -1;
+  }
 }
 
 int hprmat_hip_lu_destroy(hprmat_hip_lu_t h) {

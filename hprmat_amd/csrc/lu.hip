@@ -347,11 +347,30 @@ void split_operands(int nc, int kb, const cx<T>* L, int64_t ld, const cx<T>* U, 
 // Trailing update C[:, 0:ncols] -= L21 U12[:, c0:c0+ncols] (C is the block
 // at column c0 of A22), with the FP32/FP64 complex GEMM or the split-BF16
 // three-product GEMM over the planes made by split_operands.
+template <class T, int BM, int BN, int BK, int WM, int WN>
+void launch_gemm2(int M, int N, int K, const cx<T>* A, int64_t lda, const cx<T>* B, int64_t ldb,
+                  cx<T>* C, int64_t ldc, T alpha, int beta, hipStream_t st) {
+  static bool attr = false;
+  if (!attr) { allow_big_lds(zgemm2_kernel<T, BM, BN, BK, WM, WN, false, false>); attr = true; }
+  const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+  zgemm2_kernel<T, BM, BN, BK, WM, WN, false, false>
+      <<<tiles, 256, zgemm2_lds_bytes<BK, BM, BN, false, T>(), st>>>(M, N, K, A, lda, B, ldb, C,
+                                                                      ldc, alpha, beta, nullptr);
+}
+
 template <class T>
 void trailing(bool bfx, int nc, int ncols, int c0, int kb, const cx<T>* L, int64_t ld,
               const cx<T>* U, int64_t ldu, cx<T>* C, int64_t ldc, bool g3m,
-              const unsigned short* split, hipStream_t st, size_t lds_extra) {
+              const unsigned short* split, hipStream_t st, size_t lds_extra, int variant) {
   if (nc <= 0 || ncols <= 0) return;
+  if (!g3m && !bfx && variant == 1) {
+    launch_gemm2<T, 64, 128, 16, 32, 64>(nc, ncols, kb, L, ld, U + c0, ldu, C, ldc, T(-1), 1, st);
+    return;
+  }
+  if (!g3m && !bfx && variant == 2) {
+    launch_gemm2<T, 128, 64, 16, 64, 32>(nc, ncols, kb, L, ld, U + c0, ldu, C, ldc, T(-1), 1, st);
+    return;
+  }
   if constexpr (std::is_same<T, float>::value) {
     if (bfx) {
       constexpr int BM = 64, BN = 64;
@@ -469,7 +488,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       if (bfx) split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, st);
       if (pr) (void)hipEventRecord(pr->a, st);
       trailing<T>(bfx, nc, c_next, 0, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
-                  Akk + (int64_t)kb * ld + kb, ld, g3m, w.split, st, 0);
+                  Akk + (int64_t)kb * ld + kb, ld, g3m, w.split, st, 0, plan.trail);
       if (pr) {
         (void)hipEventRecord(pr->b, st);
         pr->flops = 8.0 * (double)nc * (double)c_next * (double)kb;
@@ -483,7 +502,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
         if (pr2) (void)hipEventRecord(pr2->a, cs);
         trailing<T>(bfx, nc, nc - c_next, c_next, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
                     Akk + (int64_t)kb * ld + kb + c_next, ld, g3m, w.split, cs,
-                    trail_lds_extra(sizeof(T) == 4));
+                    trail_lds_extra(sizeof(T) == 4), plan.trail);
         if (pr2) {
           (void)hipEventRecord(pr2->b, cs);
           pr2->flops = 8.0 * (double)nc * (double)(nc - c_next) * (double)kb;

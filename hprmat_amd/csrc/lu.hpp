@@ -7,6 +7,7 @@ namespace hpr {
 // Tunables of the blocked factorization.
 struct LuPlan {
   int nb = 256;     // outer panel width and getrs block height (<= 256, multiple of 64)
+  int trail = 0;    // trailing-update GEMM tile: 0 64x64x16, 1 64x128x16, 2 128x64x16
 };
 
 // Device scratch used by getrf: exchange area of the cooperative panel

@@ -129,6 +129,64 @@ adjoint_contract_kernel(int nch, int nr, int nrhs, const double2* __restrict__ M
   }
 }
 
+namespace {
+int r_to_s_impl(int nch, int nr, const void* dX, int64_t ldx, const double* q2, double normfac,
+                double rmax, const double* qk, int nopen, const int* open, const void* co,
+                const void* cop, void* dR, void* dS, void* stream, void* work, size_t work_cap) {
+  hipStream_t st = (hipStream_t)stream;
+  if (nch <= 0 || nr <= 0 || nopen < 0 || nopen > nch) return -1;
+  rmatrix_kernel<<<(nch * nch + 255) / 256, 256, 0, st>>>(nch, nr, (const double2*)dX, ldx, q2,
+                                                         normfac, (double2*)dR);
+  if (nopen == 0) return hipGetLastError() == hipSuccess ? 0 : -1;
+  const int64_t ld = nopen;
+  const size_t zbytes = (size_t)nopen * nopen * sizeof(double2);
+  const size_t ibytes = ((size_t)(nopen + 64) * sizeof(int) + 255) & ~(size_t)255;
+  const size_t d64 = (hpr::dinv64_elems(nopen) * sizeof(double2) + 255) & ~(size_t)255;
+  const size_t d256 = (hpr::dinv256_elems(nopen) * sizeof(double2) + 255) & ~(size_t)255;
+  const size_t dst = (hpr::ustage_elems(nopen) * sizeof(double2) + 255) & ~(size_t)255;
+  const size_t need = 4 * zbytes + 3 * ibytes + d64 + d256 + dst + hpr::scratch_bytes() + 2048;
+  if (!work) {   // library-global workspace (one caller at a time)
+    if (need > g_small.cap) {
+      if (g_small.buf) (void)hipFree(g_small.buf);
+      g_small.buf = nullptr;
+      g_small.cap = 0;
+      if (hipMalloc(&g_small.buf, need) != hipSuccess) return -2;
+      g_small.cap = need;
+    }
+    work = g_small.buf;
+  } else if (need > work_cap) {
+    return -2;
+  }
+  char* p = (char*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
+  double2* ZO = (double2*)p;
+  double2* ZI = (double2*)(p + zbytes);
+  double2* ZP = (double2*)(p + 2 * zbytes);
+  double2* ZY = (double2*)(p + 3 * zbytes);
+  p += ((4 * zbytes + 255) & ~(size_t)255);
+  int* ipiv = (int*)p;
+  int* perm = (int*)(p + ibytes);
+  int* info = (int*)(p + 2 * ibytes);
+  p += 3 * ibytes;
+  hpr::FactorWork<double> w;
+  w.dinv64 = (double2*)p;
+  w.dinv256 = (double2*)(p + d64);
+  w.ustage = (double2*)(p + d64 + d256);
+  w.perm = perm;
+  p += d64 + d256 + dst;
+  hpr::Scratch sc;
+  hpr::scratch_bind(sc, p);
+  zmatrices_kernel<<<(nopen * nopen + 255) / 256, 256, 0, st>>>(
+      nch, nopen, (const double2*)dR, rmax, qk, open, (const double2*)co, (const double2*)cop, ZO, ZI);
+  hpr::LuPlan plan;
+  if (hpr::getrf<double>(ZO, ld, nopen, ipiv, info, w, sc, plan, hpr::GEMM_4M, st)) return -1;
+  hpr::permute_cast<double, double>(nopen, nopen, perm, ZI, ZP, nullptr, st);
+  if (hpr::getrs<double>(ZO, ld, nopen, w.dinv256, ZP, ld, nopen, ZY, nullptr, plan, false, st))
+    return -1;
+  store_colmajor_kernel<<<(nopen * nopen + 255) / 256, 256, 0, st>>>(nopen, ZP, (double2*)dS);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+}  // namespace
+
 extern "C" {
 
 /* Assemble the complex-symmetric Lagrange-mesh matrix of one interval into
@@ -158,52 +216,27 @@ int hprmat_hip_assemble(int nch, int nr, const double* tc, const double* blo0, c
 int hprmat_hip_r_to_s(int nch, int nr, const void* dX, int64_t ldx, const double* q2,
                       double normfac, double rmax, const double* qk, int nopen, const int* open,
                       const void* co, const void* cop, void* dR, void* dS, void* stream) {
-  hipStream_t st = (hipStream_t)stream;
-  if (nch <= 0 || nr <= 0 || nopen < 0 || nopen > nch) return -1;
-  rmatrix_kernel<<<(nch * nch + 255) / 256, 256, 0, st>>>(nch, nr, (const double2*)dX, ldx, q2,
-                                                         normfac, (double2*)dR);
-  if (nopen == 0) return hipGetLastError() == hipSuccess ? 0 : -1;
-  const int64_t ld = nopen;
-  const size_t zbytes = (size_t)nopen * nopen * sizeof(double2);
-  const size_t ibytes = ((size_t)(nopen + 64) * sizeof(int) + 255) & ~(size_t)255;
-  const size_t d64 = (hpr::dinv64_elems(nopen) * sizeof(double2) + 255) & ~(size_t)255;
-  const size_t d256 = (hpr::dinv256_elems(nopen) * sizeof(double2) + 255) & ~(size_t)255;
-  const size_t dst = (hpr::ustage_elems(nopen) * sizeof(double2) + 255) & ~(size_t)255;
-  const size_t need = 4 * zbytes + 3 * ibytes + d64 + d256 + dst + hpr::scratch_bytes() + 2048;
-  if (need > g_small.cap) {
-    if (g_small.buf) (void)hipFree(g_small.buf);
-    g_small.buf = nullptr;
-    g_small.cap = 0;
-    if (hipMalloc(&g_small.buf, need) != hipSuccess) return -2;
-    g_small.cap = need;
-  }
-  char* p = (char*)(((uintptr_t)g_small.buf + 255) & ~(uintptr_t)255);
-  double2* ZO = (double2*)p;
-  double2* ZI = (double2*)(p + zbytes);
-  double2* ZP = (double2*)(p + 2 * zbytes);
-  double2* ZY = (double2*)(p + 3 * zbytes);
-  p += ((4 * zbytes + 255) & ~(size_t)255);
-  int* ipiv = (int*)p;
-  int* perm = (int*)(p + ibytes);
-  int* info = (int*)(p + 2 * ibytes);
-  p += 3 * ibytes;
-  hpr::FactorWork<double> w;
-  w.dinv64 = (double2*)p;
-  w.dinv256 = (double2*)(p + d64);
-  w.ustage = (double2*)(p + d64 + d256);
-  w.perm = perm;
-  p += d64 + d256 + dst;
-  hpr::Scratch sc;
-  hpr::scratch_bind(sc, p);
-  zmatrices_kernel<<<(nopen * nopen + 255) / 256, 256, 0, st>>>(
-      nch, nopen, (const double2*)dR, rmax, qk, open, (const double2*)co, (const double2*)cop, ZO, ZI);
-  hpr::LuPlan plan;
-  if (hpr::getrf<double>(ZO, ld, nopen, ipiv, info, w, sc, plan, hpr::GEMM_4M, st)) return -1;
-  hpr::permute_cast<double, double>(nopen, nopen, perm, ZI, ZP, nullptr, st);
-  if (hpr::getrs<double>(ZO, ld, nopen, w.dinv256, ZP, ld, nopen, ZY, nullptr, plan, false, st))
-    return -1;
-  store_colmajor_kernel<<<(nopen * nopen + 255) / 256, 256, 0, st>>>(nopen, ZP, (double2*)dS);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return r_to_s_impl(nch, nr, dX, ldx, q2, normfac, rmax, qk, nopen, open, co, cop, dR, dS,
+                     stream, nullptr, 0);
+}
+
+size_t hprmat_hip_r_to_s_work_bytes(int nch) {
+  const int k = nch > 0 ? nch : 1;
+  const size_t zbytes = (size_t)k * k * sizeof(double2);
+  const size_t ibytes = ((size_t)(k + 64) * sizeof(int) + 255) & ~(size_t)255;
+  const size_t d64 = (hpr::dinv64_elems(k) * sizeof(double2) + 255) & ~(size_t)255;
+  const size_t d256 = (hpr::dinv256_elems(k) * sizeof(double2) + 255) & ~(size_t)255;
+  const size_t dst = (hpr::ustage_elems(k) * sizeof(double2) + 255) & ~(size_t)255;
+  return 4 * zbytes + 3 * ibytes + d64 + d256 + dst + hpr::scratch_bytes() + 2048 + 256;
+}
+
+int hprmat_hip_r_to_s_ws(int nch, int nr, const void* dX, int64_t ldx, const double* q2,
+                         double normfac, double rmax, const double* qk, int nopen, const int* open,
+                         const void* co, const void* cop, void* dR, void* dS, void* work,
+                         size_t work_bytes, void* stream) {
+  if (!work) return -1;
+  return r_to_s_impl(nch, nr, dX, ldx, q2, normfac, rmax, qk, nopen, open, co, cop, dR, dS,
+                     stream, work, work_bytes);
 }
 
 int hprmat_hip_adjoint_contract(int nch, int nr, int nrhs, const void* dM, int64_t ldm,

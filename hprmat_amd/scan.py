@@ -1,0 +1,353 @@
+"""Batched energy scan over many partial waves with uneven matrix sizes.
+
+Workload: n_waves partial waves, each with nch channels drawn from
+{2, 4, 8, 16, 32, 64} and nr mesh points from {40, 60, 80, 100} (seeded),
+solved at n_energies energies uniform in [e_min, e_max]: every (wave, energy)
+system C(E) X = B (N = nch nr, nch right-hand sides) is assembled on the
+device, factored in complex FP32 with FP64 refinement (classic or FGMRES), and
+reduced to R and S on the device (src/rmatrix_hp.F90:246-368 per system, the
+reference's rmatrix call looped over energies).
+
+Execution (``EnergyScan``):
+* lanes: L HIP streams, each with its own LU handle and grow-only buffers
+  sized for the largest system it runs; systems are assigned to lanes
+  longest-processing-time first by their N^3 cost, so large and small
+  systems of different waves run concurrently and the latency-bound panel
+  of one system overlaps the GEMMs of others.  At most L cooperative panel
+  kernels are in flight, which keeps them within the co-residence bound.
+* graph: the systems up to N = graph_max (launch-bound: hundreds of small
+  kernels per system) are captured once into one HIP graph over their lanes
+  and replayed; the larger ones (GPU-bound) are launched directly on lanes
+  of their own, concurrently with the graph.  Captured systems are factored
+  without the look-ahead stream, so a graph only spans the lane streams.
+* plan: per size class (N, nch) the refinement kind, the trailing-update
+  GEMM tile and the look-ahead switch come from ``autotune`` (measured on a
+  few energies of that class), as does the number of lanes.
+Nothing synchronises with the host inside a scan; S of every system is left
+in device memory (``s_of(wave, energy)``).
+"""
+import ctypes
+import json
+import os
+import time
+
+import numpy as np
+import torch
+
+from . import _lib, rmatrix, solver, synth
+
+NC_CHOICES = (2, 4, 8, 16, 32, 64)
+NR_CHOICES = (40, 60, 80, 100)
+
+
+def draw_waves(n_waves=64, seed=0):
+    """(nch, nr) of each wave, drawn uniformly from the choice sets."""
+    rng = np.random.default_rng(seed)
+    nc = rng.choice(NC_CHOICES, n_waves)
+    nr = rng.choice(NR_CHOICES, n_waves)
+    return [(int(a), int(b)) for a, b in zip(nc, nr)]
+
+
+def _dev(x, dtype):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype).cuda()
+
+
+class Wave:
+    """One partial wave: synthetic system (pair-recipe potential, thresholds
+    in [0, emax]) and its device-resident per-energy channel data."""
+
+    def __init__(self, index, nch, nr, energies, a=15.0, emax=3.0, seed=0):
+        self.index, self.nch, self.nr, self.n, self.a = index, nch, nr, nch * nr, a
+        self.sysm = synth.System(nch, nr, a, emax, float(energies[0]), seed + 7919 * (index + 1))
+        self.dmesh = rmatrix.DeviceMesh(self.sysm.mesh)
+        self.lv = _dev(np.asarray(self.sysm.lval, dtype=np.int32), torch.int32)
+        self.cpot = _dev(self.sysm.cpot, torch.complex128)
+        self.b = rmatrix.rhs_colmajor(self.dmesh, nch)
+        self.energies = np.asarray(energies, dtype=np.float64)
+        ne = len(self.energies)
+        qk = np.stack([synth.wave_numbers(self.sysm.eps, e) for e in self.energies])
+        self.qk_host = qk
+        self.qk = _dev(qk, torch.float64)                       # (ne, nch)
+        xc1 = np.zeros((ne, nch))
+        opn = np.zeros((ne, nch), dtype=np.int32)
+        co = np.zeros((ne, nch), dtype=np.complex128)
+        cop = np.zeros((ne, nch), dtype=np.complex128)
+        self.nopen = np.zeros(ne, dtype=np.int64)
+        for k in range(ne):
+            o, c0, c1, x1 = rmatrix.channel_data(self.sysm.lval, qk[k], a)
+            xc1[k] = x1
+            self.nopen[k] = len(o)
+            opn[k, :len(o)] = o
+            co[k, :len(o)] = c0
+            cop[k, :len(o)] = c1
+        self.xc1 = _dev(xc1, torch.float64)
+        self.open = _dev(opn, torch.int32)
+        self.co = _dev(co, torch.complex128)
+        self.cop = _dev(cop, torch.complex128)
+        self.R = torch.zeros((ne, nch, nch), dtype=torch.complex128, device='cuda')
+        self.S = torch.zeros((ne, nch, nch), dtype=torch.complex128, device='cuda')
+
+    def s_of(self, k):
+        """S at energy index k (nopen x nopen matrix, from the column-major slot)."""
+        no = int(self.nopen[k])
+        return self.S[k].reshape(-1)[:no * no].reshape(no, no).transpose(0, 1)
+
+    @property
+    def cost(self):
+        return float(self.n) ** 3
+
+
+class Lane:
+    """One stream with its own LU handle, matrix/RHS buffers and R->S workspace."""
+
+    def __init__(self, max_n, max_nch):
+        lib = _lib.load()
+        self.stream = torch.cuda.Stream()
+        self.lu = solver.LU()
+        self.a = torch.empty(max_n * max_n, dtype=torch.complex128, device='cuda')
+        self.x = torch.empty(max_nch * max_n, dtype=torch.complex128, device='cuda')
+        self.work_bytes = int(lib.hprmat_hip_r_to_s_work_bytes(max_nch))
+        self.work = torch.empty(self.work_bytes, dtype=torch.uint8, device='cuda')
+        self.items = []
+
+
+DEFAULT_PLAN = {'method': 'fp32', 'refinement': 'fgmres', 'max_steps': 16, 'trail': 0,
+                'lookahead': 1}
+
+
+def solve_system(lane, wave, k, plan, tol=1e-12, stats=False):
+    """Assemble, factor, refine and reduce system (wave, k) on the current
+    stream; with stats=True returns the refinement statistics (synchronises)."""
+    lib = _lib.load()
+    n, nch, nr = wave.n, wave.nch, wave.nr
+    st = solver._stream_ptr()
+    a = lane.a[:n * n].view(n, n)
+    dm = wave.dmesh
+    rc = lib.hprmat_hip_assemble(
+        nch, nr, ctypes.c_void_p(dm.tc.data_ptr()), ctypes.c_void_p(dm.blo0.data_ptr()),
+        ctypes.c_void_p(dm.r.data_ptr()), ctypes.c_double(wave.a), ctypes.c_void_p(wave.lv.data_ptr()),
+        ctypes.c_void_p(wave.qk[k].data_ptr()), ctypes.c_void_p(wave.xc1[k].data_ptr()),
+        ctypes.c_void_p(wave.cpot.data_ptr()), ctypes.c_void_p(a.data_ptr()), ctypes.c_int64(n), st)
+    _lib.check(rc, 'hprmat_hip_assemble')
+    x = lane.x[:nch * n].view(nch, n)
+    x.copy_(wave.b)
+    lane.lu.set_plan(trail_tile=plan['trail'], lookahead=plan['lookahead'])
+    lane.lu.factor(a, plan['method'], equilibrate=True)
+    st_ref = lane.lu.solve_(x, plan['max_steps'], tol, 'rel_fro', True, stats=stats,
+                            refinement=plan['refinement'])
+    no = int(wave.nopen[k])
+    rc = lib.hprmat_hip_r_to_s_ws(
+        nch, nr, ctypes.c_void_p(x.data_ptr()), ctypes.c_int64(n), ctypes.c_void_p(dm.q2.data_ptr()),
+        ctypes.c_double(1.0), ctypes.c_double(wave.a), ctypes.c_void_p(wave.qk[k].data_ptr()), no,
+        ctypes.c_void_p(wave.open[k].data_ptr()), ctypes.c_void_p(wave.co[k].data_ptr()),
+        ctypes.c_void_p(wave.cop[k].data_ptr()), ctypes.c_void_p(wave.R[k].data_ptr()),
+        ctypes.c_void_p(wave.S[k].data_ptr()), ctypes.c_void_p(lane.work.data_ptr()),
+        ctypes.c_size_t(lane.work_bytes), st)
+    _lib.check(rc, 'hprmat_hip_r_to_s_ws')
+    return st_ref
+
+
+def _lpt(items, nlanes):
+    """Longest-processing-time-first assignment of (cost, payload) items."""
+    loads = [0.0] * nlanes
+    out = [[] for _ in range(nlanes)]
+    for cost, it in sorted(items, key=lambda t: -t[0]):
+        j = int(np.argmin(loads))
+        loads[j] += cost
+        out[j].append(it)
+    return out
+
+
+class EnergyScan:
+    def __init__(self, waves, lanes=4, graph_max=1280, plans=None):
+        self.waves = waves
+        self.nlanes = lanes
+        self.graph_max = graph_max
+        self.plans = plans or {}
+        self.graph = None
+        small = [(w.cost, (w, k)) for w in waves if w.n <= graph_max for k in range(len(w.energies))]
+        big = [(w.cost, (w, k)) for w in waves if w.n > graph_max for k in range(len(w.energies))]
+        self.small_lanes = self._make_lanes(_lpt(small, lanes)) if small else []
+        self.big_lanes = self._make_lanes(_lpt(big, lanes)) if big else []
+
+    @staticmethod
+    def _make_lanes(assign):
+        lanes = []
+        for items in assign:
+            if not items:
+                continue
+            lane = Lane(max(w.n for w, _ in items), max(w.nch for w, _ in items))
+            lane.items = sorted(items, key=lambda t: (-t[0].n, -t[0].nch))
+            lanes.append(lane)
+        return lanes
+
+    def plan_for(self, w):
+        p = self.plans.get((w.n, w.nch), DEFAULT_PLAN)
+        if w.n <= self.graph_max:
+            # graph-captured systems stay on their lane's stream: no look-ahead
+            # (its second stream and reused events are kept out of captures;
+            # the arithmetic is identical, the split falls on tile boundaries)
+            p = dict(p, lookahead=0)
+        return p
+
+    def _run_lanes(self, lanes):
+        root = torch.cuda.current_stream()
+        for lane in lanes:
+            lane.stream.wait_stream(root)
+            with torch.cuda.stream(lane.stream):
+                for w, k in lane.items:
+                    solve_system(lane, w, k, self.plan_for(w))
+        for lane in lanes:
+            root.wait_stream(lane.stream)
+
+    def warm(self):
+        """Eager pass over the systems of every lane with the largest N and the
+        largest N nch, so that each handle's grow-only buffers (factors,
+        right-hand sides, refinement workspace) reach their final size before
+        any graph capture (no allocation happens inside one)."""
+        for lane in self.small_lanes + self.big_lanes:
+            first = lane.items[0]
+            widest = max(lane.items, key=lambda t: t[0].n * t[0].nch)
+            with torch.cuda.stream(lane.stream):
+                for w, k in {id(first[0]): first, id(widest[0]): widest}.values():
+                    solve_system(lane, w, k, self.plan_for(w))
+        torch.cuda.synchronize()
+
+    def capture(self):
+        """Capture the small systems of the scan into one HIP graph."""
+        if not self.small_lanes:
+            return
+        self.warm()
+        g = torch.cuda.CUDAGraph()
+        cap = torch.cuda.Stream()
+        with torch.cuda.graph(g, stream=cap, capture_error_mode='relaxed'):
+            self._run_lanes(self.small_lanes)
+        torch.cuda.synchronize()
+        self.graph = g
+
+    def run(self):
+        """One full scan (graph replay for the small systems, direct launches
+        for the large ones, concurrently); returns after enqueueing."""
+        if self.small_lanes:
+            if self.graph is None:
+                self._run_lanes(self.small_lanes)
+            else:
+                self.graph.replay()
+        if self.big_lanes:
+            self._run_lanes(self.big_lanes)
+
+    @property
+    def n_systems(self):
+        return sum(len(w.energies) for w in self.waves)
+
+    @property
+    def flops(self):
+        """Algorithmic flops of the scan: sum of 8/3 N^3 + 8 N^2 nrhs."""
+        return float(sum(len(w.energies) * (8.0 / 3.0 * w.n ** 3 + 8.0 * w.n ** 2 * w.nch)
+                         for w in self.waves))
+
+
+def serial_fp64(waves, pairs, tol=1e-12):
+    """Reference loop: one plain FP64 solve per system, one stream, in order
+    (factor + solve + one FP64 residual check, then R and S)."""
+    lane = Lane(max(w.n for w, _ in pairs), max(w.nch for w, _ in pairs))
+    plan = {'method': 'fp64', 'refinement': 'ir', 'max_steps': 0, 'trail': 0, 'lookahead': 1}
+    for w, k in pairs:
+        solve_system(lane, w, k, plan, tol)
+
+
+def autotune(waves, sample=3, cache=None, lane_options=(2, 4, 8)):
+    """Pick, per size class (N, nch), the refinement kind, the trailing GEMM
+    tile and the look-ahead switch by timing `sample` energies of the class on
+    one lane; then the number of lanes by timing a slice of the whole scan.
+    Results are cached in `cache` (JSON) when given."""
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            d = json.load(f)
+        plans = {tuple(map(int, key.split('x'))): v for key, v in d['plans'].items()}
+        return plans, int(d['lanes'])
+    classes = {}
+    for w in waves:
+        classes.setdefault((w.n, w.nch), w)
+    candidates = []
+    for method, refinement in (('fp32', 'ir'), ('fp32', 'fgmres'), ('bf16x3', 'fgmres')):
+        for trail in (0, 1, 2):
+            for lookahead in (1, 0):
+                if method == 'bf16x3' and trail != 0:
+                    continue      # the tile applies to the 4-product kernels only
+                candidates.append({'method': method, 'refinement': refinement,
+                                   'max_steps': 16, 'trail': trail, 'lookahead': lookahead})
+    plans = {}
+    for key, w in sorted(classes.items()):
+        lane = Lane(w.n, w.nch)
+        ks = list(range(0, len(w.energies), max(1, len(w.energies) // sample)))[:sample]
+        best = None
+        for p in candidates:
+            if p['lookahead'] == 0 and w.n <= 512:
+                continue          # no look-ahead below two panels anyway
+            if p['trail'] != 0 and w.n <= 512:
+                continue          # a single trailing update: the default tile
+            with torch.cuda.stream(lane.stream):
+                ok = all(solve_system(lane, w, k, p, stats=True)['converged'] for k in ks)
+                if not ok:
+                    continue      # must reach the tolerance on every sampled energy
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for k in ks:
+                    solve_system(lane, w, k, p)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+            if best is None or dt < best[0]:
+                best = (dt, p)
+        if best is None:
+            best = (0.0, dict(DEFAULT_PLAN, max_steps=24))
+        plans[key] = dict(best[1])
+        del lane
+    # lanes: time a slice (every 16th energy of every wave)
+    best_l = None
+    for nl in lane_options:
+        sc = EnergyScan(_slice_waves(waves, 16), lanes=nl, graph_max=0, plans=plans)
+        sc.warm()
+        t0 = time.perf_counter()
+        sc.run()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if best_l is None or dt < best_l[0]:
+            best_l = (dt, nl)
+        del sc
+    if cache:
+        os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+        with open(cache, 'w') as f:
+            json.dump({'plans': {'%dx%d' % k: v for k, v in plans.items()}, 'lanes': best_l[1]}, f)
+    return plans, best_l[1]
+
+
+class _WaveSlice:
+    """View of a wave restricted to every step-th energy (shares device data)."""
+
+    def __init__(self, w, step):
+        self.__dict__.update(w.__dict__)
+        self._w = w
+        self._idx = list(range(0, len(w.energies), step))
+        self.energies = w.energies[self._idx]
+        self.nopen = w.nopen[self._idx]
+        self.qk = w.qk[self._idx]
+        self.xc1 = w.xc1[self._idx]
+        self.open = w.open[self._idx]
+        self.co = w.co[self._idx]
+        self.cop = w.cop[self._idx]
+        self.R = torch.zeros((len(self._idx), w.nch, w.nch), dtype=torch.complex128, device='cuda')
+        self.S = torch.zeros_like(self.R)
+
+    cost = Wave.cost
+    s_of = Wave.s_of
+
+
+def _slice_waves(waves, step):
+    return [_WaveSlice(w, step) for w in waves]
+
+
+def make_waves(n_waves=64, n_energies=256, e_min=0.5, e_max=10.0, seed=0, emax=3.0):
+    energies = np.linspace(e_min, e_max, n_energies)
+    return [Wave(i, nc, nr, energies, emax=emax, seed=seed)
+            for i, (nc, nr) in enumerate(draw_waves(n_waves, seed))]

@@ -58,6 +58,17 @@ class LU:
             self._lib.hprmat_hip_lu_destroy(h)
             self._h = None
 
+    def set_plan(self, trail_tile=None, lookahead=None):
+        """Factorization plan: trailing-update GEMM tile (0 64x64x16,
+        1 64x128x16, 2 128x64x16) and look-ahead on/off."""
+        if trail_tile is not None:
+            _lib.check(self._lib.hprmat_hip_lu_set_plan(self._h, _lib.PLAN_TRAIL_TILE,
+                                                         int(trail_tile)), 'set_plan')
+        if lookahead is not None:
+            _lib.check(self._lib.hprmat_hip_lu_set_plan(self._h, _lib.PLAN_LOOKAHEAD,
+                                                         int(bool(lookahead))), 'set_plan')
+        return self
+
     def factor(self, a_cm, method='fp64', equilibrate=False):
         """Factor the matrix whose column-major storage is ``a_cm`` (n x n).
 

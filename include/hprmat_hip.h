@@ -123,6 +123,17 @@ typedef struct hprmat_hip_lu *hprmat_hip_lu_t;
 int hprmat_hip_lu_create(hprmat_hip_lu_t *h);
 int hprmat_hip_lu_destroy(hprmat_hip_lu_t h);
 
+/* No reference counterpart (cuSOLVER chooses internally): factorization
+   plan of a handle, set by an autotuner or by hand before factoring.
+     HPRMAT_PLAN_TRAIL_TILE  trailing-update GEMM tile of the FP32/FP64
+                             4-product path: 0 = 64x64x16 (default),
+                             1 = 64x128x16, 2 = 128x64x16 (double-buffered LDS)
+     HPRMAT_PLAN_LOOKAHEAD   1 = overlap the next panel with the trailing
+                             update on a second stream (default), 0 = off
+   Returns -1 for an unknown key or value. */
+enum { HPRMAT_PLAN_TRAIL_TILE = 1, HPRMAT_PLAN_LOOKAHEAD = 2 };
+int hprmat_hip_lu_set_plan(hprmat_hip_lu_t h, int key, int value);
+
 /* Factor the n x n complex FP64 matrix at device pointer dA (column-major,
    lda) in the given precision.  dA is not modified and must stay valid while
    the handle is used for refined solves (the FP64 residual reads it).
@@ -179,6 +190,17 @@ int hprmat_hip_r_to_s(int nch, int nr, const void *dX, int64_t ldx,
                       const double *qk, int nopen, const int *open,
                       const void *co, const void *cop, void *dR, void *dS,
                       void *stream);
+/* No reference counterpart: hprmat_hip_r_to_s with a caller-owned device
+   workspace (hprmat_hip_r_to_s_work_bytes(nch) bytes) instead of the
+   library-global one, for concurrent calls on several streams and for
+   stream capture (no allocation inside).  Returns -2 when work_bytes is too
+   small. */
+size_t hprmat_hip_r_to_s_work_bytes(int nch);
+int hprmat_hip_r_to_s_ws(int nch, int nr, const void *dX, int64_t ldx,
+                         const double *q2, double normfac, double rmax,
+                         const double *qk, int nopen, const int *open,
+                         const void *co, const void *cop, void *dR, void *dS,
+                         void *work, size_t work_bytes, void *stream);
 
 /* No reference counterpart (the reference has no derivatives): adjoint
    contraction for gradients with respect to a local coupling potential.
