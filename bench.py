@@ -25,7 +25,7 @@ residual gate; per-method figures, the roofline of the dominant kernel (the
 trailing-update GEMM, timed with HIP events on the launch stream) and the
 CPU baseline (LAPACK zgesv + R, reference solver type 1) are in the line.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--scan]
 """
 import argparse
 import json
@@ -88,6 +88,12 @@ def main():
     ap.add_argument('--nr', type=int, default=200)
     ap.add_argument('--methods', default='fp64,fp32_gmres,fp32,fp32_ir5,bf16x3,bf16x3_gmres')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--scan', action='store_true',
+                    help='also run the batched energy scan (64 waves x 256 energies)')
+    ap.add_argument('--scan-waves', type=int, default=64)
+    ap.add_argument('--scan-energies', type=int, default=256)
+    ap.add_argument('--graph-max', type=int, default=10 ** 9)
+    ap.add_argument('--tune-cache', default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path '
@@ -197,9 +203,69 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
         'speedup_mixed_vs_fp64': (methods['fp64']['ms'] / hm['ms']) if 'fp64' in methods else None,
         'roofline': roof,
     }
+    if args.scan:
+        out['energy_scan'] = energy_scan(torch, args)
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline()
     print(json.dumps(out))
+
+
+def energy_scan(torch, args):
+    """Batched energy scan: 64 waves (nch in {2..64}, nr in {40..100}) x 256
+    energies in [0.5, 10], FP32 LU + FP64 refinement per system with plans
+    from the autotuner, small systems replayed from a HIP graph; against a
+    loop of plain FP64 solves timed on every 16th energy and scaled."""
+    from hprmat_amd import scan
+    t0 = time.perf_counter()
+    waves = scan.make_waves(args.scan_waves, args.scan_energies, seed=0)
+    t_prep = time.perf_counter() - t0
+    print('# scan: autotune', file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    plans, lanes = scan.autotune(waves, sample=3, cache=args.tune_cache, log=sys.stderr)
+    t_tune = time.perf_counter() - t0
+    print('# scan: %d waves prepared in %.1f s, tuned in %.1f s (%d lanes)'
+          % (len(waves), t_prep, t_tune, lanes), file=sys.stderr, flush=True)
+    sc = scan.EnergyScan(waves, lanes=lanes, graph_max=args.graph_max, plans=plans, log=sys.stderr)
+    t0 = time.perf_counter()
+    sc.capture()
+    t_cap = time.perf_counter() - t0
+    print('# scan: %d systems, %.3g flops, prep %.1f s, tune %.1f s, capture %.1f s, lanes %d'
+          % (sc.n_systems, sc.flops, t_prep, t_tune, t_cap, lanes), file=sys.stderr, flush=True)
+    print('# scan: first run', file=sys.stderr, flush=True)
+    sc.run()
+    torch.cuda.synchronize()
+    print('# scan: timed run', file=sys.stderr, flush=True)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    sc.run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    s_scan = {(w.index, k): w.s_of(k).clone() for w in waves for k in range(0, len(w.energies), 16)}
+    pairs = [(w, k) for w in waves for k in range(0, len(w.energies), 16)]
+    torch.cuda.synchronize()
+    print('# scan: serial FP64 sample', file=sys.stderr, flush=True)
+    e0.record()
+    scan.serial_fp64(waves, pairs)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_ser = e0.elapsed_time(e1)
+    ds = max(float((w.s_of(k) - s_scan[(w.index, k)]).abs().max()) if w.nopen[k] else 0.0
+             for w, k in pairs)
+    n_sys = sc.n_systems
+    out = {'systems': n_sys, 'ms': ms, 'solves_per_s': n_sys / (ms * 1e-3),
+           'tflops': sc.flops / (ms * 1e-3) / 1e12, 'lanes': lanes, 'graph_max': args.graph_max,
+           'serial_fp64': {'systems_timed': len(pairs), 'ms': ms_ser,
+                           'solves_per_s': len(pairs) / (ms_ser * 1e-3),
+                           'sample': 'every 16th energy of every wave, one stream, eager'},
+           'max_dS_vs_fp64': ds}
+    out['speedup_vs_serial_fp64'] = out['solves_per_s'] / out['serial_fp64']['solves_per_s']
+    print('# scan: %.1f ms, %.0f solves/s, %.1f TFLOP/s; serial FP64 %.0f solves/s; x%.2f; '
+          'max|dS| %.1e' % (ms, out['solves_per_s'], out['tflops'],
+                            out['serial_fp64']['solves_per_s'], out['speedup_vs_serial_fp64'], ds),
+          file=sys.stderr, flush=True)
+    return out
 
 
 if __name__ == '__main__':

@@ -15,14 +15,16 @@ Execution (``EnergyScan``):
   systems of different waves run concurrently and the latency-bound panel
   of one system overlaps the GEMMs of others.  At most L cooperative panel
   kernels are in flight, which keeps them within the co-residence bound.
-* graph: the systems up to N = graph_max (launch-bound: hundreds of small
-  kernels per system) are captured once into one HIP graph over their lanes
-  and replayed; the larger ones (GPU-bound) are launched directly on lanes
-  of their own, concurrently with the graph.  Captured systems are factored
-  without the look-ahead stream, so a graph only spans the lane streams.
-* plan: per size class (N, nch) the refinement kind, the trailing-update
-  GEMM tile and the look-ahead switch come from ``autotune`` (measured on a
-  few energies of that class), as does the number of lanes.
+* graphs: each lane's systems are captured once, on the lane's stream, into
+  chain graphs of at most NODE_BUDGET kernels and replayed on that stream,
+  so one scan costs a few hundred graph launches instead of millions of
+  kernel launches from the host (a system launches 10^2 - 10^3 kernels).
+  Captured systems are factored without the look-ahead stream (a graph only
+  spans its lane's stream); concurrency comes from the lanes.
+* plan: per size class (N, nch) the factorization arithmetic (FP32 or
+  split-BF16), the refinement kind and its launched step count, the
+  trailing-update GEMM tile and the look-ahead switch come from ``autotune``
+  (measured on a few energies of that class), as does the number of lanes.
 Nothing synchronises with the host inside a scan; S of every system is left
 in device memory (``s_of(wave, energy)``).
 """
@@ -158,17 +160,26 @@ def _lpt(items, nlanes):
     return out
 
 
+NODE_BUDGET = 60000     # kernel nodes per captured graph
+
+
+def est_nodes(w, plan):
+    """Upper estimate of the kernels one system launches (the refinement
+    launches all max_steps steps; the device skips those after convergence)."""
+    nb = (w.n + 255) // 256
+    return 30 + 50 * nb + plan['max_steps'] * (25 + 3 * nb)
+
+
 class EnergyScan:
-    def __init__(self, waves, lanes=4, graph_max=1280, plans=None):
+    def __init__(self, waves, lanes=4, graph_max=10 ** 9, plans=None, log=None):
         self.waves = waves
         self.nlanes = lanes
         self.graph_max = graph_max
         self.plans = plans or {}
-        self.graph = None
-        small = [(w.cost, (w, k)) for w in waves if w.n <= graph_max for k in range(len(w.energies))]
-        big = [(w.cost, (w, k)) for w in waves if w.n > graph_max for k in range(len(w.energies))]
-        self.small_lanes = self._make_lanes(_lpt(small, lanes)) if small else []
-        self.big_lanes = self._make_lanes(_lpt(big, lanes)) if big else []
+        self.log = log
+        items = [(w.cost, (w, k)) for w in waves for k in range(len(w.energies))]
+        self.lanes = self._make_lanes(_lpt(items, lanes)) if items else []
+        self.captured = False
 
     @staticmethod
     def _make_lanes(assign):
@@ -178,6 +189,7 @@ class EnergyScan:
                 continue
             lane = Lane(max(w.n for w, _ in items), max(w.nch for w, _ in items))
             lane.items = sorted(items, key=lambda t: (-t[0].n, -t[0].nch))
+            lane.graphs = []
             lanes.append(lane)
         return lanes
 
@@ -190,51 +202,84 @@ class EnergyScan:
             p = dict(p, lookahead=0)
         return p
 
-    def _run_lanes(self, lanes):
-        root = torch.cuda.current_stream()
-        for lane in lanes:
-            lane.stream.wait_stream(root)
-            with torch.cuda.stream(lane.stream):
-                for w, k in lane.items:
-                    solve_system(lane, w, k, self.plan_for(w))
-        for lane in lanes:
-            root.wait_stream(lane.stream)
+    def _say(self, msg):
+        if self.log:
+            print(msg, file=self.log, flush=True)
 
     def warm(self):
-        """Eager pass over the systems of every lane with the largest N and the
-        largest N nch, so that each handle's grow-only buffers (factors,
-        right-hand sides, refinement workspace) reach their final size before
-        any graph capture (no allocation happens inside one)."""
-        for lane in self.small_lanes + self.big_lanes:
+        """Eager pass that brings every handle's grow-only buffers (factors,
+        right-hand sides, refinement workspace) to their final size and runs
+        every kernel variant of the lane's plans once, so that nothing is
+        allocated or set up for the first time inside a graph capture."""
+        for lane in self.lanes:
+            reps = {}
             first = lane.items[0]
             widest = max(lane.items, key=lambda t: t[0].n * t[0].nch)
+            reps[id(first[0])] = first
+            reps[id(widest[0])] = widest
+            seen = set()
+            for w, k in lane.items:
+                key = tuple(sorted(self.plan_for(w).items()))
+                if key not in seen:
+                    seen.add(key)
+                    reps.setdefault(id(w), (w, k))
             with torch.cuda.stream(lane.stream):
-                for w, k in {id(first[0]): first, id(widest[0]): widest}.values():
+                for w, k in reps.values():
                     solve_system(lane, w, k, self.plan_for(w))
         torch.cuda.synchronize()
 
     def capture(self):
-        """Capture the small systems of the scan into one HIP graph."""
-        if not self.small_lanes:
-            return
+        """Capture each lane's systems (those up to N = graph_max) into chain
+        graphs of at most NODE_BUDGET kernels, captured on the lane's stream."""
+        self._say('# scan: warm-up')
         self.warm()
-        g = torch.cuda.CUDAGraph()
-        cap = torch.cuda.Stream()
-        with torch.cuda.graph(g, stream=cap, capture_error_mode='relaxed'):
-            self._run_lanes(self.small_lanes)
+        total = 0
+        for li, lane in enumerate(self.lanes):
+            chunks, cur, nodes = [], [], 0
+            for w, k in lane.items:
+                if w.n > self.graph_max:
+                    continue
+                c = est_nodes(w, self.plan_for(w))
+                if cur and nodes + c > NODE_BUDGET:
+                    chunks.append(cur)
+                    cur, nodes = [], 0
+                cur.append((w, k))
+                nodes += c
+            if cur:
+                chunks.append(cur)
+            lane.graphs = []
+            for chunk in chunks:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=lane.stream, capture_error_mode='relaxed'):
+                    for w, k in chunk:
+                        solve_system(lane, w, k, self.plan_for(w))
+                lane.graphs.append(g)
+            total += len(lane.graphs)
+            self._say('# scan: lane %d: %d systems in %d graphs' % (li, len(lane.items),
+                                                                    len(lane.graphs)))
         torch.cuda.synchronize()
-        self.graph = g
+        self.captured = True
+        self._say('# scan: %d graphs captured' % total)
 
     def run(self):
-        """One full scan (graph replay for the small systems, direct launches
-        for the large ones, concurrently); returns after enqueueing."""
-        if self.small_lanes:
-            if self.graph is None:
-                self._run_lanes(self.small_lanes)
-            else:
-                self.graph.replay()
-        if self.big_lanes:
-            self._run_lanes(self.big_lanes)
+        """One full scan: on every lane, its graphs replayed (or, before
+        capture, its systems launched); lanes run concurrently.  Returns after
+        enqueueing."""
+        root = torch.cuda.current_stream()
+        for lane in self.lanes:
+            lane.stream.wait_stream(root)
+            with torch.cuda.stream(lane.stream):
+                if self.captured:
+                    for w, k in lane.items:
+                        if w.n > self.graph_max:
+                            solve_system(lane, w, k, self.plan_for(w))
+                    for g in lane.graphs:
+                        g.replay()
+                else:
+                    for w, k in lane.items:
+                        solve_system(lane, w, k, self.plan_for(w))
+        for lane in self.lanes:
+            root.wait_stream(lane.stream)
 
     @property
     def n_systems(self):
@@ -256,7 +301,7 @@ def serial_fp64(waves, pairs, tol=1e-12):
         solve_system(lane, w, k, plan, tol)
 
 
-def autotune(waves, sample=3, cache=None, lane_options=(2, 4, 8)):
+def autotune(waves, sample=3, cache=None, lane_options=(2, 4, 8), log=None):
     """Pick, per size class (N, nch), the refinement kind, the trailing GEMM
     tile and the look-ahead switch by timing `sample` energies of the class on
     one lane; then the number of lanes by timing a slice of the whole scan.
@@ -279,6 +324,8 @@ def autotune(waves, sample=3, cache=None, lane_options=(2, 4, 8)):
                                    'max_steps': 16, 'trail': trail, 'lookahead': lookahead})
     plans = {}
     for key, w in sorted(classes.items()):
+        if log:
+            print('# autotune: class N=%d nch=%d' % key, file=log, flush=True)
         lane = Lane(w.n, w.nch)
         ks = list(range(0, len(w.energies), max(1, len(w.energies) // sample)))[:sample]
         best = None
@@ -288,9 +335,11 @@ def autotune(waves, sample=3, cache=None, lane_options=(2, 4, 8)):
             if p['trail'] != 0 and w.n <= 512:
                 continue          # a single trailing update: the default tile
             with torch.cuda.stream(lane.stream):
-                ok = all(solve_system(lane, w, k, p, stats=True)['converged'] for k in ks)
-                if not ok:
+                sts = [solve_system(lane, w, k, p, stats=True) for k in ks]
+                if not all(st['converged'] for st in sts):
                     continue      # must reach the tolerance on every sampled energy
+                # launched steps: the most the sample needed plus a margin
+                p = dict(p, max_steps=min(16, max(st['sweeps'] for st in sts) + 3))
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
                 for k in ks:
@@ -306,6 +355,8 @@ def autotune(waves, sample=3, cache=None, lane_options=(2, 4, 8)):
     # lanes: time a slice (every 16th energy of every wave)
     best_l = None
     for nl in lane_options:
+        if log:
+            print('# autotune: %d lanes' % nl, file=log, flush=True)
         sc = EnergyScan(_slice_waves(waves, 16), lanes=nl, graph_max=0, plans=plans)
         sc.warm()
         t0 = time.perf_counter()

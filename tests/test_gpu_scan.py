@@ -24,9 +24,9 @@ def _oracle_s(w, k):
 
 def test_scan_matches_oracle():
     waves = _waves()
-    sc = scan.EnergyScan(waves, lanes=3, graph_max=300)
-    assert sc.small_lanes and sc.big_lanes      # both paths exercised
+    sc = scan.EnergyScan(waves, lanes=3, graph_max=300)   # some systems stay uncaptured
     sc.capture()
+    assert any(l.graphs for l in sc.lanes)
     sc.run()
     torch.cuda.synchronize()
     for w in waves:
