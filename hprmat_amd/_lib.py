@@ -34,7 +34,7 @@ NATIVE_SYMBOLS = (
     'hprmat_hip_lu_info', 'hprmat_hip_lu_pivots', 'hprmat_hip_lu_solve', 'hprmat_hip_solve',
     'hprmat_hip_assemble', 'hprmat_hip_r_to_s', 'hprmat_hip_r_to_s_work_bytes',
     'hprmat_hip_r_to_s_ws', 'hprmat_hip_adjoint_contract',
-    'hprmat_hip_prof_enable', 'hprmat_hip_prof_read', 'hprmat_hip_version',
+    'hprmat_hip_zgemm', 'hprmat_hip_prof_enable', 'hprmat_hip_prof_read', 'hprmat_hip_version',
 )
 
 
@@ -101,6 +101,7 @@ def load():
                                          DPtr, DPtr, ctypes.c_size_t, P]
     lib.hprmat_hip_lu_solve.argtypes = [P, P, L, I, I, D, I, I, P, ctypes.POINTER(Stats)]
     lib.hprmat_hip_solve.argtypes = [I, P, L, P, L, I, I, I, D, I, I, P, ctypes.POINTER(Stats)]
+    lib.hprmat_hip_zgemm.argtypes = [I, I, I, I, DPtr, L, I, DPtr, L, DPtr, L, I, I, P]
     lib.hprmat_hip_version.restype = ctypes.c_char_p
     lib.hprmat_hip_prof_enable.argtypes = [I]
     lib.hprmat_hip_prof_read.argtypes = [DP, DP, IP]

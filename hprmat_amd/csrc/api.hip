@@ -438,6 +438,26 @@ int hprmat_hip_lu_create(hprmat_hip_lu_t* h) {
   return 0;
 }
 
+int hprmat_hip_zgemm(int precision, int M, int N, int K, const void* A, int64_t lda, int a_col,
+                     const void* B, int64_t ldb, void* C, int64_t ldc, int alpha_sign, int beta,
+                     void* stream) {
+  if (M < 0 || N < 0 || K < 0 || (alpha_sign != 1 && alpha_sign != -1) || (beta != 0 && beta != 1))
+    return -1;
+  hipStream_t st = (hipStream_t)stream;
+  (void)hipGetLastError();
+  int rc;
+  if (precision == 64)
+    rc = hpr::gemm<double>(M, N, K, (const double2*)A, lda, a_col != 0, (const double2*)B, ldb,
+                           (double2*)C, ldc, (double)alpha_sign, beta, false, nullptr, st);
+  else if (precision == 32)
+    rc = hpr::gemm<float>(M, N, K, (const float2*)A, lda, a_col != 0, (const float2*)B, ldb,
+                          (float2*)C, ldc, (float)alpha_sign, beta, false, nullptr, st);
+  else
+    return -1;
+  if (rc) return rc;
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int hprmat_hip_lu_set_plan(hprmat_hip_lu_t h, int key, int value) {
   if (!h) return -1;
   switch (key) {

@@ -614,6 +614,32 @@ __device__ inline void prod64(const T (*ar)[65], const T (*ai)[65], const T (*br
   }
 }
 
+// C[16 w + i][16 t + j] += sum_k A[16 w + i][k] B[k][16 t + j] for one
+// 64 x 64 x 64 complex block product on MFMA, operands read from global
+// memory (small, L2-resident blocks); A is zero beyond arows x acols.
+template <class T>
+__device__ inline void prod64_g(const cx<T>* __restrict__ A, int64_t lda, int arows, int acols,
+                                const cx<T>* __restrict__ B, int64_t ldb,
+                                typename acc_t<T>::type* re, typename acc_t<T>::type* im) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+  const int i = 16 * w + li;
+#pragma unroll 4
+  for (int k0 = 0; k0 < 64; k0 += 4) {
+    const int k = k0 + lk;
+    const cx<T> a = (i < arows && k < acols) ? A[(int64_t)i * lda + k] : mkc<T>(0, 0);
+    cx<T> b[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) b[t] = B[(int64_t)k * ldb + 16 * t + li];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      re[t] = mfma16x4(a.x, b[t].x, re[t]);
+      re[t] = mfma16x4(-a.y, b[t].y, re[t]);
+      im[t] = mfma16x4(a.x, b[t].y, im[t]);
+      im[t] = mfma16x4(a.y, b[t].x, im[t]);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Inverse of one 256 x 256 diagonal block of a row-major LU factor, one 64-
 // column block column per workgroup (grid: blocks x {L, U} x 4), from the
@@ -623,13 +649,19 @@ __device__ inline void prod64(const T (*ar)[65], const T (*ai)[65], const T (*br
 //   U: X_jj = inv(U_jj),  X_ij = -inv(U_ii) sum_{k=i+1}^{j} U_ik X_kj  (i < j)
 // Output dinv256[(b*2+u)*65536 + r*256 + c], zero outside the triangle and
 // outside a trailing partial block.
+//
+// The kernel runs on the panel's critical path while the previous trailing
+// update occupies the CUs, so it uses no LDS and a GEMM workgroup's worth of
+// registers at most: it can start wherever one GEMM workgroup retires.  The
+// intermediate sum is parked in the output block X_ij itself.  A workgroup
+// only reads X blocks of its own block column, written by itself: barriers
+// (workgroup-scope release/acquire; the waves share the CU's L1) order them.
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(256)
 blk_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int bb0,
                const cx<T>* __restrict__ dinv64, cx<T>* __restrict__ dinv256) {
   typedef typename acc_t<T>::type acc;
-  __shared__ T ar[64][65], ai[64][65], br[64][65], bi[64][65];
   const int bb = bb0 + blockIdx.x, upper = blockIdx.y, jb = blockIdx.z;
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
   const int r0 = bb * 256, kb = min(256, n - r0), nsb = (kb + 63) / 64;
@@ -645,46 +677,30 @@ blk_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int bb0,
     const cx<T>* src = inv64(jb);
     for (int e = tid; e < 4096; e += 256) X[(jb * 64 + e / 64) * 256 + jb * 64 + e % 64] = src[e];
   }
-  __threadfence();
   __syncthreads();
-  auto load = [&](const cx<T>* src, int64_t lds, int rows, int cols, T (*dr)[65], T (*di)[65]) {
-    for (int e = tid; e < 4096; e += 256) {
-      const int i = e / 64, k = e % 64;
-      cx<T> v = (i < rows && k < cols) ? src[(int64_t)i * lds + k] : mkc<T>(0, 0);
-      dr[i][k] = v.x;
-      di[i][k] = v.y;
-    }
-  };
   const int step = upper ? -1 : 1;
   for (int ib = jb + step; ib >= 0 && ib < nsb; ib += step) {
     acc re[4], im[4];
     for (int t = 0; t < 4; ++t) { re[t] = acc{}; im[t] = acc{}; }
     const int k_lo = upper ? ib + 1 : jb, k_hi = upper ? jb : ib - 1;
-    for (int kk = k_lo; kk <= k_hi; ++kk) {
-      __syncthreads();
-      load(LU + (int64_t)(r0 + 64 * ib) * ld + r0 + 64 * kk, ld, min(64, kb - 64 * ib),
-           min(64, kb - 64 * kk), ar, ai);
-      load(X + (int64_t)(64 * kk) * 256 + 64 * jb, 256, 64, 64, br, bi);
-      __syncthreads();
-      prod64<T>(ar, ai, br, bi, re, im);
-    }
-    __syncthreads();
+    for (int kk = k_lo; kk <= k_hi; ++kk)
+      prod64_g<T>(LU + (int64_t)(r0 + 64 * ib) * ld + r0 + 64 * kk, ld, min(64, kb - 64 * ib),
+                  min(64, kb - 64 * kk), X + (int64_t)(64 * kk) * 256 + 64 * jb, 256, re, im);
+    cx<T>* Xij = X + (int64_t)(64 * ib) * 256 + 64 * jb;
     for (int t = 0; t < 4; ++t)
       for (int r = 0; r < 4; ++r) {
         const int row = 16 * w + mfma_row<T>(lk, r), col = 16 * t + li;
-        br[row][col] = re[t][r];
-        bi[row][col] = im[t][r];
+        Xij[(int64_t)row * 256 + col] = mkc<T>(re[t][r], im[t][r]);
       }
-    load(inv64(ib), 64, 64, 64, ar, ai);
     __syncthreads();
     for (int t = 0; t < 4; ++t) { re[t] = acc{}; im[t] = acc{}; }
-    prod64<T>(ar, ai, br, bi, re, im);
+    prod64_g<T>(inv64(ib), 64, 64, 64, Xij, 256, re, im);
+    __syncthreads();   // every wave has read the parked sum before it is overwritten
     for (int t = 0; t < 4; ++t)
       for (int r = 0; r < 4; ++r) {
         const int row = 16 * w + mfma_row<T>(lk, r), col = 16 * t + li;
-        X[(int64_t)(64 * ib + row) * 256 + 64 * jb + col] = mkc<T>(-re[t][r], -im[t][r]);
+        Xij[(int64_t)row * 256 + col] = mkc<T>(-re[t][r], -im[t][r]);
       }
-    __threadfence();
     __syncthreads();
   }
 }
