@@ -139,7 +139,7 @@ int residual_check(int n, int nrhs, const double2* A64, int64_t lda, const doubl
 int residual_norms(int n, int nrhs, const double2* R, double* part, RefineState* state, int mode,
                    int crit, double tol, const int* skip, hipStream_t st);
 
-// Workspace of gmres_solve for n x nrhs blocks and cycles of m <= 16 steps.
+// Workspace of gmres_solve for n x nrhs blocks and cycles of m <= GMRES_MAX_M steps.
 struct GmresWork {
   double2* V = nullptr;     // (m + 1) Krylov blocks, n x nrhs each
   double2* Z = nullptr;     // m preconditioned blocks
@@ -153,7 +153,7 @@ struct GmresWork {
   double* inv = nullptr;    // nrhs
   double2* part = nullptr;  // GMRES_DOT_BLOCKS x (m + 1) x nrhs partial dots
 };
-constexpr int GMRES_MAX_M = 16;
+constexpr int GMRES_MAX_M = 24;
 constexpr int GMRES_DOT_BLOCKS = 256;
 size_t gmres_work_bytes(int n, int nrhs, int m);
 void gmres_work_bind(GmresWork& w, void* base, int n, int nrhs, int m);

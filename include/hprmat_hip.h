@@ -100,7 +100,7 @@ enum {
   HPRMAT_CRIT_REL_FRO = 1,   /* ||B - A X||_F / ||B||_F <= tol               */
   /* flag OR-ed into the rule: flexible GMRES refinement (FP64 Arnoldi per
      right-hand side, the resident factors as right preconditioner, cycles of
-     min(max_refine, 16) steps); max_refine then bounds the preconditioned
+     min(max_refine, 24) steps); max_refine then bounds the preconditioned
      matrix-vector products, `sweeps` counts them */
   HPRMAT_FGMRES = 0x10
 };
@@ -220,6 +220,15 @@ int hprmat_hip_adjoint_contract(int nch, int nr, int nrhs, const void *dM, int64
    resets). */
 int hprmat_hip_prof_enable(int on);
 int hprmat_hip_prof_read(double *ms, double *flops, int *count);
+
+/* No reference counterpart: the library's complex GEMM on the device,
+   C = beta C + alpha A B with alpha = alpha_sign (+1 or -1), beta in {0, 1},
+   row-major C (ldc) and B (ldb), A row-major or, with a_col, column-major
+   (lda); precision 64 (complex128 data) or 32 (complex64 data).  The kernel
+   of the LU trailing update and of the FP64 refinement residual. */
+int hprmat_hip_zgemm(int precision, int M, int N, int K, const void *A, int64_t lda, int a_col,
+                     const void *B, int64_t ldb, void *C, int64_t ldc, int alpha_sign, int beta,
+                     void *stream);
 
 /* Library version string. */
 const char *hprmat_hip_version(void);

@@ -188,3 +188,48 @@ def test_full_size_mixed_solve_gate():
     st_ir = lu.solve_(b0, 30, 1e-12, 'rel_fro', True)
     assert st_ir['converged'] and st_ir['res_rel'] <= 1e-12, st_ir
     assert st['sweeps'] < st_ir['sweeps'], (st['sweeps'], st_ir['sweeps'])
+
+
+def _memory_system(w0_zero):
+    s = synth.System(256, 200, 15.0, 3.0, 6.0, 0, w0_zero=w0_zero)
+    dm = rmatrix.DeviceMesh(s.mesh)
+    o, co, cop, xc1 = rmatrix.channel_data(s.lval, s.qk, s.a)
+    assert len(o) == 256                       # all channels open at E = 6
+    a_cm = rmatrix.assemble(dm, s.lval, s.qk, xc1, s.cpot)
+    b = rmatrix.rhs_colmajor(dm, s.nch)
+    return s, dm, o, co, cop, a_cm, b
+
+
+def test_memory_scale_mixed_solve():
+    # N = 51200 (nch = 256 x nr = 200, about 42 GB as complex128 in HBM),
+    # 256 right-hand sides: FP32 factorization, FGMRES refinement.  The FP64
+    # attainable relative residual of this system is about 2e-12 (u ||A|| ||X||
+    # / ||B||), so the gate is 1e-11; S (256 x 256, FP64 on the device) is
+    # symmetric and, with absorption, |S_ij| <= 1.
+    s, dm, o, co, cop, a_cm, b = _memory_system(False)
+    lu = solver.LU().factor(a_cm, 'fp32', True)
+    st = lu.solve_(b, 40, 1e-11, 'rel_fro', True, refinement='fgmres')
+    print('memory-scale mixed: res_rel %.2e products %d hist %s'
+          % (st['res_rel'], st['sweeps'], ' '.join('%.1e' % h for h in st['hist'])))
+    assert st['converged'] and st['res_rel'] <= 1e-11, st
+    _, sm = rmatrix.r_to_s(dm, b, s.nch, s.qk, o, co, cop)
+    sm = sm.cpu().numpy()
+    assert np.abs(sm - sm.T).max() <= 1e-10
+    assert np.abs(sm).max() <= 1.0 + 1e-10     # absorption: |S_ij| <= 1
+
+
+def test_memory_scale_real_potential_unitarity():
+    # W0 = 0 (real potential): C is real symmetric indefinite with E inside
+    # its spectrum, far too ill-conditioned for FP32 factors to precondition;
+    # solved with the plain FP64 method.  S must be unitary and symmetric.
+    s, dm, o, co, cop, a_cm, b = _memory_system(True)
+    lu = solver.LU().factor(a_cm, 'fp64', True)
+    st = lu.solve_(b, 5, 1e-12, 'rel_fro', True)
+    _, sm = rmatrix.r_to_s(dm, b, s.nch, s.qk, o, co, cop)
+    sm = sm.cpu().numpy()
+    defect = np.abs(sm.conj().T @ sm - np.eye(256)).max()
+    print('memory-scale W0=0: res_rel %.2e sweeps %d unitarity defect %.2e'
+          % (st['res_rel'], st['sweeps'], defect))
+    assert st['res_rel'] <= 1e-10, st
+    assert defect <= 1e-8, defect
+    assert np.abs(sm - sm.T).max() <= 1e-8
