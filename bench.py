@@ -46,6 +46,18 @@ def flops_solve(n, nrhs):
     return 8.0 / 3.0 * n ** 3 + 8.0 * n * n * nrhs
 
 
+def traffic_record(prec):
+    """Counter-measured HBM bytes of one trailing-update launch (rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised under profiles/), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                        'trailing_gemm_traffic.json')
+    try:
+        with open(path) as f:
+            return json.load(f)[prec]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(nch=64, nr=200, reps=3):
     """Reference solver type 1 (ZGESV + R extraction) on host cores, bounded sample."""
     import numpy as np
@@ -88,6 +100,9 @@ def main():
     ap.add_argument('--nr', type=int, default=200)
     ap.add_argument('--methods', default='fp64,fp32_gmres,fp32,fp32_ir5,bf16x3,bf16x3_gmres')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-probe', type=int, default=0,
+                    help='copy this many bytes device-to-device once before the run (a '
+                         'kernel of known traffic for calibrating HBM counters)')
     ap.add_argument('--scan', action='store_true',
                     help='also run the batched energy scan (64 waves x 256 energies)')
     ap.add_argument('--scan-waves', type=int, default=64)
@@ -109,6 +124,12 @@ def main():
     # not be implicitly ordered against the legacy default stream)
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
+        if args.copy_probe > 0:
+            src = torch.ones(args.copy_probe // 4, dtype=torch.float32, device=dev)
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            del src, dst
         run(args, torch, lib, _lib, rmatrix, solver, synth)
 
 
@@ -189,6 +210,14 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
             'kernel': '%s (trailing update A22 -= L21 U12, %s)' % (
                 'zgemm_bf16x3_kernel' if head.startswith('bf16x3') else 'zgemm_kernel', head),
             'flops_per_launch': g['flops_per_launch'], 'ms_per_launch': g['ms_per_launch']}
+    tr = traffic_record('fp64' if head == 'fp64' else 'fp32')
+    if tr is not None and not head.startswith('bf16x3'):
+        roof['traffic'] = tr['hbm_bytes']
+        roof['traffic_launch'] = {
+            'M': tr['M'], 'N': tr['N'], 'K': tr['K'], 'hbm_read_bytes': tr['hbm_read_bytes'],
+            'hbm_write_bytes': tr['hbm_write_bytes'], 'alg_bytes': tr['alg_bytes'],
+            'source': 'profiles/trailing_gemm_counters_N25600.md (FETCH_SIZE, WRITE_SIZE, '
+                      'copy-calibrated; first trailing update)'}
     out = {
         'metric': METRIC, 'value': hm['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': hm['ms'],
