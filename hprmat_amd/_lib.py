@@ -20,6 +20,7 @@ CRIT_ABS_MAX = 0
 CRIT_REL_FRO = 1
 PLAN_TRAIL_TILE = 1      # hprmat_hip_lu_set_plan keys
 PLAN_LOOKAHEAD = 2
+PLAN_GMRES_CYCLE = 3
 FGMRES = 0x10            # flag OR-ed into the criterion: flexible GMRES refinement
 
 REFERENCE_SYMBOLS = (
@@ -35,6 +36,8 @@ NATIVE_SYMBOLS = (
     'hprmat_hip_assemble', 'hprmat_hip_r_to_s', 'hprmat_hip_r_to_s_work_bytes',
     'hprmat_hip_r_to_s_ws', 'hprmat_hip_adjoint_contract',
     'hprmat_hip_zgemm', 'hprmat_hip_prof_enable', 'hprmat_hip_prof_read', 'hprmat_hip_version',
+    'hprmat_hip_batch_work_bytes', 'hprmat_hip_batch_solve', 'hprmat_hip_assemble_batch',
+    'hprmat_hip_r_to_s_batch',
 )
 
 
@@ -102,6 +105,14 @@ def load():
     lib.hprmat_hip_lu_solve.argtypes = [P, P, L, I, I, D, I, I, P, ctypes.POINTER(Stats)]
     lib.hprmat_hip_solve.argtypes = [I, P, L, P, L, I, I, I, D, I, I, P, ctypes.POINTER(Stats)]
     lib.hprmat_hip_zgemm.argtypes = [I, I, I, I, DPtr, L, I, DPtr, L, DPtr, L, I, I, P]
+    lib.hprmat_hip_batch_work_bytes.argtypes = [I, I, I, I]
+    lib.hprmat_hip_batch_work_bytes.restype = ctypes.c_size_t
+    lib.hprmat_hip_batch_solve.argtypes = [I, I, I, I, DPtr, L, L, DPtr, L, L, I, D, I, DPtr,
+                                           DPtr, DPtr, ctypes.c_size_t, P]
+    lib.hprmat_hip_assemble_batch.argtypes = [I, I, I, DPtr, DPtr, DPtr, D, DPtr, DPtr, DPtr,
+                                              DPtr, DPtr, L, L, P]
+    lib.hprmat_hip_r_to_s_batch.argtypes = [I, I, I, DPtr, L, L, DPtr, D, D, DPtr, DPtr, DPtr,
+                                            DPtr, DPtr, DPtr, DPtr, P]
     lib.hprmat_hip_version.restype = ctypes.c_char_p
     lib.hprmat_hip_prof_enable.argtypes = [I]
     lib.hprmat_hip_prof_read.argtypes = [DP, DP, IP]

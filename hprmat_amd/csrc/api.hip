@@ -50,6 +50,7 @@ struct hprmat_hip_lu {
   hpr::LuPlan plan;
   hpr::SideStream side;       // look-ahead stream and its two events
   int lookahead = 1;
+  int gm_cycle = 0;           // FGMRES cycle length (0: GMRES_DEFAULT_CYCLE)
 };
 
 namespace {
@@ -205,7 +206,12 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
   const bool fgmres = (crit & HPRMAT_FGMRES) != 0;
   crit &= ~HPRMAT_FGMRES;
   if (fgmres) {
-    const int m = max_refine < 1 ? 1 : (max_refine > hpr::GMRES_MAX_M ? hpr::GMRES_MAX_M : max_refine);
+    // cycles of m steps; the budget max_refine spans several cycles, so a
+    // cycle that stops early (Arnoldi estimate below tol, true residual not)
+    // is followed by a restart from the true residual
+    int m = h->gm_cycle > 0 ? h->gm_cycle : hpr::GMRES_DEFAULT_CYCLE;
+    if (m > max_refine) m = max_refine;
+    if (m < 1) m = 1;
     if (nrhs > 1024) return -1;
     if (grow(&h->gm, &h->gm_cap, hpr::gmres_work_bytes(n, nrhs, m))) return -2;
     hpr::GmresWork gw;
@@ -469,6 +475,10 @@ int hprmat_hip_lu_set_plan(hprmat_hip_lu_t h, int key, int value) {
       if (value < 0 || value > 1) return -1;
       h->lookahead = value;
       return 0;
+    case HPRMAT_PLAN_GMRES_CYCLE:
+      if (value < 0 || value > hpr::GMRES_MAX_M) return -1;
+      h->gm_cycle = value;
+      return 0;
     default:
       return -1;
   }
@@ -520,6 +530,46 @@ int hprmat_hip_solve(int method, const void* dA, int64_t lda, void* dB, int64_t 
   if (rc) return rc;
   return lu_solve(&G.native, dB, ldb, nrhs, max_refine, tol, criterion, final_check,
                   (hipStream_t)stream, stats);
+}
+
+size_t hprmat_hip_batch_work_bytes(int method, int n, int nrhs, int count) {
+  method &= 0xff;
+  if (method != HPRMAT_PLAIN_FP64 && method != HPRMAT_MIXED_FP32) return 0;
+  return hpr::batch_work_bytes(n, nrhs, count, factor_elem_bytes(method));
+}
+
+int hprmat_hip_batch_solve(int method_flags, int n, int nrhs, int count, const void* dA,
+                           int64_t lda, int64_t stride_a, void* dB, int64_t ldb, int64_t stride_b,
+                           int max_refine, double tol, int criterion, double* res_rel,
+                           int* status, void* work, size_t work_bytes, void* stream) {
+  const int method = method_flags & 0xff;
+  const bool equil = (method_flags & HPRMAT_EQUILIBRATE) != 0;
+  if (method != HPRMAT_PLAIN_FP64 && method != HPRMAT_MIXED_FP32) {
+    fprintf(stderr, "hprmat_hip: batched solve supports HPRMAT_PLAIN_FP64 and HPRMAT_MIXED_FP32\n");
+    return -1;
+  }
+  if (criterion & HPRMAT_FGMRES) {
+    fprintf(stderr, "hprmat_hip: batched solve refines with classic sweeps only\n");
+    return -1;
+  }
+  if (count == 0 || n == 0) return 0;
+  if (n < 0 || nrhs <= 0 || count < 0 || lda < n || ldb < n || max_refine < 0 ||
+      (count > 1 && (stride_a < lda * n || stride_b < ldb * nrhs)) || !dA || !dB || !work)
+    return -1;
+  const hipStream_t st = (hipStream_t)stream;
+  if (method == HPRMAT_PLAIN_FP64) {
+    hpr::BatchArgs<double> b;
+    const int rc = hpr::batch_bind<double>(b, work, work_bytes, n, nrhs, count, equil);
+    if (rc) return rc;
+    return hpr::solve_batched<double>(b, (const double2*)dA, lda, stride_a, (double2*)dB, ldb,
+                                      stride_b, nrhs, max_refine, tol, criterion, res_rel, status,
+                                      st);
+  }
+  hpr::BatchArgs<float> b;
+  const int rc = hpr::batch_bind<float>(b, work, work_bytes, n, nrhs, count, equil);
+  if (rc) return rc;
+  return hpr::solve_batched<float>(b, (const double2*)dA, lda, stride_a, (double2*)dB, ldb,
+                                   stride_b, nrhs, max_refine, tol, criterion, res_rel, status, st);
 }
 
 int hprmat_hip_prof_enable(int on) { return hpr::prof_enable(on); }

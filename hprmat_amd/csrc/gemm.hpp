@@ -176,10 +176,12 @@ template <class T, int BM, int BN, int BK, int WM, int WN, bool A_COL, bool GAUS
 __global__ void __launch_bounds__(256)
 zgemm_kernel(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
              const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
-             int64_t ldc, T alpha, int beta, const int* __restrict__ skip) {
-  if (skip && *skip) return;
-  zgemm_tile<T, BM, BN, BK, WM, WN, A_COL, GAUSS3M>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta,
-                                                   blockIdx.x);
+             int64_t ldc, T alpha, int beta, const int* __restrict__ skip, int64_t sA = 0,
+             int64_t sB = 0, int64_t sC = 0, int sSkip = 0) {
+  const int b = blockIdx.y;   // system of a batch
+  if (skip && skip[(int64_t)b * sSkip]) return;
+  zgemm_tile<T, BM, BN, BK, WM, WN, A_COL, GAUSS3M>(M, N, K, A + b * sA, lda, B + b * sB, ldb,
+                                                   C + b * sC, ldc, alpha, beta, blockIdx.x);
 }
 
 // Two independent GEMM problems (row-major A) in one launch: workgroups
@@ -195,19 +197,24 @@ template <class T> struct GemmProb {
   int64_t ldc;
   T alpha;
   int beta;
+  int64_t sA = 0, sB = 0, sC = 0;   // strides between the systems of a batch
 };
 
 template <class T, int BM, int BN, int BK, int WM, int WN>
 __global__ void __launch_bounds__(256)
-zgemm_dual_kernel(GemmProb<T> p1, GemmProb<T> p2, int tiles1, const int* __restrict__ skip) {
-  if (skip && *skip) return;
+zgemm_dual_kernel(GemmProb<T> p1, GemmProb<T> p2, int tiles1, const int* __restrict__ skip,
+                  int sSkip = 0) {
+  const int b = blockIdx.y;   // system of a batch
+  if (skip && skip[(int64_t)b * sSkip]) return;
   const int t = blockIdx.x;
   if (t < tiles1)
-    zgemm_tile<T, BM, BN, BK, WM, WN, false, false>(p1.M, p1.N, p1.K, p1.A, p1.lda, p1.B, p1.ldb,
-                                                   p1.C, p1.ldc, p1.alpha, p1.beta, t);
+    zgemm_tile<T, BM, BN, BK, WM, WN, false, false>(p1.M, p1.N, p1.K, p1.A + b * p1.sA, p1.lda,
+                                                   p1.B + b * p1.sB, p1.ldb, p1.C + b * p1.sC,
+                                                   p1.ldc, p1.alpha, p1.beta, t);
   else
-    zgemm_tile<T, BM, BN, BK, WM, WN, false, false>(p2.M, p2.N, p2.K, p2.A, p2.lda, p2.B, p2.ldb,
-                                                   p2.C, p2.ldc, p2.alpha, p2.beta, t - tiles1);
+    zgemm_tile<T, BM, BN, BK, WM, WN, false, false>(p2.M, p2.N, p2.K, p2.A + b * p2.sA, p2.lda,
+                                                   p2.B + b * p2.sB, p2.ldb, p2.C + b * p2.sC,
+                                                   p2.ldc, p2.alpha, p2.beta, t - tiles1);
 }
 
 // ---------------------------------------------------------------------------
@@ -404,6 +411,169 @@ zgemm2_kernel(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
           cx<T>* pc = C + (int64_t)gi * ldc + gj;
           const cx<T> c0 = beta ? *pc : mkc<T>(0, 0);
           *pc = mkc<T>(c0.x + alpha * pr, c0.y + alpha * pi);
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 complex GEMM on 32x32x2 MFMA (same contract as zgemm_kernel, T = float).
+//
+// v_mfma_f32_32x32x2f32 reaches the FP32 matrix peak where 16x16x4 stops at
+// about 0.89 of it (measured in a register-only loop on gfx950).  Operand
+// layout: lane l supplies A[i = l % 32][k = l / 32] and B[k = l / 32][j = l % 32];
+// result register r of lane l holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+// Each wave owns WM x WN = (32 FM) x (32 FN) of the tile with separate real and
+// imaginary accumulators (four real products).  LDS double-buffered as in
+// zgemm2_kernel; a row-major A plane is stored k-major with the column index
+// XOR-swizzled by (2k mod 32): a read of 32 consecutive i stays a permutation of
+// the 32 banks, and the k-strided stores spread over all banks.
+// ---------------------------------------------------------------------------
+template <int BK, int BM, int BN>
+constexpr size_t zgemm32_lds_bytes() {
+  return (size_t)2 * 2 * BK * (BM + BN) * sizeof(float);
+}
+
+template <int BM, int BN, int BK, int WM, int WN, bool A_COL>
+__global__ void __launch_bounds__(256)
+zgemm32_kernel(int M, int N, int K, const cx<float>* __restrict__ A, int64_t lda,
+               const cx<float>* __restrict__ B, int64_t ldb, cx<float>* __restrict__ C,
+               int64_t ldc, float alpha, int beta, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  static_assert((BM / WM) * (BN / WN) == 4, "four waves per workgroup");
+  static_assert(WM % 32 == 0 && WN % 32 == 0 && BK % 2 == 0 && BM % 32 == 0, "tile shape");
+  constexpr int NT = 256, FM = WM / 32, FN = WN / 32;
+  constexpr int EA = BM * BK / NT, EB = BK * BN / NT;
+  static_assert(EA * NT == BM * BK && EB * NT == BK * BN, "load split");
+  typedef float v16f __attribute__((ext_vector_type(16)));
+  extern __shared__ __align__(16) unsigned char z32_smem[];
+  float* As = reinterpret_cast<float*>(z32_smem);   // [buf][plane][BK][BM]
+  float* Bs = As + 2 * 2 * BK * BM;                  // [buf][plane][BK][BN]
+
+  const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+  const int total = tiles_m * tiles_n;
+  int t = blockIdx.x;
+  if ((total & 7) == 0) t = (t & 7) * (total >> 3) + (t >> 3);
+  constexpr int GROUP = 8;
+  const int group = t / (GROUP * tiles_n);
+  const int first_m = group * GROUP;
+  const int gsz = min(tiles_m - first_m, GROUP);
+  const int tm = first_m + (t % (GROUP * tiles_n)) % gsz;
+  const int tn = (t % (GROUP * tiles_n)) / gsz;
+  const int row0 = tm * BM, col0 = tn * BN;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave / (BN / WN), wc = wave % (BN / WN);
+  const int l32 = lane & 31, lh = lane >> 5;
+
+  v16f acc_re[FM][FN], acc_im[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) { acc_re[a][b][r] = 0.f; acc_im[a][b][r] = 0.f; }
+
+  cx<float> ra[EA], rb[EB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < EA; ++e) {
+      const int idx = tid + e * NT;
+      int i, k;
+      if (A_COL) { i = idx % BM; k = idx / BM; } else { i = idx / BK; k = idx % BK; }
+      const int gi = row0 + i, gk = k0 + k;
+      ra[e] = (gi < M && gk < K)
+                  ? (A_COL ? A[(int64_t)gk * lda + gi] : A[(int64_t)gi * lda + gk])
+                  : mkc<float>(0.f, 0.f);
+    }
+#pragma unroll
+    for (int e = 0; e < EB; ++e) {
+      const int idx = tid + e * NT, k = idx / BN, j = idx % BN;
+      const int gk = k0 + k, gj = col0 + j;
+      rb[e] = (gk < K && gj < N) ? B[(int64_t)gk * ldb + gj] : mkc<float>(0.f, 0.f);
+    }
+  };
+  auto store = [&](int buf) {
+    float* a0 = As + (size_t)(buf * 2) * BK * BM;
+    float* a1 = a0 + BK * BM;
+    float* b0 = Bs + (size_t)(buf * 2) * BK * BN;
+    float* b1 = b0 + BK * BN;
+#pragma unroll
+    for (int e = 0; e < EA; ++e) {
+      const int idx = tid + e * NT;
+      int i, k;
+      if (A_COL) { i = idx % BM; k = idx / BM; } else { i = idx / BK; k = idx % BK; i ^= (2 * k) & 31; }
+      a0[k * BM + i] = ra[e].x;
+      a1[k * BM + i] = ra[e].y;
+    }
+#pragma unroll
+    for (int e = 0; e < EB; ++e) {
+      const int idx = tid + e * NT, k = idx / BN, j = idx % BN;
+      b0[k * BN + j] = rb[e].x;
+      b1[k * BN + j] = rb[e].y;
+    }
+  };
+
+  load(0);
+  store(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    const bool more = k0 + BK < K;
+    if (more) load(k0 + BK);
+    const float* a0 = As + (size_t)(buf * 2) * BK * BM;
+    const float* a1 = a0 + BK * BM;
+    const float* b0 = Bs + (size_t)(buf * 2) * BK * BN;
+    const float* b1 = b0 + BK * BN;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 2) {
+      const int k = kk + lh;
+      float ar[FM], ai[FM], nai[FM], br[FN], bi[FN];
+#pragma unroll
+      for (int a = 0; a < FM; ++a) {
+        int i = wr * WM + a * 32 + l32;
+        if (!A_COL) i ^= (2 * k) & 31;
+        ar[a] = a0[k * BM + i];
+        ai[a] = a1[k * BM + i];
+        nai[a] = -ai[a];
+      }
+#pragma unroll
+      for (int b = 0; b < FN; ++b) {
+        const int j = wc * WN + b * 32 + l32;
+        br[b] = b0[k * BN + j];
+        bi[b] = b1[k * BN + j];
+      }
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+          acc_re[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[a], br[b], acc_re[a][b], 0, 0, 0);
+          acc_im[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[a], bi[b], acc_im[a][b], 0, 0, 0);
+        }
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+          acc_re[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(nai[a], bi[b], acc_re[a][b], 0, 0, 0);
+          acc_im[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai[a], br[b], acc_im[a][b], 0, 0, 0);
+        }
+    }
+    if (more) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int gi = row0 + wr * WM + a * 32 + 8 * (r / 4) + 4 * lh + (r % 4);
+        const int gj = col0 + wc * WN + b * 32 + l32;
+        if (gi < M && gj < N) {
+          cx<float>* pc = C + (int64_t)gi * ldc + gj;
+          const cx<float> c0 = beta ? *pc : mkc<float>(0.f, 0.f);
+          *pc = mkc<float>(c0.x + alpha * acc_re[a][b][r], c0.y + alpha * acc_im[a][b][r]);
         }
       }
 }

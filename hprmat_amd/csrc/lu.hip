@@ -54,7 +54,14 @@ template <class Tin, class Tout>
 __global__ void __launch_bounds__(256)
 transpose_cast_kernel(int rows, int cols, const cx<Tin>* __restrict__ in, int64_t ldi,
                       cx<Tout>* __restrict__ out, int64_t ldo, const double* __restrict__ rs,
-                      const double* __restrict__ cs) {
+                      const double* __restrict__ cs, int64_t sIn = 0, int64_t sOut = 0,
+                      int sScale = 0) {
+  in += blockIdx.z * sIn;   // system of a batch
+  out += blockIdx.z * sOut;
+  if (rs) {
+    rs += (int64_t)blockIdx.z * sScale;
+    cs += (int64_t)blockIdx.z * sScale;
+  }
   __shared__ cx<Tin> tile[32][33];
   const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -79,8 +86,10 @@ transpose_cast_kernel(int rows, int cols, const cx<Tin>* __restrict__ in, int64_
 
 template <class Tin, class Tout>
 __global__ void cast_kernel(size_t n, const cx<Tin>* __restrict__ in, cx<Tout>* __restrict__ out,
-                            const int* __restrict__ skip) {
-  if (skip && *skip) return;
+                            const int* __restrict__ skip, int64_t sV = 0, int sSkip = 0) {
+  if (skip && skip[(int64_t)blockIdx.y * sSkip]) return;
+  in += blockIdx.y * sV;
+  out += blockIdx.y * sV;
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; i < n; i += (size_t)gridDim.x * blockDim.x) {
     cx<Tin> v = in[i];
@@ -94,7 +103,15 @@ __global__ void cast_kernel(size_t n, const cx<Tin>* __restrict__ in, cx<Tout>* 
 template <class TF>
 __global__ void add_correction_kernel(int nrhs, size_t cnt, double2* __restrict__ X,
                                       const cx<TF>* __restrict__ D, const double* __restrict__ s,
-                                      int accumulate, RefineState* __restrict__ st) {
+                                      int accumulate, RefineState* __restrict__ st,
+                                      int64_t sV = 0, int sS = 0) {
+  {
+    const int b = blockIdx.y;   // system of a batch
+    X += b * sV;
+    D += b * sV;
+    if (s) s += (int64_t)b * sS;
+    st += b;
+  }
   if (accumulate && st->done) return;
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0 && accumulate) st->sweeps += 1;
@@ -110,7 +127,9 @@ __global__ void add_correction_kernel(int nrhs, size_t cnt, double2* __restrict_
 
 // s[i] = 1 / sqrt(|a_ii|) of a column-major matrix (1 where a_ii = 0).
 __global__ void diag_scale_kernel(int n, const double2* __restrict__ A, int64_t lda,
-                                  double* __restrict__ s) {
+                                  double* __restrict__ s, int64_t sA = 0, int sS = 0) {
+  A += blockIdx.y * sA;   // system of a batch
+  s += (int64_t)blockIdx.y * sS;
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const double2 a = A[(int64_t)i * lda + i];
@@ -123,8 +142,10 @@ __global__ void diag_scale_kernel(int n, const double2* __restrict__ A, int64_t 
 constexpr int NORM_BLOCKS = 1024;
 __global__ void __launch_bounds__(256)
 norm_partial_kernel(int n, int nrhs, const double2* __restrict__ R, double* __restrict__ part,
-                    const int* __restrict__ skip) {
-  if (skip && *skip) return;
+                    const int* __restrict__ skip, int64_t sR = 0, int sSkip = 0) {
+  if (skip && skip[(int64_t)blockIdx.y * sSkip]) return;
+  R += blockIdx.y * sR;   // system of a batch
+  part += (size_t)blockIdx.y * gridDim.x * 5;
   __shared__ double sv[256], sm[256], ss[256], sa[256];
   __shared__ long long si[256];
   const size_t total = (size_t)n * nrhs;
@@ -167,6 +188,8 @@ norm_partial_kernel(int n, int nrhs, const double2* __restrict__ R, double* __re
 __global__ void __launch_bounds__(256)
 norm_final_kernel(int nblocks, const double* __restrict__ part, RefineState* __restrict__ st,
                   int mode, int crit, double tol) {
+  part += (size_t)blockIdx.x * nblocks * 5;   // one workgroup per system of a batch
+  st += blockIdx.x;
   if (mode == 1 && st->done) return;
   __shared__ double sv[256], sm[256], ss[256], si[256], sa[256];
   double bv = -1.0, bi = 1e300, bm = 0.0, sq = 0.0, am = 0.0;
@@ -641,6 +664,316 @@ int residual_check(int n, int nrhs, const double2* A64, int64_t lda, const doubl
   return 0;
 }
 
+// ===========================================================================
+// Uniform batches: `count` systems of one order n, factored and solved in
+// lockstep.  Every launch carries the system index in a grid dimension, so
+// one chain of kernels serves the whole batch: a system of a few hundred to
+// a few thousand unknowns is a chain of 10^2 - 10^3 small dependent kernels
+// whose dispatch latency, not their work, dominates when systems run one at
+// a time.  No look-ahead (the batch fills the GPU).  The cooperative panel
+// leaf is launched for at most BATCH_LEAF_WG workgroups at a time; the
+// workgroups of one system are consecutive in dispatch order, so the
+// earliest unfinished system always has all of its workgroups resident.
+// ===========================================================================
+namespace {
+constexpr int BATCH_LEAF_WG = 480;
+
+inline int64_t up32(int64_t x) { return (x + 31) & ~(int64_t)31; }
+inline size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// exchange words of one system's leaf: val [2][G][2], idx [2][G],
+// row [2][G][LEAF_W][4], rowj [2][LEAF_W][4] (G = leaf workgroups)
+inline int batch_leaf_g(int n) { return n > 256 ? (n + 255) / 256 : 1; }
+inline int64_t batch_xchg_words(int n) {
+  const int64_t g = batch_leaf_g(n);
+  return up32(4 * g + 2 * g + 2 * g * LEAF_W * 4 + 2 * LEAF_W * 4);
+}
+
+template <class T>
+void gemm_b(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<T>* B,
+            int64_t ldb, cx<T>* C, int64_t ldc, T alpha, int beta, const int* skip, int sSkip,
+            int count, int64_t sA, int64_t sB, int64_t sC, hipStream_t st) {
+  if (M <= 0 || N <= 0 || K <= 0 || count <= 0) return;
+  constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32;
+  const dim3 grid(((M + BM - 1) / BM) * ((N + BN - 1) / BN), count);
+  if (a_col)
+    zgemm_kernel<T, BM, BN, BK, WM, WN, true, false><<<grid, 256, 0, st>>>(
+        M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, sA, sB, sC, sSkip);
+  else
+    zgemm_kernel<T, BM, BN, BK, WM, WN, false, false><<<grid, 256, 0, st>>>(
+        M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, sA, sB, sC, sSkip);
+}
+
+template <class T>
+void laswp_b(const BatchArgs<T>& b, int c0, int c1, int k1, int nk, hipStream_t st) {
+  if (c1 <= c0 || nk <= 0) return;
+  constexpr int CW = 128 / sizeof(cx<T>);
+  const size_t lds = (size_t)2 * nk * 128 + (size_t)4 * nk * sizeof(int);
+  static bool attr = false;
+  if (!attr) { allow_big_lds(laswp_kernel<T>); attr = true; }
+  const dim3 grid((c1 - c0 + CW - 1) / CW, b.count);
+  laswp_kernel<T><<<grid, 256, lds, st>>>(b.F, b.ld, c0, c1, k1, nk, b.ipiv, nullptr, b.sF,
+                                          b.sIv, 0);
+}
+
+inline dim3 copy_grid(size_t cnt, int count) {
+  return dim3((unsigned)min((size_t)1024, (cnt + 255) / 256), (unsigned)count);
+}
+
+void norms_b(int n, int nrhs, const double2* R, int64_t sR, double* part, RefineState* state,
+             int mode, int crit, double tol, const int* skip, int sSkip, int count,
+             hipStream_t st) {
+  const size_t total = (size_t)n * nrhs;
+  int blocks = (int)min((size_t)BATCH_NORM_BLOCKS, (total + 255) / 256);
+  if (blocks < 1) blocks = 1;
+  norm_partial_kernel<<<dim3(blocks, count), 256, 0, st>>>(n, nrhs, R, part, skip, sR, sSkip);
+  norm_final_kernel<<<count, 256, 0, st>>>(blocks, part, state, mode, crit, tol);
+}
+
+// per-system outcome: relative residual of the last evaluation and
+// 1 converged / 0 not converged / -info (exactly singular pivot)
+__global__ void batch_result_kernel(int count, const RefineState* __restrict__ st,
+                                    const int* __restrict__ info, double* __restrict__ res,
+                                    int* __restrict__ conv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const double bf = st[i].b_fro != 0.0 ? st[i].b_fro : 1.0;
+  if (res) res[i] = st[i].res_fro / bf;
+  if (conv) conv[i] = info[i] != 0 ? -info[i] : st[i].done;
+}
+}  // namespace
+
+size_t batch_work_bytes(int n, int nrhs, int count, size_t elem) {
+  if (n <= 0 || nrhs <= 0 || count <= 0) return 0;
+  const int64_t ld = (((int64_t)n + 15) / 16) * 16 + 16;
+  const int64_t sIv = (n + 1 + 63) & ~63;
+  const int64_t sV = up32((int64_t)n * nrhs);
+  size_t per = 0;
+  per += (size_t)up32(ld * n) * elem;                                   // factors
+  per += (dinv64_elems(n) + dinv256_elems(n)) * elem;                  // inverses
+  per += (size_t)up32((int64_t)ustage_elems(n)) * elem;                 // U12 staging
+  per += (size_t)sIv * (2 * sizeof(int) + sizeof(double));             // ipiv, perm, scale
+  per += (size_t)batch_xchg_words(n) * sizeof(u64);                    // leaf exchange
+  per += sizeof(RefineState) + sizeof(int) + BATCH_NORM_BLOCKS * 5 * sizeof(double);
+  per += (size_t)sV * (3 * sizeof(double2) + 2 * elem);                // Brm, Xrm, R, RF, RY
+  return per * (size_t)count + 16 * 256;                               // + alignment, err
+}
+
+template <class T>
+int batch_bind(BatchArgs<T>& b, void* base, size_t bytes, int n, int nrhs, int count, bool equil) {
+  const size_t e = sizeof(cx<T>);
+  if (!base || n <= 0 || nrhs <= 0 || count <= 0) return -1;
+  if (batch_work_bytes(n, nrhs, count, e) > bytes) return -2;
+  char* p = (char*)base;
+  auto take = [&](size_t total) {
+    p = (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
+    char* q = p;
+    p += total;
+    return q;
+  };
+  b.n = n;
+  b.count = count;
+  b.ld = (((int64_t)n + 15) / 16) * 16 + 16;
+  b.sF = up32(b.ld * n);
+  b.s64 = (int64_t)dinv64_elems(n);
+  b.s256 = (int64_t)dinv256_elems(n);
+  b.sU = up32((int64_t)ustage_elems(n));
+  b.sIv = (n + 1 + 63) & ~63;
+  b.sX = batch_xchg_words(n);
+  b.gstride = batch_leaf_g(n);
+  b.sV = up32((int64_t)n * nrhs);
+  b.F = (cx<T>*)take((size_t)b.sF * count * e);
+  b.dinv64 = (cx<T>*)take((size_t)b.s64 * count * e);
+  b.dinv256 = (cx<T>*)take((size_t)b.s256 * count * e);
+  b.ustage = (cx<T>*)take((size_t)b.sU * count * e);
+  b.ipiv = (int*)take((size_t)b.sIv * count * sizeof(int));
+  b.perm = (int*)take((size_t)b.sIv * count * sizeof(int));
+  double* sc = (double*)take((size_t)b.sIv * count * sizeof(double));
+  b.scale = equil ? sc : nullptr;
+  b.xchg = (u64*)take((size_t)b.sX * count * sizeof(u64));
+  b.state = (RefineState*)take(sizeof(RefineState) * count);
+  b.info = (int*)take(sizeof(int) * count);
+  b.part = (double*)take((size_t)BATCH_NORM_BLOCKS * 5 * count * sizeof(double));
+  b.Brm = (double2*)take((size_t)b.sV * count * sizeof(double2));
+  b.Xrm = (double2*)take((size_t)b.sV * count * sizeof(double2));
+  b.R = (double2*)take((size_t)b.sV * count * sizeof(double2));
+  b.RF = (cx<T>*)take((size_t)b.sV * count * e);
+  b.RY = (cx<T>*)take((size_t)b.sV * count * e);
+  b.err = (int*)take(sizeof(int));
+  return (size_t)(p - (char*)base) <= bytes ? 0 : -2;
+}
+
+template <class T>
+int getrf_batched(const BatchArgs<T>& b, hipStream_t st) {
+  const int n = b.n, count = b.count;
+  if (n <= 0 || count <= 0) return 0;
+  const int64_t ld = b.ld, sF = b.sF;
+  (void)hipGetLastError();   // start from a clean error state (checked at the end)
+  fill_u32_kernel<<<1024, 256, 0, st>>>((unsigned*)b.xchg, (size_t)b.sX * 2 * count, 0u);
+  fill_u32_kernel<<<1, 64, 0, st>>>((unsigned*)b.err, 1, 0u);
+  fill_u32_kernel<<<(count + 255) / 256, 256, 0, st>>>((unsigned*)b.info, (size_t)count, 0u);
+  int leaf = 0;
+  for (int k = 0; k < n; k += 256) {
+    const int kb = min(256, n - k), m = n - k;
+    for (int t = 0; t < kb; t += LEAF_W) {
+      const int wt = min(LEAF_W, kb - t);
+      cx<T>* P = b.F + (int64_t)(k + t) * ld + (k + t);
+      const int G = (m - t + 255) / 256;
+      if (G > b.gstride) return -1;
+      const int chunk = max(1, BATCH_LEAF_WG / G);
+      for (int c0 = 0; c0 < count; c0 += chunk) {
+        const int cn = min(chunk, count - c0);
+        LeafXchg x;
+        x.val = b.xchg + (int64_t)c0 * b.sX;
+        x.idx = x.val + 4 * (int64_t)b.gstride;
+        x.row = x.idx + 2 * (int64_t)b.gstride;
+        x.rowj = x.row + 2 * (int64_t)b.gstride * LEAF_W * 4;
+        x.gstride = b.gstride;
+        leaf_getf2_kernel<T, LEAF_W><<<dim3(G, cn), 256, 0, st>>>(
+            P + (int64_t)c0 * sF, ld, m - t, wt, k + t, b.ipiv + (int64_t)c0 * b.sIv + k + t,
+            b.info + c0, x, (unsigned)leaf * LEAF_W, b.err, sF, b.sIv, b.sX);
+      }
+      ++leaf;
+      if (t > 0) laswp_b<T>(b, k, k + t, k + t, wt, st);
+      if (t + wt < kb) {
+        laswp_b<T>(b, k + t + wt, k + kb, k + t, wt, st);
+        trsm_diag_kernel<T, false><<<dim3((kb - t - wt + 15) / 16, count), 256, 0, st>>>(
+            P, ld, wt, P + wt, ld, kb - t - wt, nullptr, sF);
+        gemm_b<T>(m - t - wt, kb - t - wt, wt, P + (int64_t)wt * ld, ld, false, P + wt, ld,
+                  P + (int64_t)wt * ld + wt, ld, T(-1), 1, nullptr, 0, count, sF, sF, sF, st);
+      }
+    }
+    diag_inv_kernel<T><<<dim3((kb + 63) / 64, 2, count), 256, 0, st>>>(b.F, ld, n, k / 64,
+                                                                       b.dinv64, sF, b.s64);
+    blk_inv_kernel<T><<<dim3(1, 2, 4 * count), 256, 0, st>>>(b.F, ld, n, k / 256, b.dinv64,
+                                                             b.dinv256, sF, b.s64, b.s256);
+    if (k > 0) laswp_b<T>(b, 0, k, k, kb, st);
+    if (k + kb < n) {
+      const int nc = n - k - kb;
+      cx<T>* Akk = b.F + (int64_t)k * ld + k;
+      laswp_b<T>(b, k + kb, n, k, kb, st);
+      // U12 = inv(L11) A12 (staged, row stride nc), A22 -= L21 U12, U12 back
+      gemm_b<T>(kb, nc, kb, b.dinv256 + (size_t)(k / 256) * 2 * 65536, 256, false, Akk + kb, ld,
+                b.ustage, nc, T(1), 0, nullptr, 0, count, b.s256, sF, b.sU, st);
+      gemm_b<T>(nc, nc, kb, Akk + (int64_t)kb * ld, ld, false, b.ustage, nc,
+                Akk + (int64_t)kb * ld + kb, ld, T(-1), 1, nullptr, 0, count, sF, b.sU, sF, st);
+      copy2d_kernel<T><<<copy_grid((size_t)kb * nc, count), 256, 0, st>>>(
+          kb, nc, b.ustage, nc, Akk + kb, ld, b.sU, sF);
+    }
+  }
+  // fold inv(L_bb) into the blocks of L below the diagonal (as fold_inverses)
+  for (int b0 = 0; b0 < n; b0 += 256) {
+    const int bs = min(256, n - b0), ml = n - b0 - bs;
+    if (ml <= 0) break;
+    const cx<T>* iL = b.dinv256 + (size_t)(b0 / 256) * 2 * 65536;
+    cx<T>* Lb = b.F + (int64_t)(b0 + bs) * ld + b0;
+    gemm_b<T>(ml, bs, bs, Lb, ld, false, iL, 256, b.ustage, bs, T(1), 0, nullptr, 0, count, sF,
+              b.s256, b.sU, st);
+    copy2d_kernel<T><<<copy_grid((size_t)ml * bs, count), 256, 0, st>>>(ml, bs, b.ustage, bs, Lb,
+                                                                         ld, b.sU, sF);
+  }
+  const size_t perm_lds = (size_t)n * sizeof(int);
+  const int in_lds = perm_lds <= 140 * 1024;
+  static bool attr = false;
+  if (!attr) { allow_big_lds(perm_from_ipiv_kernel); attr = true; }
+  perm_from_ipiv_kernel<<<count, 256, in_lds ? perm_lds : 0, st>>>(b.ipiv, n, b.perm, in_lds,
+                                                                    b.sIv);
+  HPR_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <class T>
+int getrs_batched(const BatchArgs<T>& b, cx<T>* B, cx<T>* Y, int nrhs, const int* skip, int sSkip,
+                  hipStream_t st) {
+  const int n = b.n, count = b.count;
+  if (n <= 0 || nrhs <= 0 || count <= 0) return 0;
+  const int64_t ld = b.ld, sV = b.sV;
+  constexpr int BM = 64, BN = 64;
+  auto tiles = [](const GemmProb<T>& p) {
+    return (p.M > 0 && p.N > 0 && p.K > 0) ? ((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN) : 0;
+  };
+  // forward (L holds W = L[>b, b] inv(L_bb)): Y_b = inv(L_bb) B_b and
+  // B_>b -= W B_b in one launch per block
+  for (int r = 0; r < n; r += 256) {
+    const int bs = min(256, n - r);
+    GemmProb<T> p1 = prob<T>(bs, nrhs, bs, b.dinv256 + (size_t)(r / 256) * 2 * 65536, 256,
+                             B + (int64_t)r * nrhs, nrhs, Y + (int64_t)r * nrhs, nrhs, T(1), 0);
+    GemmProb<T> p2 = prob<T>(n - r - bs, nrhs, bs, b.F + (int64_t)(r + bs) * ld + r, ld,
+                             B + (int64_t)r * nrhs, nrhs, B + (int64_t)(r + bs) * nrhs, nrhs,
+                             T(-1), 1);
+    p1.sA = b.s256; p1.sB = sV; p1.sC = sV;
+    p2.sA = b.sF; p2.sB = sV; p2.sC = sV;
+    const int t1 = tiles(p1), t2 = tiles(p2);
+    zgemm_dual_kernel<T, 64, 64, 16, 32, 32><<<dim3(t1 + t2, count), 256, 0, st>>>(p1, p2, t1, skip,
+                                                                                   sSkip);
+  }
+  // backward: B_b = inv(U_bb) Y_b, then Y_<b -= U[<b, b] B_b
+  for (int r = ((n - 1) / 256) * 256; r >= 0; r -= 256) {
+    const int bs = min(256, n - r);
+    gemm_b<T>(bs, nrhs, bs, b.dinv256 + ((size_t)(r / 256) * 2 + 1) * 65536, 256, false,
+              Y + (int64_t)r * nrhs, nrhs, B + (int64_t)r * nrhs, nrhs, T(1), 0, skip, sSkip,
+              count, b.s256, sV, sV, st);
+    if (r > 0)
+      gemm_b<T>(r, nrhs, bs, b.F + r, ld, false, B + (int64_t)r * nrhs, nrhs, Y, nrhs, T(-1), 1,
+                skip, sSkip, count, b.sF, sV, sV, st);
+  }
+  HPR_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <class TF>
+int solve_batched(const BatchArgs<TF>& b, const double2* A64, int64_t lda, int64_t sA,
+                  double2* dB, int64_t ldb, int64_t sB, int nrhs, int max_refine, double tol,
+                  int crit, double* res_out, int* conv_out, hipStream_t st) {
+  const int n = b.n, count = b.count;
+  if (n <= 0 || count <= 0) return 0;
+  if (nrhs <= 0 || lda < n || ldb < n || max_refine < 0) return -1;
+  const int64_t sV = b.sV;
+  const size_t cnt = (size_t)n * nrhs;
+  const dim3 eg = copy_grid(cnt, count);
+  const int* done = &b.state->done;
+  constexpr int sDone = (int)(sizeof(RefineState) / sizeof(int));
+  static_assert(sizeof(RefineState) % sizeof(int) == 0, "state stride in ints");
+  // factors of the (equilibrated) matrices, row-major, factor precision
+  if (b.scale)
+    diag_scale_kernel<<<dim3((n + 255) / 256, count), 256, 0, st>>>(n, A64, lda, b.scale, sA,
+                                                                   b.sIv);
+  transpose_cast_kernel<double, TF><<<dim3((n + 31) / 32, (n + 31) / 32, count), 256, 0, st>>>(
+      n, n, A64, lda, b.F, b.ld, b.scale, b.scale, sA, b.sF, b.sIv);
+  if (getrf_batched<TF>(b, st)) return -1;
+  // right-hand sides, row-major FP64
+  transpose_cast_kernel<double, double>
+      <<<dim3((n + 31) / 32, (nrhs + 31) / 32, count), 256, 0, st>>>(
+          n, nrhs, dB, ldb, b.Brm, nrhs, nullptr, nullptr, sB, sV, 0);
+  norms_b(n, nrhs, b.Brm, sV, b.part, b.state, 0, crit, tol, nullptr, 0, count, st);
+  permute_cast_kernel<double, TF><<<eg, 256, 0, st>>>(n, nrhs, b.perm, b.Brm, b.RF, b.scale,
+                                                      nullptr, b.sIv, sV, 0);
+  if (getrs_batched<TF>(b, b.RF, b.RY, nrhs, nullptr, 0, st)) return -1;
+  add_correction_kernel<TF><<<eg, 256, 0, st>>>(nrhs, cnt, b.Xrm, b.RF, b.scale, 0, b.state, sV,
+                                                b.sIv);
+  // classic refinement: residual in FP64, correction through the factors;
+  // a converged system skips every later launch; the last evaluation only checks
+  for (int it = 0; it <= max_refine; ++it) {
+    cast_kernel<double, double><<<eg, 256, 0, st>>>(cnt, b.Brm, b.R, done, sV, sDone);
+    gemm_b<double>(n, nrhs, n, A64, lda, true, b.Xrm, nrhs, b.R, nrhs, -1.0, 1, done, sDone,
+                   count, sA, sV, sV, st);
+    norms_b(n, nrhs, b.R, sV, b.part, b.state, 1, crit, tol, done, sDone, count, st);
+    if (it == max_refine) break;
+    permute_cast_kernel<double, TF><<<eg, 256, 0, st>>>(n, nrhs, b.perm, b.R, b.RF, b.scale, done,
+                                                        b.sIv, sV, sDone);
+    if (getrs_batched<TF>(b, b.RF, b.RY, nrhs, done, sDone, st)) return -1;
+    add_correction_kernel<TF><<<eg, 256, 0, st>>>(nrhs, cnt, b.Xrm, b.RF, b.scale, 1, b.state, sV,
+                                                  b.sIv);
+  }
+  transpose_cast_kernel<double, double>
+      <<<dim3((nrhs + 31) / 32, (n + 31) / 32, count), 256, 0, st>>>(
+          nrhs, n, b.Xrm, nrhs, dB, ldb, nullptr, nullptr, sV, sB, 0);
+  batch_result_kernel<<<(count + 255) / 256, 256, 0, st>>>(count, b.state, b.info, res_out,
+                                                           conv_out);
+  HPR_CHECK(hipGetLastError());
+  return 0;
+}
+
 #define INST(T)                                                                               \
   template int getrf<T>(cx<T>*, int64_t, int, int*, int*, const FactorWork<T>&, const Scratch&, \
                         const LuPlan&, int, hipStream_t, const SideStream*);                   \
@@ -651,7 +984,14 @@ int residual_check(int n, int nrhs, const double2* A64, int64_t lda, const doubl
   template int refine_solve<T>(const cx<T>*, int64_t, int, const cx<T>*, const int*,          \
                                const double2*, int64_t, const double2*, double2*, int, int,    \
                                double, int, RefineState*, double2*, cx<T>*, cx<T>*, double*,   \
-                               const LuPlan&, bool, hipStream_t, const double*);
+                               const LuPlan&, bool, hipStream_t, const double*);       \
+  template int batch_bind<T>(BatchArgs<T>&, void*, size_t, int, int, int, bool);              \
+  template int getrf_batched<T>(const BatchArgs<T>&, hipStream_t);                             \
+  template int getrs_batched<T>(const BatchArgs<T>&, cx<T>*, cx<T>*, int, const int*, int,     \
+                                hipStream_t);                                                  \
+  template int solve_batched<T>(const BatchArgs<T>&, const double2*, int64_t, int64_t,         \
+                                double2*, int64_t, int64_t, int, int, double, int, double*,    \
+                                int*, hipStream_t);
 INST(double)
 INST(float)
 template int transpose_cast<double, double>(int, int, const cx<double>*, int64_t, cx<double>*, int64_t, hipStream_t, const double*, const double*);

@@ -154,6 +154,7 @@ struct GmresWork {
   double2* part = nullptr;  // GMRES_DOT_BLOCKS x (m + 1) x nrhs partial dots
 };
 constexpr int GMRES_MAX_M = 24;
+constexpr int GMRES_DEFAULT_CYCLE = 16;
 constexpr int GMRES_DOT_BLOCKS = 256;
 size_t gmres_work_bytes(int n, int nrhs, int m);
 void gmres_work_bind(GmresWork& w, void* base, int n, int nrhs, int m);
@@ -170,6 +171,54 @@ int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, cons
                 int max_steps, int m, double tol, int crit, RefineState* state, double2* R,
                 cx<TF>* RF, cx<TF>* RY, double* part, const GmresWork& gw, const LuPlan& plan,
                 bool gauss3m, hipStream_t st, const double* scale);
+
+// ---- uniform batches ------------------------------------------------------
+// `count` independent systems of one order n, each at a fixed element
+// stride from the previous one in every buffer; one chain of launches (the
+// system index in a grid dimension) factors and solves all of them.
+template <class T> struct BatchArgs {
+  int n = 0, count = 0;
+  int64_t ld = 0;                                   // row stride of the factors
+  cx<T>* F = nullptr;        int64_t sF = 0;        // row-major factors
+  cx<T>* dinv64 = nullptr;   int64_t s64 = 0;       // 64 x 64 diagonal-block inverses
+  cx<T>* dinv256 = nullptr;  int64_t s256 = 0;      // 256 x 256 diagonal-block inverses
+  cx<T>* ustage = nullptr;   int64_t sU = 0;        // U12 staging
+  int* ipiv = nullptr;
+  int* perm = nullptr;       int sIv = 0;           // also the stride of `scale`
+  int* info = nullptr;                              // count entries
+  double* scale = nullptr;                          // equilibration (null: none)
+  unsigned long long* xchg = nullptr; int64_t sX = 0; int gstride = 0;
+  int* err = nullptr;
+  RefineState* state = nullptr;                     // count entries
+  double* part = nullptr;                           // count x BATCH_NORM_BLOCKS x 5
+  double2* Brm = nullptr;
+  double2* Xrm = nullptr;
+  double2* R = nullptr;
+  cx<T>* RF = nullptr;
+  cx<T>* RY = nullptr;
+  int64_t sV = 0;                                   // stride of the n x nrhs blocks
+};
+constexpr int BATCH_NORM_BLOCKS = 64;
+
+size_t batch_work_bytes(int n, int nrhs, int count, size_t elem);
+template <class T>
+int batch_bind(BatchArgs<T>& b, void* base, size_t bytes, int n, int nrhs, int count, bool equil);
+// Blocked GEPP of every system (no look-ahead: the batch fills the GPU).
+template <class T> int getrf_batched(const BatchArgs<T>& b, hipStream_t st);
+// Substitution with the factors of getrf_batched on row-major n x nrhs
+// blocks at stride b.sV (B overwritten with the solution, Y workspace);
+// systems whose skip flag (stride sSkip ints) is set are left untouched.
+template <class T>
+int getrs_batched(const BatchArgs<T>& b, cx<T>* B, cx<T>* Y, int nrhs, const int* skip, int sSkip,
+                  hipStream_t st);
+// Factor + classic FP64 refinement of every system: A64 column-major (lda,
+// stride sA), dB column-major right-hand sides (ldb, stride sB) overwritten
+// with the solutions; per system the relative residual ||B - A X||_F/||B||_F
+// (res_out) and 1 converged / 0 not converged / -info singular (conv_out).
+template <class TF>
+int solve_batched(const BatchArgs<TF>& b, const double2* A64, int64_t lda, int64_t sA,
+                  double2* dB, int64_t ldb, int64_t sB, int nrhs, int max_refine, double tol,
+                  int crit, double* res_out, int* conv_out, hipStream_t st);
 
 // HIP-event timing of getrf's trailing-update GEMM launches (for roofline
 // reporting): enable, run, then read total ms / algorithmic flops / count.

@@ -192,7 +192,19 @@ template <class T, int W>
 __global__ void __launch_bounds__(256)
 leaf_getf2_kernel(cx<T>* __restrict__ P, int64_t ld, int m, int w, int row_base,
                   int* __restrict__ ipiv, int* __restrict__ info, LeafXchg x, unsigned seq_base,
-                  int* __restrict__ err) {
+                  int* __restrict__ err, int64_t sP = 0, int sIv = 0, int64_t sX = 0) {
+  // batch of independent systems: blockIdx.y selects the system (strides sP
+  // elements of P, sIv entries of ipiv, sX exchange words; info per system)
+  {
+    const int b = blockIdx.y;
+    P += b * sP;
+    ipiv += (int64_t)b * sIv;
+    info += b;
+    x.val += b * sX;
+    x.idx += b * sX;
+    x.row += b * sX;
+    x.rowj += b * sX;
+  }
   constexpr int NT = 256, NW = NT / 64, STAGE = 32;
   constexpr int WPR = words_per<T>::n, RW = 2 * WPR;   // 32-bit words per real / complex
   // column buffers, aliased with the staging tile used before and after the loop
@@ -373,8 +385,11 @@ leaf_getf2_kernel(cx<T>* __restrict__ P, int64_t ld, int m, int w, int row_base,
 template <class T>
 __global__ void __launch_bounds__(256)
 laswp_kernel(cx<T>* __restrict__ A, int64_t ld, int c0, int c1, int k1, int nk,
-             const int* __restrict__ ipiv, const int* __restrict__ skip) {
-  if (skip && *skip) return;
+             const int* __restrict__ ipiv, const int* __restrict__ skip, int64_t sA = 0,
+             int sIv = 0, int sSkip = 0) {
+  if (skip && skip[(int64_t)blockIdx.y * sSkip]) return;
+  A += blockIdx.y * sA;
+  ipiv += (int64_t)blockIdx.y * sIv;
   constexpr int CW = 128 / sizeof(cx<T>);
   extern __shared__ unsigned char smem_raw[];
   cx<T>* buf = reinterpret_cast<cx<T>*>(smem_raw);  // [2 * nk][CW]
@@ -437,8 +452,10 @@ laswp_kernel(cx<T>* __restrict__ A, int64_t ld, int c0, int c1, int k1, int nk,
 template <class T, bool UPPER>
 __global__ void __launch_bounds__(256)
 trsm_diag_kernel(const cx<T>* __restrict__ D, int64_t ldd, int nb, cx<T>* __restrict__ S,
-                 int64_t lds, int ncols, const int* __restrict__ skip) {
+                 int64_t lds, int ncols, const int* __restrict__ skip, int64_t sA = 0) {
   if (skip && *skip) return;
+  D += blockIdx.y * sA;
+  S += blockIdx.y * sA;
   constexpr int CW = 16, NBM = 64;
   __shared__ cx<T> d[NBM][NBM + 1];
   __shared__ cx<T> s[NBM][CW];
@@ -491,7 +508,10 @@ trsm_diag_kernel(const cx<T>* __restrict__ D, int64_t ldd, int nb, cx<T>* __rest
 // sequential swap loop never waits on global memory.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-perm_from_ipiv_kernel(const int* __restrict__ ipiv, int n, int* __restrict__ perm, int in_lds) {
+perm_from_ipiv_kernel(const int* __restrict__ ipiv, int n, int* __restrict__ perm, int in_lds,
+                      int sIv = 0) {
+  ipiv += (int64_t)blockIdx.x * sIv;   // one workgroup per system
+  perm += (int64_t)blockIdx.x * sIv;
   extern __shared__ int sh_perm[];
   __shared__ int chunk[1024];
   int* p = in_lds ? sh_perm : perm;
@@ -519,8 +539,15 @@ template <class Tin, class Tout>
 __global__ void __launch_bounds__(256)
 permute_cast_kernel(int n, int nrhs, const int* __restrict__ perm, const cx<Tin>* __restrict__ in,
                     cx<Tout>* __restrict__ out, const double* __restrict__ scale,
-                    const int* __restrict__ skip) {
-  if (skip && *skip) return;
+                    const int* __restrict__ skip, int sIv = 0, int64_t sV = 0, int sSkip = 0) {
+  {
+    const int b = blockIdx.y;   // system of a batch: sIv perm/scale entries, sV elements
+    if (skip && skip[(int64_t)b * sSkip]) return;
+    perm += (int64_t)b * sIv;
+    if (scale) scale += (int64_t)b * sIv;
+    in += b * sV;
+    out += b * sV;
+  }
   const size_t total = (size_t)n * nrhs;
   for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
     const int i = (int)(e / nrhs), c = (int)(e % nrhs);
@@ -540,7 +567,9 @@ permute_cast_kernel(int n, int nrhs, const int* __restrict__ perm, const cx<Tin>
 template <class T>
 __global__ void __launch_bounds__(256)
 diag_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int blk0,
-                cx<T>* __restrict__ dinv) {
+                cx<T>* __restrict__ dinv, int64_t sA = 0, int64_t sD = 0) {
+  LU += blockIdx.z * sA;   // system of a batch
+  dinv += blockIdx.z * sD;
   __shared__ cx<T> a[64][65];
   __shared__ cx<T> tmp[64];
   const int bi = blk0 + blockIdx.x, upper = blockIdx.y, tid = threadIdx.x;
@@ -660,9 +689,16 @@ __device__ inline void prod64_g(const cx<T>* __restrict__ A, int64_t lda, int ar
 template <class T>
 __global__ void __launch_bounds__(256)
 blk_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int bb0,
-               const cx<T>* __restrict__ dinv64, cx<T>* __restrict__ dinv256) {
+               const cx<T>* __restrict__ dinv64, cx<T>* __restrict__ dinv256, int64_t sA = 0,
+               int64_t sD64 = 0, int64_t sD256 = 0) {
   typedef typename acc_t<T>::type acc;
-  const int bb = bb0 + blockIdx.x, upper = blockIdx.y, jb = blockIdx.z;
+  const int bb = bb0 + blockIdx.x, upper = blockIdx.y, jb = blockIdx.z & 3;
+  {
+    const int b = blockIdx.z >> 2;   // system of a batch (z = 4 b + jb)
+    LU += b * sA;
+    dinv64 += b * sD64;
+    dinv256 += b * sD256;
+  }
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
   const int r0 = bb * 256, kb = min(256, n - r0), nsb = (kb + 63) / 64;
   cx<T>* X = dinv256 + ((int64_t)bb * 2 + upper) * 65536;
@@ -709,7 +745,9 @@ blk_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int bb0,
 template <class T>
 __global__ void __launch_bounds__(256)
 copy2d_kernel(int rows, int cols, const cx<T>* __restrict__ src, int64_t lds, cx<T>* __restrict__ dst,
-              int64_t ldd) {
+              int64_t ldd, int64_t sSrc = 0, int64_t sDst = 0) {
+  src += blockIdx.y * sSrc;
+  dst += blockIdx.y * sDst;
   const size_t total = (size_t)rows * cols;
   for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
     const int r = (int)(e / cols), c = (int)(e % cols);

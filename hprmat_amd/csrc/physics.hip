@@ -19,7 +19,12 @@ __global__ void __launch_bounds__(256)
 assemble_kernel(int nch, int nr, const double* __restrict__ tc, const double* __restrict__ blo0,
                 const double* __restrict__ r, double rmax0, const int* __restrict__ lval,
                 const double* __restrict__ qk, const double* __restrict__ xc1,
-                const double2* __restrict__ cpot, double2* __restrict__ A, int64_t lda) {
+                const double2* __restrict__ cpot, double2* __restrict__ A, int64_t lda,
+                int64_t sA = 0) {
+  // batch over energies: blockIdx.z selects qk / xc1 (nch each) and A
+  qk += (int64_t)blockIdx.z * nch;
+  xc1 += (int64_t)blockIdx.z * nch;
+  A += blockIdx.z * sA;
   const int n = nch * nr;
   const int row = blockIdx.x * 256 + threadIdx.x;
   const int col = blockIdx.y;
@@ -99,6 +104,125 @@ __global__ void store_colmajor_kernel(int n, const double2* __restrict__ Xr, dou
   if (t >= n * n) return;
   const int i = t / n, j = t % n;
   S[(int64_t)j * n + i] = Xr[t];
+}
+
+// Batched R -> S, one workgroup per system (nch <= R2S_BATCH_MAX_NCH):
+// R = normfac q2^T X to global memory; Z_O and Z_I over the open channels in
+// LDS (row-major, nopen x nopen each); Z_O S = Z_I by Gaussian elimination
+// with partial pivoting (largest |Re|+|Im|, first on ties) and back
+// substitution, all in FP64; S stored column-major.  Per-system inputs are
+// rows of count x nch arrays (qk, open, co, cop) and nopen[count].
+constexpr int R2S_BATCH_MAX_NCH = 64;
+
+__device__ inline double2 zdiv(double2 a, double2 b) {
+  const double d = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+
+__global__ void __launch_bounds__(256)
+r_to_s_batch_kernel(int nch, int nr, const double2* __restrict__ X, int64_t ldx, int64_t sX,
+                    const double* __restrict__ q2, double normfac, double rmax,
+                    const double* __restrict__ qk, const int* __restrict__ nopen,
+                    const int* __restrict__ open, const double2* __restrict__ co,
+                    const double2* __restrict__ cop, double2* __restrict__ R,
+                    double2* __restrict__ S) {
+  const int b = blockIdx.x, tid = threadIdx.x;
+  X += b * sX;
+  qk += (int64_t)b * nch;
+  open += (int64_t)b * nch;
+  co += (int64_t)b * nch;
+  cop += (int64_t)b * nch;
+  R += (int64_t)b * nch * nch;
+  S += (int64_t)b * nch * nch;
+  const int no = nopen[b];
+  extern __shared__ double2 zsh[];
+  double2* ZO = zsh;               // [no][no]
+  double2* ZI = zsh + no * no;     // [no][no]
+  __shared__ double2 lcol[R2S_BATCH_MAX_NCH];
+  __shared__ int piv;
+  auto rdot = [&](int a, int c) {  // R(a, c) = normfac sum_r q2[r] X[a nr + r, c]
+    const double2* x = X + (int64_t)c * ldx + (int64_t)a * nr;
+    double sr = 0.0, si = 0.0;
+    for (int r = 0; r < nr; ++r) {
+      sr += q2[r] * x[r].x;
+      si += q2[r] * x[r].y;
+    }
+    return make_double2(sr * normfac, si * normfac);
+  };
+  for (int t = tid; t < nch * nch; t += 256) R[t] = rdot(t % nch, t / nch);
+  for (int t = tid; t < no * no; t += 256) {
+    const int m1 = t / no, m2 = t % no, o1 = open[m1], o2 = open[m2];
+    const double2 rr = rdot(o1, o2);
+    const double f = sqrt(qk[o1] / qk[o2]) / rmax;
+    const double2 c2 = make_double2(rr.x * f, rr.y * f);
+    const double2 cp = cop[m2], cpc = make_double2(cp.x, -cp.y);
+    double2 zo = zmul(c2, cp), zi = zmul(c2, cpc);
+    zo = make_double2(-zo.x, -zo.y);
+    zi = make_double2(-zi.x, -zi.y);
+    if (m1 == m2) {
+      zo.x += co[m1].x; zo.y += co[m1].y;
+      zi.x += co[m1].x; zi.y -= co[m1].y;
+    }
+    ZO[t] = zo;
+    ZI[t] = zi;
+  }
+  __syncthreads();
+  for (int k = 0; k < no; ++k) {
+    if (tid < 64) {   // pivot search in wave 0 (no <= 64 rows)
+      const int i = k + tid;
+      double v = -1.0;
+      int vi = 0x7fffffff;
+      if (i < no) {
+        const double2 z = ZO[i * no + k];
+        v = fabs(z.x) + fabs(z.y);
+        vi = i;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off);
+        const int oi = __shfl_xor(vi, off);
+        if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
+      }
+      if (tid == 0) piv = (v > 0.0) ? vi : -1;
+    }
+    __syncthreads();
+    const int p = piv;
+    __syncthreads();       // every thread has read piv before the next column rewrites it
+    if (p < 0) continue;   // exactly singular column: left as is (S gets inf/nan)
+    if (p != k) {
+      for (int j = tid; j < 2 * no; j += 256) {
+        double2* Z = j < no ? ZO : ZI;
+        const int c = j < no ? j : j - no;
+        const double2 t = Z[k * no + c];
+        Z[k * no + c] = Z[p * no + c];
+        Z[p * no + c] = t;
+      }
+      __syncthreads();
+    }
+    for (int i = k + 1 + tid; i < no; i += 256) lcol[i] = zdiv(ZO[i * no + k], ZO[k * no + k]);
+    __syncthreads();
+    const int rows = no - k - 1, cols = no - k - 1 + no;   // ZO columns > k, then all of ZI
+    for (int e = tid; e < rows * cols; e += 256) {
+      const int i = k + 1 + e / cols, j = e % cols;
+      double2* Z = j < no - k - 1 ? ZO : ZI;
+      const int c = j < no - k - 1 ? k + 1 + j : j - (no - k - 1);
+      const double2 l = lcol[i], u = Z[k * no + c], z = Z[i * no + c];
+      const double2 lu = zmul(l, u);
+      Z[i * no + c] = make_double2(z.x - lu.x, z.y - lu.y);
+    }
+    __syncthreads();
+  }
+  for (int k = no - 1; k >= 0; --k) {
+    const double2 d = ZO[k * no + k];
+    for (int c = tid; c < no; c += 256) ZI[k * no + c] = zdiv(ZI[k * no + c], d);
+    __syncthreads();
+    for (int e = tid; e < k * no; e += 256) {
+      const int i = e / no, c = e % no;
+      const double2 lu = zmul(ZO[i * no + k], ZI[k * no + c]), z = ZI[i * no + c];
+      ZI[i * no + c] = make_double2(z.x - lu.x, z.y - lu.y);
+    }
+    __syncthreads();
+  }
+  for (int t = tid; t < no * no; t += 256) S[(t % no) * no + t / no] = ZI[t];
 }
 
 struct SmallWork {
@@ -237,6 +361,47 @@ int hprmat_hip_r_to_s_ws(int nch, int nr, const void* dX, int64_t ldx, const dou
   if (!work) return -1;
   return r_to_s_impl(nch, nr, dX, ldx, q2, normfac, rmax, qk, nopen, open, co, cop, dR, dS,
                      stream, work, work_bytes);
+}
+
+int hprmat_hip_assemble_batch(int nch, int nr, int count, const double* tc, const double* blo0,
+                              const double* r, double rmax0, const int* lval, const double* qk,
+                              const double* xc1, const void* cpot, void* dA, int64_t lda,
+                              int64_t stride_a, void* stream) {
+  const int n = nch * nr;
+  if (count == 0) return 0;
+  if (nch <= 0 || nr <= 0 || count < 0 || count > 65535 || lda < n || n > 65535 ||
+      (count > 1 && stride_a < lda * n))
+    return -1;
+  dim3 grid((n + 255) / 256, n, count);
+  assemble_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(nch, nr, tc, blo0, r, rmax0, lval, qk,
+                                                        xc1, (const double2*)cpot, (double2*)dA,
+                                                        lda, stride_a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int hprmat_hip_r_to_s_batch(int nch, int nr, int count, const void* dX, int64_t ldx,
+                            int64_t stride_x, const double* q2, double normfac, double rmax,
+                            const double* qk, const int* nopen, const int* open, const void* co,
+                            const void* cop, void* dR, void* dS, void* stream) {
+  if (count == 0) return 0;
+  if (nch <= 0 || nr <= 0 || count < 0 || nch > R2S_BATCH_MAX_NCH ||
+      ldx < (int64_t)nch * nr || (count > 1 && stride_x < ldx * nch))
+    return -1;
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute((const void*)r_to_s_batch_kernel,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            2 * R2S_BATCH_MAX_NCH * R2S_BATCH_MAX_NCH * 16) != hipSuccess) {
+      (void)hipGetLastError();
+      return -1;
+    }
+    attr = true;
+  }
+  const size_t lds = (size_t)2 * nch * nch * sizeof(double2);   // nopen <= nch
+  r_to_s_batch_kernel<<<count, 256, lds, (hipStream_t)stream>>>(
+      nch, nr, (const double2*)dX, ldx, stride_x, q2, normfac, rmax, qk, nopen, open,
+      (const double2*)co, (const double2*)cop, (double2*)dR, (double2*)dS);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int hprmat_hip_adjoint_contract(int nch, int nr, int nrhs, const void* dM, int64_t ldm,

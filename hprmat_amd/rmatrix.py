@@ -182,6 +182,55 @@ def r_to_s(dmesh, x_cm, nch, qk, open_idx, co, cop, normfac=1.0):
     return r.transpose(0, 1), s[:nopen, :nopen].transpose(0, 1)
 
 
+def assemble_batch(dmesh, lval_dev, qk, xc1, cpot, out, count=None):
+    """Column-major storages of C at several energies in one launch: qk and
+    xc1 are (count, nch) float64 device tensors (row e for energy e), lval_dev
+    the (nch,) int32 device angular momenta, cpot the (nr, nch, nch) device
+    potential; out is a (>= count, N, N) complex128 device tensor."""
+    lib = _lib.load()
+    nr = dmesh.mesh.nr
+    nch = qk.shape[1]
+    n = nch * nr
+    c = qk.shape[0] if count is None else int(count)
+    if out.dim() != 3 or out.shape[1:] != (n, n) or out.shape[0] < c or xc1.shape[0] < c:
+        raise ValueError('out must be (count, N, N) and xc1 (count, nch)')
+    rc = lib.hprmat_hip_assemble_batch(
+        nch, nr, c, ctypes.c_void_p(dmesh.tc.data_ptr()), ctypes.c_void_p(dmesh.blo0.data_ptr()),
+        ctypes.c_void_p(dmesh.r.data_ptr()), ctypes.c_double(dmesh.mesh.a),
+        ctypes.c_void_p(lval_dev.data_ptr()), ctypes.c_void_p(qk.data_ptr()),
+        ctypes.c_void_p(xc1.data_ptr()), ctypes.c_void_p(cpot.data_ptr()),
+        ctypes.c_void_p(out.data_ptr()), ctypes.c_int64(n), ctypes.c_int64(n * n),
+        solver._stream_ptr())
+    _lib.check(rc, 'hprmat_hip_assemble_batch')
+    return out
+
+
+def r_to_s_batch(dmesh, x_cm, qk, nopen, open_idx, co, cop, r_out, s_out, count=None,
+                 normfac=1.0):
+    """R and S of several solutions of one system in one launch (nch <= 64).
+    x_cm: (count, nch, N) column-major solution storages; qk (count, nch)
+    float64, nopen (count,) int32, open_idx (count, nch) int32, co / cop
+    (count, nch) complex128 (first nopen[e] entries used), all on the device.
+    r_out / s_out: (count, nch, nch) complex128; block e receives R
+    (column-major nch x nch) and S (column-major nopen x nopen at its start)."""
+    lib = _lib.load()
+    nch = qk.shape[1]
+    c = x_cm.shape[0] if count is None else int(count)
+    n = nch * dmesh.mesh.nr
+    if x_cm.dim() != 3 or x_cm.shape[1:] != (nch, n) or x_cm.shape[0] < c:
+        raise ValueError('x_cm must be (count, nch, N)')
+    rc = lib.hprmat_hip_r_to_s_batch(
+        nch, dmesh.mesh.nr, c, ctypes.c_void_p(x_cm.data_ptr()), ctypes.c_int64(n),
+        ctypes.c_int64(n * nch), ctypes.c_void_p(dmesh.q2.data_ptr()), ctypes.c_double(normfac),
+        ctypes.c_double(dmesh.mesh.a), ctypes.c_void_p(qk.data_ptr()),
+        ctypes.c_void_p(nopen.data_ptr()), ctypes.c_void_p(open_idx.data_ptr()),
+        ctypes.c_void_p(co.data_ptr()), ctypes.c_void_p(cop.data_ptr()),
+        ctypes.c_void_p(r_out.data_ptr()), ctypes.c_void_p(s_out.data_ptr()),
+        solver._stream_ptr())
+    _lib.check(rc, 'hprmat_hip_r_to_s_batch')
+    return r_out, s_out
+
+
 def rmatrix(lval, qk, a, nr, cpot, solver_type=1, max_refine=0, dmesh=None):
     """ns = 1 pipeline on the GPU: returns dict(S, R, nopen, stats)."""
     if dmesh is None:

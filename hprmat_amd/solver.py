@@ -58,9 +58,13 @@ class LU:
             self._lib.hprmat_hip_lu_destroy(h)
             self._h = None
 
-    def set_plan(self, trail_tile=None, lookahead=None):
+    def set_plan(self, trail_tile=None, lookahead=None, gmres_cycle=None):
         """Factorization plan: trailing-update GEMM tile (0 64x64x16,
-        1 64x128x16, 2 128x64x16) and look-ahead on/off."""
+        1 64x128x16, 2 128x64x16), look-ahead on/off, and the FGMRES cycle
+        length (1..24 steps; 0 = default 16)."""
+        if gmres_cycle is not None:
+            _lib.check(self._lib.hprmat_hip_lu_set_plan(self._h, _lib.PLAN_GMRES_CYCLE,
+                                                         int(gmres_cycle)), 'set_plan')
         if trail_tile is not None:
             _lib.check(self._lib.hprmat_hip_lu_set_plan(self._h, _lib.PLAN_TRAIL_TILE,
                                                          int(trail_tile)), 'set_plan')
@@ -137,3 +141,54 @@ def solve(a, b, method='fp64', max_refine=0, tol=1e-12, criterion='rel_fro', fin
     if st['info'] != 0:
         raise RuntimeError('zero pivot at column %d' % st['info'])
     return b_cm.transpose(0, 1), st
+
+
+class Batch:
+    """Factor + solve + classic FP64 refinement of ``count`` systems of one
+    order ``n`` in lockstep (libhprmat_hip's hprmat_hip_batch_solve: one chain
+    of launches serves the whole batch).
+
+    The device workspace is allocated once here; ``solve_`` allocates nothing
+    and does not synchronise, so it can be captured into a CUDA/HIP graph.
+    Methods: 'fp64' (plain) and 'fp32' (FP32 factors, FP64 refinement).
+    """
+
+    def __init__(self, n, nrhs, count, method='fp32', equilibrate=True):
+        if method not in ('fp64', 'fp32'):
+            raise ValueError('batched method must be fp64 or fp32')
+        self._lib = _lib.load()
+        self.n, self.nrhs, self.count, self.method = int(n), int(nrhs), int(count), method
+        self.code = METHODS[method] | (_lib.EQUILIBRATE if equilibrate else 0)
+        self.work_bytes = int(self._lib.hprmat_hip_batch_work_bytes(self.code, self.n, self.nrhs,
+                                                                    self.count))
+        if self.work_bytes == 0:
+            raise ValueError('invalid batch shape')
+        self.work = torch.empty(self.work_bytes, dtype=torch.uint8, device='cuda')
+        self.res = torch.zeros(self.count, dtype=torch.float64, device='cuda')
+        self.status = torch.zeros(self.count, dtype=torch.int32, device='cuda')
+
+    def solve_(self, a_cm, b_cm, max_refine=10, tol=1e-12, criterion='rel_fro', count=None):
+        """Solve in place.  ``a_cm``: (count, n, n) column-major storages of the
+        matrices; ``b_cm``: (count, nrhs, n) column-major storages of the
+        right-hand sides, overwritten with the solutions.  ``count`` (default:
+        the batch size) solves only the first systems.  Per-system relative
+        residuals and statuses (1 converged, 0 not, -info singular) are left
+        in ``self.res`` / ``self.status`` on the device."""
+        _check_tensor(a_cm, 'a_cm')
+        _check_tensor(b_cm, 'b_cm')
+        c = self.count if count is None else int(count)
+        n, r = self.n, self.nrhs
+        if not 0 <= c <= self.count:
+            raise ValueError('count exceeds the batch size')
+        if a_cm.dim() != 3 or a_cm.shape[1:] != (n, n) or a_cm.shape[0] < c:
+            raise ValueError('a_cm must have shape (count, n, n)')
+        if b_cm.dim() != 3 or b_cm.shape[1:] != (r, n) or b_cm.shape[0] < c:
+            raise ValueError('b_cm must have shape (count, nrhs, n)')
+        rc = self._lib.hprmat_hip_batch_solve(
+            self.code, n, r, c, ctypes.c_void_p(a_cm.data_ptr()), n, n * n,
+            ctypes.c_void_p(b_cm.data_ptr()), n, n * r, int(max_refine), float(tol),
+            CRITERIA[criterion], ctypes.c_void_p(self.res.data_ptr()),
+            ctypes.c_void_p(self.status.data_ptr()), ctypes.c_void_p(self.work.data_ptr()),
+            self.work_bytes, _stream_ptr())
+        _lib.check(rc, 'hprmat_hip_batch_solve')
+        return self

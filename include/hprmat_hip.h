@@ -130,8 +130,10 @@ int hprmat_hip_lu_destroy(hprmat_hip_lu_t h);
                              1 = 64x128x16, 2 = 128x64x16 (double-buffered LDS)
      HPRMAT_PLAN_LOOKAHEAD   1 = overlap the next panel with the trailing
                              update on a second stream (default), 0 = off
+     HPRMAT_PLAN_GMRES_CYCLE steps per FGMRES cycle, 1..24 (0 = default 16);
+                             the solve's step budget spans several cycles
    Returns -1 for an unknown key or value. */
-enum { HPRMAT_PLAN_TRAIL_TILE = 1, HPRMAT_PLAN_LOOKAHEAD = 2 };
+enum { HPRMAT_PLAN_TRAIL_TILE = 1, HPRMAT_PLAN_LOOKAHEAD = 2, HPRMAT_PLAN_GMRES_CYCLE = 3 };
 int hprmat_hip_lu_set_plan(hprmat_hip_lu_t h, int key, int value);
 
 /* Factor the n x n complex FP64 matrix at device pointer dA (column-major,
@@ -202,6 +204,30 @@ int hprmat_hip_r_to_s_ws(int nch, int nr, const void *dX, int64_t ldx,
                          const void *co, const void *cop, void *dR, void *dS,
                          void *work, size_t work_bytes, void *stream);
 
+/* No reference counterpart (the reference assembles one energy per call):
+   hprmat_hip_assemble for `count` energies of one system at once; qk and
+   xc1 are count x nch (row e for system e), dA receives count column-major
+   matrices stride_a elements apart.  One launch. */
+int hprmat_hip_assemble_batch(int nch, int nr, int count, const double *tc,
+                              const double *blo0, const double *r, double rmax0,
+                              const int *lval, const double *qk, const double *xc1,
+                              const void *cpot, void *dA, int64_t lda, int64_t stride_a,
+                              void *stream);
+
+/* No reference counterpart: hprmat_hip_r_to_s for `count` solutions of one
+   system (nch <= 64) in one launch, one workgroup per system (S by Gaussian
+   elimination with partial pivoting in FP64).  dX: count column-major N x
+   nch solutions (ldx, stride_x apart); qk, open, co, cop: count x nch
+   (row e for system e; open, co, cop use the first nopen[e] entries);
+   nopen: count device ints; dR, dS: count blocks of nch*nch complex
+   (R column-major nch x nch, S column-major nopen x nopen at the block
+   start).  Returns -1 for nch > 64. */
+int hprmat_hip_r_to_s_batch(int nch, int nr, int count, const void *dX, int64_t ldx,
+                            int64_t stride_x, const double *q2, double normfac, double rmax,
+                            const double *qk, const int *nopen, const int *open,
+                            const void *co, const void *cop, void *dR, void *dS,
+                            void *stream);
+
 /* No reference counterpart (the reference has no derivatives): adjoint
    contraction for gradients with respect to a local coupling potential.
    With X = A^-1 B and M = A^-T G (for complex-symmetric A the same factors
@@ -229,6 +255,29 @@ int hprmat_hip_prof_read(double *ms, double *flops, int *count);
 int hprmat_hip_zgemm(int precision, int M, int N, int K, const void *A, int64_t lda, int a_col,
                      const void *B, int64_t ldb, void *C, int64_t ldc, int alpha_sign, int beta,
                      void *stream);
+
+/* No reference counterpart (the reference loops rmatrix over energies, one
+   zgesv per system): factor + solve + classic FP64 refinement of `count`
+   independent systems of one order n in lockstep, one chain of launches for
+   the whole batch.  dA: count column-major n x n matrices (lda, stride_a
+   elements apart); dB: count column-major n x nrhs right-hand sides (ldb,
+   stride_b apart), overwritten with the solutions.  method_flags:
+   HPRMAT_PLAIN_FP64 or HPRMAT_MIXED_FP32, optionally | HPRMAT_EQUILIBRATE;
+   criterion: HPRMAT_CRIT_* (no FGMRES).  max_refine correction sweeps at
+   most, each system stopping once it meets tol; res_rel[i] receives the
+   final ||B - A X||_F / ||B||_F and status[i] 1 (converged), 0 (not
+   converged) or -info (exactly zero pivot at column info), both device
+   arrays (may be NULL).  work: device workspace of
+   hprmat_hip_batch_work_bytes(method, n, nrhs, count) bytes (256-byte
+   aligned); nothing is allocated and nothing synchronises with the host,
+   so the call can be captured into a HIP graph.  dA must not be modified
+   until the call completes on `stream`. */
+size_t hprmat_hip_batch_work_bytes(int method, int n, int nrhs, int count);
+int hprmat_hip_batch_solve(int method_flags, int n, int nrhs, int count, const void *dA,
+                           int64_t lda, int64_t stride_a, void *dB, int64_t ldb,
+                           int64_t stride_b, int max_refine, double tol, int criterion,
+                           double *res_rel, int *status, void *work, size_t work_bytes,
+                           void *stream);
 
 /* Library version string. */
 const char *hprmat_hip_version(void);

@@ -207,7 +207,9 @@ def test_memory_scale_mixed_solve():
     # / ||B||), so the gate is 1e-11; S (256 x 256, FP64 on the device) is
     # symmetric and, with absorption, |S_ij| <= 1.
     s, dm, o, co, cop, a_cm, b = _memory_system(False)
-    lu = solver.LU().factor(a_cm, 'fp32', True)
+    # cycles of 24 steps: restarts after 16 lose the Krylov space this
+    # system needs to get below the gate
+    lu = solver.LU().set_plan(gmres_cycle=24).factor(a_cm, 'fp32', True)
     st = lu.solve_(b, 40, 1e-11, 'rel_fro', True, refinement='fgmres')
     print('memory-scale mixed: res_rel %.2e products %d hist %s'
           % (st['res_rel'], st['sweeps'], ' '.join('%.1e' % h for h in st['hist'])))

@@ -19,7 +19,8 @@ HEADER = os.path.join(ROOT, 'include', 'hprmat_hip.h')
 def header_functions():
     txt = open(HEADER).read()
     txt = re.sub(r'/\*.*?\*/', '', txt, flags=re.S)
-    return sorted(set(re.findall(r'^\s*(?:int|void|const char)\s*\*?\s*(\w+)\s*\(', txt, flags=re.M)))
+    return sorted(set(re.findall(r'^\s*(?:int|void|size_t|const char)\s*\*?\s*(\w+)\s*\(', txt,
+                                 flags=re.M)))
 
 
 def test_header_declares_reference_abi():
@@ -44,6 +45,38 @@ def test_library_loads_and_binds():
     p = subprocess.run([sys.executable, '-c', code], capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stderr
     assert 'hprmat_hip' in p.stdout
+
+
+def test_batch_workspace_and_argument_checks():
+    # host-only paths of the batch entry points: workspace arithmetic and the
+    # argument checks that return before any device call
+    code = '''
+import sys, ctypes
+sys.path.insert(0, %r)
+from hprmat_amd import _lib
+lib = _lib.load()
+wb = lib.hprmat_hip_batch_work_bytes
+one, four = wb(1, 1000, 8, 1), wb(1, 1000, 8, 4)
+assert one > 1000 * 1000 * 8 and 3 * one < four <= 4 * one + 4096, (one, four)
+assert wb(0, 1000, 8, 1) > one            # FP64 factors are twice as large
+assert wb(2, 1000, 8, 1) == 0 and wb(1, 0, 8, 1) == 0 and wb(1, 10, 0, 1) == 0
+nul = ctypes.c_void_p(0)
+assert lib.hprmat_hip_batch_solve(3, 10, 1, 1, nul, 10, 100, nul, 10, 10, 0, 0.0, 1, nul, nul,
+                                  nul, 0, nul) == -1     # split-BF16: not batched
+assert lib.hprmat_hip_batch_solve(1, 10, 1, 1, nul, 10, 100, nul, 10, 10, 0, 0.0, 0x11, nul,
+                                  nul, nul, 0, nul) == -1  # FGMRES: not batched
+assert lib.hprmat_hip_batch_solve(1, 10, 1, 2, ctypes.c_void_p(16), 10, 50, ctypes.c_void_p(16),
+                                  10, 10, 0, 0.0, 1, nul, nul, ctypes.c_void_p(16), 1 << 20,
+                                  nul) == -1               # overlapping matrices
+assert lib.hprmat_hip_batch_solve(1, 0, 1, 5, nul, 1, 0, nul, 1, 0, 0, 0.0, 1, nul, nul, nul,
+                                  0, nul) == 0             # empty systems: nothing to do
+assert lib.hprmat_hip_r_to_s_batch(65, 10, 1, nul, 650, 650 * 65, nul, 1.0, 1.0, nul, nul, nul,
+                                   nul, nul, nul, nul, nul) == -1   # nch > 64
+print('ok')
+''' % ROOT
+    p = subprocess.run([sys.executable, '-c', code], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert 'ok' in p.stdout
 
 
 @pytest.mark.parametrize('nr,a', [(10, 8.0), (60, 7.0), (200, 15.0)])
