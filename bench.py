@@ -107,8 +107,9 @@ def main():
                     help='also run the batched energy scan (64 waves x 256 energies)')
     ap.add_argument('--scan-waves', type=int, default=64)
     ap.add_argument('--scan-energies', type=int, default=256)
-    ap.add_argument('--graph-max', type=int, default=10 ** 9)
-    ap.add_argument('--tune-cache', default=None)
+    ap.add_argument('--scan-lanes', type=int, default=2)
+    ap.add_argument('--scan-refine', type=int, default=20)
+    ap.add_argument('--scan-graph', type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path '
@@ -241,36 +242,36 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
 
 def energy_scan(torch, args):
     """Batched energy scan: 64 waves (nch in {2..64}, nr in {40..100}) x 256
-    energies in [0.5, 10], FP32 LU + FP64 refinement per system with plans
-    from the autotuner, small systems replayed from a HIP graph; against a
-    loop of plain FP64 solves timed on every 16th energy and scaled."""
+    energies in [0.5, 10]; each wave's energies solved as uniform batches
+    (complex-FP32 LU + FP64 refinement, assembly and R->S on the device),
+    lanes of streams, captured in HIP graphs; systems left unconverged are
+    re-solved in FP64 inside the timed region.  Against a loop of plain
+    per-system FP64 solves timed on every 16th energy of every wave."""
     from hprmat_amd import scan
     t0 = time.perf_counter()
     waves = scan.make_waves(args.scan_waves, args.scan_energies, seed=0)
     t_prep = time.perf_counter() - t0
-    print('# scan: autotune', file=sys.stderr, flush=True)
+    sc = scan.BatchScan(waves, method='fp32', lanes=args.scan_lanes, max_refine=args.scan_refine,
+                        log=sys.stderr)
     t0 = time.perf_counter()
-    plans, lanes = scan.autotune(waves, sample=3, cache=args.tune_cache, log=sys.stderr)
-    t_tune = time.perf_counter() - t0
-    print('# scan: %d waves prepared in %.1f s, tuned in %.1f s (%d lanes)'
-          % (len(waves), t_prep, t_tune, lanes), file=sys.stderr, flush=True)
-    sc = scan.EnergyScan(waves, lanes=lanes, graph_max=args.graph_max, plans=plans, log=sys.stderr)
-    t0 = time.perf_counter()
-    sc.capture()
+    if args.scan_graph:
+        sc.capture()
+    else:
+        sc.run()
+        sc.finish()
     t_cap = time.perf_counter() - t0
-    print('# scan: %d systems, %.3g flops, prep %.1f s, tune %.1f s, capture %.1f s, lanes %d'
-          % (sc.n_systems, sc.flops, t_prep, t_tune, t_cap, lanes), file=sys.stderr, flush=True)
-    print('# scan: first run', file=sys.stderr, flush=True)
-    sc.run()
-    torch.cuda.synchronize()
-    print('# scan: timed run', file=sys.stderr, flush=True)
+    print('# scan: %d systems, %.3g flops, prep %.1f s, warm-up/capture %.1f s, lanes %d'
+          % (sc.n_systems, sc.flops, t_prep, t_cap, len(sc.lanes)), file=sys.stderr, flush=True)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    sc.run()
-    e1.record()
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
+    print('# scan: timed run', file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    sc.run()
+    n_redo = sc.finish()
+    ms = (time.perf_counter() - t0) * 1e3
+    bad = sum(int((w.status != 1).sum()) for w in waves)
+    worst = max(float(w.res.max()) for w in waves)
     s_scan = {(w.index, k): w.s_of(k).clone() for w in waves for k in range(0, len(w.energies), 16)}
     pairs = [(w, k) for w in waves for k in range(0, len(w.energies), 16)]
     torch.cuda.synchronize()
@@ -284,16 +285,18 @@ def energy_scan(torch, args):
              for w, k in pairs)
     n_sys = sc.n_systems
     out = {'systems': n_sys, 'ms': ms, 'solves_per_s': n_sys / (ms * 1e-3),
-           'tflops': sc.flops / (ms * 1e-3) / 1e12, 'lanes': lanes, 'graph_max': args.graph_max,
+           'tflops': sc.flops / (ms * 1e-3) / 1e12, 'flops': sc.flops, 'lanes': len(sc.lanes),
+           'graphs': bool(args.scan_graph), 'max_refine': args.scan_refine,
+           'fp64_fallbacks': n_redo, 'unconverged': bad, 'max_res_rel': worst,
            'serial_fp64': {'systems_timed': len(pairs), 'ms': ms_ser,
                            'solves_per_s': len(pairs) / (ms_ser * 1e-3),
                            'sample': 'every 16th energy of every wave, one stream, eager'},
            'max_dS_vs_fp64': ds}
     out['speedup_vs_serial_fp64'] = out['solves_per_s'] / out['serial_fp64']['solves_per_s']
     print('# scan: %.1f ms, %.0f solves/s, %.1f TFLOP/s; serial FP64 %.0f solves/s; x%.2f; '
-          'max|dS| %.1e' % (ms, out['solves_per_s'], out['tflops'],
-                            out['serial_fp64']['solves_per_s'], out['speedup_vs_serial_fp64'], ds),
-          file=sys.stderr, flush=True)
+          'max|dS| %.1e; fallbacks %d; max res %.1e' % (
+              ms, out['solves_per_s'], out['tflops'], out['serial_fp64']['solves_per_s'],
+              out['speedup_vs_serial_fp64'], ds, n_redo, worst), file=sys.stderr, flush=True)
     return out
 
 

@@ -86,8 +86,13 @@ class Wave:
         self.open = _dev(opn, torch.int32)
         self.co = _dev(co, torch.complex128)
         self.cop = _dev(cop, torch.complex128)
+        self.nopen_dev = _dev(self.nopen.astype(np.int32), torch.int32)
         self.R = torch.zeros((ne, nch, nch), dtype=torch.complex128, device='cuda')
         self.S = torch.zeros((ne, nch, nch), dtype=torch.complex128, device='cuda')
+        # per-energy outcome of the batched solve: relative residual and status
+        # (1 converged, 0 not converged, -info exactly singular)
+        self.res = torch.zeros(ne, dtype=torch.float64, device='cuda')
+        self.status = torch.zeros(ne, dtype=torch.int32, device='cuda')
 
     def s_of(self, k):
         """S at energy index k (nopen x nopen matrix, from the column-major slot)."""
@@ -290,6 +295,185 @@ class EnergyScan:
         """Algorithmic flops of the scan: sum of 8/3 N^3 + 8 N^2 nrhs."""
         return float(sum(len(w.energies) * (8.0 / 3.0 * w.n ** 3 + 8.0 * w.n ** 2 * w.nch)
                          for w in self.waves))
+
+
+class BatchScan:
+    """Energy scan as uniform batches: the energies of one wave are systems
+    of one order N, so they are assembled, factored, solved, refined and
+    reduced to R and S in lockstep (hprmat_hip_assemble_batch,
+    hprmat_hip_batch_solve, hprmat_hip_r_to_s_batch): one chain of launches
+    per chunk of energies instead of one chain per system.  A chunk is as
+    many energies as fit the lane's share of ``mem_bytes``.
+
+    Waves are spread over ``lanes`` streams longest-processing-time first, so
+    the launch-latency-bound chains of small waves overlap the MFMA-bound
+    chains of large ones.  ``capture()`` records each lane's chunks into HIP
+    graphs of at most NODE_BUDGET kernels.  Factorization 'fp32' (FP64
+    refinement, at most ``max_refine`` sweeps; converged systems skip the
+    rest) or 'fp64'.  ``finish()`` synchronises and re-solves any system left
+    unconverged with the plain FP64 batch (the reference's fallback to its
+    FP64 path when the fast one fails, src/rmat_solvers.F90:576-588).
+    """
+
+    def __init__(self, waves, method='fp32', lanes=2, mem_bytes=48 << 30, max_refine=20,
+                 tol=1e-12, log=None):
+        self.waves, self.method, self.max_refine, self.tol = waves, method, max_refine, tol
+        self.code = solver.batch_code(method)
+        self.log = log
+        self.captured = False
+        self.fallbacks = 0
+        budget = mem_bytes // max(1, lanes)
+        self.chunk = {}
+        for w in waves:
+            per = (16 * w.n * w.n + 16 * w.n * w.nch
+                   + solver.batch_work_bytes(method, w.n, w.nch, 1))
+            self.chunk[id(w)] = int(max(1, min(len(w.energies), budget // per)))
+        items = [(self._cost(w), w) for w in waves if len(w.energies)]
+        self.lanes = []
+        for ws in _lpt(items, lanes):
+            if not ws:
+                continue
+            lane = _BatchLane()
+            lane.waves = sorted(ws, key=lambda w: -w.n)
+            lane.calls = [(w, k0, min(self.chunk[id(w)], len(w.energies) - k0))
+                          for w in lane.waves for k0 in range(0, len(w.energies), self.chunk[id(w)])]
+            ma = max(c * w.n * w.n for w, _, c in lane.calls)
+            mx = max(c * w.n * w.nch for w, _, c in lane.calls)
+            mw = max(solver.batch_work_bytes(method, w.n, w.nch, c) for w, _, c in lane.calls)
+            lane.a = torch.empty(ma, dtype=torch.complex128, device='cuda')
+            lane.x = torch.empty(mx, dtype=torch.complex128, device='cuda')
+            lane.work = torch.empty(mw, dtype=torch.uint8, device='cuda')
+            self.lanes.append(lane)
+
+    def _cost(self, w):
+        """Estimated seconds of a wave: MFMA work plus the launch chain."""
+        ne, c = len(w.energies), self.chunk[id(w)]
+        chunks = (ne + c - 1) // c
+        return ne * (8.0 / 3.0) * w.n ** 3 / 60e12 + chunks * self._nodes(w, c) * 4e-6
+
+    def _nodes(self, w, count):
+        """Upper estimate of the kernels one chunk launches."""
+        nb = (w.n + 255) // 256
+        g = (w.n + 255) // 256
+        leaf = 8 * ((count * g + 479) // 480)
+        return 20 + nb * (45 + leaf) + (self.max_refine + 1) * (10 + 3 * nb)
+
+    def _say(self, msg):
+        if self.log:
+            print(msg, file=self.log, flush=True)
+
+    def _chunk_call(self, lane, w, k0, c):
+        n, nch = w.n, w.nch
+        a = lane.a[:c * n * n].view(c, n, n)
+        x = lane.x[:c * nch * n].view(c, nch, n)
+        e = slice(k0, k0 + c)
+        rmatrix.assemble_batch(w.dmesh, w.lv, w.qk[e], w.xc1[e], w.cpot, a)
+        x.copy_(w.b.unsqueeze(0).expand(c, -1, -1))
+        solver.batch_solve(self.code, n, nch, c, a.data_ptr(), x.data_ptr(), self.max_refine,
+                           self.tol, 'rel_fro', w.res[k0].data_ptr(), w.status[k0].data_ptr(),
+                           lane.work)
+        rmatrix.r_to_s_batch(w.dmesh, x, w.qk[e], w.nopen_dev[e], w.open[e], w.co[e], w.cop[e],
+                             w.R[e], w.S[e])
+
+    def _lane_calls(self, lane, calls):
+        with torch.cuda.stream(lane.stream):
+            for w, k0, c in calls:
+                self._chunk_call(lane, w, k0, c)
+
+    def capture(self):
+        """Eager warm-up (first-use kernel attributes), then each lane's chunks
+        captured into chain graphs of at most NODE_BUDGET kernels."""
+        self._say('# batch scan: warm-up')
+        self.run()
+        torch.cuda.synchronize()
+        total = 0
+        for li, lane in enumerate(self.lanes):
+            groups, cur, nodes = [], [], 0
+            for call in lane.calls:
+                c = self._nodes(call[0], call[2])
+                if cur and nodes + c > NODE_BUDGET:
+                    groups.append(cur)
+                    cur, nodes = [], 0
+                cur.append(call)
+                nodes += c
+            if cur:
+                groups.append(cur)
+            lane.graphs = []
+            for grp in groups:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=lane.stream, capture_error_mode='relaxed'):
+                    for w, k0, c in grp:
+                        self._chunk_call(lane, w, k0, c)
+                lane.graphs.append(g)
+            total += len(lane.graphs)
+            self._say('# batch scan: lane %d: %d waves, %d chunks, %d graphs'
+                      % (li, len(lane.waves), len(lane.calls), len(lane.graphs)))
+        torch.cuda.synchronize()
+        self.captured = True
+        self._say('# batch scan: %d graphs captured' % total)
+
+    def run(self):
+        """Enqueue one full scan on the lanes (graph replays once captured);
+        returns without synchronising."""
+        root = torch.cuda.current_stream()
+        for lane in self.lanes:
+            lane.stream.wait_stream(root)
+            if self.captured:
+                with torch.cuda.stream(lane.stream):
+                    for g in lane.graphs:
+                        g.replay()
+            else:
+                self._lane_calls(lane, lane.calls)
+        for lane in self.lanes:
+            root.wait_stream(lane.stream)
+
+    def finish(self):
+        """Synchronise; re-solve every system whose status is not 1 with the
+        plain FP64 batch (count 1, up to 10 sweeps).  Returns the number of
+        systems re-solved; statuses still not 1 afterwards are left in
+        ``wave.status``."""
+        torch.cuda.synchronize()
+        redo = []
+        for w in self.waves:
+            bad = torch.nonzero(w.status != 1).flatten().tolist()
+            redo += [(w, k) for k in bad]
+        self.fallbacks = len(redo)
+        if not redo:
+            return 0
+        code = solver.batch_code('fp64')
+        for w, k in redo:
+            n, nch = w.n, w.nch
+            a = torch.empty((1, n, n), dtype=torch.complex128, device='cuda')
+            x = w.b.unsqueeze(0).clone()
+            work = torch.empty(solver.batch_work_bytes('fp64', n, nch, 1), dtype=torch.uint8,
+                               device='cuda')
+            e = slice(k, k + 1)
+            rmatrix.assemble_batch(w.dmesh, w.lv, w.qk[e], w.xc1[e], w.cpot, a)
+            solver.batch_solve(code, n, nch, 1, a.data_ptr(), x.data_ptr(), 10, self.tol,
+                               'rel_fro', w.res[k].data_ptr(), w.status[k].data_ptr(), work)
+            rmatrix.r_to_s_batch(w.dmesh, x, w.qk[e], w.nopen_dev[e], w.open[e], w.co[e],
+                                 w.cop[e], w.R[e], w.S[e])
+        torch.cuda.synchronize()
+        return len(redo)
+
+    @property
+    def n_systems(self):
+        return sum(len(w.energies) for w in self.waves)
+
+    @property
+    def flops(self):
+        """Algorithmic flops of the scan: sum of 8/3 N^3 + 8 N^2 nrhs."""
+        return float(sum(len(w.energies) * (8.0 / 3.0 * w.n ** 3 + 8.0 * w.n ** 2 * w.nch)
+                         for w in self.waves))
+
+
+class _BatchLane:
+    """One stream of a BatchScan with its matrix, solution and workspace
+    buffers, sized for the largest chunk it runs."""
+
+    def __init__(self):
+        self.stream = torch.cuda.Stream()
+        self.graphs = []
 
 
 def serial_fp64(waves, pairs, tol=1e-12):

@@ -143,6 +143,36 @@ def solve(a, b, method='fp64', max_refine=0, tol=1e-12, criterion='rel_fro', fin
     return b_cm.transpose(0, 1), st
 
 
+def batch_code(method, equilibrate=True):
+    """Method flags of hprmat_hip_batch_solve ('fp64' or 'fp32')."""
+    if method not in ('fp64', 'fp32'):
+        raise ValueError('batched method must be fp64 or fp32')
+    return METHODS[method] | (_lib.EQUILIBRATE if equilibrate else 0)
+
+
+def batch_work_bytes(method, n, nrhs, count, equilibrate=True):
+    """Device workspace (bytes) of a uniform batch of ``count`` systems."""
+    return int(_lib.load().hprmat_hip_batch_work_bytes(batch_code(method, equilibrate), int(n),
+                                                       int(nrhs), int(count)))
+
+
+def batch_solve(code, n, nrhs, count, a_ptr, b_ptr, max_refine, tol, criterion, res_ptr,
+                status_ptr, work, work_bytes=None):
+    """Raw hprmat_hip_batch_solve on the current stream: ``count`` column-major
+    n x n matrices at ``a_ptr`` (stride n*n) and n x nrhs right-hand sides at
+    ``b_ptr`` (stride n*nrhs, overwritten with the solutions); ``work`` a
+    uint8 device tensor of at least the batch's workspace size (shared by
+    calls that are ordered on one stream).  Allocates nothing, does not
+    synchronise."""
+    wb = work.numel() if work_bytes is None else int(work_bytes)
+    rc = _lib.load().hprmat_hip_batch_solve(
+        int(code), int(n), int(nrhs), int(count), ctypes.c_void_p(a_ptr), int(n), int(n) * int(n),
+        ctypes.c_void_p(b_ptr), int(n), int(n) * int(nrhs), int(max_refine), float(tol),
+        CRITERIA[criterion], ctypes.c_void_p(res_ptr), ctypes.c_void_p(status_ptr),
+        ctypes.c_void_p(work.data_ptr()), wb, _stream_ptr())
+    _lib.check(rc, 'hprmat_hip_batch_solve')
+
+
 class Batch:
     """Factor + solve + classic FP64 refinement of ``count`` systems of one
     order ``n`` in lockstep (libhprmat_hip's hprmat_hip_batch_solve: one chain
@@ -154,11 +184,9 @@ class Batch:
     """
 
     def __init__(self, n, nrhs, count, method='fp32', equilibrate=True):
-        if method not in ('fp64', 'fp32'):
-            raise ValueError('batched method must be fp64 or fp32')
         self._lib = _lib.load()
         self.n, self.nrhs, self.count, self.method = int(n), int(nrhs), int(count), method
-        self.code = METHODS[method] | (_lib.EQUILIBRATE if equilibrate else 0)
+        self.code = batch_code(method, equilibrate)
         self.work_bytes = int(self._lib.hprmat_hip_batch_work_bytes(self.code, self.n, self.nrhs,
                                                                     self.count))
         if self.work_bytes == 0:
@@ -184,11 +212,7 @@ class Batch:
             raise ValueError('a_cm must have shape (count, n, n)')
         if b_cm.dim() != 3 or b_cm.shape[1:] != (r, n) or b_cm.shape[0] < c:
             raise ValueError('b_cm must have shape (count, nrhs, n)')
-        rc = self._lib.hprmat_hip_batch_solve(
-            self.code, n, r, c, ctypes.c_void_p(a_cm.data_ptr()), n, n * n,
-            ctypes.c_void_p(b_cm.data_ptr()), n, n * r, int(max_refine), float(tol),
-            CRITERIA[criterion], ctypes.c_void_p(self.res.data_ptr()),
-            ctypes.c_void_p(self.status.data_ptr()), ctypes.c_void_p(self.work.data_ptr()),
-            self.work_bytes, _stream_ptr())
-        _lib.check(rc, 'hprmat_hip_batch_solve')
+        batch_solve(self.code, n, r, c, a_cm.data_ptr(), b_cm.data_ptr(), max_refine, tol,
+                    criterion, self.res.data_ptr(), self.status.data_ptr(), self.work,
+                    self.work_bytes)
         return self

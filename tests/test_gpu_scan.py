@@ -84,3 +84,51 @@ def test_autotune_plans_and_cache(tmp_path):
             ref = _oracle_s(w, k)
             if ref.size:
                 assert np.abs(w.s_of(k).cpu().numpy() - ref).max() <= 1e-10
+
+
+def test_batch_scan_matches_oracle_and_replays():
+    # uniform-batch scan: one chain per chunk of energies; a small memory
+    # budget forces several chunks per wave, three lanes, then graph replays
+    waves = _waves((0.7, 2.1, 4.4, 6.0, 9.5))
+    sc = scan.BatchScan(waves, method='fp32', lanes=3, mem_bytes=3 * 2 * 400 * 400 * 16 * 2)
+    assert any(sc.chunk[id(w)] < len(w.energies) for w in waves)
+    print('batch scan: eager', flush=True)
+    sc.run()
+    assert sc.finish() == 0
+    for w in waves:
+        assert (w.status.cpu().numpy() == 1).all()
+        assert (w.res.cpu().numpy() <= 1e-12).all()
+        for k in range(len(w.energies)):
+            ref = _oracle_s(w, k)
+            got = w.s_of(k).cpu().numpy()
+            assert got.shape == ref.shape
+            if ref.size:
+                assert np.abs(got - ref).max() <= 1e-10, (w.n, w.energies[k])
+    eager = [w.S.clone() for w in waves]
+    print('batch scan: capture', flush=True)
+    sc.capture()
+    for rep in range(3):
+        for w in waves:
+            w.S.zero_()
+            w.status.zero_()
+        torch.cuda.synchronize()
+        sc.run()
+        assert sc.finish() == 0
+        for w, e in zip(waves, eager):
+            assert torch.equal(w.S, e), rep
+
+
+def test_batch_scan_fallback_resolves_unconverged():
+    # with no refinement sweep allowed the FP32 factors alone miss 1e-12:
+    # finish() must re-solve those systems in FP64 and reach the oracle's S
+    waves = _waves((1.3, 5.5))[:3]
+    sc = scan.BatchScan(waves, method='fp32', lanes=2, max_refine=0)
+    sc.run()
+    n_redo = sc.finish()
+    assert n_redo > 0
+    for w in waves:
+        assert (w.status.cpu().numpy() == 1).all()
+        for k in range(len(w.energies)):
+            ref = _oracle_s(w, k)
+            if ref.size:
+                assert np.abs(w.s_of(k).cpu().numpy() - ref).max() <= 1e-10
