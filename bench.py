@@ -17,7 +17,8 @@ Methods timed back to back (name: factorization, refinement):
   fp32_gmres  the fp32 factorization with flexible-GMRES refinement (FP64
             Arnoldi, the FP32 factors as preconditioner, <= 24 products);
   bf16x3_gmres  the bf16x3 factorization with flexible-GMRES refinement;
-  fp32_3m   complex-FP32 LU with Gauss's three-product complex GEMM.
+  fp32_3m   complex-FP32 LU with Gauss's three-product complex GEMM;
+  fp32_3m_gmres  the fp32_3m factorization with flexible-GMRES refinement.
 Each method's solution is compared with the fp64 method's (max relative
 error of X and max |dS| of the S-matrix formed from it on the device).
 ``value`` is the solves/s of the fastest non-FP64 method that reaches the
@@ -25,7 +26,7 @@ residual gate; per-method figures, the roofline of the dominant kernel (the
 trailing-update GEMM, timed with HIP events on the launch stream) and the
 CPU baseline (LAPACK zgesv + R, reference solver type 1) are in the line.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--scan]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-scan]
 """
 import argparse
 import json
@@ -35,11 +36,13 @@ import time
 
 METRIC = "solves/s & TFLOP/s, N=25600 complex LU+solve: mixed vs FP64 at resid≤1e-12"
 PEAK = {'fp64': 78.6, 'fp32': 157.3, 'fp32_ir5': 157.3, 'fp32_gmres': 157.3, 'fp32_3m': 157.3,
+        'fp32_3m_gmres': 157.3,
         'bf16x3': 2500.0, 'bf16x3_gmres': 2500.0}   # dense MFMA TFLOP/s, MI355X (bf16x3: BF16 matrix peak)
 # name -> (factorization method, max refinement sweeps)
 SPECS = {'fp64': ('fp64', 10, 'ir'), 'fp32': ('fp32', 30, 'ir'), 'fp32_ir5': ('fp32', 5, 'ir'),
          'fp32_gmres': ('fp32', 24, 'fgmres'), 'bf16x3': ('bf16x3', 30, 'ir'),
-         'bf16x3_gmres': ('bf16x3', 32, 'fgmres'), 'fp32_3m': ('fp32_3m', 30, 'ir')}
+         'bf16x3_gmres': ('bf16x3', 32, 'fgmres'), 'fp32_3m': ('fp32_3m', 30, 'ir'),
+         'fp32_3m_gmres': ('fp32_3m', 24, 'fgmres')}
 
 
 def flops_solve(n, nrhs):
@@ -98,13 +101,13 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--nch', type=int, default=128)
     ap.add_argument('--nr', type=int, default=200)
-    ap.add_argument('--methods', default='fp64,fp32_gmres,fp32,fp32_ir5,bf16x3,bf16x3_gmres')
+    ap.add_argument('--methods', default='fp64,fp32_gmres,fp32_3m_gmres,fp32,fp32_ir5,bf16x3,bf16x3_gmres')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--copy-probe', type=int, default=0,
                     help='copy this many bytes device-to-device once before the run (a '
                          'kernel of known traffic for calibrating HBM counters)')
-    ap.add_argument('--scan', action='store_true',
-                    help='also run the batched energy scan (64 waves x 256 energies)')
+    ap.add_argument('--no-scan', action='store_true',
+                    help='skip the batched energy scan (64 waves x 256 energies)')
     ap.add_argument('--scan-waves', type=int, default=64)
     ap.add_argument('--scan-energies', type=int, default=256)
     ap.add_argument('--scan-lanes', type=int, default=2)
@@ -233,8 +236,12 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
         'speedup_mixed_vs_fp64': (methods['fp64']['ms'] / hm['ms']) if 'fp64' in methods else None,
         'roofline': roof,
     }
-    if args.scan:
-        out['energy_scan'] = energy_scan(torch, args)
+    if not args.no_scan:
+        try:
+            out['energy_scan'] = energy_scan(torch, args)
+        except Exception as exc:     # the headline line is still printed
+            out['energy_scan'] = {'error': repr(exc)}
+            print('# scan failed: %r' % (exc,), file=sys.stderr, flush=True)
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline()
     print(json.dumps(out))
@@ -281,8 +288,14 @@ def energy_scan(torch, args):
     e1.record()
     torch.cuda.synchronize()
     ms_ser = e0.elapsed_time(e1)
-    ds = max(float((w.s_of(k) - s_scan[(w.index, k)]).abs().max()) if w.nopen[k] else 0.0
-             for w, k in pairs)
+    dss = [(float((w.s_of(k) - s_scan[(w.index, k)]).abs().max()) if w.nopen[k] else 0.0, w, k)
+           for w, k in pairs]
+    ds, ww, wk = max(dss, key=lambda t: t[0])
+    worst_sys = {'wave': ww.index, 'energy_index': int(wk), 'energy': float(ww.energies[wk]),
+                 'N': ww.n, 'nch': ww.nch, 'nr': ww.nr, 'nopen': int(ww.nopen[wk]),
+                 'res_rel_scan': float(ww.res[wk]), 'dS': ds}
+    unconv = [{'wave': w.index, 'energy_index': int(k), 'N': w.n, 'res_rel': float(w.res[k])}
+              for w in waves for k in torch.nonzero(w.status != 1).flatten().tolist()]
     n_sys = sc.n_systems
     out = {'systems': n_sys, 'ms': ms, 'solves_per_s': n_sys / (ms * 1e-3),
            'tflops': sc.flops / (ms * 1e-3) / 1e12, 'flops': sc.flops, 'lanes': len(sc.lanes),
@@ -291,7 +304,7 @@ def energy_scan(torch, args):
            'serial_fp64': {'systems_timed': len(pairs), 'ms': ms_ser,
                            'solves_per_s': len(pairs) / (ms_ser * 1e-3),
                            'sample': 'every 16th energy of every wave, one stream, eager'},
-           'max_dS_vs_fp64': ds}
+           'max_dS_vs_fp64': ds, 'worst_dS_system': worst_sys, 'unconverged_systems': unconv[:16]}
     out['speedup_vs_serial_fp64'] = out['solves_per_s'] / out['serial_fp64']['solves_per_s']
     print('# scan: %.1f ms, %.0f solves/s, %.1f TFLOP/s; serial FP64 %.0f solves/s; x%.2f; '
           'max|dS| %.1e; fallbacks %d; max res %.1e' % (

@@ -29,6 +29,8 @@ struct hprmat_hip_lu {
   size_t split_cap = 0;
   void* gm = nullptr;         // FGMRES workspace (HPRMAT_FGMRES solves)
   size_t gm_cap = 0;
+  void* kpart = nullptr;      // split-K partial sums of the FP64 refinement products
+  size_t kpart_cap = 0;
   int* perm = nullptr;        // final row permutation
   size_t perm_cap = 0;
   double* scale = nullptr;    // equilibration factors (when equil)
@@ -96,7 +98,7 @@ void free_lu(hprmat_hip_lu* h) {
   }
   void* ps[] = {h->fac, h->ipiv, h->dinfo, h->dinv64, h->dinv256, h->ustage, h->split, h->gm,
                 h->perm, h->scale, h->scr, h->Brm, h->Xrm, h->R, h->RF, h->RY, h->part,
-                h->state};
+                h->state, h->kpart};
   for (void* p : ps)
     if (p) hipFree(p);
   *h = hprmat_hip_lu();
@@ -199,6 +201,15 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
       return -2;
     h->rhs_cap = cnt;
   }
+  // split-K partial sums of the FP64 products (grow-only, like the rest;
+  // sized for the largest slice count, so that a warm-up on the widest
+  // system leaves nothing to allocate for smaller ones, e.g. under capture)
+  hpr::SplitWork sw;
+  sw.splits = hpr::resid_splits(n, nrhs);
+  if (sw.splits > 1) {
+    if (grow(&h->kpart, &h->kpart_cap, (size_t)hpr::RESID_MAX_SPLITS * cnt * 16)) return -2;
+    sw.part = (double2*)h->kpart;
+  }
   if (hpr::transpose_cast<double, double>(n, nrhs, (const double2*)dB, ldb, h->Brm, nrhs, st))
     return -1;
   int rc;
@@ -220,27 +231,30 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
       rc = hpr::gmres_solve<double>((const double2*)h->fac, h->ld, n, (const double2*)h->dinv256,
                                     h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, m,
                                     tol, crit, h->state, h->R, (double2*)h->RF, (double2*)h->RY,
-                                    h->part, gw, h->plan, false, st, h->equil ? h->scale : nullptr);
+                                    h->part, gw, h->plan, false, st, h->equil ? h->scale : nullptr,
+                                    sw);
     else
       rc = hpr::gmres_solve<float>((const float2*)h->fac, h->ld, n, (const float2*)h->dinv256,
                                    h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, m,
                                    tol, crit, h->state, h->R, (float2*)h->RF, (float2*)h->RY,
-                                   h->part, gw, h->plan, g3m, st, h->equil ? h->scale : nullptr);
+                                   h->part, gw, h->plan, g3m, st, h->equil ? h->scale : nullptr,
+                                   sw);
     final_check = 0;   // the FGMRES solve ends with a true residual evaluation
   } else if (h->method == HPRMAT_PLAIN_FP64)
     rc = hpr::refine_solve<double>((const double2*)h->fac, h->ld, n, (const double2*)h->dinv256,
                                    h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, tol,
                                    crit, h->state, h->R, (double2*)h->RF, (double2*)h->RY,
-                                   h->part, h->plan, false, st, h->equil ? h->scale : nullptr);
+                                   h->part, h->plan, false, st, h->equil ? h->scale : nullptr,
+                                   sw);
   else
     rc = hpr::refine_solve<float>((const float2*)h->fac, h->ld, n, (const float2*)h->dinv256,
                                   h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, tol,
                                   crit, h->state, h->R, (float2*)h->RF, (float2*)h->RY, h->part,
-                                  h->plan, g3m, st, h->equil ? h->scale : nullptr);
+                                  h->plan, g3m, st, h->equil ? h->scale : nullptr, sw);
   if (rc) return rc;
   if (final_check)
     rc = hpr::residual_check(n, nrhs, h->A64, h->lda, h->Brm, h->Xrm, h->R, h->part, h->state,
-                             crit, tol, st);
+                             crit, tol, st, sw);
   if (rc) return rc;
   if (hpr::transpose_cast<double, double>(nrhs, n, h->Xrm, nrhs, (double2*)dB, ldb, st)) return -1;
   if (stats) {
@@ -468,7 +482,7 @@ int hprmat_hip_lu_set_plan(hprmat_hip_lu_t h, int key, int value) {
   if (!h) return -1;
   switch (key) {
     case HPRMAT_PLAN_TRAIL_TILE:
-      if (value < 0 || value > 2) return -1;
+      if (value < 0 || value > 6) return -1;
       h->plan.trail = value;
       return 0;
     case HPRMAT_PLAN_LOOKAHEAD:

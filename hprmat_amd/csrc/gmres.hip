@@ -301,7 +301,7 @@ int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, cons
                 const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
                 int max_steps, int m, double tol, int crit, RefineState* state, double2* R,
                 cx<TF>* RF, cx<TF>* RY, double* part, const GmresWork& gw, const LuPlan& plan,
-                bool g3m, hipStream_t st, const double* scale) {
+                bool g3m, hipStream_t st, const double* scale, const SplitWork& sw) {
   if (m < 1 || m > GMRES_MAX_M || nrhs > 1024) return -1;
   const size_t cnt = (size_t)n * nrhs;
   const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
@@ -318,7 +318,7 @@ int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, cons
     left -= mc;
     // true residual, stopping rule, start of a cycle
     copy_kernel<<<eg, 256, 0, st>>>(cnt, B, R, done);
-    gemm<double>(n, nrhs, n, A64, lda, true, X, nrhs, R, nrhs, -1.0, 1, false, done, st);
+    if (resid_gemm(n, nrhs, A64, lda, X, R, -1.0, 1, done, sw, st)) return -1;
     residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
     dots(n, nrhs, 1, R, 0, R, gw.h, 0, gw, done, st);
     cycle_begin_kernel<<<1, 256, 0, st>>>(nrhs, mc, gw.h, gw.g, gw.inv, state);
@@ -331,7 +331,7 @@ int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, cons
       if (getrs<TF>(LU, ld, n, dinv256, RF, nrhs, nrhs, RY, halt, plan, g3m, st)) return -1;
       widen_kernel<TF><<<eg, 256, 0, st>>>(cnt, nrhs, RF, scale, Zj, halt);
       // W = A Z_j, two CGS passes, ||W||^2
-      gemm<double>(n, nrhs, n, A64, lda, true, Zj, nrhs, gw.W, nrhs, 1.0, 0, false, halt, st);
+      if (resid_gemm(n, nrhs, A64, lda, Zj, gw.W, 1.0, 0, halt, sw, st)) return -1;
       double2* h1 = gw.h;
       double2* h2 = gw.h + (size_t)(mc + 1) * nrhs;
       dots(n, nrhs, j + 1, gw.V, cnt, gw.W, h1, 0, gw, halt, st);
@@ -350,7 +350,7 @@ int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, cons
   }
   // final true residual
   copy_kernel<<<eg, 256, 0, st>>>(cnt, B, R, done);
-  gemm<double>(n, nrhs, n, A64, lda, true, X, nrhs, R, nrhs, -1.0, 1, false, done, st);
+  if (resid_gemm(n, nrhs, A64, lda, X, R, -1.0, 1, done, sw, st)) return -1;
   residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
   HPR_CHECK(hipGetLastError());
   return 0;
@@ -360,11 +360,11 @@ template int gmres_solve<double>(const cx<double>*, int64_t, int, const cx<doubl
                                  const double2*, int64_t, const double2*, double2*, int, int, int,
                                  double, int, RefineState*, double2*, cx<double>*, cx<double>*,
                                  double*, const GmresWork&, const LuPlan&, bool, hipStream_t,
-                                 const double*);
+                                 const double*, const SplitWork&);
 template int gmres_solve<float>(const cx<float>*, int64_t, int, const cx<float>*, const int*,
                                 const double2*, int64_t, const double2*, double2*, int, int, int,
                                 double, int, RefineState*, double2*, cx<float>*, cx<float>*,
                                 double*, const GmresWork&, const LuPlan&, bool, hipStream_t,
-                                const double*);
+                                const double*, const SplitWork&);
 
 }  // namespace hpr

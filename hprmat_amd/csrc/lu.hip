@@ -18,6 +18,7 @@
 // 256-row block rows: one MFMA block-solve kernel per block row using the
 // stored 64 x 64 inverses, and one MFMA GEMM update of the remaining rows
 // (cusolverDn?getrs, src/cusolver_interface.cu:393, 787).
+#include <cmath>
 #include <vector>
 
 #include <type_traits>
@@ -354,6 +355,89 @@ size_t trail_lds_extra(bool fp32) {
   return fp32 ? 32768 : 0;
 }
 
+// ---- split-K FP64 refinement product ---------------------------------------
+namespace {
+constexpr int SK_BM = 64, SK_BN = 64, SK_BK = 16;
+
+// Slice s = blockIdx.y of K: P_s = A[:, k0:k0+kn] X[k0:k0+kn, :] (A column-major).
+__global__ void __launch_bounds__(256)
+zgemm_splitk_kernel(int M, int N, int K, int Kc, const double2* __restrict__ A, int64_t lda,
+                    const double2* __restrict__ B, int64_t ldb, double2* __restrict__ P,
+                    int64_t sP, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  const int k0 = blockIdx.y * Kc;
+  const int kn = min(Kc, K - k0);
+  zgemm_tile<double, SK_BM, SK_BN, SK_BK, 32, 32, true, false>(
+      M, N, kn, A + (int64_t)k0 * lda, lda, B + (int64_t)k0 * ldb, ldb, P + blockIdx.y * sP, N,
+      1.0, 0, blockIdx.x);
+}
+
+// C = beta C + alpha (P_0 + ... + P_{S-1}), summed in slice order.
+__global__ void __launch_bounds__(256)
+splitk_reduce_kernel(size_t cnt, int S, const double2* __restrict__ P, size_t sP,
+                     double2* __restrict__ C, double alpha, int beta,
+                     const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt;
+       i += (size_t)gridDim.x * blockDim.x) {
+    double2 acc = P[i];
+    for (int s = 1; s < S; ++s) {
+      const double2 v = P[(size_t)s * sP + i];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    const double2 c = beta ? C[i] : make_double2(0.0, 0.0);
+    C[i] = make_double2(c.x + alpha * acc.x, c.y + alpha * acc.y);
+  }
+}
+}  // namespace
+
+int resid_splits(int n, int nrhs) {
+  static int slots = 0;
+  if (!slots) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, zgemm_splitk_kernel, 256, 0) !=
+            hipSuccess || cus <= 0 || per <= 0)
+      return 1;
+    slots = cus * per;
+  }
+  if (n <= 0 || nrhs <= 0) return 1;
+  const double tiles = (double)((n + SK_BM - 1) / SK_BM) * ((nrhs + SK_BN - 1) / SK_BN);
+  if (tiles >= 4.0 * slots) return 1;
+  // fraction of the last round of workgroups that is busy, less a small
+  // charge per slice for the partial-sum traffic
+  auto score = [&](int s) {
+    const double rounds = tiles * s / slots;
+    return rounds / std::ceil(rounds) - 0.01 * s;
+  };
+  int best = 1;
+  for (int s = 2; s <= RESID_MAX_SPLITS; ++s) {
+    if (n / s < 512) break;
+    if (score(s) > score(best)) best = s;
+  }
+  return best;
+}
+
+int resid_gemm(int n, int nrhs, const double2* A64, int64_t lda, const double2* X, double2* R,
+               double alpha, int beta, const int* skip, const SplitWork& sw, hipStream_t st) {
+  if (n <= 0 || nrhs <= 0) return 0;
+  if (sw.splits <= 1 || !sw.part)
+    return gemm<double>(n, nrhs, n, A64, lda, true, X, nrhs, R, nrhs, alpha, beta, false, skip,
+                        st);
+  const int S = sw.splits;
+  const int Kc = ((n + S - 1) / S + SK_BK - 1) / SK_BK * SK_BK;
+  const size_t cnt = (size_t)n * nrhs;
+  const int tiles = ((n + SK_BM - 1) / SK_BM) * ((nrhs + SK_BN - 1) / SK_BN);
+  zgemm_splitk_kernel<<<dim3(tiles, S), 256, 0, st>>>(n, nrhs, n, Kc, A64, lda, X, nrhs, sw.part,
+                                                      (int64_t)cnt, skip);
+  splitk_reduce_kernel<<<(int)min((size_t)4096, (cnt + 255) / 256), 256, 0, st>>>(
+      cnt, S, sw.part, cnt, R, alpha, beta, skip);
+  HPR_CHECK(hipGetLastError());
+  return 0;
+}
+
 // Split-BF16 planes of L21 (nc x kb at L, row stride ld) and of U12^T (U12:
 // kb x nc at U, row stride ldu) into `split` (see bf16x3.hpp).
 template <class T>
@@ -381,11 +465,36 @@ void launch_gemm2(int M, int N, int K, const cx<T>* A, int64_t lda, const cx<T>*
                                                                       ldc, alpha, beta, nullptr);
 }
 
+// FP32 trailing update on 32x32x2 MFMA (zgemm32_kernel); lds_extra as in gemm().
+template <int BM, int BN, int BK, int WM, int WN>
+void launch_gemm32(int M, int N, int K, const cx<float>* A, int64_t lda, const cx<float>* B,
+                   int64_t ldb, cx<float>* C, int64_t ldc, float alpha, int beta, hipStream_t st,
+                   size_t lds_extra) {
+  static bool attr = false;
+  if (!attr) { allow_big_lds(zgemm32_kernel<BM, BN, BK, WM, WN, false>); attr = true; }
+  const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+  const size_t lds = zgemm32_lds_bytes<BK, BM, BN>() + lds_extra;
+  zgemm32_kernel<BM, BN, BK, WM, WN, false><<<tiles, 256, lds, st>>>(M, N, K, A, lda, B, ldb, C,
+                                                                      ldc, alpha, beta, nullptr);
+}
+
 template <class T>
 void trailing(bool bfx, int nc, int ncols, int c0, int kb, const cx<T>* L, int64_t ld,
               const cx<T>* U, int64_t ldu, cx<T>* C, int64_t ldc, bool g3m,
               const unsigned short* split, hipStream_t st, size_t lds_extra, int variant) {
   if (nc <= 0 || ncols <= 0) return;
+  if constexpr (std::is_same<T, float>::value) {
+    if (!g3m && !bfx && variant >= 3) {
+      const cx<float>* B = U + c0;
+      switch (variant) {
+        case 3: launch_gemm32<64, 64, 16, 32, 32>(nc, ncols, kb, L, ld, B, ldu, C, ldc, -1.f, 1, st, lds_extra); return;
+        case 4: launch_gemm32<128, 128, 16, 64, 64>(nc, ncols, kb, L, ld, B, ldu, C, ldc, -1.f, 1, st, lds_extra); return;
+        case 5: launch_gemm32<128, 64, 16, 64, 32>(nc, ncols, kb, L, ld, B, ldu, C, ldc, -1.f, 1, st, lds_extra); return;
+        case 6: launch_gemm32<64, 128, 16, 32, 64>(nc, ncols, kb, L, ld, B, ldu, C, ldc, -1.f, 1, st, lds_extra); return;
+        default: break;
+      }
+    }
+  }
   if (!g3m && !bfx && variant == 1) {
     launch_gemm2<T, 64, 128, 16, 32, 64>(nc, ncols, kb, L, ld, U + c0, ldu, C, ldc, T(-1), 1, st);
     return;
@@ -630,7 +739,7 @@ int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, con
                  const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
                  int max_refine, double tol, int crit, RefineState* state, double2* R, cx<TF>* RF,
                  cx<TF>* RY, double* part, const LuPlan& plan, bool g3m, hipStream_t st,
-                 const double* scale) {
+                 const double* scale, const SplitWork& sw) {
   const size_t cnt = (size_t)n * nrhs;
   const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
   const int* done = &state->done;
@@ -640,7 +749,7 @@ int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, con
   add_correction_kernel<TF><<<eg, 256, 0, st>>>(nrhs, cnt, X, RF, scale, 0, state);
   for (int it = 0; it < max_refine; ++it) {
     cast_kernel<double, double><<<eg, 256, 0, st>>>(cnt, B, R, done);
-    gemm<double>(n, nrhs, n, A64, lda, true, X, nrhs, R, nrhs, -1.0, 1, false, done, st);
+    if (resid_gemm(n, nrhs, A64, lda, X, R, -1.0, 1, done, sw, st)) return -1;
     residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
     permute_cast<double, TF>(n, nrhs, perm, R, RF, done, st, scale);
     if (getrs<TF>(LU, ld, n, dinv256, RF, nrhs, nrhs, RY, done, plan, g3m, st)) return -1;
@@ -653,12 +762,12 @@ int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, con
 // Residual check without correction (final verification of a refined X).
 int residual_check(int n, int nrhs, const double2* A64, int64_t lda, const double2* B,
                    const double2* X, double2* R, double* part, RefineState* state, int crit,
-                   double tol, hipStream_t st) {
+                   double tol, hipStream_t st, const SplitWork& sw) {
   const size_t cnt = (size_t)n * nrhs;
   const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
   const int* done = &state->done;
   cast_kernel<double, double><<<eg, 256, 0, st>>>(cnt, B, R, done);
-  gemm<double>(n, nrhs, n, A64, lda, true, X, nrhs, R, nrhs, -1.0, 1, false, done, st);
+  if (resid_gemm(n, nrhs, A64, lda, X, R, -1.0, 1, done, sw, st)) return -1;
   residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
   HPR_CHECK(hipGetLastError());
   return 0;
@@ -984,7 +1093,8 @@ int solve_batched(const BatchArgs<TF>& b, const double2* A64, int64_t lda, int64
   template int refine_solve<T>(const cx<T>*, int64_t, int, const cx<T>*, const int*,          \
                                const double2*, int64_t, const double2*, double2*, int, int,    \
                                double, int, RefineState*, double2*, cx<T>*, cx<T>*, double*,   \
-                               const LuPlan&, bool, hipStream_t, const double*);       \
+                               const LuPlan&, bool, hipStream_t, const double*,               \
+                               const SplitWork&);                                              \
   template int batch_bind<T>(BatchArgs<T>&, void*, size_t, int, int, int, bool);              \
   template int getrf_batched<T>(const BatchArgs<T>&, hipStream_t);                             \
   template int getrs_batched<T>(const BatchArgs<T>&, cx<T>*, cx<T>*, int, const int*, int,     \

@@ -7,7 +7,10 @@ namespace hpr {
 // Tunables of the blocked factorization.
 struct LuPlan {
   int nb = 256;     // outer panel width and getrs block height (<= 256, multiple of 64)
-  int trail = 0;    // trailing-update GEMM tile: 0 64x64x16, 1 64x128x16, 2 128x64x16
+  // trailing-update GEMM: 0 64x64x16, 1 64x128x16, 2 128x64x16 (16x16x4 MFMA);
+  // FP32 only, 32x32x2 MFMA: 3 64x64x16, 4 128x128x16, 5 128x64x16, 6 64x128x16
+  // (FP64 uses variant 0 for 3..6)
+  int trail = 0;
 };
 
 // Device scratch used by getrf: exchange area of the cooperative panel
@@ -114,6 +117,24 @@ int gemm(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<
          int64_t ldb, cx<T>* C, int64_t ldc, T alpha, int beta, bool gauss3m,
          const int* skip, hipStream_t st, size_t lds_extra = 0);
 
+// Split-K partial sums of the FP64 refinement product R = beta R + alpha A X
+// (A column-major n x n, X and R row-major n x nrhs): `splits` slices of K
+// are multiplied by separate workgroups into `part` (splits x n x nrhs) and
+// summed in a fixed order, so a product that has too few output tiles to
+// fill the GPU still runs at full occupancy and results stay reproducible.
+struct SplitWork {
+  double2* part = nullptr;
+  int splits = 1;
+};
+// Number of K slices for the product of an n x n matrix with nrhs columns
+// (1: no split); resid_splits(n, nrhs) * n * nrhs complex doubles of `part`.
+int resid_splits(int n, int nrhs);
+constexpr int RESID_MAX_SPLITS = 8;
+// R = beta R + alpha A64 X (skipped when *skip is set), split-K when sw has
+// more than one slice and a partial buffer.
+int resid_gemm(int n, int nrhs, const double2* A64, int64_t lda, const double2* X, double2* R,
+               double alpha, int beta, const int* skip, const SplitWork& sw, hipStream_t st);
+
 // Mixed/plain refinement around resident factors.  A64 is the FP64 matrix in
 // column-major order (lda); B and X are row-major n x nrhs (ld = nrhs).
 // Work buffers: R (n x nrhs FP64), RF and RY (n x nrhs in factor precision),
@@ -125,13 +146,13 @@ int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, con
                  const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
                  int max_refine, double tol, int crit, RefineState* state, double2* R,
                  cx<TF>* RF, cx<TF>* RY, double* part, const LuPlan& plan, bool gauss3m,
-                 hipStream_t st, const double* scale);
+                 hipStream_t st, const double* scale, const SplitWork& sw = SplitWork());
 
 // One residual evaluation of a refined X (skipped when state->done is set):
 // R = B - A64 X, norms and the stopping rule recorded in state.
 int residual_check(int n, int nrhs, const double2* A64, int64_t lda, const double2* B,
                    const double2* X, double2* R, double* part, RefineState* state, int crit,
-                   double tol, hipStream_t st);
+                   double tol, hipStream_t st, const SplitWork& sw = SplitWork());
 
 // Norms of an n x nrhs FP64 block R (row-major): mode 0 records ||B|| and
 // resets the state, mode 1 records a residual evaluation and the stopping
@@ -170,7 +191,8 @@ int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, cons
                 const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
                 int max_steps, int m, double tol, int crit, RefineState* state, double2* R,
                 cx<TF>* RF, cx<TF>* RY, double* part, const GmresWork& gw, const LuPlan& plan,
-                bool gauss3m, hipStream_t st, const double* scale);
+                bool gauss3m, hipStream_t st, const double* scale,
+                const SplitWork& sw = SplitWork());
 
 // ---- uniform batches ------------------------------------------------------
 // `count` independent systems of one order n, each at a fixed element

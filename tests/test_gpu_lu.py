@@ -235,3 +235,25 @@ def test_memory_scale_real_potential_unitarity():
     assert st['res_rel'] <= 1e-10, st
     assert defect <= 1e-8, defect
     assert np.abs(sm - sm.T).max() <= 1e-8
+
+
+@pytest.mark.parametrize('tile', [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize('method', ['fp64', 'fp32'])
+def test_trailing_tile_variants_match_lapack(tile, method):
+    # every trailing-update kernel variant (16x16x4 and 32x32x2 MFMA tiles;
+    # FP64 maps 3..6 to the default tile), with look-ahead, ragged edges
+    n = 1337
+    print('trail tile %d method %s n %d' % (tile, method, n), flush=True)
+    a, b = rand_system(n, 4, 4242 + tile)
+    x_ref, info = osolve.zgesv(a, b)
+    assert info == 0
+    lu = solver.LU().set_plan(trail_tile=tile, lookahead=1)
+    lu.factor(solver.colmajor(torch.from_numpy(a).cuda()), method)
+    b_cm = torch.from_numpy(np.ascontiguousarray(b.T)).cuda()
+    st = lu.solve_(b_cm, 0 if method == 'fp64' else 12, 1e-13, 'rel_fro', True)
+    x = b_cm.cpu().numpy().T
+    assert st['info'] == 0
+    assert np.abs(x - x_ref).max() / np.abs(x_ref).max() <= 1e-12, st
+    if method == 'fp32':
+        # unrefined FP32 factors: first residual at FP32 level
+        assert st['hist'][0] <= 1e-4, st['hist'][:2]
