@@ -87,3 +87,11 @@ template <> struct acc_t<double> { typedef double4_t type; };
 template <> struct acc_t<float> { typedef float4_t type; };
 
 static inline int64_t cdiv_i64(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Raise the issue priority of the calling wave within its SIMD (s_setprio 3)
+// when `on`: kernels on the latency-bound panel chain of the factorization
+// use it so that their instructions win arbitration against the MFMA-bound
+// trailing-update waves that share the SIMDs with them.
+__device__ __forceinline__ void raise_wave_priority(int on) {
+  if (on) __builtin_amdgcn_s_setprio(3);
+}

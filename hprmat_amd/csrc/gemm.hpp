@@ -177,7 +177,8 @@ __global__ void __launch_bounds__(256)
 zgemm_kernel(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
              const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
              int64_t ldc, T alpha, int beta, const int* __restrict__ skip, int64_t sA = 0,
-             int64_t sB = 0, int64_t sC = 0, int sSkip = 0) {
+             int64_t sB = 0, int64_t sC = 0, int sSkip = 0, int prio = 0) {
+  raise_wave_priority(prio);
   const int b = blockIdx.y;   // system of a batch
   if (skip && skip[(int64_t)b * sSkip]) return;
   zgemm_tile<T, BM, BN, BK, WM, WN, A_COL, GAUSS3M>(M, N, K, A + b * sA, lda, B + b * sB, ldb,

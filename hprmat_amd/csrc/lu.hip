@@ -243,7 +243,8 @@ constexpr size_t XW_VAL = 2 * MAX_LEAF_G * 2, XW_IDX = 2 * MAX_LEAF_G,
 constexpr size_t XW_TOTAL = XW_VAL + XW_IDX + XW_ROW + XW_ROWJ;
 
 template <class T> int launch_leaf(cx<T>* P, int64_t ld, int m, int w, int row_base, int* ipiv,
-                                   int* info, const Scratch& s, int leaf, hipStream_t st) {
+                                   int* info, const Scratch& s, int leaf, hipStream_t st,
+                                   int prio = 0) {
   constexpr int NT = 256;   // one row per thread
   const int G = (m + NT - 1) / NT;
   if (G < 1 || G > MAX_LEAF_G) return -1;   // all leaf workgroups must be co-resident
@@ -255,19 +256,20 @@ template <class T> int launch_leaf(cx<T>* P, int64_t ld, int m, int w, int row_b
   x.rowj = x.row + XW_ROW;
   x.gstride = MAX_LEAF_G;
   leaf_getf2_kernel<T, LEAF_W><<<G, NT, 0, st>>>(P, ld, m, w, row_base, ipiv, info, x,
-                                                  (unsigned)leaf * LEAF_W, s.err);
+                                                  (unsigned)leaf * LEAF_W, s.err, 0, 0, 0, prio);
   return 0;
 }
 
 template <class T> void launch_laswp(cx<T>* A, int64_t ld, int c0, int c1, int k1, int nk,
-                                     const int* ipiv, const int* skip, hipStream_t st) {
+                                     const int* ipiv, const int* skip, hipStream_t st,
+                                     int prio = 0) {
   if (c1 <= c0 || nk <= 0) return;
   constexpr int CW = 128 / sizeof(cx<T>);
   size_t lds = (size_t)2 * nk * 128 + (size_t)4 * nk * sizeof(int);
   static bool attr = false;
   if (!attr) { allow_big_lds(laswp_kernel<T>); attr = true; }
   int grid = (c1 - c0 + CW - 1) / CW;
-  laswp_kernel<T><<<grid, 256, lds, st>>>(A, ld, c0, c1, k1, nk, ipiv, skip);
+  laswp_kernel<T><<<grid, 256, lds, st>>>(A, ld, c0, c1, k1, nk, ipiv, skip, 0, 0, 0, prio);
 }
 
 // ---- optional HIP-event timing of the trailing-update GEMM ------------------
@@ -328,20 +330,32 @@ void scratch_bind(Scratch& s, void* base) {
 template <class T>
 int gemm(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<T>* B,
          int64_t ldb, cx<T>* C, int64_t ldc, T alpha, int beta, bool g3m, const int* skip,
-         hipStream_t st, size_t lds_extra) {
+         hipStream_t st, size_t lds_extra, int prio) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
   constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32;
   int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
   // lds_extra reserves unused LDS per workgroup, which caps the number of
   // workgroups per CU and so leaves room for the concurrently running panel
   if (a_col) {
-    if (g3m) zgemm_kernel<T, BM, BN, BK, WM, WN, true, true><<<tiles, 256, lds_extra, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip);
-    else zgemm_kernel<T, BM, BN, BK, WM, WN, true, false><<<tiles, 256, lds_extra, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip);
+    if (g3m) zgemm_kernel<T, BM, BN, BK, WM, WN, true, true><<<tiles, 256, lds_extra, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, 0, 0, 0, 0, prio);
+    else zgemm_kernel<T, BM, BN, BK, WM, WN, true, false><<<tiles, 256, lds_extra, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, 0, 0, 0, 0, prio);
   } else {
-    if (g3m) zgemm_kernel<T, BM, BN, BK, WM, WN, false, true><<<tiles, 256, lds_extra, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip);
-    else zgemm_kernel<T, BM, BN, BK, WM, WN, false, false><<<tiles, 256, lds_extra, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip);
+    if (g3m) zgemm_kernel<T, BM, BN, BK, WM, WN, false, true><<<tiles, 256, lds_extra, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, 0, 0, 0, 0, prio);
+    else zgemm_kernel<T, BM, BN, BK, WM, WN, false, false><<<tiles, 256, lds_extra, st>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, 0, 0, 0, 0, prio);
   }
   return 0;
+}
+
+// Wave priority of the panel chain while the look-ahead stream updates the
+// trailing matrix (HPRMAT_HIP_PANEL_PRIO): 0 none, 1 the panel kernels (leaf,
+// interchanges, triangular solves, inverses), 2 also the main-stream GEMMs
+// of the panel (in-panel updates, U12, the next panel's columns).
+int panel_prio_level() {
+  static int v = [] {
+    const char* e = getenv("HPRMAT_HIP_PANEL_PRIO");
+    return e ? atoi(e) : 2;
+  }();
+  return v;
 }
 
 // Extra LDS of the look-ahead trailing update (bytes, HPRMAT_HIP_TRAIL_LDS;
@@ -440,14 +454,21 @@ int resid_gemm(int n, int nrhs, const double2* A64, int64_t lda, const double2* 
 
 // Split-BF16 planes of L21 (nc x kb at L, row stride ld) and of U12^T (U12:
 // kb x nc at U, row stride ldu) into `split` (see bf16x3.hpp).
+// With split_l false only U12's columns [c0, c0 + cols) are split (the
+// planes keep the layout of all nc columns).
 template <class T>
 void split_operands(int nc, int kb, const cx<T>* L, int64_t ld, const cx<T>* U, int64_t ldu,
-                    unsigned short* split, hipStream_t st) {
+                    unsigned short* split, hipStream_t st, bool split_l = true, int c0 = 0,
+                    int cols = -1) {
   if constexpr (std::is_same<T, float>::value) {
     const size_t plane = (size_t)nc * 256;
-    const dim3 grid(256 / 32, (nc + 31) / 32);
-    split_bf16_kernel<false><<<grid, 256, 0, st>>>(nc, kb, L, ld, split, plane, 256);
-    split_bf16_kernel<true><<<grid, 256, 0, st>>>(nc, kb, U, ldu, split + 4 * plane, plane, 256);
+    if (cols < 0) cols = nc - c0;
+    if (split_l)
+      split_bf16_kernel<false><<<dim3(256 / 32, (nc + 31) / 32), 256, 0, st>>>(
+          nc, kb, L, ld, split, plane, 256);
+    if (cols > 0)
+      split_bf16_kernel<true><<<dim3(256 / 32, (cols + 31) / 32), 256, 0, st>>>(
+          cols, kb, U + c0, ldu, split + 4 * plane + (size_t)c0 * 256, plane, 256);
   }
 }
 
@@ -481,7 +502,8 @@ void launch_gemm32(int M, int N, int K, const cx<float>* A, int64_t lda, const c
 template <class T>
 void trailing(bool bfx, int nc, int ncols, int c0, int kb, const cx<T>* L, int64_t ld,
               const cx<T>* U, int64_t ldu, cx<T>* C, int64_t ldc, bool g3m,
-              const unsigned short* split, hipStream_t st, size_t lds_extra, int variant) {
+              const unsigned short* split, hipStream_t st, size_t lds_extra, int variant,
+              int prio = 0) {
   if (nc <= 0 || ncols <= 0) return;
   if constexpr (std::is_same<T, float>::value) {
     if (!g3m && !bfx && variant >= 3) {
@@ -516,7 +538,8 @@ void trailing(bool bfx, int nc, int ncols, int c0, int kb, const cx<T>* L, int64
       return;
     }
   }
-  gemm<T>(nc, ncols, kb, L, ld, false, U + c0, ldu, C, ldc, T(-1), 1, g3m, nullptr, st, lds_extra);
+  gemm<T>(nc, ncols, kb, L, ld, false, U + c0, ldu, C, ldc, T(-1), 1, g3m, nullptr, st, lds_extra,
+          prio);
 }
 
 template <class T>
@@ -571,6 +594,8 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
   if (bfx && !w.split) return -1;
   if (nb != 256) return -1;
   const bool la = side && side->stream && n > 2 * nb;  // look-ahead
+  const int pk = la && panel_prio_level() >= 1;   // priority of the panel kernels
+  const int pg = la && panel_prio_level() >= 2;   // ... and of the main-stream GEMMs
   (void)hipGetLastError();  // start from a clean error state (checked at the end)
   fill_u32_kernel<<<64, 256, 0, st>>>((unsigned*)s.xchg, s.xchg_bytes / sizeof(unsigned), 0u);
   fill_u32_kernel<<<1, 64, 0, st>>>((unsigned*)s.err, 1, 0u);
@@ -585,67 +610,107 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
     for (int t = 0; t < kb; t += LEAF_W) {
       const int wt = min(LEAF_W, kb - t);
       cx<T>* P = A + (int64_t)(k + t) * ld + (k + t);
-      if (launch_leaf<T>(P, ld, m - t, wt, k + t, ipiv + k + t, info, s, leaf++, st)) return -1;
-      if (t > 0) launch_laswp<T>(A, ld, k, k + t, k + t, wt, ipiv, nullptr, st);
+      if (launch_leaf<T>(P, ld, m - t, wt, k + t, ipiv + k + t, info, s, leaf++, st, pk))
+        return -1;
+      if (t > 0) launch_laswp<T>(A, ld, k, k + t, k + t, wt, ipiv, nullptr, st, pk);
       if (t + wt < kb) {
-        launch_laswp<T>(A, ld, k + t + wt, k + kb, k + t, wt, ipiv, nullptr, st);
+        launch_laswp<T>(A, ld, k + t + wt, k + kb, k + t, wt, ipiv, nullptr, st, pk);
         trsm_diag_kernel<T, false><<<(kb - t - wt + 15) / 16, 256, 0, st>>>(
-            P, ld, wt, P + wt, ld, kb - t - wt, nullptr);
+            P, ld, wt, P + wt, ld, kb - t - wt, nullptr, 0, pk);
         gemm<T>(m - t - wt, kb - t - wt, wt, P + (int64_t)wt * ld, ld, false, P + wt, ld,
-                P + (int64_t)wt * ld + wt, ld, T(-1), 1, g3m, nullptr, st);
+                P + (int64_t)wt * ld + wt, ld, T(-1), 1, g3m, nullptr, st, 0, pg);
       }
     }
-    // -- inverses of the panel's diagonal blocks: 64 x 64, then 256 x 256
-    diag_inv_kernel<T><<<dim3((kb + 63) / 64, 2), 256, 0, st>>>(A, ld, n, k / 64, w.dinv64);
-    blk_inv_kernel<T><<<dim3(1, 2, 4), 256, 0, st>>>(A, ld, n, k / 256, w.dinv64, w.dinv256);
-    // columns outside the panel may still be updated (and read) on the side
-    // stream: wait before interchanging rows there
-    if (rest_pending) {
-      HPR_CHECK(hipStreamWaitEvent(st, side->rest, 0));
-      rest_pending = false;
-    }
-    if (k > 0) launch_laswp<T>(A, ld, 0, k, k, kb, ipiv, nullptr, st);
-    if (k + kb < n) {
-      const int nc = n - k - kb;
-      cx<T>* Akk = A + (int64_t)k * ld + k;
-      launch_laswp<T>(A, ld, k + kb, n, k, kb, ipiv, nullptr, st);
-      // U12 = inv(L11) A12, staged (row stride nc), then A22 -= L21 U12
-      gemm<T>(kb, nc, kb, w.dinv256 + (size_t)(k / 256) * 2 * 65536, 256, false, Akk + kb, ld,
-              w.ustage, nc, T(1), 0, g3m, nullptr, st);
-      // next panel's columns first (on the main stream), the rest on the side
-      // (the timed "trailing update" is the launch over the columns beyond the
-      // next panel, or the whole update when there is no look-ahead)
-      const int c_next = (la && nc > nb) ? nb : nc;
-      ProfRec* pr = (g_prof_on && c_next == nc) ? prof_slot() : nullptr;
-      if (bfx) split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, st);
-      if (pr) (void)hipEventRecord(pr->a, st);
-      trailing<T>(bfx, nc, c_next, 0, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
-                  Akk + (int64_t)kb * ld + kb, ld, g3m, w.split, st, 0, plan.trail);
-      if (pr) {
-        (void)hipEventRecord(pr->b, st);
-        pr->flops = 8.0 * (double)nc * (double)c_next * (double)kb;
+    const int nc = n - k - kb;
+    cx<T>* Akk = A + (int64_t)k * ld + k;
+    const cx<T>* iL = w.dinv256 + (size_t)(k / 256) * 2 * 65536;
+    // Inverses of the panel's diagonal blocks (64 x 64, then 256 x 256):
+    // inv(L11) now, it is needed for U12; inv(U11) only by the solves.
+    auto inverses = [&](int part, hipStream_t s_, int prio_) {
+      diag_inv_kernel<T><<<dim3((kb + 63) / 64, 1), 256, 0, s_>>>(A, ld, n, k / 64, w.dinv64, 0,
+                                                                  0, prio_, part);
+      blk_inv_kernel<T><<<dim3(1, 1, 4), 256, 0, s_>>>(A, ld, n, k / 256, w.dinv64, w.dinv256,
+                                                       0, 0, 0, prio_, part);
+    };
+    inverses(0, st, pk);
+    // U12 = inv(L11) A12 for the trailing columns [c0, c0 + cols) of the
+    // step, staged in ustage (row stride nc), then C -= L21 U12 over them
+    // and U12 copied back into A's rows k .. k + kb.
+    auto u12 = [&](int c0, int cols, hipStream_t s_, int prio_) {
+      gemm<T>(kb, cols, kb, iL, 256, false, Akk + kb + c0, ld, w.ustage + c0, nc, T(1), 0, g3m,
+              nullptr, s_, 0, prio_);
+    };
+    auto copy_back = [&](int c0, int cols, hipStream_t s_) {
+      const size_t cnt = (size_t)kb * cols;
+      copy2d_kernel<T><<<(int)min((size_t)4096, (cnt + 255) / 256), 256, 0, s_>>>(
+          kb, cols, w.ustage + c0, nc, Akk + kb + c0, ld);
+    };
+    if (la && nc > nb) {
+      // Look-ahead step.  Main stream: the next panel's columns only (swap,
+      // U12, update, copy back), so that the next panel starts as early as
+      // possible.  Side stream: the left interchanges, inv(U11), and the swap,
+      // U12 and update of every column beyond the next panel.
+      // (split-BF16: the side stream also reads the L21 planes made below, so
+      // it starts after them)
+      if (!bfx) HPR_CHECK(hipEventRecord(side->ready, st));
+      // the next panel's columns were last updated on the side stream, and
+      // ustage / the split planes were last read there
+      if (rest_pending) {
+        HPR_CHECK(hipStreamWaitEvent(st, side->rest, 0));
+        rest_pending = false;
       }
-      hipStream_t cs = st;
-      if (c_next < nc) {
+      launch_laswp<T>(A, ld, k + kb, k + 2 * kb, k, kb, ipiv, nullptr, st, pk);
+      u12(0, nb, st, pg);
+      if (bfx) {
+        split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, st, true, 0,
+                          nb);
         HPR_CHECK(hipEventRecord(side->ready, st));
-        HPR_CHECK(hipStreamWaitEvent(side->stream, side->ready, 0));
-        cs = side->stream;
-        ProfRec* pr2 = g_prof_on ? prof_slot() : nullptr;
-        if (pr2) (void)hipEventRecord(pr2->a, cs);
-        trailing<T>(bfx, nc, nc - c_next, c_next, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
-                    Akk + (int64_t)kb * ld + kb + c_next, ld, g3m, w.split, cs,
-                    trail_lds_extra(sizeof(T) == 4), plan.trail);
-        if (pr2) {
-          (void)hipEventRecord(pr2->b, cs);
-          pr2->flops = 8.0 * (double)nc * (double)(nc - c_next) * (double)kb;
-        }
       }
-      const size_t cnt = (size_t)kb * nc;
-      copy2d_kernel<T><<<(int)min((size_t)4096, (cnt + 255) / 256), 256, 0, cs>>>(
-          kb, nc, w.ustage, nc, Akk + kb, ld);
-      if (cs != st) {
-        HPR_CHECK(hipEventRecord(side->rest, cs));
-        rest_pending = true;
+      trailing<T>(bfx, nc, nb, 0, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
+                  Akk + (int64_t)kb * ld + kb, ld, g3m, w.split, st, 0, plan.trail, pg);
+      copy_back(0, nb, st);
+      hipStream_t cs = side->stream;
+      HPR_CHECK(hipStreamWaitEvent(cs, side->ready, 0));
+      if (k > 0) launch_laswp<T>(A, ld, 0, k, k, kb, ipiv, nullptr, cs);
+      launch_laswp<T>(A, ld, k + 2 * kb, n, k, kb, ipiv, nullptr, cs);
+      inverses(1, cs, 0);
+      u12(nb, nc - nb, cs, 0);
+      if (bfx)
+        split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, cs, false,
+                          nb, nc - nb);
+      ProfRec* pr2 = g_prof_on ? prof_slot() : nullptr;
+      if (pr2) (void)hipEventRecord(pr2->a, cs);
+      trailing<T>(bfx, nc, nc - nb, nb, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
+                  Akk + (int64_t)kb * ld + kb + nb, ld, g3m, w.split, cs,
+                  trail_lds_extra(sizeof(T) == 4), plan.trail);
+      if (pr2) {
+        (void)hipEventRecord(pr2->b, cs);
+        pr2->flops = 8.0 * (double)nc * (double)(nc - nb) * (double)kb;
+      }
+      copy_back(nb, nc - nb, cs);
+      HPR_CHECK(hipEventRecord(side->rest, cs));
+      rest_pending = true;
+    } else {
+      // no look-ahead (or the last steps): everything on the main stream
+      inverses(1, st, pk);
+      if (rest_pending) {
+        HPR_CHECK(hipStreamWaitEvent(st, side->rest, 0));
+        rest_pending = false;
+      }
+      if (k > 0) launch_laswp<T>(A, ld, 0, k, k, kb, ipiv, nullptr, st, pk);
+      if (nc > 0) {
+        launch_laswp<T>(A, ld, k + kb, n, k, kb, ipiv, nullptr, st, pk);
+        u12(0, nc, st, pg);
+        ProfRec* pr = g_prof_on ? prof_slot() : nullptr;
+        if (bfx) split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, st);
+        if (pr) (void)hipEventRecord(pr->a, st);
+        trailing<T>(bfx, nc, nc, 0, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
+                    Akk + (int64_t)kb * ld + kb, ld, g3m, w.split, st, 0, plan.trail);
+        if (pr) {
+          (void)hipEventRecord(pr->b, st);
+          pr->flops = 8.0 * (double)nc * (double)nc * (double)kb;
+        }
+        copy_back(0, nc, st);
       }
     }
   }
@@ -1089,7 +1154,7 @@ int solve_batched(const BatchArgs<TF>& b, const double2* A64, int64_t lda, int64
   template int getrs<T>(const cx<T>*, int64_t, int, const cx<T>*, cx<T>*, int64_t, int, cx<T>*, \
                         const int*, const LuPlan&, bool, hipStream_t);                         \
   template int gemm<T>(int, int, int, const cx<T>*, int64_t, bool, const cx<T>*, int64_t,      \
-                       cx<T>*, int64_t, T, int, bool, const int*, hipStream_t, size_t);        \
+                       cx<T>*, int64_t, T, int, bool, const int*, hipStream_t, size_t, int);   \
   template int refine_solve<T>(const cx<T>*, int64_t, int, const cx<T>*, const int*,          \
                                const double2*, int64_t, const double2*, double2*, int, int,    \
                                double, int, RefineState*, double2*, cx<T>*, cx<T>*, double*,   \

@@ -115,7 +115,7 @@ int diag_scale(int n, const double2* A, int64_t lda, double* s, hipStream_t st);
 template <class T>
 int gemm(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<T>* B,
          int64_t ldb, cx<T>* C, int64_t ldc, T alpha, int beta, bool gauss3m,
-         const int* skip, hipStream_t st, size_t lds_extra = 0);
+         const int* skip, hipStream_t st, size_t lds_extra = 0, int prio = 0);
 
 // Split-K partial sums of the FP64 refinement product R = beta R + alpha A X
 // (A column-major n x n, X and R row-major n x nrhs): `splits` slices of K

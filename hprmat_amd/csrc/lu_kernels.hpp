@@ -192,7 +192,9 @@ template <class T, int W>
 __global__ void __launch_bounds__(256)
 leaf_getf2_kernel(cx<T>* __restrict__ P, int64_t ld, int m, int w, int row_base,
                   int* __restrict__ ipiv, int* __restrict__ info, LeafXchg x, unsigned seq_base,
-                  int* __restrict__ err, int64_t sP = 0, int sIv = 0, int64_t sX = 0) {
+                  int* __restrict__ err, int64_t sP = 0, int sIv = 0, int64_t sX = 0,
+                  int prio = 0) {
+  raise_wave_priority(prio);
   // batch of independent systems: blockIdx.y selects the system (strides sP
   // elements of P, sIv entries of ipiv, sX exchange words; info per system)
   {
@@ -386,7 +388,8 @@ template <class T>
 __global__ void __launch_bounds__(256)
 laswp_kernel(cx<T>* __restrict__ A, int64_t ld, int c0, int c1, int k1, int nk,
              const int* __restrict__ ipiv, const int* __restrict__ skip, int64_t sA = 0,
-             int sIv = 0, int sSkip = 0) {
+             int sIv = 0, int sSkip = 0, int prio = 0) {
+  raise_wave_priority(prio);
   if (skip && skip[(int64_t)blockIdx.y * sSkip]) return;
   A += blockIdx.y * sA;
   ipiv += (int64_t)blockIdx.y * sIv;
@@ -452,7 +455,9 @@ laswp_kernel(cx<T>* __restrict__ A, int64_t ld, int c0, int c1, int k1, int nk,
 template <class T, bool UPPER>
 __global__ void __launch_bounds__(256)
 trsm_diag_kernel(const cx<T>* __restrict__ D, int64_t ldd, int nb, cx<T>* __restrict__ S,
-                 int64_t lds, int ncols, const int* __restrict__ skip, int64_t sA = 0) {
+                 int64_t lds, int ncols, const int* __restrict__ skip, int64_t sA = 0,
+                 int prio = 0) {
+  raise_wave_priority(prio);
   if (skip && *skip) return;
   D += blockIdx.y * sA;
   S += blockIdx.y * sA;
@@ -567,12 +572,15 @@ permute_cast_kernel(int n, int nrhs, const int* __restrict__ perm, const cx<Tin>
 template <class T>
 __global__ void __launch_bounds__(256)
 diag_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int blk0,
-                cx<T>* __restrict__ dinv, int64_t sA = 0, int64_t sD = 0) {
+                cx<T>* __restrict__ dinv, int64_t sA = 0, int64_t sD = 0, int prio = 0,
+                int part0 = 0) {
+  raise_wave_priority(prio);
   LU += blockIdx.z * sA;   // system of a batch
   dinv += blockIdx.z * sD;
   __shared__ cx<T> a[64][65];
   __shared__ cx<T> tmp[64];
-  const int bi = blk0 + blockIdx.x, upper = blockIdx.y, tid = threadIdx.x;
+  // part0 + blockIdx.y: 0 the unit-lower factor, 1 the upper factor
+  const int bi = blk0 + blockIdx.x, upper = part0 + blockIdx.y, tid = threadIdx.x;
   const int r0 = bi * 64, bs = min(64, n - r0);
   for (int e = tid; e < 64 * 64; e += 256) {
     int i = e / 64, k = e % 64;
@@ -690,9 +698,11 @@ template <class T>
 __global__ void __launch_bounds__(256)
 blk_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int bb0,
                const cx<T>* __restrict__ dinv64, cx<T>* __restrict__ dinv256, int64_t sA = 0,
-               int64_t sD64 = 0, int64_t sD256 = 0) {
+               int64_t sD64 = 0, int64_t sD256 = 0, int prio = 0, int part0 = 0) {
+  raise_wave_priority(prio);
   typedef typename acc_t<T>::type acc;
-  const int bb = bb0 + blockIdx.x, upper = blockIdx.y, jb = blockIdx.z & 3;
+  // part0 + blockIdx.y: 0 the unit-lower factor, 1 the upper factor
+  const int bb = bb0 + blockIdx.x, upper = part0 + blockIdx.y, jb = blockIdx.z & 3;
   {
     const int b = blockIdx.z >> 2;   // system of a batch (z = 4 b + jb)
     LU += b * sA;
