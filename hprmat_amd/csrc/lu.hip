@@ -845,12 +845,16 @@ int residual_check(int n, int nrhs, const double2* A64, int64_t lda, const doubl
 // a few thousand unknowns is a chain of 10^2 - 10^3 small dependent kernels
 // whose dispatch latency, not their work, dominates when systems run one at
 // a time.  No look-ahead (the batch fills the GPU).  The cooperative panel
-// leaf is launched for at most BATCH_LEAF_WG workgroups at a time; the
-// workgroups of one system are consecutive in dispatch order, so the
-// earliest unfinished system always has all of its workgroups resident.
+// leaf of every system (G = ceil(m / 256) workgroups each) is launched in
+// one grid of up to BATCH_LEAF_WG workgroups: the workgroups of one system
+// are consecutive in dispatch order and a system waits only on its own
+// workgroups, so the earliest unfinished system always has all of its
+// workgroups resident (workgroups of later systems queue behind it), as long
+// as G leaf workgroups of each concurrently running batch fit on the GPU at
+// once (G <= 25 for n <= 6400; about 512 FP64 leaf workgroups are resident).
 // ===========================================================================
 namespace {
-constexpr int BATCH_LEAF_WG = 480;
+constexpr int BATCH_LEAF_WG = 4096;
 
 inline int64_t up32(int64_t x) { return (x + 31) & ~(int64_t)31; }
 inline size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
