@@ -212,7 +212,10 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
     roof = {'bound': 'mfma', 'achieved': g['tflops'], 'peak': PEAK[head],
             'unit': 'TFLOP/s', 'frac': g['tflops'] / PEAK[head], 'traffic': None,
             'kernel': '%s (trailing update A22 -= L21 U12, %s)' % (
-                'zgemm_bf16x3_kernel' if head.startswith('bf16x3') else 'zgemm_kernel', head),
+                'zgemm_bf16x3_kernel' if head.startswith('bf16x3') else
+                'trailing_update_kernel<%s, 64, 64, 16, 32, 32, %s>' % (
+                    'double' if head == 'fp64' else 'float',
+                    'true' if head.startswith('fp32_3m') else 'false'), head),
             'flops_per_launch': g['flops_per_launch'], 'ms_per_launch': g['ms_per_launch']}
     tr = traffic_record('fp64' if head == 'fp64' else 'fp32')
     if tr is not None and not head.startswith('bf16x3'):

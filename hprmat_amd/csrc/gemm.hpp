@@ -185,6 +185,18 @@ zgemm_kernel(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
                                                    C + b * sC, ldc, alpha, beta, blockIdx.x);
 }
 
+// The LU's trailing update A22 -= L21 U12 (the launches timed for the
+// roofline record): zgemm_kernel's tile under its own kernel name, so that
+// kernel traces list these launches apart from the library's other GEMMs.
+template <class T, int BM, int BN, int BK, int WM, int WN, bool GAUSS3M>
+__global__ void __launch_bounds__(256)
+trailing_update_kernel(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
+                       const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
+                       int64_t ldc, T alpha) {
+  zgemm_tile<T, BM, BN, BK, WM, WN, false, GAUSS3M>(M, N, K, A, lda, B, ldb, C, ldc, alpha, 1,
+                                                   blockIdx.x);
+}
+
 // Two independent GEMM problems (row-major A) in one launch: workgroups
 // [0, tiles1) take problem 1, the rest problem 2.  Used where two products
 // of one block step read the same operand and write disjoint outputs.

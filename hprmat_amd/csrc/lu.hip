@@ -503,7 +503,7 @@ template <class T>
 void trailing(bool bfx, int nc, int ncols, int c0, int kb, const cx<T>* L, int64_t ld,
               const cx<T>* U, int64_t ldu, cx<T>* C, int64_t ldc, bool g3m,
               const unsigned short* split, hipStream_t st, size_t lds_extra, int variant,
-              int prio = 0) {
+              int prio = 0, bool timed = false) {
   if (nc <= 0 || ncols <= 0) return;
   if constexpr (std::is_same<T, float>::value) {
     if (!g3m && !bfx && variant >= 3) {
@@ -537,6 +537,18 @@ void trailing(bool bfx, int nc, int ncols, int c0, int kb, const cx<T>* L, int64
           -1.f, 1);
       return;
     }
+  }
+  if (timed) {
+    // the same tile as gemm() under the trailing_update_kernel name
+    constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32;
+    const int tiles = ((nc + BM - 1) / BM) * ((ncols + BN - 1) / BN);
+    if (g3m)
+      trailing_update_kernel<T, BM, BN, BK, WM, WN, true><<<tiles, 256, lds_extra, st>>>(
+          nc, ncols, kb, L, ld, U + c0, ldu, C, ldc, T(-1));
+    else
+      trailing_update_kernel<T, BM, BN, BK, WM, WN, false><<<tiles, 256, lds_extra, st>>>(
+          nc, ncols, kb, L, ld, U + c0, ldu, C, ldc, T(-1));
+    return;
   }
   gemm<T>(nc, ncols, kb, L, ld, false, U + c0, ldu, C, ldc, T(-1), 1, g3m, nullptr, st, lds_extra,
           prio);
@@ -682,7 +694,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       if (pr2) (void)hipEventRecord(pr2->a, cs);
       trailing<T>(bfx, nc, nc - nb, nb, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
                   Akk + (int64_t)kb * ld + kb + nb, ld, g3m, w.split, cs,
-                  trail_lds_extra(sizeof(T) == 4), plan.trail);
+                  trail_lds_extra(sizeof(T) == 4), plan.trail, 0, true);
       if (pr2) {
         (void)hipEventRecord(pr2->b, cs);
         pr2->flops = 8.0 * (double)nc * (double)(nc - nb) * (double)kb;
@@ -705,7 +717,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
         if (bfx) split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, st);
         if (pr) (void)hipEventRecord(pr->a, st);
         trailing<T>(bfx, nc, nc, 0, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
-                    Akk + (int64_t)kb * ld + kb, ld, g3m, w.split, st, 0, plan.trail);
+                    Akk + (int64_t)kb * ld + kb, ld, g3m, w.split, st, 0, plan.trail, 0, true);
         if (pr) {
           (void)hipEventRecord(pr->b, st);
           pr->flops = 8.0 * (double)nc * (double)nc * (double)kb;

@@ -131,12 +131,14 @@ def rhs_colmajor(dmesh, nch):
 # solver_type -> (method, max_refine, tol, criterion, equilibrate, refinement).
 # Types 1/2 keep the semantics of the reference's CPU types (plain FP64; FP32
 # + FP64 refinement with ||r||_F/||b||_F < 1e-10, two sweeps) but run on the
-# GPU; type 3 (Woodbury, CPU-only) is not provided.  'plain' and 'mixed' are
+# GPU; type 1 adds at most two FP64 sweeps to ||r||_F/||b||_F <= 1e-12 so that
+# it matches LAPACK zgesv's accuracy (the substitution multiplies by explicit
+# diagonal-block inverses); type 3 (Woodbury, CPU-only) is not provided.  'plain' and 'mixed' are
 # this library's refined methods: FP64 factors with classic refinement, or
 # FP32 factors with flexible-GMRES refinement, both of the equilibrated
 # matrix, until ||r||_F/||b||_F <= 1e-12.
 SOLVER_TYPES = {
-    1: ('fp64', 0, 0.0, 'rel_fro', False, 'ir'),
+    1: ('fp64', 2, 1.0e-12, 'rel_fro', False, 'ir'),
     2: ('fp32', 2, 1.0e-10, 'rel_fro', False, 'ir'),
     4: ('fp32', None, 1.0e-10, 'abs_max', False, 'ir'),
     5: ('bf16x3', 5, 1.0e-12, 'abs_max', False, 'ir'),

@@ -478,9 +478,10 @@ class _BatchLane:
 
 def serial_fp64(waves, pairs, tol=1e-12):
     """Reference loop: one plain FP64 solve per system, one stream, in order
-    (factor + solve + one FP64 residual check, then R and S)."""
+    (factor + solve + FP64 refinement to the residual gate, at most two
+    sweeps, as the plain method does; then R and S)."""
     lane = Lane(max(w.n for w, _ in pairs), max(w.nch for w, _ in pairs))
-    plan = {'method': 'fp64', 'refinement': 'ir', 'max_steps': 0, 'trail': 0, 'lookahead': 1}
+    plan = {'method': 'fp64', 'refinement': 'ir', 'max_steps': 2, 'trail': 0, 'lookahead': 1}
     for w, k in pairs:
         solve_system(lane, w, k, plan, tol)
 

@@ -61,7 +61,12 @@ int gpu_solve_tf32_(double *A, double *B, int *n, int *nrhs, int *max_refine,
 /* src/cusolver_interface.cu:1600-1639: memory-fit warning, then mixed. */
 int gpu_solve_auto_(double *A, double *B, int *n, int *nrhs, int *max_refine,
                     double *tol, int *info);
-/* src/cusolver_interface.cu:737-795: plain FP64 LU + solve, no refinement. */
+/* src/cusolver_interface.cu:737-795: plain FP64 LU + solve.  The reference
+   applies no refinement (LAPACK-quality substitution); here the substitution
+   multiplies by explicit diagonal-block inverses, so at most two FP64
+   refinement sweeps follow until ||B - A X||_F / ||B||_F <= 1e-12, which
+   gives LAPACK zgesv's accuracy (or better) on kappa ~ 1e9 systems.  Not
+   reaching the gate is not an error. */
 int gpu_solve_double_(double *A, double *B, int *n, int *nrhs, int *info);
 /* src/cusolver_interface.cu:797-804 */
 int gpu_is_available_(void);
@@ -271,7 +276,9 @@ int hprmat_hip_zgemm(int precision, int M, int N, int K, const void *A, int64_t 
    hprmat_hip_batch_work_bytes(method, n, nrhs, count) bytes (256-byte
    aligned); nothing is allocated and nothing synchronises with the host,
    so the call can be captured into a HIP graph.  dA must not be modified
-   until the call completes on `stream`. */
+   until the call completes on `stream`.  Batches running concurrently on
+   several streams need streams * ceil(n / 256) <= 512: the panel leaf of
+   one system of each batch keeps ceil(n / 256) workgroups resident. */
 size_t hprmat_hip_batch_work_bytes(int method, int n, int nrhs, int count);
 int hprmat_hip_batch_solve(int method_flags, int n, int nrhs, int count, const void *dA,
                            int64_t lda, int64_t stride_a, void *dB, int64_t ldb,

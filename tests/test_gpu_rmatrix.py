@@ -110,3 +110,35 @@ def test_golden_ex2_reid_on_gpu():
         d2 = math.atan2(s[1, 1].imag, s[1, 1].real) / 2
         for got, want in [(d1, g['delta1_rad']), (d2, g['delta2_rad']), (abs(s[0, 1]), g['eta12'])]:
             assert abs(got - want) <= 5e-5 * abs(want) + 1e-9, (g, got, want)
+
+
+def test_plain_fp64_matches_zgesv_on_ill_conditioned_scan_system():
+    # the N = 6400 partial wave (64 channels x 100 points, thresholds in
+    # [0, 3], seed 7919) of the energy scan at E = 2.288 (kappa ~ 1e9): the
+    # reference's plain FP64 path (solver_type 1, gpu_solve_double_) must give
+    # LAPACK zgesv's S to 1e-10; the unrefined substitution through explicit
+    # diagonal-block inverses alone missed that (5e-10)
+    from hprmat_amd import synth as hsynth
+    e = float(np.linspace(0.5, 10.0, 256)[48])
+    sysm = hsynth.System(64, 100, 15.0, 3.0, e, 7919)
+    ref = ormat.rmatrix(sysm.lval, sysm.qk, 15.0, 100, sysm.cpot)
+    print('plain fp64 N=6400 E=%.4f nopen=%d' % (e, ref['nopen']), flush=True)
+    out = rmatrix.rmatrix(sysm.lval, sysm.qk, 15.0, 100, sysm.cpot, solver_type=1)
+    assert out['nopen'] == ref['nopen'] == 50
+    assert out['stats']['res_rel'] <= 1e-12
+    assert np.abs(out['S'].cpu().numpy() - ref['S']).max() <= 1e-10
+    # the reference ABI entry point on host arrays: same parity
+    import ctypes
+    from hprmat_amd import _lib
+    lib = _lib.load()
+    c = np.asfortranarray(ref['C'])
+    b = np.asfortranarray(ref['B'].copy())
+    info = ctypes.c_int(7)
+    rc = lib.gpu_solve_double_(c.ctypes.data_as(ctypes.c_void_p), b.ctypes.data_as(ctypes.c_void_p),
+                               ctypes.byref(ctypes.c_int(6400)), ctypes.byref(ctypes.c_int(64)),
+                               ctypes.byref(info))
+    assert rc == 0 and info.value == 0
+    assert np.linalg.norm(ref['B'] - ref['C'] @ b) / np.linalg.norm(ref['B']) <= 1e-12
+    r = ormat.extract_r(b, ref['mesh']['q2'][0], 64)
+    s = ormat.r_to_s(r, 15.0, sysm.qk, ref['npo'], ref['co'], ref['cop'])
+    assert np.abs(s - ref['S']).max() <= 1e-10
