@@ -18,7 +18,10 @@ Methods timed back to back (name: factorization, refinement):
             Arnoldi, the FP32 factors as preconditioner, <= 24 products);
   bf16x3_gmres  the bf16x3 factorization with flexible-GMRES refinement;
   fp32_3m   complex-FP32 LU with Gauss's three-product complex GEMM;
-  fp32_3m_gmres  the fp32_3m factorization with flexible-GMRES refinement.
+  fp32_3m_gmres  the fp32_3m factorization with flexible-GMRES refinement;
+  bf16x6    FP32 LU whose trailing update uses three-piece split-BF16 (six
+            products per real product, FP32 accuracy), classic FP64 IR;
+  bf16x6_gmres  the bf16x6 factorization with flexible-GMRES refinement.
 Each method's solution is compared with the fp64 method's (max relative
 error of X and max |dS| of the S-matrix formed from it on the device).
 ``value`` is the solves/s of the fastest non-FP64 method that reaches the
@@ -35,14 +38,21 @@ import sys
 import time
 
 METRIC = "solves/s & TFLOP/s, N=25600 complex LU+solve: mixed vs FP64 at resid≤1e-12"
+# Ceiling of the trailing update in algorithmic complex TFLOP/s (8 flops per
+# complex multiply-add) on MI355X's dense matrix peaks: FP64 78.6, FP32 157.3;
+# the split-BF16 kernels execute 3 (bf16x3) or 6 (bf16x6) BF16 products per
+# real product, so their ceiling is the BF16 peak 2516.6 divided by that.
+BF16_PEAK = 2516.6
 PEAK = {'fp64': 78.6, 'fp32': 157.3, 'fp32_ir5': 157.3, 'fp32_gmres': 157.3, 'fp32_3m': 157.3,
         'fp32_3m_gmres': 157.3,
-        'bf16x3': 2500.0, 'bf16x3_gmres': 2500.0}   # dense MFMA TFLOP/s, MI355X (bf16x3: BF16 matrix peak)
+        'bf16x3': BF16_PEAK / 3, 'bf16x3_gmres': BF16_PEAK / 3,
+        'bf16x6': BF16_PEAK / 6, 'bf16x6_gmres': BF16_PEAK / 6}
 # name -> (factorization method, max refinement sweeps)
 SPECS = {'fp64': ('fp64', 10, 'ir'), 'fp32': ('fp32', 30, 'ir'), 'fp32_ir5': ('fp32', 5, 'ir'),
          'fp32_gmres': ('fp32', 24, 'fgmres'), 'bf16x3': ('bf16x3', 30, 'ir'),
          'bf16x3_gmres': ('bf16x3', 32, 'fgmres'), 'fp32_3m': ('fp32_3m', 30, 'ir'),
-         'fp32_3m_gmres': ('fp32_3m', 24, 'fgmres')}
+         'fp32_3m_gmres': ('fp32_3m', 24, 'fgmres'),
+         'bf16x6': ('bf16x6', 30, 'ir'), 'bf16x6_gmres': ('bf16x6', 24, 'fgmres')}
 
 
 def flops_solve(n, nrhs):
@@ -101,7 +111,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--nch', type=int, default=128)
     ap.add_argument('--nr', type=int, default=200)
-    ap.add_argument('--methods', default='fp64,fp32_gmres,fp32_3m_gmres,fp32,fp32_ir5,bf16x3,bf16x3_gmres')
+    ap.add_argument('--methods', default='fp64,fp32_gmres,bf16x6_gmres,fp32_3m_gmres,fp32,'
+                                         'fp32_ir5,bf16x3,bf16x3_gmres')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--copy-probe', type=int, default=0,
                     help='copy this many bytes device-to-device once before the run (a '
@@ -212,13 +223,17 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
     roof = {'bound': 'mfma', 'achieved': g['tflops'], 'peak': PEAK[head],
             'unit': 'TFLOP/s', 'frac': g['tflops'] / PEAK[head], 'traffic': None,
             'kernel': '%s (trailing update A22 -= L21 U12, %s)' % (
-                'zgemm_bf16x3_kernel' if head.startswith('bf16x3') else
+                'zgemm_bf16x3_kernel<64, 64, 32, 32>' if head.startswith('bf16x3') else
+                'zgemm_bf16x6_kernel<128, 64, 64, 32>' if head.startswith('bf16x6') else
                 'trailing_update_kernel<%s, 64, 64, 16, 32, 32, %s>' % (
                     'double' if head == 'fp64' else 'float',
                     'true' if head.startswith('fp32_3m') else 'false'), head),
             'flops_per_launch': g['flops_per_launch'], 'ms_per_launch': g['ms_per_launch']}
     tr = traffic_record('fp64' if head == 'fp64' else 'fp32')
-    if tr is not None and not head.startswith('bf16x3'):
+    if head.startswith('bf16'):
+        roof['peak_note'] = 'BF16 dense matrix peak %.1f TFLOP/s / %d BF16 products per real product' % (
+            BF16_PEAK, 3 if head.startswith('bf16x3') else 6)
+    if tr is not None and not head.startswith('bf16'):
         roof['traffic'] = tr['hbm_bytes']
         roof['traffic_launch'] = {
             'M': tr['M'], 'N': tr['N'], 'K': tr['K'], 'hbm_read_bytes': tr['hbm_read_bytes'],
@@ -229,7 +244,9 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
         'metric': METRIC, 'value': hm['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': hm['ms'],
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'c64 LU + c128 refinement' if head != 'fp64' else 'c128',
+        'dtype': ('c128' if head == 'fp64' else
+                  'c64 LU (split-bf16 update) + c128 refinement' if head.startswith('bf16')
+                  else 'c64 LU + c128 refinement'),
         'data': 'synthetic',
         'config': {'workload': 'single Lagrange-mesh solve N=%d (nch=%d x nr=%d), nrhs=%d, '
                                'gate ||r||_F/||b||_F<=1e-12' % (n, args.nch, args.nr, nrhs),

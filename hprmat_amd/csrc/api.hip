@@ -25,7 +25,7 @@ struct hprmat_hip_lu {
   size_t dinv256_cap = 0;
   void* ustage = nullptr;     // U12 staging block (256 x n)
   size_t ustage_cap = 0;
-  void* split = nullptr;      // split-BF16 operand planes (HPRMAT_SPLIT_BF16)
+  void* split = nullptr;      // split-BF16 operand planes (HPRMAT_SPLIT_BF16, _BF16X6)
   size_t split_cap = 0;
   void* gm = nullptr;         // FGMRES workspace (HPRMAT_FGMRES solves)
   size_t gm_cap = 0;
@@ -69,11 +69,10 @@ size_t factor_elem_bytes(int method) { return method == HPRMAT_PLAIN_FP64 ? 16 :
 
 int64_t padded_ld(int n) { return (((int64_t)n + 15) / 16) * 16 + 16; }
 
-// The look-ahead stream runs the bulk of every trailing update.  It is
-// restricted (CU mask) to all but a reserved set of CUs (every 8th, 32 of 256
-// by default; HPRMAT_HIP_RESERVE_CU overrides, 0 disables), so the
-// latency-bound panel kernels on the main stream always find free CUs
-// instead of waiting for GEMM workgroups to drain.  Default priority.
+// The look-ahead stream runs the bulk of every trailing update (with the
+// interchanges, U12 and inv(U11) that only the bulk needs) beside the panel
+// chain on the caller's stream.  A plain non-blocking stream at the default
+// priority, created on the first factorization of a handle.
 int ensure_side(hprmat_hip_lu* h) {
   if (h->side.stream) return 0;
   hipError_t e;
@@ -109,7 +108,7 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
   if (n < 0 || lda < n) return -1;
   const int method = method_flags & 0xff;
   const int equil = (method_flags & HPRMAT_EQUILIBRATE) ? 1 : 0;
-  if (method < HPRMAT_PLAIN_FP64 || method > HPRMAT_SPLIT_BF16) {
+  if (method < HPRMAT_PLAIN_FP64 || method > HPRMAT_SPLIT_BF16X6) {
     fprintf(stderr, "hprmat_hip: factorization method %d is not available\n", method);
     return -1;
   }
@@ -133,7 +132,7 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
   if (grow(&h->ustage, &h->ustage_cap, hpr::ustage_elems(n) * eb)) return -2;
   if (grow((void**)&h->perm, &h->perm_cap, (size_t)(n + 1) * sizeof(int))) return -2;
   if (grow((void**)&h->scale, &h->scale_cap, (size_t)(n + 1) * sizeof(double))) return -2;
-  if (method == HPRMAT_SPLIT_BF16 &&
+  if ((method == HPRMAT_SPLIT_BF16 || method == HPRMAT_SPLIT_BF16X6) &&
       grow(&h->split, &h->split_cap, hpr::split_elems(n) * sizeof(unsigned short)))
     return -2;
   h->equil = equil;
@@ -176,6 +175,7 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
     w.split = (unsigned short*)h->split;
     const int gmode = method == HPRMAT_MIXED_FP32_3M ? hpr::GEMM_3M
                       : method == HPRMAT_SPLIT_BF16   ? hpr::GEMM_BF16X3
+                      : method == HPRMAT_SPLIT_BF16X6 ? hpr::GEMM_BF16X6
                                                       : hpr::GEMM_4M;
     if (!rc)
       rc = hpr::getrf<float>(F, ld, n, h->ipiv, h->dinfo, w, h->sc, h->plan, gmode, st, side);

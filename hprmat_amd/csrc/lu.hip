@@ -459,16 +459,25 @@ int resid_gemm(int n, int nrhs, const double2* A64, int64_t lda, const double2* 
 template <class T>
 void split_operands(int nc, int kb, const cx<T>* L, int64_t ld, const cx<T>* U, int64_t ldu,
                     unsigned short* split, hipStream_t st, bool split_l = true, int c0 = 0,
-                    int cols = -1) {
+                    int cols = -1, int pieces = 2) {
   if constexpr (std::is_same<T, float>::value) {
     const size_t plane = (size_t)nc * 256;
+    const size_t uoff = (size_t)2 * pieces * plane;   // U12^T planes follow L21's
     if (cols < 0) cols = nc - c0;
-    if (split_l)
-      split_bf16_kernel<false><<<dim3(256 / 32, (nc + 31) / 32), 256, 0, st>>>(
-          nc, kb, L, ld, split, plane, 256);
-    if (cols > 0)
-      split_bf16_kernel<true><<<dim3(256 / 32, (cols + 31) / 32), 256, 0, st>>>(
-          cols, kb, U + c0, ldu, split + 4 * plane + (size_t)c0 * 256, plane, 256);
+    const dim3 gl(256 / 32, (nc + 31) / 32), gu(256 / 32, (cols + 31) / 32);
+    if (pieces == 3) {
+      if (split_l)
+        split_bf16_kernel<false, 3><<<gl, 256, 0, st>>>(nc, kb, L, ld, split, plane, 256);
+      if (cols > 0)
+        split_bf16_kernel<true, 3><<<gu, 256, 0, st>>>(cols, kb, U + c0, ldu,
+                                                       split + uoff + (size_t)c0 * 256, plane, 256);
+    } else {
+      if (split_l)
+        split_bf16_kernel<false, 2><<<gl, 256, 0, st>>>(nc, kb, L, ld, split, plane, 256);
+      if (cols > 0)
+        split_bf16_kernel<true, 2><<<gu, 256, 0, st>>>(cols, kb, U + c0, ldu,
+                                                       split + uoff + (size_t)c0 * 256, plane, 256);
+    }
   }
 }
 
@@ -503,7 +512,7 @@ template <class T>
 void trailing(bool bfx, int nc, int ncols, int c0, int kb, const cx<T>* L, int64_t ld,
               const cx<T>* U, int64_t ldu, cx<T>* C, int64_t ldc, bool g3m,
               const unsigned short* split, hipStream_t st, size_t lds_extra, int variant,
-              int prio = 0, bool timed = false) {
+              int prio = 0, bool timed = false, int pieces = 2) {
   if (nc <= 0 || ncols <= 0) return;
   if constexpr (std::is_same<T, float>::value) {
     if (!g3m && !bfx && variant >= 3) {
@@ -526,6 +535,17 @@ void trailing(bool bfx, int nc, int ncols, int c0, int kb, const cx<T>* L, int64
     return;
   }
   if constexpr (std::is_same<T, float>::value) {
+    if (bfx && pieces == 3) {
+      constexpr int BM = 128, BN = 64;
+      const size_t plane = (size_t)nc * 256, lds = zgemm_bf16x6_lds_bytes<BM, BN>();
+      static bool attr = false;
+      if (!attr) { allow_big_lds(zgemm_bf16x6_kernel<BM, BN, 64, 32>); attr = true; }
+      const int tiles = ((nc + BM - 1) / BM) * ((ncols + BN - 1) / BN);
+      zgemm_bf16x6_kernel<BM, BN, 64, 32><<<tiles, 256, lds, st>>>(
+          nc, ncols, 256, split, plane, split + 6 * plane + (size_t)c0 * 256, plane, C, ldc,
+          -1.f);
+      return;
+    }
     if (bfx) {
       constexpr int BM = 64, BN = 64;
       const size_t plane = (size_t)nc * 256, lds = zgemm_bf16x3_lds_bytes<BM, BN>();
@@ -602,7 +622,8 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
   if (n <= 0) return 0;
   const int nb = plan.nb;
   const bool g3m = gmode == GEMM_3M;
-  const bool bfx = gmode == GEMM_BF16X3 && std::is_same<T, float>::value;
+  const bool bfx = (gmode == GEMM_BF16X3 || gmode == GEMM_BF16X6) && std::is_same<T, float>::value;
+  const int pieces = gmode == GEMM_BF16X6 ? 3 : 2;   // BF16 pieces per FP32 operand
   if (bfx && !w.split) return -1;
   if (nb != 256) return -1;
   const bool la = side && side->stream && n > 2 * nb;  // look-ahead
@@ -675,11 +696,12 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       u12(0, nb, st, pg);
       if (bfx) {
         split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, st, true, 0,
-                          nb);
+                          nb, pieces);
         HPR_CHECK(hipEventRecord(side->ready, st));
       }
       trailing<T>(bfx, nc, nb, 0, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
-                  Akk + (int64_t)kb * ld + kb, ld, g3m, w.split, st, 0, plan.trail, pg);
+                  Akk + (int64_t)kb * ld + kb, ld, g3m, w.split, st, 0, plan.trail, pg, false,
+                  pieces);
       copy_back(0, nb, st);
       hipStream_t cs = side->stream;
       HPR_CHECK(hipStreamWaitEvent(cs, side->ready, 0));
@@ -689,12 +711,12 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       u12(nb, nc - nb, cs, 0);
       if (bfx)
         split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, cs, false,
-                          nb, nc - nb);
+                          nb, nc - nb, pieces);
       ProfRec* pr2 = g_prof_on ? prof_slot() : nullptr;
       if (pr2) (void)hipEventRecord(pr2->a, cs);
       trailing<T>(bfx, nc, nc - nb, nb, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
                   Akk + (int64_t)kb * ld + kb + nb, ld, g3m, w.split, cs,
-                  trail_lds_extra(sizeof(T) == 4), plan.trail, 0, true);
+                  trail_lds_extra(sizeof(T) == 4), plan.trail, 0, true, pieces);
       if (pr2) {
         (void)hipEventRecord(pr2->b, cs);
         pr2->flops = 8.0 * (double)nc * (double)(nc - nb) * (double)kb;
@@ -714,10 +736,13 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
         launch_laswp<T>(A, ld, k + kb, n, k, kb, ipiv, nullptr, st, pk);
         u12(0, nc, st, pg);
         ProfRec* pr = g_prof_on ? prof_slot() : nullptr;
-        if (bfx) split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, st);
+        if (bfx)
+          split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, st, true,
+                            0, -1, pieces);
         if (pr) (void)hipEventRecord(pr->a, st);
         trailing<T>(bfx, nc, nc, 0, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
-                    Akk + (int64_t)kb * ld + kb, ld, g3m, w.split, st, 0, plan.trail, 0, true);
+                    Akk + (int64_t)kb * ld + kb, ld, g3m, w.split, st, 0, plan.trail, 0, true,
+                    pieces);
         if (pr) {
           (void)hipEventRecord(pr->b, st);
           pr->flops = 8.0 * (double)nc * (double)nc * (double)kb;

@@ -53,17 +53,18 @@ inline size_t ustage_elems(int n) { return (size_t)256 * (n > 0 ? n : 1); }
 
 // Split-BF16 operand planes of one trailing update: L21 and U12^T, four
 // BF16 planes each (n rows of 256 elements).
-inline size_t split_elems(int n) { return (size_t)2 * 4 * 256 * (n > 0 ? n : 1); }
+// (sized for the three-piece split: six planes per operand)
+inline size_t split_elems(int n) { return (size_t)2 * 6 * 256 * (n > 0 ? n : 1); }
 
 // Arithmetic of the FP32 trailing update A22 -= L21 U12.
-enum GemmMode { GEMM_4M = 0, GEMM_3M = 1, GEMM_BF16X3 = 2 };
+enum GemmMode { GEMM_4M = 0, GEMM_3M = 1, GEMM_BF16X3 = 2, GEMM_BF16X6 = 3 };
 
 template <class T> struct FactorWork {
   cx<T>* dinv64 = nullptr;
   cx<T>* dinv256 = nullptr;
   cx<T>* ustage = nullptr;
   int* perm = nullptr;
-  unsigned short* split = nullptr;   // GEMM_BF16X3 only
+  unsigned short* split = nullptr;   // GEMM_BF16X3 / GEMM_BF16X6 only
 };
 
 // Second stream of the look-ahead: the trailing update of all but the next

@@ -25,7 +25,7 @@ def gpu_solve(a, b, method, max_refine=0, tol=1e-13, equilibrate=False):
 
 
 @pytest.mark.parametrize('n', [1, 7, 32, 33, 64, 255, 256, 257, 500, 1100])
-@pytest.mark.parametrize('method', ['fp64', 'fp32', 'fp32_3m', 'bf16x3'])
+@pytest.mark.parametrize('method', ['fp64', 'fp32', 'fp32_3m', 'bf16x3', 'bf16x6'])
 def test_solve_matches_lapack(n, method):
     a, b = rand_system(n, 5, n)
     x_ref, info = osolve.zgesv(a, b)
@@ -63,6 +63,27 @@ def test_split_bf16_trailing_update_precision(n):
         assert st['info'] == 0
         errs[method] = np.abs(x - x_ref).max() / np.abs(x_ref).max()
     assert 2 * errs['fp32'] < errs['bf16x3'] < 200 * errs['fp32'], errs
+
+
+@pytest.mark.parametrize('n', [700, 1500, 3000])
+def test_split_bf16x6_trailing_update_precision(n):
+    # Three BF16 pieces (hi, mid, lo) carry 24 significant bits and the six
+    # kept piece products drop only terms below 2^-24 relative, so the
+    # unrefined bf16x6 factorization is as accurate as the FP32 one (within a
+    # small factor either way) and far more accurate than bf16x3; its result
+    # differs from FP32's in the last bits, which shows its kernel ran.
+    a, b = rand_system(n, 4, 57 + n, dd=0.0)
+    x_ref, _ = osolve.zgesv(a, b)
+    errs, xs = {}, {}
+    for method in ('fp32', 'bf16x3', 'bf16x6'):
+        x, st = gpu_solve(a, b, method, max_refine=0)
+        assert st['info'] == 0
+        xs[method] = x
+        errs[method] = np.abs(x - x_ref).max() / np.abs(x_ref).max()
+    print('n %d errors %s' % (n, errs), flush=True)
+    assert errs['bf16x6'] < 4 * errs['fp32'], errs
+    assert errs['bf16x6'] * 3 < errs['bf16x3'], errs
+    assert not np.array_equal(xs['bf16x6'], xs['fp32'])
 
 
 @pytest.mark.parametrize('n', [16500, 20000])
