@@ -233,13 +233,19 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
     if head.startswith('bf16'):
         roof['peak_note'] = 'BF16 dense matrix peak %.1f TFLOP/s / %d BF16 products per real product' % (
             BF16_PEAK, 3 if head.startswith('bf16x3') else 6)
+    if head.startswith('fp32_3m'):
+        roof['peak_note'] = ('Gauss three-product complex GEMM: the matrix cores execute 6 of the 8 '
+                             'algorithmic flops of a complex multiply-add; executed rate = 0.75 x '
+                             'achieved = %.1f TFLOP/s (%.2f of the FP32 matrix peak)' % (
+                                 0.75 * g['tflops'], 0.75 * g['tflops'] / PEAK[head]))
     if tr is not None and not head.startswith('bf16'):
         roof['traffic'] = tr['hbm_bytes']
         roof['traffic_launch'] = {
             'M': tr['M'], 'N': tr['N'], 'K': tr['K'], 'hbm_read_bytes': tr['hbm_read_bytes'],
             'hbm_write_bytes': tr['hbm_write_bytes'], 'alg_bytes': tr['alg_bytes'],
             'source': 'profiles/trailing_gemm_counters_N25600.md (FETCH_SIZE, WRITE_SIZE, '
-                      'copy-calibrated; first trailing update)'}
+                      'copy-calibrated; first trailing update; measured on the four-product '
+                      'kernel, whose tiles read A, B and C exactly as the three-product one)'}
     out = {
         'metric': METRIC, 'value': hm['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': hm['ms'],

@@ -138,7 +138,7 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
   h->equil = equil;
   const double* sc = nullptr;
   if (equil) {
-    if (hpr::diag_scale(n, (const double2*)dA, lda, h->scale, st)) return -1;
+    if (hpr::equil_scale(n, (const double2*)dA, lda, h->scale, st)) return -1;
     sc = h->scale;
   }
   h->n = n;

@@ -126,16 +126,31 @@ __global__ void add_correction_kernel(int nrhs, size_t cnt, double2* __restrict_
   }
 }
 
-// s[i] = 1 / sqrt(|a_ii|) of a column-major matrix (1 where a_ii = 0).
-__global__ void diag_scale_kernel(int n, const double2* __restrict__ A, int64_t lda,
-                                  double* __restrict__ s, int64_t sA = 0, int sS = 0) {
+// Symmetric equilibration factors s_i = ||row i of A||_2^(-1/2) (1 for a zero
+// row) of a column-major FP64 matrix: the factored matrix is diag(s) A diag(s).
+// One workgroup per 64 rows; its four waves sum every fourth column (each
+// wave reads 64 consecutive rows of a column, 1 KB), the four partial sums are
+// added in a fixed order, so the factors are reproducible bit for bit.
+__global__ void __launch_bounds__(256)
+row_scale_kernel(int n, const double2* __restrict__ A, int64_t lda, double* __restrict__ s,
+                 int64_t sA = 0, int sS = 0) {
   A += blockIdx.y * sA;   // system of a batch
   s += (int64_t)blockIdx.y * sS;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const double2 a = A[(int64_t)i * lda + i];
-  const double m = sqrt(a.x * a.x + a.y * a.y);
-  s[i] = m > 0.0 ? 1.0 / sqrt(m) : 1.0;
+  __shared__ double part[4][64];
+  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + r;
+  double acc = 0.0;
+  if (i < n)
+    for (int j = q; j < n; j += 4) {
+      const double2 a = A[(int64_t)j * lda + i];
+      acc += a.x * a.x + a.y * a.y;
+    }
+  part[q][r] = acc;
+  __syncthreads();
+  if (q == 0 && i < n) {
+    const double t = ((part[0][r] + part[1][r]) + part[2][r]) + part[3][r];
+    s[i] = t > 0.0 ? 1.0 / sqrt(sqrt(t)) : 1.0;
+  }
 }
 
 // Residual statistics, stage 1: per-block (max |Re|+|Im|, its column-major
@@ -809,9 +824,9 @@ int permute_cast(int n, int nrhs, const int* perm, const cx<Tin>* in, cx<Tout>* 
   return 0;
 }
 
-int diag_scale(int n, const double2* A, int64_t lda, double* s, hipStream_t st) {
+int equil_scale(int n, const double2* A, int64_t lda, double* s, hipStream_t st) {
   if (n <= 0) return 0;
-  diag_scale_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, A, lda, s);
+  row_scale_kernel<<<(n + 63) / 64, 256, 0, st>>>(n, A, lda, s);
   HPR_CHECK(hipGetLastError());
   return 0;
 }
@@ -1151,7 +1166,7 @@ int solve_batched(const BatchArgs<TF>& b, const double2* A64, int64_t lda, int64
   static_assert(sizeof(RefineState) % sizeof(int) == 0, "state stride in ints");
   // factors of the (equilibrated) matrices, row-major, factor precision
   if (b.scale)
-    diag_scale_kernel<<<dim3((n + 255) / 256, count), 256, 0, st>>>(n, A64, lda, b.scale, sA,
+    row_scale_kernel<<<dim3((n + 63) / 64, count), 256, 0, st>>>(n, A64, lda, b.scale, sA,
                                                                    b.sIv);
   transpose_cast_kernel<double, TF><<<dim3((n + 31) / 32, (n + 31) / 32, count), 256, 0, st>>>(
       n, n, A64, lda, b.F, b.ld, b.scale, b.scale, sA, b.sF, b.sIv);

@@ -108,9 +108,10 @@ int transpose_cast(int rows, int cols, const cx<Tin>* in, int64_t ldi, cx<Tout>*
                    int64_t ldo, hipStream_t st, const double* rs = nullptr,
                    const double* cs = nullptr);
 
-// Symmetric equilibration factors s[i] = 1 / sqrt(|a_ii|) (1 where a_ii = 0)
-// of a column-major FP64 matrix: the factored matrix is diag(s) A diag(s).
-int diag_scale(int n, const double2* A, int64_t lda, double* s, hipStream_t st);
+// Symmetric equilibration factors s[i] = ||row i of A||_2^(-1/2) (1 for a
+// zero row) of a column-major FP64 matrix: the factored matrix is
+// diag(s) A diag(s) (still complex symmetric when A is).
+int equil_scale(int n, const double2* A, int64_t lda, double* s, hipStream_t st);
 
 // Complex GEMM launcher: C = beta C + alpha A B (row-major; A column-major when a_col).
 template <class T>

@@ -79,9 +79,10 @@ class LU:
     def factor(self, a_cm, method='fp64', equilibrate=False):
         """Factor the matrix whose column-major storage is ``a_cm`` (n x n).
 
-        ``equilibrate`` factors diag(s) A diag(s) with s_i = |a_ii|^-1/2, which
-        keeps the diagonal blocks of U well conditioned on Lagrange-mesh
-        matrices (the refinement residual still uses A itself).
+        ``equilibrate`` factors diag(s) A diag(s) with s_i = ||row i||_2^-1/2
+        (symmetric, so A^T = A is kept), which lowers the effective condition
+        number of Lagrange-mesh matrices and so the refinement work of the
+        reduced-precision factors (the refinement residual still uses A).
         """
         _check_tensor(a_cm, 'a_cm')
         n = a_cm.shape[0]
