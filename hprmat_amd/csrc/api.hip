@@ -213,7 +213,11 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
   if (hpr::transpose_cast<double, double>(n, nrhs, (const double2*)dB, ldb, h->Brm, nrhs, st))
     return -1;
   int rc;
-  const bool g3m = h->method == HPRMAT_MIXED_FP32_3M;
+  // The substitutions always use the four-product kernels, whatever the
+  // factorization's arithmetic: they are launch-latency-bound chains (one
+  // two-problem launch per 256-row block forward), which the Gauss form would
+  // lengthen without saving time.
+  const bool g3m = false;
   const bool fgmres = (crit & HPRMAT_FGMRES) != 0;
   crit &= ~HPRMAT_FGMRES;
   if (fgmres) {

@@ -224,17 +224,16 @@ def _memory_system(w0_zero):
 def test_memory_scale_mixed_solve():
     # N = 51200 (nch = 256 x nr = 200, about 42 GB as complex128 in HBM),
     # 256 right-hand sides: FP32 factorization, FGMRES refinement.  The FP64
-    # attainable relative residual of this system is about 2e-12 (u ||A|| ||X||
-    # / ||B||), so the gate is 1e-11; S (256 x 256, FP64 on the device) is
+    # attainable relative residual of this system is 1.35e-12 (FP64 LU + FP64
+    # refinement stalls there too), so the gate is 2e-12 (8 products with the
+    # row-2-norm equilibration); S (256 x 256, FP64 on the device) is
     # symmetric and, with absorption, |S_ij| <= 1.
     s, dm, o, co, cop, a_cm, b = _memory_system(False)
-    # cycles of 24 steps: restarts after 16 lose the Krylov space this
-    # system needs to get below the gate
     lu = solver.LU().set_plan(gmres_cycle=24).factor(a_cm, 'fp32', True)
-    st = lu.solve_(b, 40, 1e-11, 'rel_fro', True, refinement='fgmres')
+    st = lu.solve_(b, 40, 2e-12, 'rel_fro', True, refinement='fgmres')
     print('memory-scale mixed: res_rel %.2e products %d hist %s'
           % (st['res_rel'], st['sweeps'], ' '.join('%.1e' % h for h in st['hist'])))
-    assert st['converged'] and st['res_rel'] <= 1e-11, st
+    assert st['converged'] and st['res_rel'] <= 2e-12, st
     _, sm = rmatrix.r_to_s(dm, b, s.nch, s.qk, o, co, cop)
     sm = sm.cpu().numpy()
     assert np.abs(sm - sm.T).max() <= 1e-10
