@@ -40,6 +40,10 @@ NATIVE_SYMBOLS = (
     'hprmat_hip_batch_work_bytes', 'hprmat_hip_batch_solve', 'hprmat_hip_assemble_batch',
     'hprmat_hip_r_to_s_batch',
 )
+PHYSICS_SYMBOLS = (
+    'hprmat_init', 'hprmat_set_solver', 'hprmat_get_solver', 'hprmat_solve',
+    'hprmat_solve_linear', 'hprmat_version', 'hprmat_solver_info',
+)
 
 
 class Stats(ctypes.Structure):
@@ -117,7 +121,21 @@ def load():
     lib.hprmat_hip_version.restype = ctypes.c_char_p
     lib.hprmat_hip_prof_enable.argtypes = [I]
     lib.hprmat_hip_prof_read.argtypes = [DP, DP, IP]
-    for name in REFERENCE_SYMBOLS + NATIVE_SYMBOLS:
+    lib.hprmat_init.argtypes = [I, I, D, P]
+    lib.hprmat_init.restype = None
+    lib.hprmat_set_solver.argtypes = [I]
+    lib.hprmat_set_solver.restype = None
+    lib.hprmat_get_solver.argtypes = []
+    lib.hprmat_get_solver.restype = I
+    lib.hprmat_solve.argtypes = [I, P, P, P, D, I, I, P, I, P, IP, I]
+    lib.hprmat_solve.restype = None
+    lib.hprmat_solve_linear.argtypes = [P, P, I, I, D, P, I]
+    lib.hprmat_solve_linear.restype = None
+    lib.hprmat_version.argtypes = [ctypes.c_char_p, I]
+    lib.hprmat_version.restype = None
+    lib.hprmat_solver_info.argtypes = [I, ctypes.c_char_p, I]
+    lib.hprmat_solver_info.restype = None
+    for name in REFERENCE_SYMBOLS + NATIVE_SYMBOLS + PHYSICS_SYMBOLS:
         if getattr(lib, name, None) is None:
             raise RuntimeError('libhprmat_hip.so lacks symbol ' + name)
     _lib = lib

@@ -292,6 +292,38 @@ int hprmat_hip_batch_solve(int method_flags, int n, int nrhs, int count, const v
 /* Library version string. */
 const char *hprmat_hip_version(void);
 
+/* ---------------- Part 3: host-pointer physics interface --------------- */
+/* The reference's C binding (bindings/c/hprmat.h:30-166, implemented in
+   bindings/c/hprmat_c.F90:25-261), same names and argument meaning; complex
+   arrays are double _Complex (interleaved re, im), column-major.  The
+   N x N system is assembled, factored, solved and reduced to R / S on the
+   GPU.  Provided for one interval (ns = 1) and eta = 0; other inputs print
+   a message and return nopen = -1 (hprmat_solve) or a zero R-matrix
+   (hprmat_solve_linear) where the reference stops the program.  Solver
+   types: 1 dense FP64 (+ at most two FP64 sweeps to ||r||_F/||b||_F <=
+   1e-12), 2 FP32 LU + two FP64 sweeps (1e-10), 3 as 1 (the reference's
+   CPU Woodbury solver gives the FP64 result), 4 FP32 LU + FGMRES to 1e-12;
+   0 = the type set by hprmat_set_solver (default 1). */
+/* bindings/c/hprmat.h:30 (rmat_ini): the nr*ns mesh abscissas. */
+void hprmat_init(int nr, int ns, double rmax, double *zrma);
+/* bindings/c/hprmat.h:37-45 */
+void hprmat_set_solver(int solver);
+int hprmat_get_solver(void);
+/* bindings/c/hprmat.h:78-83 (rmatrix, src/rmatrix_hp.F90:175-368):
+   cpot[ir + c*cpot_dim1 + c'*cpot_dim1*nch] = V_cc'(r_ir); cu (nch x nch)
+   receives S in its leading nopen x nopen block (open channels in input
+   order), zeros elsewhere. */
+void hprmat_solve(int nch, int *lval, double *qk, double *eta, double rmax, int nr, int ns,
+                  void *cpot, int cpot_dim1, void *cu, int *nopen, int solver);
+/* bindings/c/hprmat.h:162-164 (solve_rmatrix, src/rmat_solvers.F90:49-90):
+   C X = B with column c of B = B_vector in the rows of channel c, then
+   R(i,j) = normfac sum_r B_vector(r) X(i*nlag + r, j), Rmat column-major. */
+void hprmat_solve_linear(void *cmat, void *B_vector, int nch, int nlag, double normfac,
+                         void *Rmat, int solver);
+/* bindings/c/hprmat_c.F90:273-287 and :295: version and solver description. */
+void hprmat_version(char *version, int maxlen);
+void hprmat_solver_info(int solver, char *info, int maxlen);
+
 #ifdef __cplusplus
 }
 #endif
