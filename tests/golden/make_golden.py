@@ -1,9 +1,12 @@
-"""Extract the golden phase shifts of the reference's example outputs.
+"""Extract the golden phase shifts and wave functions of the reference's
+example outputs.
 
-Reads (as text) rmat_pierre/Ex2/output2.txt and rmat_pierre/Ex5/output5.txt
-from the HPRMAT source tree and writes tests/golden/rmat_pierre.json with the
-inputs of each one-interval (ns = 1) run and the printed results (5
-significant digits, es12.4).  These are Descouvemont's original outputs,
+Reads (as text) rmat_pierre/Ex2/output2.txt, rmat_pierre/Ex2/wf2.txt and
+rmat_pierre/Ex5/output5.txt from the HPRMAT source tree and writes
+tests/golden/rmat_pierre.json with the inputs of each run and the printed
+results (5 significant digits, es12.4): the one-interval (ns = 1) runs, the
+multi-interval Ex2 runs (ns = 2 and 3) and the internal wave functions of the
+ns = 3 Ex2 run (the rmax < 0 entry of rmat_pierre/Ex2/data2).  These are Descouvemont's original outputs,
 which HPRMAT reproduces (README.md:113-121).
 
 Usage: python tests/golden/make_golden.py [/root/reference]
@@ -36,16 +39,40 @@ def blocks(path):
     return runs
 
 
-def ex2():
+def ex2(multi=False):
     out = []
     for run in blocks(os.path.join(REF, 'rmat_pierre/Ex2/output2.txt')):
-        if run['ns'] != 1:
+        if (run['ns'] != 1) != multi:
             continue
         for line in run['lines']:
             e = float(re.search(r'E \(MeV\)=\s*([\d.]+)', line).group(1))
             d1, d2, eta12 = [float(x) for x in re.findall(NUM, line)]
-            out.append(dict(j=run['j'], nr=run['nr'], rmax=run['rmax'], e=e,
-                            delta1_rad=d1, delta2_rad=d2, eta12=eta12))
+            rec = dict(j=run['j'], nr=run['nr'], rmax=run['rmax'], e=e,
+                       delta1_rad=d1, delta2_rad=d2, eta12=eta12)
+            if multi:
+                rec['ns'] = run['ns']
+            out.append(rec)
+    return out
+
+
+def ex2_wf():
+    """Internal wave functions (r, channel 1, channel 2) per energy of
+    rmat_pierre/Ex2/wf2.txt: j = 1, nr = 25, ns = 3, rmax = 8, entrance
+    channel 1."""
+    out, cur, internal = [], None, False
+    for line in open(os.path.join(REF, 'rmat_pierre/Ex2/wf2.txt')):
+        if line.startswith('Wave function, E='):
+            e = float(re.search(r'E=\s*([\d.]+)', line).group(1))
+            cur = dict(j=1, nr=25, ns=3, rmax=8.0, e=e, entrance=1, r=[], wf=[])
+            out.append(cur)
+        elif 'Internal wave functions' in line:
+            internal = True
+        elif 'External wave functions' in line:
+            internal = False
+        elif internal and line.strip():
+            v = [float(x) for x in line.split()]
+            cur['r'].append(v[0])
+            cur['wf'].append([v[1], v[2], v[3], v[4]])
     return out
 
 
@@ -63,8 +90,11 @@ def ex5():
 
 if __name__ == '__main__':
     data = {
-        'source': 'rmat_pierre/Ex2/output2.txt, rmat_pierre/Ex5/output5.txt (es12.4)',
+        'source': 'rmat_pierre/Ex2/output2.txt, rmat_pierre/Ex2/wf2.txt, '
+                  'rmat_pierre/Ex5/output5.txt (es12.4)',
         'ex2_reid_nn': ex2(),
+        'ex2_reid_nn_intervals': ex2(multi=True),
+        'ex2_wave_functions': ex2_wf(),
         'ex5_yamaguchi': ex5(),
     }
     with open(OUT, 'w') as f:
