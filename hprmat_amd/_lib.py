@@ -38,11 +38,13 @@ NATIVE_SYMBOLS = (
     'hprmat_hip_r_to_s_ws', 'hprmat_hip_adjoint_contract',
     'hprmat_hip_zgemm', 'hprmat_hip_prof_enable', 'hprmat_hip_prof_read', 'hprmat_hip_version',
     'hprmat_hip_batch_work_bytes', 'hprmat_hip_batch_solve', 'hprmat_hip_assemble_batch',
-    'hprmat_hip_r_to_s_batch',
+    'hprmat_hip_r_to_s_batch', 'hprmat_hip_propagate_work_bytes', 'hprmat_hip_propagate',
+    'hprmat_hip_mesh',
 )
 PHYSICS_SYMBOLS = (
     'hprmat_init', 'hprmat_set_solver', 'hprmat_get_solver', 'hprmat_solve',
-    'hprmat_solve_linear', 'hprmat_version', 'hprmat_solver_info',
+    'hprmat_solve_linear', 'hprmat_version', 'hprmat_solver_info', 'hprmat_solve_full',
+    'hprmat_wavefunction',
 )
 
 
@@ -118,6 +120,12 @@ def load():
                                               DPtr, DPtr, L, L, P]
     lib.hprmat_hip_r_to_s_batch.argtypes = [I, I, I, DPtr, L, L, DPtr, D, D, DPtr, DPtr, DPtr,
                                             DPtr, DPtr, DPtr, DPtr, P]
+    lib.hprmat_hip_propagate_work_bytes.argtypes = [I, I, I]
+    lib.hprmat_hip_propagate_work_bytes.restype = ctypes.c_size_t
+    lib.hprmat_hip_propagate.argtypes = [I, I, I, DPtr, L, L, DPtr, DPtr, D, D, DPtr, I, DPtr,
+                                         DPtr, DPtr, DPtr, DPtr, I, DPtr, DPtr, I, DPtr, DPtr,
+                                         DPtr, L, L, DPtr, DPtr, ctypes.c_size_t, P]
+    lib.hprmat_hip_mesh.argtypes = [I, I, D, P, P, P, P, P, P]
     lib.hprmat_hip_version.restype = ctypes.c_char_p
     lib.hprmat_hip_prof_enable.argtypes = [I]
     lib.hprmat_hip_prof_read.argtypes = [DP, DP, IP]
@@ -130,6 +138,10 @@ def load():
     lib.hprmat_solve.argtypes = [I, P, P, P, D, I, I, P, I, P, IP, I]
     lib.hprmat_solve.restype = None
     lib.hprmat_solve_linear.argtypes = [P, P, I, I, D, P, I]
+    lib.hprmat_solve_full.argtypes = [I, P, P, P, D, I, I, P, I, P, P, I, I, I, P, P, I, IP, I]
+    lib.hprmat_solve_full.restype = None
+    lib.hprmat_wavefunction.argtypes = [I, P, P, P, D, I, I, P, I, P, I, I, P, I, I, I, D, P]
+    lib.hprmat_wavefunction.restype = None
     lib.hprmat_solve_linear.restype = None
     lib.hprmat_version.argtypes = [ctypes.c_char_p, I]
     lib.hprmat_version.restype = None

@@ -12,6 +12,7 @@
 // nopen x nopen solve done by this library's own getrf/getrs.
 #include "hprmat_hip.h"
 #include "lu.hpp"
+#include "small_gesv.hpp"
 
 namespace {
 
@@ -112,12 +113,7 @@ __global__ void store_colmajor_kernel(int n, const double2* __restrict__ Xr, dou
 // with partial pivoting (largest |Re|+|Im|, first on ties) and back
 // substitution, all in FP64; S stored column-major.  Per-system inputs are
 // rows of count x nch arrays (qk, open, co, cop) and nopen[count].
-constexpr int R2S_BATCH_MAX_NCH = 64;
-
-__device__ inline double2 zdiv(double2 a, double2 b) {
-  const double d = b.x * b.x + b.y * b.y;
-  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
-}
+constexpr int R2S_BATCH_MAX_NCH = hpr::SMALL_GESV_MAX_N;
 
 __global__ void __launch_bounds__(256)
 r_to_s_batch_kernel(int nch, int nr, const double2* __restrict__ X, int64_t ldx, int64_t sX,
@@ -167,61 +163,7 @@ r_to_s_batch_kernel(int nch, int nr, const double2* __restrict__ X, int64_t ldx,
     ZI[t] = zi;
   }
   __syncthreads();
-  for (int k = 0; k < no; ++k) {
-    if (tid < 64) {   // pivot search in wave 0 (no <= 64 rows)
-      const int i = k + tid;
-      double v = -1.0;
-      int vi = 0x7fffffff;
-      if (i < no) {
-        const double2 z = ZO[i * no + k];
-        v = fabs(z.x) + fabs(z.y);
-        vi = i;
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(v, off);
-        const int oi = __shfl_xor(vi, off);
-        if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
-      }
-      if (tid == 0) piv = (v > 0.0) ? vi : -1;
-    }
-    __syncthreads();
-    const int p = piv;
-    __syncthreads();       // every thread has read piv before the next column rewrites it
-    if (p < 0) continue;   // exactly singular column: left as is (S gets inf/nan)
-    if (p != k) {
-      for (int j = tid; j < 2 * no; j += 256) {
-        double2* Z = j < no ? ZO : ZI;
-        const int c = j < no ? j : j - no;
-        const double2 t = Z[k * no + c];
-        Z[k * no + c] = Z[p * no + c];
-        Z[p * no + c] = t;
-      }
-      __syncthreads();
-    }
-    for (int i = k + 1 + tid; i < no; i += 256) lcol[i] = zdiv(ZO[i * no + k], ZO[k * no + k]);
-    __syncthreads();
-    const int rows = no - k - 1, cols = no - k - 1 + no;   // ZO columns > k, then all of ZI
-    for (int e = tid; e < rows * cols; e += 256) {
-      const int i = k + 1 + e / cols, j = e % cols;
-      double2* Z = j < no - k - 1 ? ZO : ZI;
-      const int c = j < no - k - 1 ? k + 1 + j : j - (no - k - 1);
-      const double2 l = lcol[i], u = Z[k * no + c], z = Z[i * no + c];
-      const double2 lu = zmul(l, u);
-      Z[i * no + c] = make_double2(z.x - lu.x, z.y - lu.y);
-    }
-    __syncthreads();
-  }
-  for (int k = no - 1; k >= 0; --k) {
-    const double2 d = ZO[k * no + k];
-    for (int c = tid; c < no; c += 256) ZI[k * no + c] = zdiv(ZI[k * no + c], d);
-    __syncthreads();
-    for (int e = tid; e < k * no; e += 256) {
-      const int i = e / no, c = e % no;
-      const double2 lu = zmul(ZO[i * no + k], ZI[k * no + c]), z = ZI[i * no + c];
-      ZI[i * no + c] = make_double2(z.x - lu.x, z.y - lu.y);
-    }
-    __syncthreads();
-  }
+  hpr::lds_gesv(ZO, ZI, no, no, lcol, &piv);
   for (int t = tid; t < no * no; t += 256) S[(t % no) * no + t / no] = ZI[t];
 }
 

@@ -1,6 +1,6 @@
-"""Lagrange-mesh R-matrix front end of the GPU solve path (one interval).
+"""Lagrange-mesh R-matrix front end of the GPU solve path.
 
-Mirrors the reference's physics entry points for ns = 1:
+Mirrors the reference's physics entry points (one interval unless noted):
 * ``rmat_ini``       - src/rmatrix_hp.F90:61-159 (mesh, T+L, Bloch, q2)
 * ``channel_data``   - src/rmatrix_hp.F90:224-244 at eta = 0 (closed forms)
 * ``assemble``       - src/rmatrix_hp.F90:246-293, on the device
@@ -8,6 +8,8 @@ Mirrors the reference's physics entry points for ns = 1:
                        every type served by libhprmat_hip on the GPU
 * ``r_to_s``         - src/rmat_solvers.F90:597-609 + src/rmatrix_hp.F90:336-368
 * ``rmatrix``        - the whole ns = 1 pipeline returning (S, nopen)
+* ``rmatrix_intervals`` - src/rmatrix_hp.F90:175-431 for ns >= 1 intervals:
+                       propagation, S and the wave functions on the device
 
 Host work is O(nr^2 + nr nch^2); the N x N matrix is built, factored and
 solved on the device.
@@ -244,3 +246,94 @@ def rmatrix(lval, qk, a, nr, cpot, solver_type=1, max_refine=0, dmesh=None):
     st = solve_colmajor(a_cm, b_cm, solver_type, max_refine)
     r, s = r_to_s(dmesh, b_cm, nch, qk, open_idx, co, cop)
     return dict(S=s, R=r, nopen=len(open_idx), stats=st, open=open_idx)
+
+
+def closed_w(lval, qk, a):
+    """W(|k| a) = (2/pi) rho k_l(rho) per channel (zero for open channels):
+    the closed-channel Whittaker function at eta = 0, src/rmatrix_hp.F90:240."""
+    w = np.zeros(len(qk))
+    for c, (l, k) in enumerate(zip(lval, qk)):
+        if k < 0:
+            rho = -k * a
+            w[c] = (2.0 / math.pi) * rho * special.spherical_kn(int(l), rho)
+    return w
+
+
+def rmatrix_intervals(lval, qk, a, nr, ns, cpot, solver_type=1, nvc=None, max_refine=0):
+    """rmatrix (src/rmatrix_hp.F90:175-431) for ns >= 1 intervals on the GPU.
+
+    Every interval's system C_s is assembled on the device and solved for
+    its 2 nch boundary columns [q1 | q2] (solve_rmatrix_hp_4,
+    src/rmatrix_hp.F90:590-632, with the solver_type's method); the R blocks,
+    the propagation chain, S and, for the entrance channels ``nvc`` (0-based
+    among the open channels), the internal wave functions and bound-state
+    amplitudes follow in hprmat_hip_propagate.  cpot: (nr ns, nch, nch).
+    nch <= 64.  Returns dict(S, R, nopen, cf (nr ns, nch, len(nvc)) or None,
+    amps (nclo, len(nvc)) or None, X (ns, 2 nch, N) column-major solutions).
+    """
+    from . import physics
+    lib = _lib.load()
+    lval = np.asarray(lval)
+    qk = np.asarray(qk, dtype=np.float64)
+    nch = len(lval)
+    n = nch * nr
+    h = a / ns
+    m = physics.mesh(nr, ns, a)
+    tc, bl = _dev(m['tc'], torch.float64), _dev(m['bloch'], torch.float64)
+    rr, q1, q2 = _dev(m['r'], torch.float64), _dev(m['q1'], torch.float64), _dev(m['q2'], torch.float64)
+    wle = _dev(m['wle'], torch.float64)
+    open_idx, co, cop, xc1 = channel_data(lval, qk, a)
+    closed = [c for c in range(nch) if qk[c] <= 0]
+    nopen, nclo = len(open_idx), len(closed)
+    lv = _dev(lval.astype(np.int32), torch.int32)
+    qd, xd = _dev(qk, torch.float64), _dev(xc1, torch.float64)
+    cp = cpot if isinstance(cpot, torch.Tensor) else _dev(np.ascontiguousarray(cpot), torch.complex128)
+    if cp.shape != (nr * ns, nch, nch) or not cp.is_contiguous():
+        raise ValueError('cpot must be a contiguous (nr*ns, nch, nch) array')
+    a_cm = torch.empty((n, n), dtype=torch.complex128, device='cuda')
+    x = torch.zeros((ns, 2 * nch, n), dtype=torch.complex128, device='cuda')
+    for c in range(nch):
+        x[:, c, c * nr:(c + 1) * nr] = q1.to(torch.complex128)
+        x[:, nch + c, c * nr:(c + 1) * nr] = q2.to(torch.complex128)
+    st = []
+    for s in range(ns):
+        rc = lib.hprmat_hip_assemble(
+            nch, nr, ctypes.c_void_p(tc[s].data_ptr()), ctypes.c_void_p(bl[s].data_ptr()),
+            ctypes.c_void_p(rr[s].data_ptr()), ctypes.c_double(1.0),
+            ctypes.c_void_p(lv.data_ptr()), ctypes.c_void_p(qd.data_ptr()),
+            ctypes.c_void_p(xd.data_ptr()), ctypes.c_void_p(cp[s * nr].data_ptr()),
+            ctypes.c_void_p(a_cm.data_ptr()), ctypes.c_int64(n), solver._stream_ptr())
+        _lib.check(rc, 'hprmat_hip_assemble')
+        st.append(solve_colmajor(a_cm, x[s], solver_type, max_refine))
+    nc = 0 if nvc is None else len(nvc)
+    if nc and nopen == 0:
+        raise ValueError('wave functions need an open entrance channel')
+    if nc and any(not 0 <= v < nopen for v in nvc):
+        raise ValueError('entrance channels are 0-based indices among the open channels')
+    if nc and np.any(qk == 0):
+        raise ValueError('a channel at threshold (k = 0) has no bound-state amplitude')
+    wb = int(lib.hprmat_hip_propagate_work_bytes(nch, ns, nc))
+    if wb == 0:
+        raise ValueError('propagation needs 1 <= nch <= 64')
+    work = torch.empty(wb, dtype=torch.uint8, device='cuda')
+    r = torch.empty((nch, nch), dtype=torch.complex128, device='cuda')
+    s_cm = torch.empty((max(nopen, 1), max(nopen, 1)), dtype=torch.complex128, device='cuda')
+    ivec = lambda v: _dev(np.asarray(v if len(v) else [0], dtype=np.int32), torch.int32)
+    o, cl, nv = ivec(list(open_idx)), ivec(closed), ivec(list(nvc) if nc else [])
+    c0 = _dev(co if nopen else np.zeros(1, complex), torch.complex128)
+    c1 = _dev(cop if nopen else np.zeros(1, complex), torch.complex128)
+    x2 = _dev(closed_w(lval, qk, a), torch.float64)
+    cf = torch.zeros((max(nc, 1), nch, nr * ns), dtype=torch.complex128, device='cuda')
+    amp = torch.zeros((max(nc, 1), max(nclo, 1)), dtype=torch.complex128, device='cuda')
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    rc = lib.hprmat_hip_propagate(
+        nch, nr, ns, p(x), n, 2 * nch * n, p(q1), p(q2), 1.0, h, p(qd), nopen, p(o), p(c0), p(c1),
+        p(r), p(s_cm), nc, p(nv), p(wle), nclo if nc else 0, p(cl), p(x2), p(cf), nr * ns, nch,
+        p(amp), p(work), wb, solver._stream_ptr())
+    _lib.check(rc, 'hprmat_hip_propagate')
+    out = dict(S=s_cm[:nopen, :nopen].transpose(0, 1), R=r.transpose(0, 1), nopen=nopen, cf=None,
+               amps=None, X=x, stats=st, open=open_idx)
+    if nc:
+        out['cf'] = cf.permute(2, 1, 0)
+        out['amps'] = amp[:, :nclo].transpose(0, 1)
+    return out

@@ -292,13 +292,38 @@ int hprmat_hip_batch_solve(int method_flags, int n, int nrhs, int count, const v
 /* Library version string. */
 const char *hprmat_hip_version(void);
 
+/* No reference counterpart as one call (the reference runs it on the host,
+   src/rmatrix_hp.F90:319-431): R-matrix propagation over ns intervals, S
+   and, when nc > 0, the wave functions, on the device.  dX holds the ns
+   interval solutions C_s^-1 [q1_s blocks | q2_s blocks] (column-major N x
+   2 nch, N = nch nr, ldx, stride_x apart); q1, q2 (ns x nr), qk (nch),
+   open / co / cop (nopen: open channels and G + iF, rho O' at rmax = ns
+   rmax0), nvc (nc, 0-based among the open channels), wle (nr), closed /
+   xc2 (nclo closed channels and W(|k| rmax)) are device arrays.  Outputs:
+   dR (nch x nch, the propagated R-matrix times rmax, optional), dS (nopen x
+   nopen), dCf (ld1 x ld2 x nc wave functions at the mesh points), dAmp
+   (nclo x nc bound-state amplitudes).  nch <= 64.  work: a device buffer of
+   hprmat_hip_propagate_work_bytes(nch, ns, nc) bytes.  Nothing is
+   allocated or synchronised. */
+size_t hprmat_hip_propagate_work_bytes(int nch, int ns, int nc);
+int hprmat_hip_propagate(int nch, int nr, int ns, const void *dX, int64_t ldx, int64_t stride_x,
+                         const double *q1, const double *q2, double normfac, double rmax0,
+                         const double *qk, int nopen, const int *open, const void *co,
+                         const void *cop, void *dR, void *dS, int nc, const int *nvc,
+                         const double *wle, int nclo, const int *closed, const double *xc2,
+                         void *dCf, int64_t ld1, int64_t ld2, void *dAmp, void *work,
+                         size_t work_bytes, void *stream);
+
 /* ---------------- Part 3: host-pointer physics interface --------------- */
 /* The reference's C binding (bindings/c/hprmat.h:30-166, implemented in
    bindings/c/hprmat_c.F90:25-261), same names and argument meaning; complex
-   arrays are double _Complex (interleaved re, im), column-major.  The
-   N x N system is assembled, factored, solved and reduced to R / S on the
-   GPU.  Provided for one interval (ns = 1) and eta = 0; other inputs print
-   a message and return nopen = -1 (hprmat_solve) or a zero R-matrix
+   arrays are double _Complex (interleaved re, im), column-major, and
+   channel indices (nvc, iv, nom) are 1-based as in the reference.  Every
+   interval's system is assembled, factored and solved on the GPU, and R, S,
+   the propagation over intervals and the wave functions are formed there.
+   Provided for eta = 0 and local potentials; several intervals (ns > 1) or
+   wave functions need nch <= 64.  Other inputs print a message and return
+   nopen = -1 (hprmat_solve, hprmat_solve_full) or a zero R-matrix
    (hprmat_solve_linear) where the reference stops the program.  Solver
    types: 1 dense FP64 (+ at most two FP64 sweeps to ||r||_F/||b||_F <=
    1e-12), 2 FP32 LU + two FP64 sweeps (1e-10), 3 as 1 (the reference's
@@ -315,11 +340,38 @@ int hprmat_get_solver(void);
    order), zeros elsewhere. */
 void hprmat_solve(int nch, int *lval, double *qk, double *eta, double rmax, int nr, int ns,
                   void *cpot, int cpot_dim1, void *cu, int *nopen, int solver);
+/* bindings/c/hprmat.h:108-114 (rmatrix with twf = .true.,
+   src/rmatrix_hp.F90:370-431): as hprmat_solve, plus the internal wave
+   functions cf[r + c cf_dim1 + ic cf_dim1 cf_dim2] at the nr*ns mesh points
+   for the nc_entrance entrance channels nvc (1-based among the open
+   channels) and, for each entrance column ic, the bound-state amplitudes of
+   the closed channels in cu rows nopen.. .  cpnl must be NULL / cpnl_dim1 0.
+   The wave functions are those of Descouvemont's code and its golden
+   outputs (C^-1 applied to the boundary terms, rmat_pierre/rmatrix.f:296). */
+void hprmat_solve_full(int nch, int *lval, double *qk, double *eta, double rmax, int nr, int ns,
+                       void *cpot, int cpot_dim1, void *cu, void *cf, int cf_dim1, int cf_dim2,
+                       int nc_entrance, int *nvc, void *cpnl, int cpnl_dim1, int *nopen,
+                       int solver);
+/* bindings/c/hprmat.h:142-147 (wf_print, src/rmatrix_hp.F90:678-734): the
+   wave function of channel iv for entrance column nom at r = h, 2h, ...,
+   npoin h: natural cubic spline of cf inside rmax, the asymptotic form from
+   cu outside.  Host work only. */
+void hprmat_wavefunction(int nch, int *lval, double *qk, double *eta, double rmax, int nr, int ns,
+                         void *cu, int nopen, void *cf, int cf_dim1, int cf_dim2, double *zrma,
+                         int iv, int nom, int npoin, double h, void *cwftab);
 /* bindings/c/hprmat.h:162-164 (solve_rmatrix, src/rmat_solvers.F90:49-90):
    C X = B with column c of B = B_vector in the rows of channel c, then
    R(i,j) = normfac sum_r B_vector(r) X(i*nlag + r, j), Rmat column-major. */
 void hprmat_solve_linear(void *cmat, void *B_vector, int nch, int nlag, double normfac,
                          void *Rmat, int solver);
+/* No reference counterpart (the reference keeps them in module arrays,
+   src/rmatrix_hp.F90:61-159): the mesh matrices of ns intervals of (0, rmax)
+   for callers that assemble on the device with hprmat_hip_assemble: per
+   interval s, tc and bloch (ns x nr x nr, row-major blocks; bloch is the
+   closed-channel Bloch matrix with its radius factors, so the assembly takes
+   rmax0 = 1), r, q1, q2 (ns x nr) and the (0, 1) weights wle (nr). */
+int hprmat_hip_mesh(int nr, int ns, double rmax, double *tc, double *bloch, double *r,
+                    double *q1, double *q2, double *wle);
 /* bindings/c/hprmat_c.F90:273-287 and :295: version and solver description. */
 void hprmat_version(char *version, int maxlen);
 void hprmat_solver_info(int solver, char *info, int maxlen);

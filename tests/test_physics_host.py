@@ -47,3 +47,77 @@ def test_solve_rejects_unsupported_inputs(kwargs):
     args.update(kwargs)
     cu, nopen = physics.solve(**args)
     assert nopen == -1 and not cu.any()
+
+
+# ---- several intervals and wave functions (host parts) ----
+
+from oracle import lagrange as olag, propagation as oprop  # noqa: E402
+
+
+@pytest.mark.parametrize('nr,ns', [(7, 1), (8, 2), (9, 3), (10, 4)])
+def test_interval_mesh_matches_oracle(nr, ns):
+    a = 11.0
+    m = physics.mesh(nr, ns, a)
+    o = olag.rmat_ini(nr, ns, a)
+    for s in range(ns):
+        assert np.allclose(m['tc'][s], o['tc'][s], rtol=1e-12, atol=1e-12 * np.abs(o['tc'][s]).max())
+        assert np.allclose(m['bloch'][s], oprop.bloch_closed(o, s), rtol=1e-12, atol=1e-14)
+        assert np.allclose(m['r'][s], oprop.interval_radii(o, s), rtol=1e-14)
+        assert np.allclose(m['q1'][s], o['q1'][s], rtol=1e-12, atol=1e-15)
+        assert np.allclose(m['q2'][s], o['q2'][s], rtol=1e-12)
+    assert np.allclose(m['wle'], o['wle'], rtol=1e-12)
+
+
+def test_wavefunction_spline_and_asymptotics_match_oracle():
+    # hprmat_wavefunction is host work: interpolate the oracle's wave
+    # functions and continue them with the oracle's S
+    hm = 20.736 / 0.5
+    o = olag.rmat_ini(25, 3, 8.0)
+    from tests.test_oracle_propagation import reid
+    qk = np.array([np.sqrt(24.0 / hm)] * 2)
+    out = oprop.rmatrix_ns([0, 2], qk, 8.0, 25, 3, reid(1, o['zrma']), nvc=[0])
+    for iv in (1, 2):
+        got = physics.wavefunction([0, 2], qk, None, 8.0, 25, 3, out['cu'], 2, out['cf'],
+                                   o['zrma'], iv, 1, 60, 0.2)
+        want = oprop.wf_print([0, 2], qk, 8.0, out['cu'], 2, out['cf'], o['zrma'], iv - 1, 0, 60, 0.2)
+        assert np.abs(got - want).max() < 1e-11 * np.abs(want).max()
+
+
+def test_wavefunction_closed_channel_asymptotics():
+    # a closed channel outside the radius: amplitude x W(|k| r) = e^-|k|r (l = 0)
+    nch, nr, ns = 2, 6, 2
+    qk = np.array([0.8, -0.5])
+    cu = np.zeros((nch, nch), complex)
+    cu[1, 0] = 0.3 - 0.2j
+    cf = np.zeros((nr * ns, nch, 1), complex)
+    z = physics.init(nr, ns, 4.0)
+    out = physics.wavefunction([0, 0], qk, None, 4.0, nr, ns, cu, 1, cf, z, 2, 1, 30, 0.2)
+    r = 0.2 * np.arange(1, 31)
+    sel = r > 4.0
+    assert np.allclose(out[sel], cu[1, 0] * np.exp(-0.5 * r[sel]), rtol=1e-12)
+    assert not out[~sel].any()
+
+
+def test_solve_full_rejects_unsupported_inputs():
+    nch, nr = 2, 10
+    cp = np.zeros((nr * 2, nch, nch), complex)
+    lib = physics._lib.load()
+    cu = np.ones((nch, nch), complex, order='F')
+    cf = np.zeros((nr * 2, nch, 1), complex, order='F')
+    nopen = physics.ctypes.c_int(7)
+    lval = np.zeros(nch, np.int32)
+    qk = np.array([0.5, 0.7])
+    eta = np.zeros(nch)
+    nvc = np.array([1], np.int32)
+    cpnl = np.zeros((4, nch, nch), complex, order='F')
+    lib.hprmat_solve_full(nch, physics._ptr(lval), physics._ptr(qk), physics._ptr(eta), 8.0, nr,
+                          2, physics._ptr(np.asfortranarray(cp)), nr * 2, physics._ptr(cu),
+                          physics._ptr(cf), nr * 2, nch, 1, physics._ptr(nvc),
+                          physics._ptr(cpnl), 4, physics.ctypes.byref(nopen), 1)
+    assert nopen.value == -1 and not cu.any()
+    # entrance channel not open; nch > 64 with several intervals
+    cu, cf2, no = physics.solve_full([0, 0], [0.5, -0.7], None, 8.0, nr, cp, [2], ns=2)
+    assert no == -1
+    big = np.zeros((nr * 2, 65, 65), complex)
+    cu, no = physics.solve(np.zeros(65, int), np.full(65, 0.5), None, 8.0, nr, big, ns=2)
+    assert no == -1
