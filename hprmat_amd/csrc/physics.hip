@@ -52,6 +52,23 @@ assemble_kernel(int nch, int nr, const double* __restrict__ tc, const double* __
   A[(int64_t)col * lda + row] = make_double2(vr, vi);
 }
 
+// Non-local potential (src/rmatrix_hp.F90:283-289): C[(c,i),(c',j)] +=
+// sqrt(w_i w_j) a0 K_cc'(r_i, r_j), K as [i][j][c][c'] complex.
+__global__ void __launch_bounds__(256)
+nonlocal_kernel(int nch, int nr, const double* __restrict__ wle, double rmax0,
+                const double2* __restrict__ K, double2* __restrict__ A, int64_t lda) {
+  const int n = nch * nr;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  const int col = blockIdx.y;
+  if (row >= n) return;
+  const int c = row / nr, i = row % nr;
+  const int c2 = col / nr, j = col % nr;
+  const double f = sqrt(wle[i] * wle[j]) * rmax0;
+  const double2 k = K[(((int64_t)i * nr + j) * nch + c) * nch + c2];
+  double2* a = A + (int64_t)col * lda + row;
+  *a = make_double2(a->x + f * k.x, a->y + f * k.y);
+}
+
 // R(a, b) = normfac * sum_r q2[r] X[a*nr + r, b]  (X column-major, R column-major)
 __global__ void rmatrix_kernel(int nch, int nr, const double2* __restrict__ X, int64_t ldx,
                                const double* __restrict__ q2, double normfac,
@@ -270,6 +287,20 @@ int hprmat_hip_assemble(int nch, int nr, const double* tc, const double* blo0, c
   dim3 grid((n + 255) / 256, n);
   assemble_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(nch, nr, tc, blo0, r, rmax0, lval, qk, xc1,
                                                         (const double2*)cpot, (double2*)dA, lda);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+/* Add a non-local potential to an assembled one-interval matrix:
+   C[(c,i),(c',j)] += sqrt(wle_i wle_j) rmax0 K[i][j][c][c'] (device arrays;
+   wle the (0, 1) weights, K nr x nr x nch x nch complex).  Replaces the
+   cpnl loop of src/rmatrix_hp.F90:283-289. */
+int hprmat_hip_add_nonlocal(int nch, int nr, const double* wle, double rmax0, const void* K,
+                            void* dA, int64_t lda, void* stream) {
+  const int n = nch * nr;
+  if (nch <= 0 || nr <= 0 || lda < n || !wle || !K || !dA) return -1;
+  dim3 grid((n + 255) / 256, n);
+  nonlocal_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(nch, nr, wle, rmax0, (const double2*)K,
+                                                         (double2*)dA, lda);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -16,8 +16,8 @@
 // reduce to, src/rmatrix_hp.F90:224-244), and the spline of
 // hprmat_wavefunction (O(nr ns + npoin)).
 //
-// Scope: eta = 0 and local potentials; several intervals or wave functions
-// need nch <= 64.  Other inputs are reported on stderr and answered with
+// Scope: eta = 0; several intervals or wave functions need nch <= 64, and a
+// non-local potential one interval (as in the reference).  Other inputs are reported on stderr and answered with
 // nopen = -1 (hprmat_solve, hprmat_solve_full) or a zero R-matrix
 // (hprmat_solve_linear); the reference stops the program on errors instead.
 #include <hip/hip_runtime.h>
@@ -304,6 +304,8 @@ double spline_at(const double* x, const double* y, const std::vector<double>& y2
   return a * y[lo] + b * y[hi] + ((a * a * a - a) * y2[lo] + (b * b * b - b) * y2[hi]) * h * h / 6.0;
 }
 
+size_t nn2_of(int nr) { return (size_t)nr * nr; }
+
 // Wave-function request of hprmat_solve_full: nc entrance channels nvc
 // (1-based indices among the open channels), cf (dim1 x dim2 x nc).
 struct WaveOut {
@@ -320,7 +322,8 @@ struct WaveOut {
 // boundary columns and hprmat_hip_propagate forms S (and the wave functions).
 bool physics_solve(const char* who, int nch, const int* lval, const double* qk, const double* eta,
                    double rmax, int nr, int ns, const void* cpot, int cpot_dim1, void* cu,
-                   int* nopen, int solver, const WaveOut& wv) {
+                   int* nopen, int solver, const WaveOut& wv, const void* cpnl = nullptr,
+                   int cpnl_dim1 = 0) {
   if (nopen) *nopen = -1;
   if (cu && nch > 0) memset(cu, 0, sizeof(zd) * (size_t)nch * nch);
   SolverPlan plan;
@@ -336,6 +339,15 @@ bool physics_solve(const char* who, int nch, const int* lval, const double* qk, 
     }
   if (!plan_for(solver > 0 ? solver : g_solver, plan)) {
     fprintf(stderr, "%s: solver type %d is not available\n", who, solver);
+    return false;
+  }
+  const bool nonlocal = cpnl_dim1 > 0;
+  if (nonlocal && (!cpnl || cpnl_dim1 < nr * nr)) {
+    fprintf(stderr, "%s: invalid non-local potential arguments\n", who);
+    return false;
+  }
+  if (nonlocal && ns != 1) {
+    fprintf(stderr, "%s: ns must be 1 for non-local potentials\n", who);
     return false;
   }
   const bool chain = ns > 1 || wv.nc > 0;
@@ -389,6 +401,16 @@ bool physics_solve(const char* who, int nch, const int* lval, const double* qk, 
       for (int d = 0; d < nch; ++d)
         pot[((size_t)i * nch + c) * nch + d] =
             cp[i + (size_t)c * cpot_dim1 + (size_t)d * cpot_dim1 * nch];
+  // non-local kernel: cpnl[(ir nr + ir') + c dim1 + c' dim1 nch] -> [ir][ir'][c][c']
+  std::vector<zd> knl(nonlocal ? nn2_of(nr) * nch * nch : 1);
+  if (nonlocal) {
+    const zd* kp = (const zd*)cpnl;
+    for (size_t p = 0; p < nn2_of(nr); ++p)
+      for (int c = 0; c < nch; ++c)
+        for (int d = 0; d < nch; ++d)
+          knl[(p * nch + c) * nch + d] =
+              kp[p + (size_t)c * cpnl_dim1 + (size_t)d * cpnl_dim1 * nch];
+  }
   // right-hand sides per interval: [q1 blocks | q2 blocks] or the q2 blocks
   std::vector<zd> rhs((size_t)ns * n * nrhs, zd(0.0, 0.0));
   for (int s = 0; s < ns; ++s)
@@ -432,10 +454,11 @@ bool physics_solve(const char* who, int nch, const int* lval, const double* qk, 
   zd* dS = d.alloc<zd>((size_t)nch * nch);
   zd* dcf = d.alloc<zd>(cfn);
   zd* damp = d.alloc<zd>((size_t)nch * (wv.nc > 0 ? wv.nc : 1));
+  zd* dknl = d.alloc<zd>(knl.size());
   void* dwork = chain ? d.alloc<char>(pw) : nullptr;
   if (!dtc || !dbl || !dr || !dq1 || !dq2 || !dw || !dl || !dqk || !dxc || !dxc2 || !dpot ||
       !dA || !dX || !dopen || !dclosed || !dnvc || !dco || !dcop || !dR || !dS || !dcf || !damp ||
-      (chain && !dwork)) {
+      !dknl || (chain && !dwork)) {
     fprintf(stderr, "%s: device allocation failed\n", who);
     return false;
   }
@@ -445,7 +468,7 @@ bool physics_solve(const char* who, int nch, const int* lval, const double* qk, 
             upload(dl, lval, (size_t)nch) && upload(dqk, qk, (size_t)nch) &&
             upload(dxc, xc1.data(), (size_t)nch) && upload(dxc2, xc2.data(), (size_t)nch) &&
             upload(dpot, pot.data(), pot.size()) && upload(dX, rhs.data(), rhs.size()) &&
-            upload(dnvc, nvc0.data(), nvc0.size());
+            upload(dnvc, nvc0.data(), nvc0.size()) && (!nonlocal || upload(dknl, knl.data(), knl.size()));
   if (no > 0)
     ok = ok && upload(dopen, open.data(), (size_t)no) && upload(dco, co.data(), (size_t)no) &&
          upload(dcop, cop.data(), (size_t)no);
@@ -455,6 +478,7 @@ bool physics_solve(const char* who, int nch, const int* lval, const double* qk, 
   for (int s = 0; ok && s < ns; ++s)
     ok = !hprmat_hip_assemble(nch, nr, dtc + s * nn2, dbl + s * nn2, dr + (size_t)s * nr, 1.0, dl,
                               dqk, dxc, dpot + (size_t)s * nr * nch * nch, dA, n, g_stream) &&
+         (!nonlocal || !hprmat_hip_add_nonlocal(nch, nr, dw, m.h, dknl, dA, n, g_stream)) &&
          !factor_solve(plan, dA, n, dX + (size_t)s * n * nrhs, nrhs);
   if (ok) {
     if (chain)
@@ -538,12 +562,6 @@ void hprmat_solve_full(int nch, int* lval, double* qk, double* eta, double rmax,
                        void* cpot, int cpot_dim1, void* cu, void* cf, int cf_dim1, int cf_dim2,
                        int nc_entrance, int* nvc, void* cpnl, int cpnl_dim1, int* nopen,
                        int solver) {
-  if (cpnl_dim1 != 0 || (cpnl && cpnl_dim1 > 0)) {
-    if (nopen) *nopen = -1;
-    if (cu && nch > 0) memset(cu, 0, sizeof(zd) * (size_t)nch * nch);
-    fprintf(stderr, "hprmat_solve_full: non-local potentials are not provided\n");
-    return;
-  }
   WaveOut wv;
   wv.nc = nc_entrance;
   wv.nvc = nvc;
@@ -557,7 +575,7 @@ void hprmat_solve_full(int nch, int* lval, double* qk, double* eta, double rmax,
     return;
   }
   physics_solve("hprmat_solve_full", nch, lval, qk, eta, rmax, nr, ns, cpot, cpot_dim1, cu, nopen,
-                solver, wv);
+                solver, wv, cpnl, cpnl_dim1);
 }
 
 void hprmat_wavefunction(int nch, int* lval, double* qk, double* eta, double rmax, int nr, int ns,

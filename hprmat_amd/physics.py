@@ -56,11 +56,12 @@ def solve(lval, qk, eta, rmax, nr, cpot, ns=1, solver=0):
     return cu, nopen.value
 
 
-def solve_full(lval, qk, eta, rmax, nr, cpot, nvc, ns=1, solver=0):
+def solve_full(lval, qk, eta, rmax, nr, cpot, nvc, ns=1, solver=0, cpnl=None):
     """(cu, cf, nopen) of hprmat_solve_full.  nvc: entrance channels,
     1-based among the open channels; cf: (nr*ns, nch, len(nvc)) complex
     wave functions at the mesh points; cu: S in cu[:nopen, :nopen], the
-    bound-state amplitudes in cu[nopen:, :len(nvc)]."""
+    bound-state amplitudes in cu[nopen:, :len(nvc)].  cpnl: optional
+    non-local potential (nr, nr, nch, nch), K_cc'(r_i, r_j) (ns = 1)."""
     lval = np.ascontiguousarray(lval, dtype=np.int32)
     nch = len(lval)
     qk = np.ascontiguousarray(qk, dtype=np.float64)
@@ -70,9 +71,14 @@ def solve_full(lval, qk, eta, rmax, nr, cpot, nvc, ns=1, solver=0):
     cu = np.zeros((nch, nch), dtype=np.complex128, order='F')
     cf = np.zeros((nr * ns, nch, len(nv)), dtype=np.complex128, order='F')
     nopen = ctypes.c_int(-1)
+    kp, kdim = None, 0
+    if cpnl is not None:
+        k = np.asarray(cpnl, dtype=np.complex128)
+        kf = np.asfortranarray(k.reshape(nr * nr, nch, nch))   # row ir*nr + ir'
+        kp, kdim = _ptr(kf), nr * nr
     _lib.load().hprmat_solve_full(nch, _ptr(lval), _ptr(qk), _ptr(eta), float(rmax), int(nr),
                                   int(ns), _ptr(cp), cp.shape[0], _ptr(cu), _ptr(cf), nr * ns, nch,
-                                  len(nv), _ptr(nv), None, 0, ctypes.byref(nopen), int(solver))
+                                  len(nv), _ptr(nv), kp, kdim, ctypes.byref(nopen), int(solver))
     return cu, cf, nopen.value
 
 

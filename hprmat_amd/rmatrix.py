@@ -235,13 +235,36 @@ def r_to_s_batch(dmesh, x_cm, qk, nopen, open_idx, co, cop, r_out, s_out, count=
     return r_out, s_out
 
 
-def rmatrix(lval, qk, a, nr, cpot, solver_type=1, max_refine=0, dmesh=None):
-    """ns = 1 pipeline on the GPU: returns dict(S, R, nopen, stats)."""
+def add_nonlocal(dmesh, cpnl, a_cm):
+    """C += sqrt(w_i w_j) a K_cc'(r_i, r_j) on the device (src/rmatrix_hp.F90:
+    283-289); cpnl: (nr, nr, nch, nch) complex."""
+    return _add_nonlocal(dmesh.mesh.nr, dmesh.mesh.w, dmesh.mesh.a, cpnl, a_cm)
+
+
+def _add_nonlocal(nr, w, a0, cpnl, a_cm):
+    k = cpnl if isinstance(cpnl, torch.Tensor) else _dev(np.ascontiguousarray(cpnl), torch.complex128)
+    nch = k.shape[2]
+    if k.shape != (nr, nr, nch, nch) or not k.is_contiguous() or a_cm.shape != (nch * nr, nch * nr):
+        raise ValueError('cpnl must be a contiguous (nr, nr, nch, nch) array for an (nch nr)^2 matrix')
+    wd = _dev(np.asarray(w, dtype=np.float64), torch.float64)
+    rc = _lib.load().hprmat_hip_add_nonlocal(
+        nch, nr, ctypes.c_void_p(wd.data_ptr()), ctypes.c_double(a0),
+        ctypes.c_void_p(k.data_ptr()), ctypes.c_void_p(a_cm.data_ptr()),
+        ctypes.c_int64(nch * nr), solver._stream_ptr())
+    _lib.check(rc, 'hprmat_hip_add_nonlocal')
+    return a_cm
+
+
+def rmatrix(lval, qk, a, nr, cpot, solver_type=1, max_refine=0, dmesh=None, cpnl=None):
+    """ns = 1 pipeline on the GPU: returns dict(S, R, nopen, stats).
+    cpnl: optional non-local potential (nr, nr, nch, nch)."""
     if dmesh is None:
         dmesh = DeviceMesh(rmat_ini(nr, a))
     nch = len(lval)
     open_idx, co, cop, xc1 = channel_data(lval, qk, a)
     a_cm = assemble(dmesh, lval, qk, xc1, cpot)
+    if cpnl is not None:
+        add_nonlocal(dmesh, cpnl, a_cm)
     b_cm = rhs_colmajor(dmesh, nch)
     st = solve_colmajor(a_cm, b_cm, solver_type, max_refine)
     r, s = r_to_s(dmesh, b_cm, nch, qk, open_idx, co, cop)
@@ -259,7 +282,8 @@ def closed_w(lval, qk, a):
     return w
 
 
-def rmatrix_intervals(lval, qk, a, nr, ns, cpot, solver_type=1, nvc=None, max_refine=0):
+def rmatrix_intervals(lval, qk, a, nr, ns, cpot, solver_type=1, nvc=None, max_refine=0,
+                      cpnl=None):
     """rmatrix (src/rmatrix_hp.F90:175-431) for ns >= 1 intervals on the GPU.
 
     Every interval's system C_s is assembled on the device and solved for
@@ -267,7 +291,8 @@ def rmatrix_intervals(lval, qk, a, nr, ns, cpot, solver_type=1, nvc=None, max_re
     src/rmatrix_hp.F90:590-632, with the solver_type's method); the R blocks,
     the propagation chain, S and, for the entrance channels ``nvc`` (0-based
     among the open channels), the internal wave functions and bound-state
-    amplitudes follow in hprmat_hip_propagate.  cpot: (nr ns, nch, nch).
+    amplitudes follow in hprmat_hip_propagate.  cpot: (nr ns, nch, nch);
+    cpnl: optional non-local potential (nr, nr, nch, nch), ns = 1 only.
     nch <= 64.  Returns dict(S, R, nopen, cf (nr ns, nch, len(nvc)) or None,
     amps (nclo, len(nvc)) or None, X (ns, 2 nch, N) column-major solutions).
     """
@@ -278,6 +303,8 @@ def rmatrix_intervals(lval, qk, a, nr, ns, cpot, solver_type=1, nvc=None, max_re
     nch = len(lval)
     n = nch * nr
     h = a / ns
+    if cpnl is not None and ns != 1:
+        raise ValueError('ns must be 1 for non-local potentials')
     m = physics.mesh(nr, ns, a)
     tc, bl = _dev(m['tc'], torch.float64), _dev(m['bloch'], torch.float64)
     rr, q1, q2 = _dev(m['r'], torch.float64), _dev(m['q1'], torch.float64), _dev(m['q2'], torch.float64)
@@ -304,6 +331,8 @@ def rmatrix_intervals(lval, qk, a, nr, ns, cpot, solver_type=1, nvc=None, max_re
             ctypes.c_void_p(xd.data_ptr()), ctypes.c_void_p(cp[s * nr].data_ptr()),
             ctypes.c_void_p(a_cm.data_ptr()), ctypes.c_int64(n), solver._stream_ptr())
         _lib.check(rc, 'hprmat_hip_assemble')
+        if cpnl is not None:
+            _add_nonlocal(nr, m['wle'], h, cpnl, a_cm)
         st.append(solve_colmajor(a_cm, x[s], solver_type, max_refine))
     nc = 0 if nvc is None else len(nvc)
     if nc and nopen == 0:

@@ -189,6 +189,13 @@ int hprmat_hip_assemble(int nch, int nr, const double *tc, const double *blo0,
                         const double *qk, const double *xc1, const void *cpot,
                         void *dA, int64_t lda, void *stream);
 
+/* Add a non-local potential to a matrix assembled by hprmat_hip_assemble
+   (one interval): C[(c,i),(c',j)] += sqrt(wle_i wle_j) rmax0 K[i][j][c][c'],
+   device arrays, wle the (0, 1) Gauss weights, K nr x nr x nch x nch
+   complex.  Replaces the cpnl loop of src/rmatrix_hp.F90:283-289. */
+int hprmat_hip_add_nonlocal(int nch, int nr, const double *wle, double rmax0, const void *K,
+                            void *dA, int64_t lda, void *stream);
+
 /* R = normfac q2^T X (nch x nch) and S = Z_O^-1 Z_I (nopen x nopen), both
    column-major FP64 on the device.  dX is the column-major N x nch solution
    (ldx), q2 the nr boundary values, qk the nch wave numbers, open the
@@ -321,7 +328,7 @@ int hprmat_hip_propagate(int nch, int nr, int ns, const void *dX, int64_t ldx, i
    channel indices (nvc, iv, nom) are 1-based as in the reference.  Every
    interval's system is assembled, factored and solved on the GPU, and R, S,
    the propagation over intervals and the wave functions are formed there.
-   Provided for eta = 0 and local potentials; several intervals (ns > 1) or
+   Provided for eta = 0; several intervals (ns > 1) or
    wave functions need nch <= 64.  Other inputs print a message and return
    nopen = -1 (hprmat_solve, hprmat_solve_full) or a zero R-matrix
    (hprmat_solve_linear) where the reference stops the program.  Solver
@@ -345,7 +352,9 @@ void hprmat_solve(int nch, int *lval, double *qk, double *eta, double rmax, int 
    functions cf[r + c cf_dim1 + ic cf_dim1 cf_dim2] at the nr*ns mesh points
    for the nc_entrance entrance channels nvc (1-based among the open
    channels) and, for each entrance column ic, the bound-state amplitudes of
-   the closed channels in cu rows nopen.. .  cpnl must be NULL / cpnl_dim1 0.
+   the closed channels in cu rows nopen.. .  cpnl (cpnl_dim1 >= nr*nr rows,
+   row ir*nr + ir' for the mesh pair (ir, ir'), then nch x nch) is the
+   optional non-local potential, one interval only (as in the reference).
    The wave functions are those of Descouvemont's code and its golden
    outputs (C^-1 applied to the boundary terms, rmat_pierre/rmatrix.f:296). */
 void hprmat_solve_full(int nch, int *lval, double *qk, double *eta, double rmax, int nr, int ns,

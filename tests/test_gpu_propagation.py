@@ -121,3 +121,34 @@ def test_c_interface_golden_ex2_wave_functions():
         assert nopen == 2
         got = np.stack([cf[:, 0, 0].real, cf[:, 0, 0].imag, cf[:, 1, 0].real, cf[:, 1, 0].imag], 1)
         assert np.all(np.abs(got - np.array(g['wf'])) <= 5.01e-5 * np.abs(np.array(g['wf']))), g['e']
+
+
+def _yamaguchi(nr, a):
+    al, be = 0.2316053, 1.3918324
+    z = olag.rmat_ini(nr, 1, a)['zrma']
+    k = np.zeros((nr, nr, 1, 1), complex)
+    k[:, :, 0, 0] = -2 * be * (al + be) ** 2 * np.exp(-be * (z[:, None] + z[None, :]))
+    return k
+
+
+def test_golden_ex5_nonlocal_on_gpu():
+    # Yamaguchi separable non-local potential (the reference's example 5):
+    # through rmatrix (device assembly + hprmat_hip_add_nonlocal), the
+    # interval pipeline and the C entry point hprmat_solve_full
+    from oracle import rmatrix as ormat
+    hm = 41.472
+    for g in json.load(open(GOLDEN))['ex5_yamaguchi']:
+        nr, a = g['nr'], g['rmax']
+        k = _yamaguchi(nr, a)
+        qk = np.array([math.sqrt(g['e'] / hm)])
+        cp = np.zeros((nr, 1, 1), complex)
+        ref = ormat.rmatrix([0], qk, a, nr, cp, cpnl=k)['S']
+        s1 = _np(rmatrix.rmatrix([0], qk, a, nr, cp, cpnl=k)['S'])
+        s2 = _np(rmatrix.rmatrix_intervals([0], qk, a, nr, 1, cp, cpnl=k)['S'])
+        cu, cf, nopen = physics.solve_full([0], qk, None, a, nr, cp, [1], solver=1, cpnl=k)
+        assert nopen == 1
+        for s in (s1, s2, cu[:1, :1]):
+            assert abs(s[0, 0] - ref[0, 0]) < 1e-10
+            delta = math.atan2(s[0, 0].imag, s[0, 0].real) / 2 * 180 / math.pi
+            assert abs(delta - g['delta_deg']) <= 5e-5 * abs(g['delta_deg'])
+            assert abs(abs(s[0, 0]) - g['modulus']) < 1e-4

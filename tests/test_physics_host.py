@@ -121,3 +121,11 @@ def test_solve_full_rejects_unsupported_inputs():
     big = np.zeros((nr * 2, 65, 65), complex)
     cu, no = physics.solve(np.zeros(65, int), np.full(65, 0.5), None, 8.0, nr, big, ns=2)
     assert no == -1
+
+
+def test_solve_full_rejects_nonlocal_with_several_intervals():
+    nr = 8
+    cp = np.zeros((nr * 2, 1, 1), complex)
+    k = np.zeros((nr, nr, 1, 1), complex)
+    cu, cf, no = physics.solve_full([0], [0.5], None, 8.0, nr, cp, [1], ns=2, cpnl=k)
+    assert no == -1
