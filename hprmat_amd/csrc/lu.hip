@@ -673,14 +673,14 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
     cx<T>* Akk = A + (int64_t)k * ld + k;
     const cx<T>* iL = w.dinv256 + (size_t)(k / 256) * 2 * 65536;
     // Inverses of the panel's diagonal blocks (64 x 64, then 256 x 256):
-    // inv(L11) now, it is needed for U12; inv(U11) only by the solves.
-    auto inverses = [&](int part, hipStream_t s_, int prio_) {
-      diag_inv_kernel<T><<<dim3((kb + 63) / 64, 1), 256, 0, s_>>>(A, ld, n, k / 64, w.dinv64, 0,
-                                                                  0, prio_, part);
-      blk_inv_kernel<T><<<dim3(1, 1, 4), 256, 0, s_>>>(A, ld, n, k / 256, w.dinv64, w.dinv256,
-                                                       0, 0, 0, prio_, part);
-    };
-    inverses(0, st, pk);
+    // inv(L11) now, it is needed for U12; inv(U11) only by the solves, so
+    // those are formed for every block in one pass after the last step
+    // (U11 is final once its panel is factored: later interchanges only
+    // touch rows below it), off the update chain.
+    diag_inv_kernel<T><<<dim3((kb + 63) / 64, 1), 256, 0, st>>>(A, ld, n, k / 64, w.dinv64, 0, 0,
+                                                                pk, 0);
+    blk_inv_kernel<T><<<dim3(1, 1, 4), 256, 0, st>>>(A, ld, n, k / 256, w.dinv64, w.dinv256, 0, 0,
+                                                     0, pk, 0);
     // U12 = inv(L11) A12 for the trailing columns [c0, c0 + cols) of the
     // step, staged in ustage (row stride nc), then C -= L21 U12 over them
     // and U12 copied back into A's rows k .. k + kb.
@@ -696,8 +696,8 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
     if (la && nc > nb) {
       // Look-ahead step.  Main stream: the next panel's columns only (swap,
       // U12, update, copy back), so that the next panel starts as early as
-      // possible.  Side stream: the left interchanges, inv(U11), and the swap,
-      // U12 and update of every column beyond the next panel.
+      // possible.  Side stream: the swap, U12 and update of every column
+      // beyond the next panel.
       // (split-BF16: the side stream also reads the L21 planes made below, so
       // it starts after them)
       if (!bfx) HPR_CHECK(hipEventRecord(side->ready, st));
@@ -720,9 +720,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       copy_back(0, nb, st);
       hipStream_t cs = side->stream;
       HPR_CHECK(hipStreamWaitEvent(cs, side->ready, 0));
-      if (k > 0) launch_laswp<T>(A, ld, 0, k, k, kb, ipiv, nullptr, cs);
       launch_laswp<T>(A, ld, k + 2 * kb, n, k, kb, ipiv, nullptr, cs);
-      inverses(1, cs, 0);
       u12(nb, nc - nb, cs, 0);
       if (bfx)
         split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, cs, false,
@@ -739,10 +737,16 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       copy_back(nb, nc - nb, cs);
       HPR_CHECK(hipEventRecord(side->rest, cs));
       rest_pending = true;
+      // the finished L columns left of the panel are not read again until
+      // the solves: their interchanges follow the update on the side stream,
+      // after the event the main stream waits for (run concurrently with the
+      // update they would be starved by its workgroups)
+      if (k > 0) launch_laswp<T>(A, ld, 0, k, k, kb, ipiv, nullptr, cs);
     } else {
-      // no look-ahead (or the last steps): everything on the main stream
-      inverses(1, st, pk);
+      // no look-ahead (or the last steps): everything on the main stream,
+      // after all side-stream work (the left interchanges included)
       if (rest_pending) {
+        HPR_CHECK(hipEventRecord(side->rest, side->stream));
         HPR_CHECK(hipStreamWaitEvent(st, side->rest, 0));
         rest_pending = false;
       }
@@ -766,7 +770,14 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       }
     }
   }
-  if (rest_pending) HPR_CHECK(hipStreamWaitEvent(st, side->rest, 0));
+  if (rest_pending) {   // all side-stream work, the left interchanges included
+    HPR_CHECK(hipEventRecord(side->rest, side->stream));
+    HPR_CHECK(hipStreamWaitEvent(st, side->rest, 0));
+  }
+  // inv(U_bb) of every diagonal block: 64 x 64 blocks, then 256 x 256
+  diag_inv_kernel<T><<<dim3((n + 63) / 64, 1), 256, 0, st>>>(A, ld, n, 0, w.dinv64, 0, 0, 0, 1);
+  blk_inv_kernel<T><<<dim3((n + 255) / 256, 1, 4), 256, 0, st>>>(A, ld, n, 0, w.dinv64, w.dinv256,
+                                                                  0, 0, 0, 0, 1);
   fold_inverses<T>(A, ld, n, w.dinv256, w.ustage, st);
   const size_t perm_lds = (size_t)n * sizeof(int);
   const int in_lds = perm_lds <= 140 * 1024;
