@@ -70,9 +70,9 @@ size_t factor_elem_bytes(int method) { return method == HPRMAT_PLAIN_FP64 ? 16 :
 int64_t padded_ld(int n) { return (((int64_t)n + 15) / 16) * 16 + 16; }
 
 // The look-ahead stream runs the bulk of every trailing update (with the
-// interchanges, U12 and inv(U11) that only the bulk needs) beside the panel
-// chain on the caller's stream.  A plain non-blocking stream at the default
-// priority, created on the first factorization of a handle.
+// interchanges and U12 that only the bulk needs) beside the panel chain on
+// the caller's stream.  A plain non-blocking stream at the default priority,
+// created on the first factorization of a handle.
 int ensure_side(hprmat_hip_lu* h) {
   if (h->side.stream) return 0;
   hipError_t e;
@@ -218,6 +218,9 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
   // two-problem launch per 256-row block forward), which the Gauss form would
   // lengthen without saving time.
   const bool g3m = false;
+  // a solve that reports statistics synchronises anyway: it may stop
+  // launching refinement work as soon as the flags say so
+  const bool poll = stats != nullptr;
   const bool fgmres = (crit & HPRMAT_FGMRES) != 0;
   crit &= ~HPRMAT_FGMRES;
   if (fgmres) {
@@ -236,25 +239,25 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
                                     h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, m,
                                     tol, crit, h->state, h->R, (double2*)h->RF, (double2*)h->RY,
                                     h->part, gw, h->plan, false, st, h->equil ? h->scale : nullptr,
-                                    sw);
+                                    sw, poll);
     else
       rc = hpr::gmres_solve<float>((const float2*)h->fac, h->ld, n, (const float2*)h->dinv256,
                                    h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, m,
                                    tol, crit, h->state, h->R, (float2*)h->RF, (float2*)h->RY,
                                    h->part, gw, h->plan, g3m, st, h->equil ? h->scale : nullptr,
-                                   sw);
+                                   sw, poll);
     final_check = 0;   // the FGMRES solve ends with a true residual evaluation
   } else if (h->method == HPRMAT_PLAIN_FP64)
     rc = hpr::refine_solve<double>((const double2*)h->fac, h->ld, n, (const double2*)h->dinv256,
                                    h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, tol,
                                    crit, h->state, h->R, (double2*)h->RF, (double2*)h->RY,
                                    h->part, h->plan, false, st, h->equil ? h->scale : nullptr,
-                                   sw);
+                                   sw, poll);
   else
     rc = hpr::refine_solve<float>((const float2*)h->fac, h->ld, n, (const float2*)h->dinv256,
                                   h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, tol,
                                   crit, h->state, h->R, (float2*)h->RF, (float2*)h->RY, h->part,
-                                  h->plan, g3m, st, h->equil ? h->scale : nullptr, sw);
+                                  h->plan, g3m, st, h->equil ? h->scale : nullptr, sw, poll);
   if (rc) return rc;
   if (final_check)
     rc = hpr::residual_check(n, nrhs, h->A64, h->lda, h->Brm, h->Xrm, h->R, h->part, h->state,

@@ -301,7 +301,7 @@ int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, cons
                 const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
                 int max_steps, int m, double tol, int crit, RefineState* state, double2* R,
                 cx<TF>* RF, cx<TF>* RY, double* part, const GmresWork& gw, const LuPlan& plan,
-                bool g3m, hipStream_t st, const double* scale, const SplitWork& sw) {
+                bool g3m, hipStream_t st, const double* scale, const SplitWork& sw, bool poll) {
   if (m < 1 || m > GMRES_MAX_M || nrhs > 1024) return -1;
   const size_t cnt = (size_t)n * nrhs;
   const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
@@ -312,6 +312,8 @@ int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, cons
   permute_cast<double, TF>(n, nrhs, perm, B, RF, nullptr, st, scale);
   if (getrs<TF>(LU, ld, n, dinv256, RF, nrhs, nrhs, RY, nullptr, plan, g3m, st)) return -1;
   widen_kernel<TF><<<eg, 256, 0, st>>>(cnt, nrhs, RF, scale, X, halt);
+  const FlagPoll fp = flag_poll(st, poll);
+  bool converged = false;
   int left = max_steps;
   while (left > 0) {
     const int mc = min(m, left);
@@ -320,6 +322,10 @@ int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, cons
     copy_kernel<<<eg, 256, 0, st>>>(cnt, B, R, done);
     if (resid_gemm(n, nrhs, A64, lda, X, R, -1.0, 1, done, sw, st)) return -1;
     residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
+    if (fp.read(state, st, &RefineState::done)) {
+      converged = true;
+      break;
+    }
     dots(n, nrhs, 1, R, 0, R, gw.h, 0, gw, done, st);
     cycle_begin_kernel<<<1, 256, 0, st>>>(nrhs, mc, gw.h, gw.g, gw.inv, state);
     colscale_kernel<<<eg, 256, 0, st>>>(cnt, nrhs, R, gw.inv, gw.V, halt);
@@ -341,6 +347,7 @@ int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, cons
       dots(n, nrhs, 1, gw.W, 0, gw.W, gw.y, 0, gw, halt, st);   // ||W||^2 into y (scratch)
       arnoldi_step_kernel<<<1, 256, 0, st>>>(nrhs, mc, j, gw.h, gw.y, gw.H, gw.cs, gw.sn, gw.g,
                                               gw.inv, state, tol);
+      if (j + 1 < mc && fp.read(state, st, &RefineState::gm_halt)) break;
       if (j + 1 < mc)
         colscale_kernel<<<eg, 256, 0, st>>>(cnt, nrhs, gw.W, gw.inv, gw.V + (size_t)(j + 1) * cnt,
                                             halt);
@@ -349,9 +356,11 @@ int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, cons
     update_x_kernel<<<eg, 256, 0, st>>>(cnt, nrhs, gw.Z, cnt, gw.y, X, state);
   }
   // final true residual
-  copy_kernel<<<eg, 256, 0, st>>>(cnt, B, R, done);
-  if (resid_gemm(n, nrhs, A64, lda, X, R, -1.0, 1, done, sw, st)) return -1;
-  residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
+  if (!converged) {
+    copy_kernel<<<eg, 256, 0, st>>>(cnt, B, R, done);
+    if (resid_gemm(n, nrhs, A64, lda, X, R, -1.0, 1, done, sw, st)) return -1;
+    residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
+  }
   HPR_CHECK(hipGetLastError());
   return 0;
 }
@@ -360,11 +369,11 @@ template int gmres_solve<double>(const cx<double>*, int64_t, int, const cx<doubl
                                  const double2*, int64_t, const double2*, double2*, int, int, int,
                                  double, int, RefineState*, double2*, cx<double>*, cx<double>*,
                                  double*, const GmresWork&, const LuPlan&, bool, hipStream_t,
-                                 const double*, const SplitWork&);
+                                 const double*, const SplitWork&, bool);
 template int gmres_solve<float>(const cx<float>*, int64_t, int, const cx<float>*, const int*,
                                 const double2*, int64_t, const double2*, double2*, int, int, int,
                                 double, int, RefineState*, double2*, cx<float>*, cx<float>*,
                                 double*, const GmresWork&, const LuPlan&, bool, hipStream_t,
-                                const double*, const SplitWork&);
+                                const double*, const SplitWork&, bool);
 
 }  // namespace hpr

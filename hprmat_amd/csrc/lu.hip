@@ -862,12 +862,37 @@ int residual_norms(int n, int nrhs, const double2* R, double* part, RefineState*
   return 0;
 }
 
+FlagPoll flag_poll(hipStream_t st, bool allowed) {
+  FlagPoll p;
+  if (!allowed) return p;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    return p;
+  }
+  p.on = cs == hipStreamCaptureStatusNone;
+  return p;
+}
+
+// The flag after the work queued so far (0 when polling is off or the copy
+// fails: the device flags still gate the remaining launches).
+int FlagPoll::read(const RefineState* st, hipStream_t s, int RefineState::*field) const {
+  if (!on) return 0;
+  int v = 0;
+  if (hipMemcpyAsync(&v, &(st->*field), sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return v;
+}
+
 template <class TF>
 int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, const int* perm,
                  const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
                  int max_refine, double tol, int crit, RefineState* state, double2* R, cx<TF>* RF,
                  cx<TF>* RY, double* part, const LuPlan& plan, bool g3m, hipStream_t st,
-                 const double* scale, const SplitWork& sw) {
+                 const double* scale, const SplitWork& sw, bool poll) {
   const size_t cnt = (size_t)n * nrhs;
   const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
   const int* done = &state->done;
@@ -875,10 +900,12 @@ int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, con
   permute_cast<double, TF>(n, nrhs, perm, B, RF, nullptr, st, scale);
   if (getrs<TF>(LU, ld, n, dinv256, RF, nrhs, nrhs, RY, nullptr, plan, g3m, st)) return -1;
   add_correction_kernel<TF><<<eg, 256, 0, st>>>(nrhs, cnt, X, RF, scale, 0, state);
+  const FlagPoll fp = flag_poll(st, poll);
   for (int it = 0; it < max_refine; ++it) {
     cast_kernel<double, double><<<eg, 256, 0, st>>>(cnt, B, R, done);
     if (resid_gemm(n, nrhs, A64, lda, X, R, -1.0, 1, done, sw, st)) return -1;
     residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
+    if (fp.read(state, st, &RefineState::done)) break;
     permute_cast<double, TF>(n, nrhs, perm, R, RF, done, st, scale);
     if (getrs<TF>(LU, ld, n, dinv256, RF, nrhs, nrhs, RY, done, plan, g3m, st)) return -1;
     add_correction_kernel<TF><<<eg, 256, 0, st>>>(nrhs, cnt, X, RF, scale, 1, state);
@@ -1226,7 +1253,7 @@ int solve_batched(const BatchArgs<TF>& b, const double2* A64, int64_t lda, int64
                                const double2*, int64_t, const double2*, double2*, int, int,    \
                                double, int, RefineState*, double2*, cx<T>*, cx<T>*, double*,   \
                                const LuPlan&, bool, hipStream_t, const double*,               \
-                               const SplitWork&);                                              \
+                               const SplitWork&, bool);                                        \
   template int batch_bind<T>(BatchArgs<T>&, void*, size_t, int, int, int, bool);              \
   template int getrf_batched<T>(const BatchArgs<T>&, hipStream_t);                             \
   template int getrs_batched<T>(const BatchArgs<T>&, cx<T>*, cx<T>*, int, const int*, int,     \

@@ -41,6 +41,19 @@ struct RefineState {
 
 enum Criterion { CRIT_ABS_MAX = 0, CRIT_REL_FRO = 1 };
 
+// Host view of the refinement flags.  When the caller allows it (a solve
+// that synchronises anyway) and the stream is not being captured, the
+// refinement loops read `done` / `gm_halt` back after each sweep or Arnoldi
+// step and stop launching once they are set: one synchronisation per step,
+// against milliseconds of FP64 work per step at the sizes that refine, and
+// no trail of no-op launches up to the sweep budget.  Otherwise the device
+// flags alone turn the remaining launches into no-ops.
+struct FlagPoll {
+  bool on = false;
+  int read(const RefineState* st, hipStream_t s, int RefineState::*field) const;
+};
+FlagPoll flag_poll(hipStream_t st, bool allowed);
+
 size_t scratch_bytes();
 void scratch_bind(Scratch& s, void* base);
 
@@ -148,7 +161,8 @@ int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, con
                  const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
                  int max_refine, double tol, int crit, RefineState* state, double2* R,
                  cx<TF>* RF, cx<TF>* RY, double* part, const LuPlan& plan, bool gauss3m,
-                 hipStream_t st, const double* scale, const SplitWork& sw = SplitWork());
+                 hipStream_t st, const double* scale, const SplitWork& sw = SplitWork(),
+                 bool poll = false);
 
 // One residual evaluation of a refined X (skipped when state->done is set):
 // R = B - A64 X, norms and the stopping rule recorded in state.
@@ -187,14 +201,15 @@ void gmres_work_bind(GmresWork& w, void* base, int n, int nrhs, int m);
 // products and orthogonalization).  Arguments as refine_solve; max_steps
 // bounds the preconditioned matrix-vector products (cycles of m steps, each
 // cycle starting from the true residual B - A X).  X ends with a true
-// residual evaluation recorded in state.  Nothing synchronises with the host.
+// residual evaluation recorded in state.  Nothing synchronises with the host
+// unless `poll` is set (see FlagPoll).
 template <class TF>
 int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, const int* perm,
                 const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
                 int max_steps, int m, double tol, int crit, RefineState* state, double2* R,
                 cx<TF>* RF, cx<TF>* RY, double* part, const GmresWork& gw, const LuPlan& plan,
                 bool gauss3m, hipStream_t st, const double* scale,
-                const SplitWork& sw = SplitWork());
+                const SplitWork& sw = SplitWork(), bool poll = false);
 
 // ---- uniform batches ------------------------------------------------------
 // `count` independent systems of one order n, each at a fixed element
