@@ -39,7 +39,7 @@ NATIVE_SYMBOLS = (
     'hprmat_hip_zgemm', 'hprmat_hip_prof_enable', 'hprmat_hip_prof_read', 'hprmat_hip_version',
     'hprmat_hip_batch_work_bytes', 'hprmat_hip_batch_solve', 'hprmat_hip_assemble_batch',
     'hprmat_hip_r_to_s_batch', 'hprmat_hip_propagate_work_bytes', 'hprmat_hip_propagate',
-    'hprmat_hip_mesh', 'hprmat_hip_add_nonlocal',
+    'hprmat_hip_mesh', 'hprmat_hip_add_nonlocal', 'hprmat_hip_lu_set_operator',
 )
 PHYSICS_SYMBOLS = (
     'hprmat_init', 'hprmat_set_solver', 'hprmat_get_solver', 'hprmat_solve',
@@ -126,6 +126,8 @@ def load():
                                          DPtr, DPtr, DPtr, DPtr, I, DPtr, DPtr, I, DPtr, DPtr,
                                          DPtr, L, L, DPtr, DPtr, ctypes.c_size_t, P]
     lib.hprmat_hip_mesh.argtypes = [I, I, D, P, P, P, P, P, P]
+    lib.hprmat_hip_lu_set_operator.argtypes = [P, I, I, DPtr, DPtr, DPtr, D, DPtr, DPtr, DPtr,
+                                               DPtr]
     lib.hprmat_hip_add_nonlocal.argtypes = [I, I, DPtr, D, DPtr, DPtr, L, P]
     lib.hprmat_hip_version.restype = ctypes.c_char_p
     lib.hprmat_hip_prof_enable.argtypes = [I]

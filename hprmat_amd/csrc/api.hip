@@ -51,6 +51,7 @@ struct hprmat_hip_lu {
   RefineState* state = nullptr;
   hpr::LuPlan plan;
   hpr::SideStream side;       // look-ahead stream and its two events
+  hpr::MeshOperator mop;      // the factored matrix by its Lagrange-mesh parts (nch = 0: none)
   int lookahead = 1;
   int gm_cycle = 0;           // FGMRES cycle length (0: GMRES_DEFAULT_CYCLE)
 };
@@ -205,7 +206,8 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
   // sized for the largest slice count, so that a warm-up on the widest
   // system leaves nothing to allocate for smaller ones, e.g. under capture)
   hpr::SplitWork sw;
-  sw.splits = hpr::resid_splits(n, nrhs);
+  if (h->mop.nch > 0 && (int64_t)h->mop.nch * h->mop.nr == n) sw.mesh = &h->mop;
+  sw.splits = sw.mesh ? 1 : hpr::resid_splits(n, nrhs);
   if (sw.splits > 1) {
     if (grow(&h->kpart, &h->kpart_cap, (size_t)hpr::RESID_MAX_SPLITS * cnt * 16)) return -2;
     sw.part = (double2*)h->kpart;
@@ -509,6 +511,33 @@ int hprmat_hip_lu_set_plan(hprmat_hip_lu_t h, int key, int value) {
     default:
       return -1;
   }
+}
+
+int hprmat_hip_lu_set_operator(hprmat_hip_lu_t h, int nch, int nr, const double* tc,
+                               const double* blo0, const double* r, double rmax0,
+                               const int* lval, const double* qk, const double* xc1,
+                               const void* cpot) {
+  if (!h) return -1;
+  if (!tc) {   // back to the dense matrix
+    h->mop = hpr::MeshOperator();
+    return 0;
+  }
+  if (nch <= 0 || nr <= 0 || nr > 65535 || nch > 65535 || !blo0 || !r || !(rmax0 > 0.0) ||
+      !lval || !qk || !xc1 || !cpot)
+    return -1;
+  hpr::MeshOperator m;
+  m.nch = nch;
+  m.nr = nr;
+  m.tc = tc;
+  m.blo0 = blo0;
+  m.r = r;
+  m.rmax0 = rmax0;
+  m.lval = lval;
+  m.qk = qk;
+  m.xc1 = xc1;
+  m.cpot = (const double2*)cpot;
+  h->mop = m;
+  return 0;
 }
 
 int hprmat_hip_lu_destroy(hprmat_hip_lu_t h) {

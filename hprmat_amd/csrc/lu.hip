@@ -452,6 +452,10 @@ int resid_splits(int n, int nrhs) {
 int resid_gemm(int n, int nrhs, const double2* A64, int64_t lda, const double2* X, double2* R,
                double alpha, int beta, const int* skip, const SplitWork& sw, hipStream_t st) {
   if (n <= 0 || nrhs <= 0) return 0;
+  if (sw.mesh) {
+    if ((int64_t)sw.mesh->nch * sw.mesh->nr != n) return -1;
+    return mesh_apply(*sw.mesh, nrhs, X, R, alpha, beta, skip, st);
+  }
   if (sw.splits <= 1 || !sw.part)
     return gemm<double>(n, nrhs, n, A64, lda, true, X, nrhs, R, nrhs, alpha, beta, false, skip,
                         st);

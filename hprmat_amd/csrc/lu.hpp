@@ -137,16 +137,38 @@ int gemm(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<
 // are multiplied by separate workgroups into `part` (splits x n x nrhs) and
 // summed in a fixed order, so a product that has too few output tiles to
 // fill the GPU still runs at full occupancy and results stay reproducible.
+// The Lagrange-mesh matrix described by its parts (device arrays, as for
+// hprmat_hip_assemble): channel blocks from tc, blo0, r, rmax0, lval, qk,
+// xc1 and the couplings cpot[i][c][c'] (nr x nch x nch complex).
+struct MeshOperator {
+  int nch = 0, nr = 0;
+  const double* tc = nullptr;
+  const double* blo0 = nullptr;
+  const double* r = nullptr;
+  double rmax0 = 1.0;
+  const int* lval = nullptr;
+  const double* qk = nullptr;
+  const double* xc1 = nullptr;
+  const double2* cpot = nullptr;
+};
+// R = beta R + alpha C X (X, R row-major N x nrhs, N = nch nr) from the parts
+// of C, skipped when *skip is set.
+int mesh_apply(const MeshOperator& op, int nrhs, const double2* X, double2* R, double alpha,
+               int beta, const int* skip, hipStream_t st);
+
+// How the FP64 refinement product is formed: split-K partial sums, or, with
+// `mesh`, the Lagrange-mesh operator instead of the dense matrix.
 struct SplitWork {
   double2* part = nullptr;
   int splits = 1;
+  const MeshOperator* mesh = nullptr;
 };
 // Number of K slices for the product of an n x n matrix with nrhs columns
 // (1: no split); resid_splits(n, nrhs) * n * nrhs complex doubles of `part`.
 int resid_splits(int n, int nrhs);
 constexpr int RESID_MAX_SPLITS = 8;
 // R = beta R + alpha A64 X (skipped when *skip is set), split-K when sw has
-// more than one slice and a partial buffer.
+// more than one slice and a partial buffer; through sw.mesh when it is set.
 int resid_gemm(int n, int nrhs, const double2* A64, int64_t lda, const double2* X, double2* R,
                double alpha, int beta, const int* skip, const SplitWork& sw, hipStream_t st);
 

@@ -76,6 +76,33 @@ class LU:
                                                          int(bool(lookahead))), 'set_plan')
         return self
 
+    def set_operator(self, dmesh, lval, qk, xc1, cpot):
+        """Form the refinement's FP64 products from the Lagrange-mesh parts of
+        the factored matrix (hprmat_hip_lu_set_operator): dmesh a
+        rmatrix.DeviceMesh, lval (int32), qk, xc1 (float64) and cpot
+        ((nr, nch, nch) complex128) device tensors; None clears.  The tensors
+        are kept alive by the handle."""
+        if dmesh is None:
+            _lib.check(self._lib.hprmat_hip_lu_set_operator(self._h, 0, 0, None, None, None, 1.0,
+                                                            None, None, None, None),
+                       'hprmat_hip_lu_set_operator')
+            self._op = None
+            return self
+        nch, nr = int(lval.numel()), int(dmesh.mesh.nr)
+        for t, dt in ((lval, torch.int32), (qk, torch.float64), (xc1, torch.float64),
+                      (cpot, torch.complex128)):
+            if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != dt or not t.is_contiguous():
+                raise TypeError('operator parts must be contiguous CUDA tensors of the stated types')
+        if cpot.shape != (nr, nch, nch) or qk.numel() != nch or xc1.numel() != nch:
+            raise ValueError('operator parts do not match (nr, nch)')
+        p = lambda t: ctypes.c_void_p(t.data_ptr())
+        rc = self._lib.hprmat_hip_lu_set_operator(self._h, nch, nr, p(dmesh.tc), p(dmesh.blo0),
+                                                  p(dmesh.r), float(dmesh.mesh.a), p(lval), p(qk),
+                                                  p(xc1), p(cpot))
+        _lib.check(rc, 'hprmat_hip_lu_set_operator')
+        self._op = (dmesh, lval, qk, xc1, cpot)
+        return self
+
     def factor(self, a_cm, method='fp64', equilibrate=False):
         """Factor the matrix whose column-major storage is ``a_cm`` (n x n).
 

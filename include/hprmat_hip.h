@@ -144,6 +144,20 @@ int hprmat_hip_lu_destroy(hprmat_hip_lu_t h);
 enum { HPRMAT_PLAN_TRAIL_TILE = 1, HPRMAT_PLAN_LOOKAHEAD = 2, HPRMAT_PLAN_GMRES_CYCLE = 3 };
 int hprmat_hip_lu_set_plan(hprmat_hip_lu_t h, int key, int value);
 
+/* No reference counterpart: describe the factored matrix by its
+   Lagrange-mesh parts, the device inputs of hprmat_hip_assemble (one
+   interval; no non-local potential).  The refinement's FP64 products C X
+   are then formed from the channel blocks and the couplings (8 N (nr + nch)
+   nrhs flops) instead of the dense matrix (8 N^2 nrhs); the factors, the
+   substitutions and the stopping rule are unchanged.  Applies to solves
+   with nch * nr = n; the arrays must stay valid and describe the matrix
+   that was factored.  tc = NULL returns to the dense matrix.  Returns -1 on
+   invalid arguments. */
+int hprmat_hip_lu_set_operator(hprmat_hip_lu_t h, int nch, int nr, const double *tc,
+                               const double *blo0, const double *r, double rmax0,
+                               const int *lval, const double *qk, const double *xc1,
+                               const void *cpot);
+
 /* Factor the n x n complex FP64 matrix at device pointer dA (column-major,
    lda) in the given precision.  dA is not modified and must stay valid while
    the handle is used for refined solves (the FP64 residual reads it).
