@@ -257,11 +257,27 @@ template <class T> bool upload(T* d, const T* h, size_t count) {
   return hipMemcpyAsync(d, h, count * sizeof(T), hipMemcpyHostToDevice, g_stream) == hipSuccess;
 }
 
+// The Lagrange-mesh parts of one interval's matrix (device arrays): the
+// refinement's FP64 products are formed from them instead of the dense
+// matrix (hprmat_hip_lu_set_operator).
+struct MeshParts {
+  int nch = 0, nr = 0;
+  const double *tc = nullptr, *bloch = nullptr, *r = nullptr;
+  const int* lval = nullptr;
+  const double *qk = nullptr, *xc1 = nullptr;
+  const void* cpot = nullptr;
+};
+
 // Factor the column-major N x N dA and solve for the N x nrhs dX in place.
-int factor_solve(const SolverPlan& p, const void* dA, int n, void* dX, int nrhs) {
+int factor_solve(const SolverPlan& p, const void* dA, int n, void* dX, int nrhs,
+                 const MeshParts* mp = nullptr) {
   hprmat_hip_lu_t h = nullptr;
   if (hprmat_hip_lu_create(&h) != 0) return -1;
-  int rc = hprmat_hip_lu_factor(h, p.method, dA, n, n, g_stream);
+  int rc = 0;
+  if (mp)   // the Bloch matrix carries its radius factors: unit scale
+    rc = hprmat_hip_lu_set_operator(h, mp->nch, mp->nr, mp->tc, mp->bloch, mp->r, 1.0, mp->lval,
+                                    mp->qk, mp->xc1, mp->cpot);
+  if (!rc) rc = hprmat_hip_lu_factor(h, p.method, dA, n, n, g_stream);
   hprmat_hip_stats st;
   if (!rc) rc = hprmat_hip_lu_solve(h, dX, n, nrhs, p.max_refine, p.tol, p.criterion, 1, g_stream,
                                     &st);
@@ -475,11 +491,22 @@ bool physics_solve(const char* who, int nch, const int* lval, const double* qk, 
   if (ncl > 0) ok = ok && upload(dclosed, closed.data(), (size_t)ncl);
   // each interval: assemble (the Bloch matrix already carries its radius
   // factors, hence the unit scale) and solve; the intervals are independent
-  for (int s = 0; ok && s < ns; ++s)
-    ok = !hprmat_hip_assemble(nch, nr, dtc + s * nn2, dbl + s * nn2, dr + (size_t)s * nr, 1.0, dl,
-                              dqk, dxc, dpot + (size_t)s * nr * nch * nch, dA, n, g_stream) &&
+  for (int s = 0; ok && s < ns; ++s) {
+    MeshParts mp;
+    mp.nch = nch;
+    mp.nr = nr;
+    mp.tc = dtc + s * nn2;
+    mp.bloch = dbl + s * nn2;
+    mp.r = dr + (size_t)s * nr;
+    mp.lval = dl;
+    mp.qk = dqk;
+    mp.xc1 = dxc;
+    mp.cpot = dpot + (size_t)s * nr * nch * nch;
+    ok = !hprmat_hip_assemble(nch, nr, mp.tc, mp.bloch, mp.r, 1.0, dl, dqk, dxc,
+                              (const zd*)mp.cpot, dA, n, g_stream) &&
          (!nonlocal || !hprmat_hip_add_nonlocal(nch, nr, dw, m.h, dknl, dA, n, g_stream)) &&
-         !factor_solve(plan, dA, n, dX + (size_t)s * n * nrhs, nrhs);
+         !factor_solve(plan, dA, n, dX + (size_t)s * n * nrhs, nrhs, nonlocal ? nullptr : &mp);
+  }
   if (ok) {
     if (chain)
       ok = !hprmat_hip_propagate(nch, nr, ns, dX, n, (int64_t)n * nrhs, dq1, dq2, 1.0, m.h, dqk, no,

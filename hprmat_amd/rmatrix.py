@@ -150,14 +150,19 @@ SOLVER_TYPES = {
 }
 
 
-def solve_colmajor(a_cm, b_cm, solver_type=1, max_refine=0):
-    """Solve C X = B in place (column-major storages) for a solver_type."""
+def solve_colmajor(a_cm, b_cm, solver_type=1, max_refine=0, operator=None):
+    """Solve C X = B in place (column-major storages) for a solver_type.
+    operator: optional Lagrange-mesh parts of C, the arguments of
+    solver.LU.set_operator_parts, for the refinement's FP64 products."""
     if solver_type not in SOLVER_TYPES:
         raise NotImplementedError('solver_type %r is not provided by libhprmat_hip' % (solver_type,))
     method, mr, tol, crit, eq, refinement = SOLVER_TYPES[solver_type]
     if mr is None:
         mr = max_refine
-    lu = solver.LU().factor(a_cm, method, eq)
+    lu = solver.LU()
+    if operator is not None:
+        lu.set_operator_parts(*operator)
+    lu.factor(a_cm, method, eq)
     st = lu.solve_(b_cm, mr, tol, crit, final_check=eq, refinement=refinement)
     if st['info'] != 0:
         raise RuntimeError('zero pivot at column %d' % st['info'])
@@ -255,18 +260,36 @@ def _add_nonlocal(nr, w, a0, cpnl, a_cm):
     return a_cm
 
 
-def rmatrix(lval, qk, a, nr, cpot, solver_type=1, max_refine=0, dmesh=None, cpnl=None):
+def _operator_parts(tc, blo0, r, rmax0, lval, qk, xc1, cpot):
+    """Device tensors of the Lagrange-mesh operator (solver.LU.set_operator_parts)."""
+    def dev(v, dt, npdt):
+        if isinstance(v, torch.Tensor) and v.is_cuda:
+            return v.to(dt).contiguous()
+        return _dev(np.asarray(v, dtype=npdt), dt)
+    return (tc, blo0, r, float(rmax0), dev(lval, torch.int32, np.int32),
+            dev(qk, torch.float64, np.float64), dev(xc1, torch.float64, np.float64),
+            dev(cpot, torch.complex128, np.complex128))
+
+
+def rmatrix(lval, qk, a, nr, cpot, solver_type=1, max_refine=0, dmesh=None, cpnl=None,
+            structured=True):
     """ns = 1 pipeline on the GPU: returns dict(S, R, nopen, stats).
-    cpnl: optional non-local potential (nr, nr, nch, nch)."""
+    cpnl: optional non-local potential (nr, nr, nch, nch).  structured: the
+    refinement's FP64 products use the Lagrange-mesh parts of C instead of
+    the dense matrix (local potentials only)."""
     if dmesh is None:
         dmesh = DeviceMesh(rmat_ini(nr, a))
     nch = len(lval)
     open_idx, co, cop, xc1 = channel_data(lval, qk, a)
-    a_cm = assemble(dmesh, lval, qk, xc1, cpot)
+    cp = cpot if isinstance(cpot, torch.Tensor) else _dev(np.ascontiguousarray(cpot), torch.complex128)
+    a_cm = assemble(dmesh, lval, qk, xc1, cp)
     if cpnl is not None:
         add_nonlocal(dmesh, cpnl, a_cm)
+    op = None
+    if structured and cpnl is None:
+        op = _operator_parts(dmesh.tc, dmesh.blo0, dmesh.r, dmesh.mesh.a, lval, qk, xc1, cp)
     b_cm = rhs_colmajor(dmesh, nch)
-    st = solve_colmajor(a_cm, b_cm, solver_type, max_refine)
+    st = solve_colmajor(a_cm, b_cm, solver_type, max_refine, operator=op)
     r, s = r_to_s(dmesh, b_cm, nch, qk, open_idx, co, cop)
     return dict(S=s, R=r, nopen=len(open_idx), stats=st, open=open_idx)
 
@@ -283,7 +306,7 @@ def closed_w(lval, qk, a):
 
 
 def rmatrix_intervals(lval, qk, a, nr, ns, cpot, solver_type=1, nvc=None, max_refine=0,
-                      cpnl=None):
+                      cpnl=None, structured=True):
     """rmatrix (src/rmatrix_hp.F90:175-431) for ns >= 1 intervals on the GPU.
 
     Every interval's system C_s is assembled on the device and solved for
@@ -293,6 +316,8 @@ def rmatrix_intervals(lval, qk, a, nr, ns, cpot, solver_type=1, nvc=None, max_re
     among the open channels), the internal wave functions and bound-state
     amplitudes follow in hprmat_hip_propagate.  cpot: (nr ns, nch, nch);
     cpnl: optional non-local potential (nr, nr, nch, nch), ns = 1 only.
+    structured: the refinement's FP64 products use the Lagrange-mesh parts
+    of each C_s instead of the dense matrix (local potentials only).
     nch <= 64.  Returns dict(S, R, nopen, cf (nr ns, nch, len(nvc)) or None,
     amps (nclo, len(nvc)) or None, X (ns, 2 nch, N) column-major solutions).
     """
@@ -331,9 +356,12 @@ def rmatrix_intervals(lval, qk, a, nr, ns, cpot, solver_type=1, nvc=None, max_re
             ctypes.c_void_p(xd.data_ptr()), ctypes.c_void_p(cp[s * nr].data_ptr()),
             ctypes.c_void_p(a_cm.data_ptr()), ctypes.c_int64(n), solver._stream_ptr())
         _lib.check(rc, 'hprmat_hip_assemble')
+        op = None
         if cpnl is not None:
             _add_nonlocal(nr, m['wle'], h, cpnl, a_cm)
-        st.append(solve_colmajor(a_cm, x[s], solver_type, max_refine))
+        elif structured:
+            op = _operator_parts(tc[s], bl[s], rr[s], 1.0, lv, qd, xd, cp[s * nr:(s + 1) * nr])
+        st.append(solve_colmajor(a_cm, x[s], solver_type, max_refine, operator=op))
     nc = 0 if nvc is None else len(nvc)
     if nc and nopen == 0:
         raise ValueError('wave functions need an open entrance channel')

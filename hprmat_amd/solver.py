@@ -88,19 +88,26 @@ class LU:
                        'hprmat_hip_lu_set_operator')
             self._op = None
             return self
-        nch, nr = int(lval.numel()), int(dmesh.mesh.nr)
-        for t, dt in ((lval, torch.int32), (qk, torch.float64), (xc1, torch.float64),
+        return self.set_operator_parts(dmesh.tc, dmesh.blo0, dmesh.r, dmesh.mesh.a, lval, qk, xc1,
+                                       cpot)
+
+    def set_operator_parts(self, tc, blo0, r, rmax0, lval, qk, xc1, cpot):
+        """set_operator from the raw parts of one interval (the arguments of
+        hprmat_hip_assemble as float64 / int32 / complex128 CUDA tensors)."""
+        nch, nr = int(lval.numel()), int(r.numel())
+        for t, dt in ((tc, torch.float64), (blo0, torch.float64), (r, torch.float64),
+                      (lval, torch.int32), (qk, torch.float64), (xc1, torch.float64),
                       (cpot, torch.complex128)):
             if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != dt or not t.is_contiguous():
                 raise TypeError('operator parts must be contiguous CUDA tensors of the stated types')
-        if cpot.shape != (nr, nch, nch) or qk.numel() != nch or xc1.numel() != nch:
+        if (tc.numel() != nr * nr or blo0.numel() != nr * nr or cpot.shape != (nr, nch, nch)
+                or qk.numel() != nch or xc1.numel() != nch):
             raise ValueError('operator parts do not match (nr, nch)')
         p = lambda t: ctypes.c_void_p(t.data_ptr())
-        rc = self._lib.hprmat_hip_lu_set_operator(self._h, nch, nr, p(dmesh.tc), p(dmesh.blo0),
-                                                  p(dmesh.r), float(dmesh.mesh.a), p(lval), p(qk),
-                                                  p(xc1), p(cpot))
+        rc = self._lib.hprmat_hip_lu_set_operator(self._h, nch, nr, p(tc), p(blo0), p(r),
+                                                  float(rmax0), p(lval), p(qk), p(xc1), p(cpot))
         _lib.check(rc, 'hprmat_hip_lu_set_operator')
-        self._op = (dmesh, lval, qk, xc1, cpot)
+        self._op = (tc, blo0, r, lval, qk, xc1, cpot)
         return self
 
     def factor(self, a_cm, method='fp64', equilibrate=False):
