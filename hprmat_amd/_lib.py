@@ -40,12 +40,22 @@ NATIVE_SYMBOLS = (
     'hprmat_hip_batch_work_bytes', 'hprmat_hip_batch_solve', 'hprmat_hip_assemble_batch',
     'hprmat_hip_r_to_s_batch', 'hprmat_hip_propagate_work_bytes', 'hprmat_hip_propagate',
     'hprmat_hip_mesh', 'hprmat_hip_add_nonlocal', 'hprmat_hip_lu_set_operator',
+    'hprmat_hip_batch_solve_mesh',
 )
 PHYSICS_SYMBOLS = (
     'hprmat_init', 'hprmat_set_solver', 'hprmat_get_solver', 'hprmat_solve',
     'hprmat_solve_linear', 'hprmat_version', 'hprmat_solver_info', 'hprmat_solve_full',
     'hprmat_wavefunction',
 )
+
+
+class MeshParts(ctypes.Structure):
+    """hprmat_hip_mesh_parts: the Lagrange-mesh parts of a batch of matrices
+    of one partial wave (device pointers)."""
+    _fields_ = [('nch', ctypes.c_int), ('nr', ctypes.c_int), ('tc', ctypes.c_void_p),
+                ('blo0', ctypes.c_void_p), ('r', ctypes.c_void_p), ('rmax0', ctypes.c_double),
+                ('lval', ctypes.c_void_p), ('qk', ctypes.c_void_p), ('xc1', ctypes.c_void_p),
+                ('stride_q', ctypes.c_int64), ('cpot', ctypes.c_void_p)]
 
 
 class Stats(ctypes.Structure):
@@ -116,6 +126,9 @@ def load():
     lib.hprmat_hip_batch_work_bytes.restype = ctypes.c_size_t
     lib.hprmat_hip_batch_solve.argtypes = [I, I, I, I, DPtr, L, L, DPtr, L, L, I, D, I, DPtr,
                                            DPtr, DPtr, ctypes.c_size_t, P]
+    lib.hprmat_hip_batch_solve_mesh.argtypes = [I, I, I, I, DPtr, L, L, DPtr, L, L, I, D, I,
+                                                DPtr, DPtr, DPtr, ctypes.c_size_t,
+                                                ctypes.POINTER(MeshParts), P]
     lib.hprmat_hip_assemble_batch.argtypes = [I, I, I, DPtr, DPtr, DPtr, D, DPtr, DPtr, DPtr,
                                               DPtr, DPtr, L, L, P]
     lib.hprmat_hip_r_to_s_batch.argtypes = [I, I, I, DPtr, L, L, DPtr, D, D, DPtr, DPtr, DPtr,

@@ -594,10 +594,11 @@ size_t hprmat_hip_batch_work_bytes(int method, int n, int nrhs, int count) {
   return hpr::batch_work_bytes(n, nrhs, count, factor_elem_bytes(method));
 }
 
-int hprmat_hip_batch_solve(int method_flags, int n, int nrhs, int count, const void* dA,
-                           int64_t lda, int64_t stride_a, void* dB, int64_t ldb, int64_t stride_b,
-                           int max_refine, double tol, int criterion, double* res_rel,
-                           int* status, void* work, size_t work_bytes, void* stream) {
+namespace {
+int batch_solve_impl(int method_flags, int n, int nrhs, int count, const void* dA, int64_t lda,
+                     int64_t stride_a, void* dB, int64_t ldb, int64_t stride_b, int max_refine,
+                     double tol, int criterion, double* res_rel, int* status, void* work,
+                     size_t work_bytes, void* stream, const hpr::MeshOperator* mesh) {
   const int method = method_flags & 0xff;
   const bool equil = (method_flags & HPRMAT_EQUILIBRATE) != 0;
   if (method != HPRMAT_PLAIN_FP64 && method != HPRMAT_MIXED_FP32) {
@@ -619,13 +620,49 @@ int hprmat_hip_batch_solve(int method_flags, int n, int nrhs, int count, const v
     if (rc) return rc;
     return hpr::solve_batched<double>(b, (const double2*)dA, lda, stride_a, (double2*)dB, ldb,
                                       stride_b, nrhs, max_refine, tol, criterion, res_rel, status,
-                                      st);
+                                      st, mesh);
   }
   hpr::BatchArgs<float> b;
   const int rc = hpr::batch_bind<float>(b, work, work_bytes, n, nrhs, count, equil);
   if (rc) return rc;
   return hpr::solve_batched<float>(b, (const double2*)dA, lda, stride_a, (double2*)dB, ldb,
-                                   stride_b, nrhs, max_refine, tol, criterion, res_rel, status, st);
+                                   stride_b, nrhs, max_refine, tol, criterion, res_rel, status, st,
+                                   mesh);
+}
+}  // namespace
+
+int hprmat_hip_batch_solve(int method_flags, int n, int nrhs, int count, const void* dA,
+                           int64_t lda, int64_t stride_a, void* dB, int64_t ldb, int64_t stride_b,
+                           int max_refine, double tol, int criterion, double* res_rel,
+                           int* status, void* work, size_t work_bytes, void* stream) {
+  return batch_solve_impl(method_flags, n, nrhs, count, dA, lda, stride_a, dB, ldb, stride_b,
+                          max_refine, tol, criterion, res_rel, status, work, work_bytes, stream,
+                          nullptr);
+}
+
+int hprmat_hip_batch_solve_mesh(int method_flags, int n, int nrhs, int count, const void* dA,
+                                int64_t lda, int64_t stride_a, void* dB, int64_t ldb,
+                                int64_t stride_b, int max_refine, double tol, int criterion,
+                                double* res_rel, int* status, void* work, size_t work_bytes,
+                                const hprmat_hip_mesh_parts* parts, void* stream) {
+  if (!parts || parts->nch <= 0 || parts->nr <= 0 || (int64_t)parts->nch * parts->nr != n ||
+      !parts->tc || !parts->blo0 || !parts->r || !(parts->rmax0 > 0.0) || !parts->lval ||
+      !parts->qk || !parts->xc1 || !parts->cpot || parts->stride_q < (count > 1 ? parts->nch : 0))
+    return -1;
+  hpr::MeshOperator m;
+  m.nch = parts->nch;
+  m.nr = parts->nr;
+  m.tc = parts->tc;
+  m.blo0 = parts->blo0;
+  m.r = parts->r;
+  m.rmax0 = parts->rmax0;
+  m.lval = parts->lval;
+  m.qk = parts->qk;
+  m.xc1 = parts->xc1;
+  m.cpot = (const double2*)parts->cpot;
+  m.sq = parts->stride_q;
+  return batch_solve_impl(method_flags, n, nrhs, count, dA, lda, stride_a, dB, ldb, stride_b,
+                          max_refine, tol, criterion, res_rel, status, work, work_bytes, stream, &m);
 }
 
 int hprmat_hip_prof_enable(int on) { return hpr::prof_enable(on); }

@@ -1196,9 +1196,11 @@ int getrs_batched(const BatchArgs<T>& b, cx<T>* B, cx<T>* Y, int nrhs, const int
 template <class TF>
 int solve_batched(const BatchArgs<TF>& b, const double2* A64, int64_t lda, int64_t sA,
                   double2* dB, int64_t ldb, int64_t sB, int nrhs, int max_refine, double tol,
-                  int crit, double* res_out, int* conv_out, hipStream_t st) {
+                  int crit, double* res_out, int* conv_out, hipStream_t st,
+                  const MeshOperator* mesh) {
   const int n = b.n, count = b.count;
   if (n <= 0 || count <= 0) return 0;
+  if (mesh && (int64_t)mesh->nch * mesh->nr != n) return -1;
   if (nrhs <= 0 || lda < n || ldb < n || max_refine < 0) return -1;
   const int64_t sV = b.sV;
   const size_t cnt = (size_t)n * nrhs;
@@ -1227,8 +1229,13 @@ int solve_batched(const BatchArgs<TF>& b, const double2* A64, int64_t lda, int64
   // a converged system skips every later launch; the last evaluation only checks
   for (int it = 0; it <= max_refine; ++it) {
     cast_kernel<double, double><<<eg, 256, 0, st>>>(cnt, b.Brm, b.R, done, sV, sDone);
-    gemm_b<double>(n, nrhs, n, A64, lda, true, b.Xrm, nrhs, b.R, nrhs, -1.0, 1, done, sDone,
-                   count, sA, sV, sV, st);
+    if (mesh) {
+      if (mesh_apply_batched(*mesh, count, nrhs, b.Xrm, sV, b.R, sV, -1.0, 1, done, sDone, st))
+        return -1;
+    } else {
+      gemm_b<double>(n, nrhs, n, A64, lda, true, b.Xrm, nrhs, b.R, nrhs, -1.0, 1, done, sDone,
+                     count, sA, sV, sV, st);
+    }
     norms_b(n, nrhs, b.R, sV, b.part, b.state, 1, crit, tol, done, sDone, count, st);
     if (it == max_refine) break;
     permute_cast_kernel<double, TF><<<eg, 256, 0, st>>>(n, nrhs, b.perm, b.R, b.RF, b.scale, done,
@@ -1264,7 +1271,7 @@ int solve_batched(const BatchArgs<TF>& b, const double2* A64, int64_t lda, int64
                                 hipStream_t);                                                  \
   template int solve_batched<T>(const BatchArgs<T>&, const double2*, int64_t, int64_t,         \
                                 double2*, int64_t, int64_t, int, int, double, int, double*,    \
-                                int*, hipStream_t);
+                                int*, hipStream_t, const MeshOperator*);
 INST(double)
 INST(float)
 template int transpose_cast<double, double>(int, int, const cx<double>*, int64_t, cx<double>*, int64_t, hipStream_t, const double*, const double*);

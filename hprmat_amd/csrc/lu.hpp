@@ -150,11 +150,17 @@ struct MeshOperator {
   const double* qk = nullptr;
   const double* xc1 = nullptr;
   const double2* cpot = nullptr;
+  int64_t sq = 0;   // batches: stride of the per-system rows of qk and xc1
 };
 // R = beta R + alpha C X (X, R row-major N x nrhs, N = nch nr) from the parts
 // of C, skipped when *skip is set.
 int mesh_apply(const MeshOperator& op, int nrhs, const double2* X, double2* R, double alpha,
                int beta, const int* skip, hipStream_t st);
+// The same for `count` systems of one partial wave (X, R at strides sX, sR;
+// qk / xc1 rows at stride op.sq; system e skipped when skip[e sSkip] is set).
+int mesh_apply_batched(const MeshOperator& op, int count, int nrhs, const double2* X,
+                       int64_t sX, double2* R, int64_t sR, double alpha, int beta,
+                       const int* skip, int sSkip, hipStream_t st);
 
 // How the FP64 refinement product is formed: split-K partial sums, or, with
 // `mesh`, the Lagrange-mesh operator instead of the dense matrix.
@@ -276,10 +282,13 @@ int getrs_batched(const BatchArgs<T>& b, cx<T>* B, cx<T>* Y, int nrhs, const int
 // stride sA), dB column-major right-hand sides (ldb, stride sB) overwritten
 // with the solutions; per system the relative residual ||B - A X||_F/||B||_F
 // (res_out) and 1 converged / 0 not converged / -info singular (conv_out).
+// With mesh, the FP64 products are formed from the Lagrange-mesh parts of
+// the matrices (mesh_apply_batched) instead of A64.
 template <class TF>
 int solve_batched(const BatchArgs<TF>& b, const double2* A64, int64_t lda, int64_t sA,
                   double2* dB, int64_t ldb, int64_t sB, int nrhs, int max_refine, double tol,
-                  int crit, double* res_out, int* conv_out, hipStream_t st);
+                  int crit, double* res_out, int* conv_out, hipStream_t st,
+                  const MeshOperator* mesh = nullptr);
 
 // HIP-event timing of getrf's trailing-update GEMM launches (for roofline
 // reporting): enable, run, then read total ms / algorithmic flops / count.

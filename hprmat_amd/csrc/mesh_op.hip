@@ -15,10 +15,12 @@
 //
 // Two LDS-tiled FP64 kernels (VALU; the operands are real x complex or
 // small complex blocks), 64 x 64 outputs per workgroup, 4 x 4 per thread:
-//   block:  R_c = beta R_c + alpha D_c X_c          grid (nrhs/64, nr/64, nch)
+//   block:  R_c = beta R_c + alpha D_c X_c          grid (nrhs/64, nr/64, nch x systems)
 //   couple: R(i) += alpha V(i) X(i), X(i)[c'] = X[c' nr + i]
-//                                                    grid (nrhs/64, nch/64, nr)
-// X and R are row-major N x nrhs with row index c nr + i.
+//                                                    grid (nrhs/64, nch/64, nr x systems)
+// X and R are row-major N x nrhs with row index c nr + i.  A batch of
+// systems shares the mesh, the angular momenta and the couplings (one
+// partial wave at several energies); qk and xc1 have one row per system.
 #include "lu.hpp"
 
 namespace hpr {
@@ -31,13 +33,19 @@ __global__ void __launch_bounds__(256)
 mesh_block_kernel(int nch, int nr, int nrhs, const double* __restrict__ tc,
                   const double* __restrict__ blo0, const double* __restrict__ r, double rmax0,
                   const int* __restrict__ lval, const double* __restrict__ qk,
-                  const double* __restrict__ xc1, const double2* __restrict__ X, int64_t ldx,
-                  double2* __restrict__ R, int64_t ldr, double alpha, int beta,
-                  const int* __restrict__ skip) {
-  if (skip && *skip) return;
+                  const double* __restrict__ xc1, int64_t sq, const double2* __restrict__ X,
+                  int64_t ldx, int64_t sX, double2* __restrict__ R, int64_t ldr, int64_t sR,
+                  double alpha, int beta, const int* __restrict__ skip, int sSkip, int e0) {
+  // blockIdx.z = c + nch * (system - e0)
+  const int c = blockIdx.z % nch, sys = e0 + blockIdx.z / nch;
+  if (skip && skip[(int64_t)sys * sSkip]) return;
+  qk += sys * sq;
+  xc1 += sys * sq;
+  X += sys * sX;
+  R += sys * sR;
   __shared__ double dt[KT][MT + 1];     // D_c[i0 + ii][j0 + jj] as dt[jj][ii]
   __shared__ double2 xt[KT][MT];        // X_c[j0 + jj][k0 + kk]
-  const int c = blockIdx.z, i0 = blockIdx.y * MT, k0 = blockIdx.x * MT, tid = threadIdx.x;
+  const int i0 = blockIdx.y * MT, k0 = blockIdx.x * MT, tid = threadIdx.x;
   const int ti = (tid / 16) * 4, tk = (tid % 16) * 4;
   const double q = qk[c];
   const double l = (double)lval[c];
@@ -98,12 +106,17 @@ mesh_block_kernel(int nch, int nr, int nrhs, const double* __restrict__ tc,
 
 __global__ void __launch_bounds__(256)
 mesh_couple_kernel(int nch, int nr, int nrhs, const double2* __restrict__ cpot,
-                   const double2* __restrict__ X, int64_t ldx, double2* __restrict__ R,
-                   int64_t ldr, double alpha, const int* __restrict__ skip) {
-  if (skip && *skip) return;
+                   const double2* __restrict__ X, int64_t ldx, int64_t sX,
+                   double2* __restrict__ R, int64_t ldr, int64_t sR, double alpha,
+                   const int* __restrict__ skip, int sSkip, int e0) {
+  // blockIdx.z = i + nr * (system - e0)
+  const int i = blockIdx.z % nr, sys = e0 + blockIdx.z / nr;
+  if (skip && skip[(int64_t)sys * sSkip]) return;
+  X += sys * sX;
+  R += sys * sR;
   __shared__ double2 vt[KT][MT + 1];    // V(i)[c0 + cc][d0 + dd] as vt[dd][cc]
   __shared__ double2 xt[KT][MT];        // X[(d0 + dd) nr + i][k0 + kk]
-  const int i = blockIdx.z, c0 = blockIdx.y * MT, k0 = blockIdx.x * MT, tid = threadIdx.x;
+  const int c0 = blockIdx.y * MT, k0 = blockIdx.x * MT, tid = threadIdx.x;
   const int tc_ = (tid / 16) * 4, tk = (tid % 16) * 4;
   const double2* v = cpot + (int64_t)i * nch * nch;
   double2 acc[4][4];
@@ -146,19 +159,30 @@ mesh_couple_kernel(int nch, int nr, int nrhs, const double2* __restrict__ cpot,
 
 }  // namespace
 
-int mesh_apply(const MeshOperator& op, int nrhs, const double2* X, double2* R, double alpha,
-               int beta, const int* skip, hipStream_t st) {
-  if (op.nch <= 0 || op.nr <= 0 || nrhs <= 0) return -1;
+int mesh_apply_batched(const MeshOperator& op, int count, int nrhs, const double2* X,
+                       int64_t sX, double2* R, int64_t sR, double alpha, int beta,
+                       const int* skip, int sSkip, hipStream_t st) {
+  if (op.nch <= 0 || op.nr <= 0 || nrhs <= 0 || count < 0) return -1;
   if (op.nr > 65535 || op.nch > 65535) return -1;
-  const dim3 gb((nrhs + MT - 1) / MT, (op.nr + MT - 1) / MT, op.nch);
-  mesh_block_kernel<<<gb, 256, 0, st>>>(op.nch, op.nr, nrhs, op.tc, op.blo0, op.r, op.rmax0,
-                                        op.lval, op.qk, op.xc1, X, nrhs, R, nrhs, alpha, beta,
-                                        skip);
-  const dim3 gc((nrhs + MT - 1) / MT, (op.nch + MT - 1) / MT, op.nr);
-  mesh_couple_kernel<<<gc, 256, 0, st>>>(op.nch, op.nr, nrhs, op.cpot, X, nrhs, R, nrhs, alpha,
-                                         skip);
+  // systems per launch: the grid's z extent (nch or nr blocks per system) stays <= 65535
+  const int per = max(1, 65535 / max(op.nch, op.nr));
+  for (int e0 = 0; e0 < count; e0 += per) {
+    const int ce = min(per, count - e0);
+    const dim3 gb((nrhs + MT - 1) / MT, (op.nr + MT - 1) / MT, op.nch * ce);
+    mesh_block_kernel<<<gb, 256, 0, st>>>(op.nch, op.nr, nrhs, op.tc, op.blo0, op.r, op.rmax0,
+                                          op.lval, op.qk, op.xc1, op.sq, X, nrhs, sX, R, nrhs, sR,
+                                          alpha, beta, skip, sSkip, e0);
+    const dim3 gc((nrhs + MT - 1) / MT, (op.nch + MT - 1) / MT, op.nr * ce);
+    mesh_couple_kernel<<<gc, 256, 0, st>>>(op.nch, op.nr, nrhs, op.cpot, X, nrhs, sX, R, nrhs, sR,
+                                           alpha, skip, sSkip, e0);
+  }
   HPR_CHECK(hipGetLastError());
   return 0;
+}
+
+int mesh_apply(const MeshOperator& op, int nrhs, const double2* X, double2* R, double alpha,
+               int beta, const int* skip, hipStream_t st) {
+  return mesh_apply_batched(op, 1, nrhs, X, 0, R, 0, alpha, beta, skip, 0, st);
 }
 
 }  // namespace hpr

@@ -303,7 +303,9 @@ class BatchScan:
     reduced to R and S in lockstep (hprmat_hip_assemble_batch,
     hprmat_hip_batch_solve, hprmat_hip_r_to_s_batch): one chain of launches
     per chunk of energies instead of one chain per system.  A chunk is as
-    many energies as fit the lane's share of ``mem_bytes``.
+    many energies as fit the lane's share of ``mem_bytes``.  With
+    ``mesh_operator`` the refinement's FP64 products are formed from the
+    Lagrange-mesh parts of the matrices (hprmat_hip_batch_solve_mesh).
 
     Waves are spread over ``lanes`` streams longest-processing-time first, so
     the launch-latency-bound chains of small waves overlap the MFMA-bound
@@ -316,8 +318,9 @@ class BatchScan:
     """
 
     def __init__(self, waves, method='fp32', lanes=2, mem_bytes=48 << 30, max_refine=20,
-                 tol=1e-12, log=None):
+                 tol=1e-12, log=None, mesh_operator=True):
         self.waves, self.method, self.max_refine, self.tol = waves, method, max_refine, tol
+        self.mesh_operator = mesh_operator
         self.code = solver.batch_code(method)
         self.log = log
         self.captured = False
@@ -369,9 +372,11 @@ class BatchScan:
         e = slice(k0, k0 + c)
         rmatrix.assemble_batch(w.dmesh, w.lv, w.qk[e], w.xc1[e], w.cpot, a)
         x.copy_(w.b.unsqueeze(0).expand(c, -1, -1))
+        mesh = (solver.mesh_parts(w.dmesh, w.lv, w.qk[e], w.xc1[e], w.cpot)
+                if self.mesh_operator else None)
         solver.batch_solve(self.code, n, nch, c, a.data_ptr(), x.data_ptr(), self.max_refine,
                            self.tol, 'rel_fro', w.res[k0].data_ptr(), w.status[k0].data_ptr(),
-                           lane.work)
+                           lane.work, mesh=mesh)
         rmatrix.r_to_s_batch(w.dmesh, x, w.qk[e], w.nopen_dev[e], w.open[e], w.co[e], w.cop[e],
                              w.R[e], w.S[e])
 

@@ -194,21 +194,41 @@ def batch_work_bytes(method, n, nrhs, count, equilibrate=True):
                                                        int(nrhs), int(count)))
 
 
+def mesh_parts(dmesh, lval, qk, xc1, cpot):
+    """hprmat_hip_mesh_parts of a batch of one partial wave: dmesh a
+    rmatrix.DeviceMesh, lval (nch,) int32, qk and xc1 (count, nch) float64,
+    cpot (nr, nch, nch) complex128, all contiguous device tensors (the
+    caller keeps them alive while the batch runs)."""
+    nch, nr = int(lval.numel()), int(dmesh.mesh.nr)
+    if qk.dim() != 2 or qk.shape[1] != nch or xc1.shape != qk.shape or cpot.shape != (nr, nch, nch):
+        raise ValueError('mesh parts do not match (nr, nch)')
+    p = lambda t: t.data_ptr()
+    return _lib.MeshParts(nch, nr, p(dmesh.tc), p(dmesh.blo0), p(dmesh.r), float(dmesh.mesh.a),
+                          p(lval), p(qk), p(xc1), int(qk.stride(0)), p(cpot))
+
+
 def batch_solve(code, n, nrhs, count, a_ptr, b_ptr, max_refine, tol, criterion, res_ptr,
-                status_ptr, work, work_bytes=None):
+                status_ptr, work, work_bytes=None, mesh=None):
     """Raw hprmat_hip_batch_solve on the current stream: ``count`` column-major
     n x n matrices at ``a_ptr`` (stride n*n) and n x nrhs right-hand sides at
     ``b_ptr`` (stride n*nrhs, overwritten with the solutions); ``work`` a
     uint8 device tensor of at least the batch's workspace size (shared by
-    calls that are ordered on one stream).  Allocates nothing, does not
+    calls that are ordered on one stream).  With ``mesh`` (mesh_parts) the
+    refinement's FP64 products use the Lagrange-mesh parts of the matrices
+    (hprmat_hip_batch_solve_mesh).  Allocates nothing, does not
     synchronise."""
     wb = work.numel() if work_bytes is None else int(work_bytes)
-    rc = _lib.load().hprmat_hip_batch_solve(
-        int(code), int(n), int(nrhs), int(count), ctypes.c_void_p(a_ptr), int(n), int(n) * int(n),
-        ctypes.c_void_p(b_ptr), int(n), int(n) * int(nrhs), int(max_refine), float(tol),
-        CRITERIA[criterion], ctypes.c_void_p(res_ptr), ctypes.c_void_p(status_ptr),
-        ctypes.c_void_p(work.data_ptr()), wb, _stream_ptr())
-    _lib.check(rc, 'hprmat_hip_batch_solve')
+    args = (int(code), int(n), int(nrhs), int(count), ctypes.c_void_p(a_ptr), int(n),
+            int(n) * int(n), ctypes.c_void_p(b_ptr), int(n), int(n) * int(nrhs), int(max_refine),
+            float(tol), CRITERIA[criterion], ctypes.c_void_p(res_ptr),
+            ctypes.c_void_p(status_ptr), ctypes.c_void_p(work.data_ptr()), wb)
+    lib = _lib.load()
+    if mesh is None:
+        rc = lib.hprmat_hip_batch_solve(*args, _stream_ptr())
+        _lib.check(rc, 'hprmat_hip_batch_solve')
+    else:
+        rc = lib.hprmat_hip_batch_solve_mesh(*args, ctypes.byref(mesh), _stream_ptr())
+        _lib.check(rc, 'hprmat_hip_batch_solve_mesh')
 
 
 class Batch:

@@ -310,6 +310,26 @@ int hprmat_hip_batch_solve(int method_flags, int n, int nrhs, int count, const v
                            double *res_rel, int *status, void *work, size_t work_bytes,
                            void *stream);
 
+/* No reference counterpart: hprmat_hip_batch_solve for `count` systems of
+   one partial wave (one energy each) whose refinement products are formed
+   from their Lagrange-mesh parts (hprmat_hip_lu_set_operator's form): the
+   mesh, angular momenta and couplings are shared, qk and xc1 have one row
+   of nch per system, stride_q apart.  n = nch * nr. */
+typedef struct {
+  int nch, nr;
+  const double *tc, *blo0, *r;
+  double rmax0;
+  const int *lval;
+  const double *qk, *xc1;
+  int64_t stride_q;
+  const void *cpot;
+} hprmat_hip_mesh_parts;
+int hprmat_hip_batch_solve_mesh(int method_flags, int n, int nrhs, int count, const void *dA,
+                                int64_t lda, int64_t stride_a, void *dB, int64_t ldb,
+                                int64_t stride_b, int max_refine, double tol, int criterion,
+                                double *res_rel, int *status, void *work, size_t work_bytes,
+                                const hprmat_hip_mesh_parts *parts, void *stream);
+
 /* Library version string. */
 const char *hprmat_hip_version(void);
 
