@@ -237,19 +237,40 @@ bool plan_for(int solver, SolverPlan& p) {
   }
 }
 
-// Device buffers of one call, freed on scope exit (after the stream drains:
-// an error path may return with work still queued).
+// Device buffers of the interface, kept between calls and grown when a call
+// needs more, like the reference's persistent GPU buffers
+// (src/cusolver_interface.cu:104-130): the k-th buffer of a call reuses slot
+// k, so repeated calls (an energy scan through hprmat_solve) allocate
+// nothing after the largest one.  Every call drains the stream before it
+// returns (also on error paths, in the destructor), so a slot is never
+// reallocated under work in flight.  One caller at a time, as for the
+// reference's module state.
+struct Slot {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+std::vector<Slot> g_slots;
+
 struct DevBufs {
-  std::vector<void*> ptrs;
+  size_t next = 0;
   ~DevBufs() {
     if (g_stream) (void)hipStreamSynchronize(g_stream);
-    for (void* p : ptrs) (void)hipFree(p);
   }
   template <class T> T* alloc(size_t count) {
-    void* p = nullptr;
-    if (hipMalloc(&p, count * sizeof(T) + 16) != hipSuccess) return nullptr;
-    ptrs.push_back(p);
-    return (T*)p;
+    const size_t bytes = count * sizeof(T) + 16;
+    if (next >= g_slots.size()) g_slots.resize(next + 1);
+    Slot& sl = g_slots[next++];
+    if (sl.cap < bytes) {
+      if (sl.p) (void)hipFree(sl.p);
+      sl.p = nullptr;
+      sl.cap = 0;
+      if (hipMalloc(&sl.p, bytes) != hipSuccess) {
+        sl.p = nullptr;
+        return nullptr;
+      }
+      sl.cap = bytes;
+    }
+    return (T*)sl.p;
   }
 };
 
@@ -268,15 +289,24 @@ struct MeshParts {
   const void* cpot = nullptr;
 };
 
+// The interface's LU handle, kept between calls (its buffers grow only).
+hprmat_hip_lu_t g_lu = nullptr;
+
 // Factor the column-major N x N dA and solve for the N x nrhs dX in place.
 int factor_solve(const SolverPlan& p, const void* dA, int n, void* dX, int nrhs,
                  const MeshParts* mp = nullptr) {
-  hprmat_hip_lu_t h = nullptr;
-  if (hprmat_hip_lu_create(&h) != 0) return -1;
-  int rc = 0;
+  if (!g_lu && hprmat_hip_lu_create(&g_lu) != 0) {
+    g_lu = nullptr;
+    return -1;
+  }
+  hprmat_hip_lu_t h = g_lu;
+  int rc;
   if (mp)   // the Bloch matrix carries its radius factors: unit scale
     rc = hprmat_hip_lu_set_operator(h, mp->nch, mp->nr, mp->tc, mp->bloch, mp->r, 1.0, mp->lval,
                                     mp->qk, mp->xc1, mp->cpot);
+  else
+    rc = hprmat_hip_lu_set_operator(h, 0, 0, nullptr, nullptr, nullptr, 1.0, nullptr, nullptr,
+                                    nullptr, nullptr);
   if (!rc) rc = hprmat_hip_lu_factor(h, p.method, dA, n, n, g_stream);
   hprmat_hip_stats st;
   if (!rc) rc = hprmat_hip_lu_solve(h, dX, n, nrhs, p.max_refine, p.tol, p.criterion, 1, g_stream,
@@ -285,7 +315,6 @@ int factor_solve(const SolverPlan& p, const void* dA, int n, void* dX, int nrhs,
     fprintf(stderr, "hprmat: zero pivot at column %d\n", st.info);
     rc = -1;
   }
-  hprmat_hip_lu_destroy(h);
   return rc;
 }
 
@@ -489,6 +518,7 @@ bool physics_solve(const char* who, int nch, const int* lval, const double* qk, 
     ok = ok && upload(dopen, open.data(), (size_t)no) && upload(dco, co.data(), (size_t)no) &&
          upload(dcop, cop.data(), (size_t)no);
   if (ncl > 0) ok = ok && upload(dclosed, closed.data(), (size_t)ncl);
+  if (wv.nc > 0) ok = ok && hipMemsetAsync(dcf, 0, sizeof(zd) * cfn, g_stream) == hipSuccess;
   // each interval: assemble (the Bloch matrix already carries its radius
   // factors, hence the unit scale) and solve; the intervals are independent
   for (int s = 0; ok && s < ns; ++s) {

@@ -82,3 +82,22 @@ def test_solve_linear_matches_oracle_r(solver):
     q2 = ref['mesh']['q2'][0]
     r = physics.solve_linear(ref['C'], q2.astype(np.complex128), 4, 60, 1.0, solver)
     assert np.abs(r - ref['R']).max() <= 1e-10 * np.abs(ref['R']).max()
+
+
+def test_repeated_calls_reuse_buffers_across_sizes():
+    # the interface keeps its device buffers and LU handle between calls:
+    # alternate sizes (grow, shrink, grow) and entry points, each checked
+    import time
+    cases = [(4, 60, 0), (8, 80, 1), (2, 40, 2), (8, 80, 3), (4, 60, 4)]
+    times = []
+    for nch, nr, seed in cases:
+        lval, qk, cpot, _ = osynth.system(nch, nr, 15.0, 3.0, 2.0, seed)
+        ref = ormat.rmatrix(lval, qk, 15.0, nr, cpot)
+        no = ref['nopen']
+        t0 = time.perf_counter()
+        cu, nopen = physics.solve(lval, qk, None, 15.0, nr, cpot, solver=1)
+        times.append(time.perf_counter() - t0)
+        assert nopen == no and np.abs(cu[:no, :no] - ref['S']).max() <= 1e-10, (nch, nr)
+        r = physics.solve_linear(ref['C'], ref['mesh']['q2'][0].astype(np.complex128), nch, nr)
+        assert np.abs(r - ref['R']).max() <= 1e-10 * np.abs(ref['R']).max()
+    print('hprmat_solve call times (s):', ' '.join('%.4f' % t for t in times), flush=True)
