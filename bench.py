@@ -132,6 +132,7 @@ def main():
     ap.add_argument('--scan-energies', type=int, default=256)
     ap.add_argument('--scan-lanes', type=int, default=2)
     ap.add_argument('--scan-refine', type=int, default=20)
+    ap.add_argument('--scan-method', default='fp32_3m', choices=['fp32', 'fp32_3m'])
     ap.add_argument('--scan-graph', type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -311,7 +312,8 @@ def energy_scan(torch, args):
     t0 = time.perf_counter()
     waves = scan.make_waves(args.scan_waves, args.scan_energies, seed=0)
     t_prep = time.perf_counter() - t0
-    sc = scan.BatchScan(waves, method='fp32', lanes=args.scan_lanes, max_refine=args.scan_refine,
+    sc = scan.BatchScan(waves, method=args.scan_method, lanes=args.scan_lanes,
+                        max_refine=args.scan_refine,
                         log=sys.stderr)
     t0 = time.perf_counter()
     if args.scan_graph:
@@ -353,6 +355,7 @@ def energy_scan(torch, args):
     out = {'systems': n_sys, 'ms': ms, 'solves_per_s': n_sys / (ms * 1e-3),
            'tflops': sc.flops / (ms * 1e-3) / 1e12, 'flops': sc.flops, 'lanes': len(sc.lanes),
            'graphs': bool(args.scan_graph), 'max_refine': args.scan_refine,
+           'method': args.scan_method,
            'fp64_fallbacks': n_redo, 'unconverged': bad, 'max_res_rel': worst,
            'serial_fp64': {'systems_timed': len(pairs), 'ms': ms_ser,
                            'solves_per_s': len(pairs) / (ms_ser * 1e-3),

@@ -590,7 +590,9 @@ int hprmat_hip_solve(int method, const void* dA, int64_t lda, void* dB, int64_t 
 
 size_t hprmat_hip_batch_work_bytes(int method, int n, int nrhs, int count) {
   method &= 0xff;
-  if (method != HPRMAT_PLAIN_FP64 && method != HPRMAT_MIXED_FP32) return 0;
+  if (method != HPRMAT_PLAIN_FP64 && method != HPRMAT_MIXED_FP32 &&
+      method != HPRMAT_MIXED_FP32_3M)
+    return 0;
   return hpr::batch_work_bytes(n, nrhs, count, factor_elem_bytes(method));
 }
 
@@ -601,8 +603,10 @@ int batch_solve_impl(int method_flags, int n, int nrhs, int count, const void* d
                      size_t work_bytes, void* stream, const hpr::MeshOperator* mesh) {
   const int method = method_flags & 0xff;
   const bool equil = (method_flags & HPRMAT_EQUILIBRATE) != 0;
-  if (method != HPRMAT_PLAIN_FP64 && method != HPRMAT_MIXED_FP32) {
-    fprintf(stderr, "hprmat_hip: batched solve supports HPRMAT_PLAIN_FP64 and HPRMAT_MIXED_FP32\n");
+  if (method != HPRMAT_PLAIN_FP64 && method != HPRMAT_MIXED_FP32 &&
+      method != HPRMAT_MIXED_FP32_3M) {
+    fprintf(stderr, "hprmat_hip: batched solve supports HPRMAT_PLAIN_FP64, HPRMAT_MIXED_FP32 "
+                    "and HPRMAT_MIXED_FP32_3M\n");
     return -1;
   }
   if (criterion & HPRMAT_FGMRES) {
@@ -625,6 +629,7 @@ int batch_solve_impl(int method_flags, int n, int nrhs, int count, const void* d
   hpr::BatchArgs<float> b;
   const int rc = hpr::batch_bind<float>(b, work, work_bytes, n, nrhs, count, equil);
   if (rc) return rc;
+  b.g3m = method == HPRMAT_MIXED_FP32_3M;
   return hpr::solve_batched<float>(b, (const double2*)dA, lda, stride_a, (double2*)dB, ldb,
                                    stride_b, nrhs, max_refine, tol, criterion, res_rel, status, st,
                                    mesh);

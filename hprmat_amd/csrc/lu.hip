@@ -964,12 +964,15 @@ inline int64_t batch_xchg_words(int n) {
 template <class T>
 void gemm_b(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<T>* B,
             int64_t ldb, cx<T>* C, int64_t ldc, T alpha, int beta, const int* skip, int sSkip,
-            int count, int64_t sA, int64_t sB, int64_t sC, hipStream_t st) {
+            int count, int64_t sA, int64_t sB, int64_t sC, hipStream_t st, bool g3m = false) {
   if (M <= 0 || N <= 0 || K <= 0 || count <= 0) return;
   constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32;
   const dim3 grid(((M + BM - 1) / BM) * ((N + BN - 1) / BN), count);
   if (a_col)
     zgemm_kernel<T, BM, BN, BK, WM, WN, true, false><<<grid, 256, 0, st>>>(
+        M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, sA, sB, sC, sSkip);
+  else if (g3m)
+    zgemm_kernel<T, BM, BN, BK, WM, WN, false, true><<<grid, 256, 0, st>>>(
         M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, sA, sB, sC, sSkip);
   else
     zgemm_kernel<T, BM, BN, BK, WM, WN, false, false><<<grid, 256, 0, st>>>(
@@ -1128,7 +1131,8 @@ int getrf_batched(const BatchArgs<T>& b, hipStream_t st) {
       gemm_b<T>(kb, nc, kb, b.dinv256 + (size_t)(k / 256) * 2 * 65536, 256, false, Akk + kb, ld,
                 b.ustage, nc, T(1), 0, nullptr, 0, count, b.s256, sF, b.sU, st);
       gemm_b<T>(nc, nc, kb, Akk + (int64_t)kb * ld, ld, false, b.ustage, nc,
-                Akk + (int64_t)kb * ld + kb, ld, T(-1), 1, nullptr, 0, count, sF, b.sU, sF, st);
+                Akk + (int64_t)kb * ld + kb, ld, T(-1), 1, nullptr, 0, count, sF, b.sU, sF, st,
+                b.g3m);
       copy2d_kernel<T><<<copy_grid((size_t)kb * nc, count), 256, 0, st>>>(
           kb, nc, b.ustage, nc, Akk + kb, ld, b.sU, sF);
     }

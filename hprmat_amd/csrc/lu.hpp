@@ -264,6 +264,7 @@ template <class T> struct BatchArgs {
   cx<T>* RF = nullptr;
   cx<T>* RY = nullptr;
   int64_t sV = 0;                                   // stride of the n x nrhs blocks
+  bool g3m = false;                                 // Gauss 3-product trailing update
 };
 constexpr int BATCH_NORM_BLOCKS = 64;
 

@@ -182,9 +182,10 @@ def solve(a, b, method='fp64', max_refine=0, tol=1e-12, criterion='rel_fro', fin
 
 
 def batch_code(method, equilibrate=True):
-    """Method flags of hprmat_hip_batch_solve ('fp64' or 'fp32')."""
-    if method not in ('fp64', 'fp32'):
-        raise ValueError('batched method must be fp64 or fp32')
+    """Method flags of hprmat_hip_batch_solve ('fp64', 'fp32' or 'fp32_3m': FP32
+    factors whose trailing updates use Gauss's three-product complex GEMM)."""
+    if method not in ('fp64', 'fp32', 'fp32_3m'):
+        raise ValueError('batched method must be fp64, fp32 or fp32_3m')
     return METHODS[method] | (_lib.EQUILIBRATE if equilibrate else 0)
 
 

@@ -291,7 +291,8 @@ int hprmat_hip_zgemm(int precision, int M, int N, int K, const void *A, int64_t 
    the whole batch.  dA: count column-major n x n matrices (lda, stride_a
    elements apart); dB: count column-major n x nrhs right-hand sides (ldb,
    stride_b apart), overwritten with the solutions.  method_flags:
-   HPRMAT_PLAIN_FP64 or HPRMAT_MIXED_FP32, optionally | HPRMAT_EQUILIBRATE;
+   HPRMAT_PLAIN_FP64, HPRMAT_MIXED_FP32 or HPRMAT_MIXED_FP32_3M (Gauss's
+   three-product trailing update), optionally | HPRMAT_EQUILIBRATE;
    criterion: HPRMAT_CRIT_* (no FGMRES).  max_refine correction sweeps at
    most, each system stopping once it meets tol; res_rel[i] receives the
    final ||B - A X||_F / ||B||_F and status[i] 1 (converged), 0 (not

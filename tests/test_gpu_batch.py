@@ -24,7 +24,7 @@ def _cm(x):
     return torch.as_tensor(np.ascontiguousarray(x.transpose(0, 2, 1))).cuda()
 
 
-@pytest.mark.parametrize('method', ['fp64', 'fp32'])
+@pytest.mark.parametrize('method', ['fp64', 'fp32', 'fp32_3m'])
 @pytest.mark.parametrize('n,count', [(80, 1), (300, 5), (520, 2), (1100, 3)])
 def test_batch_solve_matches_lapack(method, n, count):
     # n = 300 and 520: the panel leaf spans two and three workgroups and the

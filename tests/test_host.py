@@ -59,7 +59,8 @@ wb = lib.hprmat_hip_batch_work_bytes
 one, four = wb(1, 1000, 8, 1), wb(1, 1000, 8, 4)
 assert one > 1000 * 1000 * 8 and 3 * one < four <= 4 * one + 4096, (one, four)
 assert wb(0, 1000, 8, 1) > one            # FP64 factors are twice as large
-assert wb(2, 1000, 8, 1) == 0 and wb(1, 0, 8, 1) == 0 and wb(1, 10, 0, 1) == 0
+assert wb(2, 1000, 8, 1) == one            # Gauss 3-product FP32: the FP32 workspace
+assert wb(3, 1000, 8, 1) == 0 and wb(1, 0, 8, 1) == 0 and wb(1, 10, 0, 1) == 0
 nul = ctypes.c_void_p(0)
 assert lib.hprmat_hip_batch_solve(3, 10, 1, 1, nul, 10, 100, nul, 10, 10, 0, 0.0, 1, nul, nul,
                                   nul, 0, nul) == -1     # split-BF16: not batched
