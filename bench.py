@@ -131,7 +131,7 @@ def main():
     ap.add_argument('--scan-waves', type=int, default=64)
     ap.add_argument('--scan-energies', type=int, default=256)
     ap.add_argument('--scan-lanes', type=int, default=2)
-    ap.add_argument('--scan-refine', type=int, default=20)
+    ap.add_argument('--scan-refine', type=int, default=15)
     ap.add_argument('--scan-method', default='fp32_3m', choices=['fp32', 'fp32_3m'])
     ap.add_argument('--scan-graph', type=int, default=1)
     args = ap.parse_args()
