@@ -289,6 +289,9 @@ struct MeshParts {
   const void* cpot = nullptr;
 };
 
+// Intervals of at most this order are solved as one uniform batch.
+constexpr int kBatchIntervalMaxN = 4096;
+
 // The interface's LU handle, kept between calls (its buffers grow only).
 hprmat_hip_lu_t g_lu = nullptr;
 
@@ -521,7 +524,43 @@ bool physics_solve(const char* who, int nch, const int* lval, const double* qk, 
   if (wv.nc > 0) ok = ok && hipMemsetAsync(dcf, 0, sizeof(zd) * cfn, g_stream) == hipSuccess;
   // each interval: assemble (the Bloch matrix already carries its radius
   // factors, hence the unit scale) and solve; the intervals are independent
-  for (int s = 0; ok && s < ns; ++s) {
+  // small intervals of one order: one uniform batch (one launch chain for
+  // all ns systems) with the solver type's precision and gate; systems the
+  // batch leaves unconverged are re-solved below with the FP64 plan
+  const bool batched = ns > 1 && !nonlocal && n <= kBatchIntervalMaxN;
+  std::vector<int> redo;
+  if (ok && batched) {
+    const int code = plan.method == HPRMAT_PLAIN_FP64
+                         ? (HPRMAT_PLAIN_FP64 | HPRMAT_EQUILIBRATE)
+                         : ((plan.criterion & HPRMAT_FGMRES) ? HPRMAT_MIXED_FP32_3M
+                                                             : HPRMAT_MIXED_FP32) |
+                               HPRMAT_EQUILIBRATE;
+    const size_t wb = hprmat_hip_batch_work_bytes(code, n, nrhs, ns);
+    zd* dAall = d.alloc<zd>((size_t)ns * n * n);
+    double* dres = d.alloc<double>(ns);
+    int* dst = d.alloc<int>(ns);
+    void* dbw = wb ? d.alloc<char>(wb) : nullptr;
+    ok = dAall && dres && dst && dbw;
+    for (int s = 0; ok && s < ns; ++s)
+      ok = !hprmat_hip_assemble(nch, nr, dtc + s * nn2, dbl + s * nn2, dr + (size_t)s * nr, 1.0, dl,
+                                dqk, dxc, dpot + (size_t)s * nr * nch * nch,
+                                dAall + (size_t)s * n * n, n, g_stream);
+    ok = ok && !hprmat_hip_batch_solve(code, n, nrhs, ns, dAall, n, (int64_t)n * n, dX, n,
+                                       (int64_t)n * nrhs, plan.max_refine, plan.tol,
+                                       HPRMAT_CRIT_REL_FRO, dres, dst, dbw, wb, g_stream);
+    std::vector<int> status(ns, 0);
+    ok = ok && hipMemcpyAsync(status.data(), dst, sizeof(int) * ns, hipMemcpyDeviceToHost,
+                              g_stream) == hipSuccess &&
+         hipStreamSynchronize(g_stream) == hipSuccess;
+    for (int s = 0; ok && s < ns; ++s)
+      if (status[s] != 1) redo.push_back(s);
+    SolverPlan p64;
+    plan_for(1, p64);
+    for (int s : redo)
+      ok = ok && upload(dX + (size_t)s * n * nrhs, &rhs[(size_t)s * n * nrhs], (size_t)n * nrhs) &&
+           !factor_solve(p64, dAall + (size_t)s * n * n, n, dX + (size_t)s * n * nrhs, nrhs);
+  }
+  for (int s = 0; ok && !batched && s < ns; ++s) {
     MeshParts mp;
     mp.nch = nch;
     mp.nr = nr;

@@ -305,13 +305,24 @@ def closed_w(lval, qk, a):
     return w
 
 
+# Intervals of at most this order are solved as one uniform batch; larger
+# ones one at a time (each then fills the GPU, with look-ahead and FGMRES).
+BATCH_INTERVAL_MAX_N = 4096
+# solver_type -> (batched method, tolerance, sweep budget) of the batch: the
+# single-system plans' precision and gate with classic sweeps
+_BATCH_PLANS = {1: ('fp64', 1.0e-12, 2), 2: ('fp32', 1.0e-10, 2), 4: ('fp32_3m', 1.0e-12, 24),
+                'plain': ('fp64', 1.0e-12, 5), 'mixed': ('fp32_3m', 1.0e-12, 24)}
+
+
 def rmatrix_intervals(lval, qk, a, nr, ns, cpot, solver_type=1, nvc=None, max_refine=0,
                       cpnl=None, structured=True):
     """rmatrix (src/rmatrix_hp.F90:175-431) for ns >= 1 intervals on the GPU.
 
     Every interval's system C_s is assembled on the device and solved for
     its 2 nch boundary columns [q1 | q2] (solve_rmatrix_hp_4,
-    src/rmatrix_hp.F90:590-632, with the solver_type's method); the R blocks,
+    src/rmatrix_hp.F90:590-632, with the solver_type's precision): as one
+    uniform batch when N <= BATCH_INTERVAL_MAX_N (local potentials), else
+    one system at a time; the R blocks,
     the propagation chain, S and, for the entrance channels ``nvc`` (0-based
     among the open channels), the internal wave functions and bound-state
     amplitudes follow in hprmat_hip_propagate.  cpot: (nr ns, nch, nch);
@@ -347,21 +358,41 @@ def rmatrix_intervals(lval, qk, a, nr, ns, cpot, solver_type=1, nvc=None, max_re
     for c in range(nch):
         x[:, c, c * nr:(c + 1) * nr] = q1.to(torch.complex128)
         x[:, nch + c, c * nr:(c + 1) * nr] = q2.to(torch.complex128)
-    st = []
-    for s in range(ns):
+    x0 = x.clone() if ns > 1 else x
+    def assemble_interval(s, out):
         rc = lib.hprmat_hip_assemble(
             nch, nr, ctypes.c_void_p(tc[s].data_ptr()), ctypes.c_void_p(bl[s].data_ptr()),
             ctypes.c_void_p(rr[s].data_ptr()), ctypes.c_double(1.0),
             ctypes.c_void_p(lv.data_ptr()), ctypes.c_void_p(qd.data_ptr()),
             ctypes.c_void_p(xd.data_ptr()), ctypes.c_void_p(cp[s * nr].data_ptr()),
-            ctypes.c_void_p(a_cm.data_ptr()), ctypes.c_int64(n), solver._stream_ptr())
+            ctypes.c_void_p(out.data_ptr()), ctypes.c_int64(n), solver._stream_ptr())
         _lib.check(rc, 'hprmat_hip_assemble')
-        op = None
-        if cpnl is not None:
-            _add_nonlocal(nr, m['wle'], h, cpnl, a_cm)
-        elif structured:
-            op = _operator_parts(tc[s], bl[s], rr[s], 1.0, lv, qd, xd, cp[s * nr:(s + 1) * nr])
-        st.append(solve_colmajor(a_cm, x[s], solver_type, max_refine, operator=op))
+
+    st = []
+    batched = (ns > 1 and n <= BATCH_INTERVAL_MAX_N and cpnl is None
+               and solver_type in _BATCH_PLANS)
+    if batched:
+        # the ns interval systems are independent and of one order: one
+        # uniform batch (one chain of launches for all of them)
+        method, tol, budget = _BATCH_PLANS[solver_type]
+        a_all = torch.empty((ns, n, n), dtype=torch.complex128, device='cuda')
+        for s in range(ns):
+            assemble_interval(s, a_all[s])
+        bt = solver.Batch(n, 2 * nch, ns, method, equilibrate=True)
+        bt.solve_(a_all, x, max_refine=budget, tol=tol)
+        status = bt.status.cpu().numpy()
+        for s in np.nonzero(status != 1)[0]:   # stragglers: the single-system FP64 path
+            x[s].copy_(x0[s])
+            st.append(solve_colmajor(a_all[s], x[s], 1, 0))
+    else:
+        for s in range(ns):
+            assemble_interval(s, a_cm)
+            op = None
+            if cpnl is not None:
+                _add_nonlocal(nr, m['wle'], h, cpnl, a_cm)
+            elif structured:
+                op = _operator_parts(tc[s], bl[s], rr[s], 1.0, lv, qd, xd, cp[s * nr:(s + 1) * nr])
+            st.append(solve_colmajor(a_cm, x[s], solver_type, max_refine, operator=op))
     nc = 0 if nvc is None else len(nvc)
     if nc and nopen == 0:
         raise ValueError('wave functions need an open entrance channel')
