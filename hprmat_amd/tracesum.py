@@ -17,12 +17,23 @@ scaled by what the same counter reads for the device-to-device copy in the
 file (the largest ``__amd_rocclr_copyBuffer``, of --copy-bytes).  Several
 counter files (one pass each) are merged per kernel.
 
+With --traffic N the counter files (a FETCH_SIZE pass and a WRITE_SIZE pass)
+hold the trailing-update launches of one blocked factorization of order N
+(block --nb): the j-th launch of --kernel updates the nc x ncols block with
+nc = N - (j + 1) nb and ncols = nc, or nc - nb on a look-ahead step (its grid
+of 64 x 64 tiles tells which), and its algorithmic bytes are one read of
+L21 (nc x nb), U12 (nb x ncols) and C (nc x ncols) and one write of C.  The
+summary compares the copy-calibrated HBM bytes with those, per launch and in
+total (--json writes the same figures).
+
 Usage: python -m hprmat_amd.tracesum TRACE.csv [--kernel zgemm_kernel<float] [--min-wg 1000]
        python -m hprmat_amd.tracesum --counters COUNTERS.csv [MORE.csv ...] [--copy-bytes N]
+       python -m hprmat_amd.tracesum --traffic 25600 FETCH.csv WRITE.csv --kernel K [--json OUT]
 """
 import argparse
 import collections
 import csv
+import json
 import sys
 
 
@@ -135,6 +146,102 @@ def print_counters(paths, copy_bytes):
                   % (name[:80], a['n'], a['ns'] / 1e6, rd, wr, (rd + wr) / sec if sec else 0))
 
 
+def dispatches(path):
+    """Dispatches of a counter-collection CSV in submission order."""
+    disp = collections.OrderedDict()
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            d = disp.setdefault(int(r['Dispatch_Id']), {
+                'name': r['Kernel_Name'].replace('(anonymous namespace)::', '')
+                        .split('(')[0].replace('void ', ''),
+                'wgs': int(r['Grid_Size']) // max(1, int(r['Workgroup_Size'])),
+                'ns': int(r['End_Timestamp']) - int(r['Start_Timestamp'])})
+            d[r['Counter_Name']] = float(r['Counter_Value'])
+    return [disp[k] for k in sorted(disp)]
+
+
+def factor_traffic(paths, copy_bytes, kernel, n, nb, elem, tile=64):
+    """Copy-calibrated HBM bytes against algorithmic bytes for the trailing
+    updates of one factorization (one counter per file)."""
+    cdiv = lambda a, b: (a + b - 1) // b
+    per = None
+    scales = {}
+    for path in paths:
+        ds = dispatches(path)
+        copies = [d for d in ds if 'copyBuffer' in d['name']]
+        sel = [d for d in ds if d['name'].startswith(kernel)]
+        if per is None:
+            per = []
+            for j, d in enumerate(sel):
+                nc = n - (j + 1) * nb
+                if nc <= 0:
+                    raise ValueError('more launches of %s than steps of N=%d' % (kernel, n))
+                shapes = [c for c in (nc, nc - nb)
+                          if c > 0 and cdiv(nc, tile) * cdiv(c, tile) == d['wgs']]
+                if not shapes:
+                    raise ValueError('launch %d: %d workgroups fit no update of %d rows'
+                                     % (j, d['wgs'], nc))
+                ncols = shapes[0]
+                per.append({'M': nc, 'N': ncols, 'K': nb, 'ns': d['ns'],
+                            'alg_read': elem * (nc * nb + nb * ncols + nc * ncols),
+                            'alg_write': elem * nc * ncols})
+        elif len(sel) != len(per):
+            raise ValueError('%s: %d launches, %d in the first file' % (path, len(sel), len(per)))
+        for c in ('FETCH_SIZE', 'WRITE_SIZE'):
+            vals = [d[c] for d in copies if c in d]
+            if not vals or not any(c in d for d in sel):
+                continue
+            scales[c] = copy_bytes / 1024.0 / max(vals)
+            for p, d in zip(per, sel):
+                p[c] = d[c] * scales[c] * 1024.0
+    if not per:
+        raise ValueError('no launches of %s' % kernel)
+    for c in ('FETCH_SIZE', 'WRITE_SIZE'):
+        if c not in scales:
+            raise ValueError('no %s pass with a calibration copy' % c)
+    tot = lambda key: sum(p[key] for p in per)
+    cnt = len(per)
+    rd, wr, ard, awr = tot('FETCH_SIZE'), tot('WRITE_SIZE'), tot('alg_read'), tot('alg_write')
+    first = per[0]
+    return {
+        'kernel': kernel, 'N': n, 'nb': nb, 'launches': cnt,
+        'fetch_scale': scales['FETCH_SIZE'], 'write_scale': scales['WRITE_SIZE'],
+        'hbm_read_bytes': rd / cnt, 'hbm_write_bytes': wr / cnt, 'hbm_bytes': (rd + wr) / cnt,
+        'alg_read_bytes': ard / cnt, 'alg_write_bytes': awr / cnt, 'alg_bytes': (ard + awr) / cnt,
+        'read_ratio': rd / ard, 'write_ratio': wr / awr,
+        'kernel_ms': tot('ns') / 1e6,
+        'first_launch': {'M': first['M'], 'N': first['N'], 'K': nb,
+                         'hbm_read_bytes': first['FETCH_SIZE'],
+                         'hbm_write_bytes': first['WRITE_SIZE'],
+                         'alg_read_bytes': first['alg_read'], 'alg_write_bytes': first['alg_write'],
+                         'read_ratio': first['FETCH_SIZE'] / first['alg_read'],
+                         'write_ratio': first['WRITE_SIZE'] / first['alg_write']},
+    }
+
+
+def print_traffic(rec):
+    f = rec['first_launch']
+    print('HBM scale factors from the copy: FETCH_SIZE x%.3f, WRITE_SIZE x%.3f'
+          % (rec['fetch_scale'], rec['write_scale']))
+    print()
+    print('| `%s`, N = %d | launches | HBM read GB | HBM write GB | alg read GB | '
+          'alg write GB | read / alg | write / alg |' % (rec['kernel'], rec['N']))
+    print('|---|---:|---:|---:|---:|---:|---:|---:|')
+    c = rec['launches']
+    print('| all launches (sum) | %d | %.3f | %.3f | %.3f | %.3f | %.2f | %.2f |'
+          % (c, c * rec['hbm_read_bytes'] / 1e9, c * rec['hbm_write_bytes'] / 1e9,
+             c * rec['alg_read_bytes'] / 1e9, c * rec['alg_write_bytes'] / 1e9,
+             rec['read_ratio'], rec['write_ratio']))
+    print('| per launch (mean) | 1 | %.3f | %.3f | %.3f | %.3f | %.2f | %.2f |'
+          % (rec['hbm_read_bytes'] / 1e9, rec['hbm_write_bytes'] / 1e9,
+             rec['alg_read_bytes'] / 1e9, rec['alg_write_bytes'] / 1e9,
+             rec['read_ratio'], rec['write_ratio']))
+    print('| first launch %d x %d x %d | 1 | %.3f | %.3f | %.3f | %.3f | %.2f | %.2f |'
+          % (f['M'], f['N'], f['K'], f['hbm_read_bytes'] / 1e9, f['hbm_write_bytes'] / 1e9,
+             f['alg_read_bytes'] / 1e9, f['alg_write_bytes'] / 1e9, f['read_ratio'],
+             f['write_ratio']))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('trace', nargs='+')
@@ -143,7 +250,19 @@ def main(argv=None):
     ap.add_argument('--top', type=int, default=16)
     ap.add_argument('--counters', action='store_true')
     ap.add_argument('--copy-bytes', type=int, default=1 << 30)
+    ap.add_argument('--traffic', type=int, default=0, metavar='N')
+    ap.add_argument('--nb', type=int, default=256)
+    ap.add_argument('--elem-bytes', type=int, default=8)
+    ap.add_argument('--json', default='')
     args = ap.parse_args(argv)
+    if args.traffic:
+        rec = factor_traffic(args.trace, args.copy_bytes, args.kernel, args.traffic, args.nb,
+                             args.elem_bytes)
+        print_traffic(rec)
+        if args.json:
+            with open(args.json, 'w') as f:
+                json.dump(rec, f, indent=1)
+        return 0
     if args.counters:
         print_counters(args.trace, args.copy_bytes)
         return 0
