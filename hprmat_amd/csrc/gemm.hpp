@@ -230,6 +230,36 @@ zgemm_dual_kernel(GemmProb<T> p1, GemmProb<T> p2, int tiles1, const int* __restr
                                                    p2.ldc, p2.alpha, p2.beta, t - tiles1);
 }
 
+// Product with the inverse D of a triangular diagonal block (the LU's
+// 256 x 256 block inverses, row-major, lower when LOWER, else upper) in
+// problem p1, with p2 an independent product as in zgemm_dual_kernel.  p1 is
+// tiled 32 x 32 (eight row tiles of a full block instead of four, four waves
+// of 16 x 16 each) and each row tile reads only D's nonzero triangle: row
+// tile i0 of D X needs k < i0 + 32 (lower) or k >= i0 (upper), 9/16 of the
+// block's work on average.  The zero blocks skipped add exact zeros, so the
+// result is the full product's.  Workgroups [0, tiles1) take p1, the rest p2
+// (64 x 64 tiles).
+template <class T, bool LOWER>
+__global__ void __launch_bounds__(256)
+tri_dual_kernel(GemmProb<T> p1, GemmProb<T> p2, int tiles1, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  const int t = blockIdx.x;
+  if (t < tiles1) {
+    constexpr int TB = 32;
+    const int tn = (p1.N + TB - 1) / TB;
+    const int i0 = (t / tn) * TB;
+    const int k0 = LOWER ? 0 : i0;
+    const int k1 = LOWER ? min(i0 + TB, p1.K) : p1.K;
+    zgemm_tile<T, TB, TB, 16, 16, 16, false, false>(
+        min(TB, p1.M - i0), p1.N, k1 - k0, p1.A + (int64_t)i0 * p1.lda + k0, p1.lda,
+        p1.B + (int64_t)k0 * p1.ldb, p1.ldb, p1.C + (int64_t)i0 * p1.ldc, p1.ldc, p1.alpha,
+        p1.beta, t % tn);
+  } else {
+    zgemm_tile<T, 64, 64, 16, 32, 32, false, false>(p2.M, p2.N, p2.K, p2.A, p2.lda, p2.B, p2.ldb,
+                                                   p2.C, p2.ldc, p2.alpha, p2.beta, t - tiles1);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Second-generation complex GEMM (same contract as zgemm_kernel).
 //

@@ -604,6 +604,22 @@ void gemm_dual(const GemmProb<T>& p1, const GemmProb<T>& p2, const int* skip, hi
   zgemm_dual_kernel<T, BM, BN, BK, WM, WN><<<t1 + t2, 256, 0, st>>>(p1, p2, t1, skip);
 }
 
+// D X into p1 (D a 256 x 256 triangular block inverse, p1.M = p1.K <= 256)
+// and the independent product p2 (skipped when empty) in one launch.
+template <class T>
+int tri_dual(bool lower, const GemmProb<T>& p1, const GemmProb<T>& p2, const int* skip,
+             hipStream_t st) {
+  if (p1.M <= 0 || p1.N <= 0) return 0;
+  if (p1.K != p1.M || p1.M > 256) return -1;
+  const int t1 = ((p1.M + 31) / 32) * ((p1.N + 31) / 32);
+  const int t2 = (p2.M > 0 && p2.N > 0 && p2.K > 0) ? ((p2.M + 63) / 64) * ((p2.N + 63) / 64) : 0;
+  if (lower)
+    tri_dual_kernel<T, true><<<t1 + t2, 256, 0, st>>>(p1, p2, t1, skip);
+  else
+    tri_dual_kernel<T, false><<<t1 + t2, 256, 0, st>>>(p1, p2, t1, skip);
+  return 0;
+}
+
 template <class T>
 GemmProb<T> prob(int M, int N, int K, const cx<T>* A, int64_t lda, const cx<T>* B, int64_t ldb,
                  cx<T>* C, int64_t ldc, T alpha, int beta) {
@@ -814,13 +830,19 @@ int getrs(const cx<T>* LU, int64_t ld, int n, const cx<T>* dinv256, cx<T>* B, in
       gemm<T>(p2.M, p2.N, p2.K, p2.A, p2.lda, false, p2.B, p2.ldb, p2.C, p2.ldc, p2.alpha,
               p2.beta, true, skip, st);
     } else {
-      gemm_dual<T>(p1, p2, skip, st);
+      if (tri_dual<T>(true, p1, p2, skip, st)) return -1;
     }
   }
   for (int b = ((n - 1) / rs) * rs; b >= 0; b -= rs) {
     const int bs = min(rs, n - b);
-    gemm<T>(bs, nrhs, bs, dinv256 + ((size_t)(b / 256) * 2 + 1) * 65536, 256, false,
-            Y + (int64_t)b * ldb, ldb, B + (int64_t)b * ldb, ldb, T(1), 0, g3m, skip, st);
+    const cx<T>* iU = dinv256 + ((size_t)(b / 256) * 2 + 1) * 65536;
+    if (g3m)
+      gemm<T>(bs, nrhs, bs, iU, 256, false, Y + (int64_t)b * ldb, ldb, B + (int64_t)b * ldb, ldb,
+              T(1), 0, true, skip, st);
+    else if (tri_dual<T>(false, prob<T>(bs, nrhs, bs, iU, 256, Y + (int64_t)b * ldb, ldb,
+                                        B + (int64_t)b * ldb, ldb, T(1), 0),
+                         prob<T>(0, 0, 0, iU, 256, Y, ldb, B, ldb, T(1), 0), skip, st))
+      return -1;
     if (b > 0)
       gemm<T>(b, nrhs, bs, LU + b, ld, false, B + (int64_t)b * ldb, ldb, Y, ldb, T(-1), 1, g3m,
               skip, st);
