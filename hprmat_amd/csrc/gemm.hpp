@@ -238,8 +238,8 @@ zgemm_dual_kernel(GemmProb<T> p1, GemmProb<T> p2, int tiles1, const int* __restr
 // tile i0 of D X needs k < i0 + 32 (lower) or k >= i0 (upper), 9/16 of the
 // block's work on average.  The zero blocks skipped add exact zeros, so the
 // result is the full product's.  Workgroups [0, tiles1) take p1, the rest p2
-// (64 x 64 tiles).
-template <class T, bool LOWER>
+// (BM2 x BN2 tiles, waves of WM2 x WN2).
+template <class T, bool LOWER, int BM2, int BN2, int WM2, int WN2>
 __global__ void __launch_bounds__(256)
 tri_dual_kernel(GemmProb<T> p1, GemmProb<T> p2, int tiles1, const int* __restrict__ skip) {
   if (skip && *skip) return;
@@ -255,8 +255,9 @@ tri_dual_kernel(GemmProb<T> p1, GemmProb<T> p2, int tiles1, const int* __restric
         p1.B + (int64_t)k0 * p1.ldb, p1.ldb, p1.C + (int64_t)i0 * p1.ldc, p1.ldc, p1.alpha,
         p1.beta, t % tn);
   } else {
-    zgemm_tile<T, 64, 64, 16, 32, 32, false, false>(p2.M, p2.N, p2.K, p2.A, p2.lda, p2.B, p2.ldb,
-                                                   p2.C, p2.ldc, p2.alpha, p2.beta, t - tiles1);
+    zgemm_tile<T, BM2, BN2, 16, WM2, WN2, false, false>(p2.M, p2.N, p2.K, p2.A, p2.lda, p2.B,
+                                                       p2.ldb, p2.C, p2.ldc, p2.alpha, p2.beta,
+                                                       t - tiles1);
   }
 }
 

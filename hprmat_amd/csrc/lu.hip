@@ -604,6 +604,31 @@ void gemm_dual(const GemmProb<T>& p1, const GemmProb<T>& p2, const int* skip, hi
   zgemm_dual_kernel<T, BM, BN, BK, WM, WN><<<t1 + t2, 256, 0, st>>>(p1, p2, t1, skip);
 }
 
+// Tile of the substitutions' block updates (M x N, K = 256): 0 = 64 x 64,
+// 1 = 32 x 64, 2 = 32 x 32 (HPRMAT_HIP_SUBST_TILE forces one; -1 = by size).
+// A launch takes about one tile time per round of workgroups over the 256
+// CUs plus a fixed ~12 us, so small tiles win until the 64 x 64 grid holds
+// several rounds: at N = 25600, nrhs = 128 the 32 x 32 tiles take an update
+// of M = 8960 rows from 45.7 to 32.3 us and M = 768 from 29.7 to 15.3 us,
+// while 64 x 64 stays 2-4 us ahead above about 670 of its tiles (M > 21500).
+int subst_tile(int M, int N) {
+  static int force = [] {
+    const char* e = getenv("HPRMAT_HIP_SUBST_TILE");
+    return e ? atoi(e) : -1;
+  }();
+  if (force >= 0 && force <= 2) return force;
+  const int64_t tiles64 = (int64_t)((M + 63) / 64) * ((N + 63) / 64);
+  return tiles64 >= 672 ? 0 : 2;
+}
+
+template <class T, bool LOWER, int BM2, int BN2, int WM2, int WN2>
+void tri_dual_launch(const GemmProb<T>& p1, const GemmProb<T>& p2, int t1, const int* skip,
+                     hipStream_t st) {
+  const int t2 = (p2.M > 0 && p2.N > 0 && p2.K > 0)
+                     ? ((p2.M + BM2 - 1) / BM2) * ((p2.N + BN2 - 1) / BN2) : 0;
+  tri_dual_kernel<T, LOWER, BM2, BN2, WM2, WN2><<<t1 + t2, 256, 0, st>>>(p1, p2, t1, skip);
+}
+
 // D X into p1 (D a 256 x 256 triangular block inverse, p1.M = p1.K <= 256)
 // and the independent product p2 (skipped when empty) in one launch.
 template <class T>
@@ -612,12 +637,35 @@ int tri_dual(bool lower, const GemmProb<T>& p1, const GemmProb<T>& p2, const int
   if (p1.M <= 0 || p1.N <= 0) return 0;
   if (p1.K != p1.M || p1.M > 256) return -1;
   const int t1 = ((p1.M + 31) / 32) * ((p1.N + 31) / 32);
-  const int t2 = (p2.M > 0 && p2.N > 0 && p2.K > 0) ? ((p2.M + 63) / 64) * ((p2.N + 63) / 64) : 0;
-  if (lower)
-    tri_dual_kernel<T, true><<<t1 + t2, 256, 0, st>>>(p1, p2, t1, skip);
-  else
-    tri_dual_kernel<T, false><<<t1 + t2, 256, 0, st>>>(p1, p2, t1, skip);
+  const int tile = subst_tile(p2.M, p2.N);
+  if (lower) {
+    if (tile == 2) tri_dual_launch<T, true, 32, 32, 16, 16>(p1, p2, t1, skip, st);
+    else if (tile == 1) tri_dual_launch<T, true, 32, 64, 16, 32>(p1, p2, t1, skip, st);
+    else tri_dual_launch<T, true, 64, 64, 32, 32>(p1, p2, t1, skip, st);
+  } else {
+    if (tile == 2) tri_dual_launch<T, false, 32, 32, 16, 16>(p1, p2, t1, skip, st);
+    else if (tile == 1) tri_dual_launch<T, false, 32, 64, 16, 32>(p1, p2, t1, skip, st);
+    else tri_dual_launch<T, false, 64, 64, 32, 32>(p1, p2, t1, skip, st);
+  }
   return 0;
+}
+
+// C = beta C + alpha A B (row-major A) with the substitution update tile.
+template <class T>
+void subst_gemm(int M, int N, int K, const cx<T>* A, int64_t lda, const cx<T>* B, int64_t ldb,
+                cx<T>* C, int64_t ldc, T alpha, int beta, const int* skip, hipStream_t st) {
+  if (M <= 0 || N <= 0 || K <= 0) return;
+  const int tile = subst_tile(M, N);
+  if (tile == 2)
+    zgemm_kernel<T, 32, 32, 16, 16, 16, false, false><<<((M + 31) / 32) * ((N + 31) / 32), 256,
+                                                        0, st>>>(M, N, K, A, lda, B, ldb, C, ldc,
+                                                                 alpha, beta, skip);
+  else if (tile == 1)
+    zgemm_kernel<T, 32, 64, 16, 16, 32, false, false><<<((M + 31) / 32) * ((N + 63) / 64), 256,
+                                                        0, st>>>(M, N, K, A, lda, B, ldb, C, ldc,
+                                                                 alpha, beta, skip);
+  else
+    gemm<T>(M, N, K, A, lda, false, B, ldb, C, ldc, alpha, beta, false, skip, st);
 }
 
 template <class T>
@@ -843,9 +891,14 @@ int getrs(const cx<T>* LU, int64_t ld, int n, const cx<T>* dinv256, cx<T>* B, in
                                         B + (int64_t)b * ldb, ldb, T(1), 0),
                          prob<T>(0, 0, 0, iU, 256, Y, ldb, B, ldb, T(1), 0), skip, st))
       return -1;
-    if (b > 0)
-      gemm<T>(b, nrhs, bs, LU + b, ld, false, B + (int64_t)b * ldb, ldb, Y, ldb, T(-1), 1, g3m,
-              skip, st);
+    if (b > 0) {
+      if (g3m)
+        gemm<T>(b, nrhs, bs, LU + b, ld, false, B + (int64_t)b * ldb, ldb, Y, ldb, T(-1), 1, true,
+                skip, st);
+      else
+        subst_gemm<T>(b, nrhs, bs, LU + b, ld, B + (int64_t)b * ldb, ldb, Y, ldb, T(-1), 1, skip,
+                      st);
+    }
   }
   HPR_CHECK(hipGetLastError());
   return 0;

@@ -277,3 +277,18 @@ def test_trailing_tile_variants_match_lapack(tile, method):
     if method == 'fp32':
         # unrefined FP32 factors: first residual at FP32 level
         assert st['hist'][0] <= 1e-4, st['hist'][:2]
+
+
+@pytest.mark.parametrize('method', ['fp64', 'fp32'])
+def test_substitution_tiles_wide_rhs(method):
+    # the block updates of the substitutions take 64 x 64 tiles while their
+    # grid holds >= 672 of them and 32 x 32 tiles below: n = 3000 with 1024
+    # right-hand sides uses both within one solve (ragged last block of 184)
+    n, nrhs = 3000, 1024
+    print('substitution tiles method %s n %d nrhs %d' % (method, n, nrhs), flush=True)
+    a, b = rand_system(n, nrhs, 77)
+    x_ref, info = osolve.zgesv(a, b)
+    assert info == 0
+    x, st = gpu_solve(a, b, method, max_refine=0 if method == 'fp64' else 12)
+    assert st['info'] == 0
+    assert np.abs(x - x_ref).max() / np.abs(x_ref).max() <= 1e-12, st
