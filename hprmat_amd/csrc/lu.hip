@@ -349,6 +349,18 @@ int gemm(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<
   if (M <= 0 || N <= 0 || K <= 0) return 0;
   constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32;
   int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+  if (!a_col && lds_extra == 0 && tiles < 672) {
+    // a grid of a few rounds over the CUs: 32 x 32 tiles (four times the
+    // workgroups, each element summed in the same order)
+    const int t32 = ((M + 31) / 32) * ((N + 31) / 32);
+    if (g3m)
+      zgemm_kernel<T, 32, 32, BK, 16, 16, false, true><<<t32, 256, 0, st>>>(
+          M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, 0, 0, 0, 0, prio);
+    else
+      zgemm_kernel<T, 32, 32, BK, 16, 16, false, false><<<t32, 256, 0, st>>>(
+          M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, 0, 0, 0, 0, prio);
+    return 0;
+  }
   // lds_extra reserves unused LDS per workgroup, which caps the number of
   // workgroups per CU and so leaves room for the concurrently running panel
   if (a_col) {
