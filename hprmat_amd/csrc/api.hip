@@ -25,6 +25,8 @@ struct hprmat_hip_lu {
   size_t dinv256_cap = 0;
   void* ustage = nullptr;     // U12 staging block (256 x n)
   size_t ustage_cap = 0;
+  void* ftmp = nullptr;       // staging of the look-ahead stream's folds (256 x n)
+  size_t ftmp_cap = 0;
   void* split = nullptr;      // split-BF16 operand planes (HPRMAT_SPLIT_BF16, _BF16X6)
   size_t split_cap = 0;
   void* gm = nullptr;         // FGMRES workspace (HPRMAT_FGMRES solves)
@@ -96,7 +98,7 @@ void free_lu(hprmat_hip_lu* h) {
     (void)hipEventDestroy(h->side.rest);
     (void)hipStreamDestroy(h->side.stream);
   }
-  void* ps[] = {h->fac, h->ipiv, h->dinfo, h->dinv64, h->dinv256, h->ustage, h->split, h->gm,
+  void* ps[] = {h->fac, h->ipiv, h->dinfo, h->dinv64, h->dinv256, h->ustage, h->ftmp, h->split, h->gm,
                 h->perm, h->scale, h->scr, h->Brm, h->Xrm, h->R, h->RF, h->RY, h->part,
                 h->state, h->kpart};
   for (void* p : ps)
@@ -131,6 +133,7 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
   if (grow(&h->dinv64, &h->dinv64_cap, hpr::dinv64_elems(n) * eb)) return -2;
   if (grow(&h->dinv256, &h->dinv256_cap, hpr::dinv256_elems(n) * eb)) return -2;
   if (grow(&h->ustage, &h->ustage_cap, hpr::ustage_elems(n) * eb)) return -2;
+  if (h->lookahead && grow(&h->ftmp, &h->ftmp_cap, hpr::ustage_elems(n) * eb)) return -2;
   if (grow((void**)&h->perm, &h->perm_cap, (size_t)(n + 1) * sizeof(int))) return -2;
   if (grow((void**)&h->scale, &h->scale_cap, (size_t)(n + 1) * sizeof(double))) return -2;
   if ((method == HPRMAT_SPLIT_BF16 || method == HPRMAT_SPLIT_BF16X6) &&
@@ -161,6 +164,7 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
     w.dinv64 = (double2*)h->dinv64;
     w.dinv256 = (double2*)h->dinv256;
     w.ustage = (double2*)h->ustage;
+    w.ftmp = side ? (double2*)h->ftmp : nullptr;
     w.perm = h->perm;
     if (!rc)
       rc = hpr::getrf<double>(F, ld, n, h->ipiv, h->dinfo, w, h->sc, h->plan, hpr::GEMM_4M, st,
@@ -172,6 +176,7 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
     w.dinv64 = (float2*)h->dinv64;
     w.dinv256 = (float2*)h->dinv256;
     w.ustage = (float2*)h->ustage;
+    w.ftmp = side ? (float2*)h->ftmp : nullptr;
     w.perm = h->perm;
     w.split = (unsigned short*)h->split;
     const int gmode = method == HPRMAT_MIXED_FP32_3M ? hpr::GEMM_3M

@@ -697,18 +697,27 @@ GemmProb<T> prob(int M, int N, int K, const cx<T>* A, int64_t lda, const cx<T>* 
 // folded (its diagonal spans many orders of magnitude on Lagrange-mesh
 // matrices, and U[<b, b] inv(U_bb) in FP32 would cost refinement sweeps).
 // Row-wise transform, staged through `tmp` (256 x n elements), copied back.
+// The fold is a right multiplication of whole rows, so it commutes with the
+// row interchanges of later steps: the look-ahead stream folds finished block
+// columns during the factorization (fold_block), and fold_inverses does the
+// rest, from block column b_first on, after the last step.
 template <class T>
-void fold_inverses(cx<T>* A, int64_t ld, int n, const cx<T>* dinv256, cx<T>* tmp, hipStream_t st) {
-  for (int b0 = 0; b0 < n; b0 += 256) {
-    const int bs = min(256, n - b0), ml = n - b0 - bs;
-    if (ml <= 0) break;
-    const cx<T>* iL = dinv256 + (size_t)(b0 / 256) * 2 * 65536;
-    cx<T>* Lb = A + (int64_t)(b0 + bs) * ld + b0;
-    gemm<T>(ml, bs, bs, Lb, ld, false, iL, 256, tmp, bs, T(1), 0, false, nullptr, st);
-    const size_t cnt = (size_t)ml * bs;
-    copy2d_kernel<T><<<(int)min((size_t)4096, (cnt + 255) / 256), 256, 0, st>>>(ml, bs, tmp, bs,
-                                                                                 Lb, ld);
-  }
+void fold_block(cx<T>* A, int64_t ld, int n, int b0, const cx<T>* dinv256, cx<T>* tmp,
+                hipStream_t st) {
+  const int bs = min(256, n - b0), ml = n - b0 - bs;
+  if (ml <= 0) return;
+  const cx<T>* iL = dinv256 + (size_t)(b0 / 256) * 2 * 65536;
+  cx<T>* Lb = A + (int64_t)(b0 + bs) * ld + b0;
+  gemm<T>(ml, bs, bs, Lb, ld, false, iL, 256, tmp, bs, T(1), 0, false, nullptr, st);
+  const size_t cnt = (size_t)ml * bs;
+  copy2d_kernel<T><<<(int)min((size_t)4096, (cnt + 255) / 256), 256, 0, st>>>(ml, bs, tmp, bs,
+                                                                               Lb, ld);
+}
+
+template <class T>
+void fold_inverses(cx<T>* A, int64_t ld, int n, const cx<T>* dinv256, cx<T>* tmp, hipStream_t st,
+                   int b_first = 0) {
+  for (int b0 = b_first; b0 < n; b0 += 256) fold_block<T>(A, ld, n, b0, dinv256, tmp, st);
 }
 
 template <class T>
@@ -730,6 +739,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
   fill_u32_kernel<<<1, 64, 0, st>>>((unsigned*)info, 1, 0u);
   int leaf = 0;
   bool rest_pending = false;
+  int folded = 0;   // block columns [0, folded) already hold L[>b, b] inv(L_bb)
   for (int k = 0; k < n; k += nb) {
     const int kb = min(nb, n - k), m = n - k;
     // -- panel: 32-column cooperative leaves, right-looking inside the panel.
@@ -822,6 +832,13 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       // after the event the main stream waits for (run concurrently with the
       // update they would be starved by its workgroups)
       if (k > 0) launch_laswp<T>(A, ld, 0, k, k, kb, ipiv, nullptr, cs);
+      // in the second half, where the panel chain outlasts the update, this
+      // stream has slack: fold inv(L_bb) into up to two finished block
+      // columns b0 < k per step (the current panel's column is still read
+      // by this step's updates)
+      if (w.ftmp && 2 * nc <= n)
+        for (int f = 0; f < 2 && folded + 256 <= k; ++f, folded += 256)
+          fold_block<T>(A, ld, n, folded, w.dinv256, w.ftmp, cs);
     } else {
       // no look-ahead (or the last steps): everything on the main stream,
       // after all side-stream work (the left interchanges included)
@@ -858,7 +875,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
   diag_inv_kernel<T><<<dim3((n + 63) / 64, 1), 256, 0, st>>>(A, ld, n, 0, w.dinv64, 0, 0, 0, 1);
   blk_inv_kernel<T><<<dim3((n + 255) / 256, 1, 4), 256, 0, st>>>(A, ld, n, 0, w.dinv64, w.dinv256,
                                                                   0, 0, 0, 0, 1);
-  fold_inverses<T>(A, ld, n, w.dinv256, w.ustage, st);
+  fold_inverses<T>(A, ld, n, w.dinv256, w.ustage, st, folded);
   const size_t perm_lds = (size_t)n * sizeof(int);
   const int in_lds = perm_lds <= 140 * 1024;
   static bool attr = false;

@@ -78,6 +78,9 @@ template <class T> struct FactorWork {
   cx<T>* ustage = nullptr;
   int* perm = nullptr;
   unsigned short* split = nullptr;   // GEMM_BF16X3 / GEMM_BF16X6 only
+  // staging of the look-ahead stream's folds of inv(L_bb) into finished
+  // block columns (ustage_elems(n)); null: every fold after the last step
+  cx<T>* ftmp = nullptr;
 };
 
 // Second stream of the look-ahead: the trailing update of all but the next
