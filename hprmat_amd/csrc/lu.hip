@@ -605,17 +605,6 @@ void trailing(bool bfx, int nc, int ncols, int c0, int kb, const cx<T>* L, int64
           prio);
 }
 
-template <class T>
-void gemm_dual(const GemmProb<T>& p1, const GemmProb<T>& p2, const int* skip, hipStream_t st) {
-  constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32;
-  auto tiles = [](const GemmProb<T>& p) {
-    return (p.M > 0 && p.N > 0 && p.K > 0) ? ((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN) : 0;
-  };
-  const int t1 = tiles(p1), t2 = tiles(p2);
-  if (t1 + t2 == 0) return;
-  zgemm_dual_kernel<T, BM, BN, BK, WM, WN><<<t1 + t2, 256, 0, st>>>(p1, p2, t1, skip);
-}
-
 // Tile of the substitutions' block updates (M x N, K = 256): 0 = 64 x 64,
 // 1 = 32 x 64, 2 = 32 x 32 (HPRMAT_HIP_SUBST_TILE forces one; -1 = by size).
 // A launch takes about one tile time per round of workgroups over the 256
