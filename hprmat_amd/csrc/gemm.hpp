@@ -241,8 +241,16 @@ zgemm_dual_kernel(GemmProb<T> p1, GemmProb<T> p2, int tiles1, const int* __restr
 // (BM2 x BN2 tiles, waves of WM2 x WN2).
 template <class T, bool LOWER, int BM2, int BN2, int WM2, int WN2>
 __global__ void __launch_bounds__(256)
-tri_dual_kernel(GemmProb<T> p1, GemmProb<T> p2, int tiles1, const int* __restrict__ skip) {
-  if (skip && *skip) return;
+tri_dual_kernel(GemmProb<T> p1, GemmProb<T> p2, int tiles1, const int* __restrict__ skip,
+                int sSkip = 0) {
+  const int b = blockIdx.y;   // system of a batch
+  if (skip && skip[(int64_t)b * sSkip]) return;
+  p1.A += b * p1.sA;
+  p1.B += b * p1.sB;
+  p1.C += b * p1.sC;
+  p2.A += b * p2.sA;
+  p2.B += b * p2.sB;
+  p2.C += b * p2.sC;
   const int t = blockIdx.x;
   if (t < tiles1) {
     constexpr int TB = 32;

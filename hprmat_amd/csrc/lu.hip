@@ -1060,6 +1060,18 @@ void gemm_b(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const 
             int count, int64_t sA, int64_t sB, int64_t sC, hipStream_t st, bool g3m = false) {
   if (M <= 0 || N <= 0 || K <= 0 || count <= 0) return;
   constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32;
+  if (N <= 32 && !g3m) {
+    // narrow products (a batch's right-hand sides): 64 x 32 tiles, no idle
+    // half tile, each element summed in the same order
+    const dim3 g32(((M + 63) / 64) * ((N + 31) / 32), count);
+    if (a_col)
+      zgemm_kernel<T, 64, 32, BK, 32, 16, true, false><<<g32, 256, 0, st>>>(
+          M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, sA, sB, sC, sSkip);
+    else
+      zgemm_kernel<T, 64, 32, BK, 32, 16, false, false><<<g32, 256, 0, st>>>(
+          M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, sA, sB, sC, sSkip);
+    return;
+  }
   const dim3 grid(((M + BM - 1) / BM) * ((N + BN - 1) / BN), count);
   if (a_col)
     zgemm_kernel<T, BM, BN, BK, WM, WN, true, false><<<grid, 256, 0, st>>>(
@@ -1272,16 +1284,29 @@ int getrs_batched(const BatchArgs<T>& b, cx<T>* B, cx<T>* Y, int nrhs, const int
                              T(-1), 1);
     p1.sA = b.s256; p1.sB = sV; p1.sC = sV;
     p2.sA = b.sF; p2.sB = sV; p2.sC = sV;
-    const int t1 = tiles(p1), t2 = tiles(p2);
-    zgemm_dual_kernel<T, 64, 64, 16, 32, 32><<<dim3(t1 + t2, count), 256, 0, st>>>(p1, p2, t1, skip,
-                                                                                   sSkip);
+    // the product with inv(L_bb) on 32 x 32 tiles over its nonzero triangle
+    // (tri_dual_kernel), the update on 64 x 32 tiles for narrow batches
+    const int t1 = ((bs + 31) / 32) * ((nrhs + 31) / 32);
+    if (nrhs <= 32) {
+      const int t2 = p2.M > 0 ? ((p2.M + 63) / 64) * ((nrhs + 31) / 32) : 0;
+      tri_dual_kernel<T, true, 64, 32, 32, 16><<<dim3(t1 + t2, count), 256, 0, st>>>(
+          p1, p2, t1, skip, sSkip);
+    } else {
+      const int t2 = tiles(p2);
+      tri_dual_kernel<T, true, 64, 64, 32, 32><<<dim3(t1 + t2, count), 256, 0, st>>>(
+          p1, p2, t1, skip, sSkip);
+    }
   }
   // backward: B_b = inv(U_bb) Y_b, then Y_<b -= U[<b, b] B_b
   for (int r = ((n - 1) / 256) * 256; r >= 0; r -= 256) {
     const int bs = min(256, n - r);
-    gemm_b<T>(bs, nrhs, bs, b.dinv256 + ((size_t)(r / 256) * 2 + 1) * 65536, 256, false,
-              Y + (int64_t)r * nrhs, nrhs, B + (int64_t)r * nrhs, nrhs, T(1), 0, skip, sSkip,
-              count, b.s256, sV, sV, st);
+    GemmProb<T> pu = prob<T>(bs, nrhs, bs, b.dinv256 + ((size_t)(r / 256) * 2 + 1) * 65536, 256,
+                             Y + (int64_t)r * nrhs, nrhs, B + (int64_t)r * nrhs, nrhs, T(1), 0);
+    pu.sA = b.s256; pu.sB = sV; pu.sC = sV;
+    const GemmProb<T> none = prob<T>(0, 0, 0, pu.A, 256, Y, nrhs, B, nrhs, T(1), 0);
+    const int tu = ((bs + 31) / 32) * ((nrhs + 31) / 32);
+    tri_dual_kernel<T, false, 64, 64, 32, 32><<<dim3(tu, count), 256, 0, st>>>(pu, none, tu, skip,
+                                                                               sSkip);
     if (r > 0)
       gemm_b<T>(r, nrhs, bs, b.F + r, ld, false, B + (int64_t)r * nrhs, nrhs, Y, nrhs, T(-1), 1,
                 skip, sSkip, count, b.sF, sV, sV, st);
