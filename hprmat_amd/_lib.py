@@ -23,6 +23,7 @@ PLAN_TRAIL_TILE = 1      # hprmat_hip_lu_set_plan keys
 PLAN_LOOKAHEAD = 2
 PLAN_GMRES_CYCLE = 3
 FGMRES = 0x10            # flag OR-ed into the criterion: flexible GMRES refinement
+STATUS_TIMEOUT = -0x40000000   # batch status: a panel leaf's bounded wait gave up
 
 REFERENCE_SYMBOLS = (
     'gpu_solver_init_', 'gpu_solver_finalize_', 'gpu_host_unregister_',
@@ -45,7 +46,7 @@ NATIVE_SYMBOLS = (
 PHYSICS_SYMBOLS = (
     'hprmat_init', 'hprmat_set_solver', 'hprmat_get_solver', 'hprmat_solve',
     'hprmat_solve_linear', 'hprmat_version', 'hprmat_solver_info', 'hprmat_solve_full',
-    'hprmat_wavefunction',
+    'hprmat_wavefunction', 'hprmat_gpu_finalize',
 )
 
 
@@ -62,12 +63,13 @@ class Stats(ctypes.Structure):
     _fields_ = [('info', ctypes.c_int), ('sweeps', ctypes.c_int), ('checks', ctypes.c_int),
                 ('converged', ctypes.c_int), ('res_max', ctypes.c_double),
                 ('res_rel', ctypes.c_double), ('b_fro', ctypes.c_double),
-                ('hist', ctypes.c_double * 32)]
+                ('hist', ctypes.c_double * 32), ('diverged', ctypes.c_int),
+                ('pad_', ctypes.c_int)]
 
     def as_dict(self):
         return dict(info=self.info, sweeps=self.sweeps, checks=self.checks,
                     converged=bool(self.converged), res_max=self.res_max,
-                    res_rel=self.res_rel, b_fro=self.b_fro,
+                    res_rel=self.res_rel, b_fro=self.b_fro, diverged=bool(self.diverged),
                     hist=[self.hist[i] for i in range(min(self.checks, 32))])
 
 
@@ -163,6 +165,8 @@ def load():
     lib.hprmat_version.restype = None
     lib.hprmat_solver_info.argtypes = [I, ctypes.c_char_p, I]
     lib.hprmat_solver_info.restype = None
+    lib.hprmat_gpu_finalize.argtypes = []
+    lib.hprmat_gpu_finalize.restype = None
     for name in REFERENCE_SYMBOLS + NATIVE_SYMBOLS + PHYSICS_SYMBOLS:
         if getattr(lib, name, None) is None:
             raise RuntimeError('libhprmat_hip.so lacks symbol ' + name)

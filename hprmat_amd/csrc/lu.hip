@@ -126,6 +126,35 @@ __global__ void add_correction_kernel(int nrhs, size_t cnt, double2* __restrict_
   }
 }
 
+// Takes back the correction last added by add_correction_kernel (still in D)
+// from a system whose refinement stopped at residual evaluation `check`
+// because the residual grew (RefineState::diverged == check): X returns to
+// the previous, better iterate, and the sweep count with it.
+template <class TF>
+__global__ void undo_correction_kernel(int nrhs, size_t cnt, double2* __restrict__ X,
+                                       const cx<TF>* __restrict__ D, const double* __restrict__ s,
+                                       RefineState* __restrict__ st, int check, int64_t sV = 0,
+                                       int sS = 0) {
+  {
+    const int b = blockIdx.y;   // system of a batch
+    X += b * sV;
+    D += b * sV;
+    if (s) s += (int64_t)b * sS;
+    st += b;
+  }
+  if (st->diverged != check) return;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) st->sweeps -= 1;
+  for (; i < cnt; i += (size_t)gridDim.x * blockDim.x) {
+    cx<TF> d = D[i];
+    const double f = s ? s[i / nrhs] : 1.0;
+    double2 x = X[i];
+    x.x -= (double)d.x * f;
+    x.y -= (double)d.y * f;
+    X[i] = x;
+  }
+}
+
 // Symmetric equilibration factors s_i = ||row i of A||_2^(-1/2) (1 for a zero
 // row) of a column-major FP64 matrix: the factored matrix is diag(s) A diag(s).
 // One workgroup per 64 rows; its four waves sum every fourth column (each
@@ -201,9 +230,13 @@ norm_partial_kernel(int n, int nrhs, const double2* __restrict__ R, double* __re
 
 // Stage 2 (one block): combine the partials in a fixed order; mode 0 stores
 // the RHS norms, mode 1 the residual norms and the convergence decision.
+// With `guard` (classic refinement), a residual whose Frobenius norm exceeds
+// the previous evaluation's stops the refinement: `diverged` records the
+// evaluation, `done` is set, and the stored norms stay those of the previous
+// (better) iterate, which undo_correction_kernel restores.
 __global__ void __launch_bounds__(256)
 norm_final_kernel(int nblocks, const double* __restrict__ part, RefineState* __restrict__ st,
-                  int mode, int crit, double tol) {
+                  int mode, int crit, double tol, int guard) {
   part += (size_t)blockIdx.x * nblocks * 5;   // one workgroup per system of a batch
   st += blockIdx.x;
   if (mode == 1 && st->done) return;
@@ -235,17 +268,23 @@ norm_final_kernel(int nblocks, const double* __restrict__ part, RefineState* __r
       st->b_fro = fro;
       st->b_max = sqrt(sa[0]);
       st->done = 0; st->sweeps = 0; st->checks = 0;
-      st->gm_halt = 0; st->gm_steps = 0;
+      st->gm_halt = 0; st->gm_steps = 0; st->diverged = 0;
       st->res_max = 0.0; st->res_fro = 0.0; st->res_absmax = 0.0;
     } else {
-      st->res_max = sm[0];
-      st->res_absmax = sqrt(sa[0]);
-      st->res_fro = fro;
       double bf = st->b_fro != 0.0 ? st->b_fro : 1.0;
       if (st->checks < 32) st->hist[st->checks] = fro / bf;
+      const bool grew = guard && st->checks > 0 && fro > st->res_fro;
       st->checks += 1;
-      bool conv = (crit == CRIT_ABS_MAX) ? (sm[0] < tol) : (fro / bf <= tol);
-      if (conv) st->done = 1;
+      if (grew) {
+        st->diverged = st->checks;
+        st->done = 1;
+      } else {
+        st->res_max = sm[0];
+        st->res_absmax = sqrt(sa[0]);
+        st->res_fro = fro;
+        bool conv = (crit == CRIT_ABS_MAX) ? (sm[0] < tol) : (fro / bf <= tol);
+        if (conv) st->done = 1;
+      }
     }
   }
 }
@@ -383,6 +422,17 @@ int panel_prio_level() {
     return e ? atoi(e) : 2;
   }();
   return v;
+}
+
+// Initial value of the panel leaves' error flag.  0 in normal use;
+// HPRMAT_HIP_FORCE_BARRIER_TIMEOUT=1 starts a factorization with the flag
+// already raised, so every cross-workgroup wait that is not satisfied at
+// once gives up (each leaf column then takes no pivot and no update, and
+// every kernel ends): the host-side way to exercise the timeout report
+// (stats.info = -3, batch status HPRMAT_STATUS_TIMEOUT).  Read per call.
+unsigned barrier_err_init() {
+  const char* e = getenv("HPRMAT_HIP_FORCE_BARRIER_TIMEOUT");
+  return (e && e[0] == '1') ? 1u : 0u;
 }
 
 // Extra LDS of the look-ahead trailing update (bytes, HPRMAT_HIP_TRAIL_LDS;
@@ -724,7 +774,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
   const int pg = la && panel_prio_level() >= 2;   // ... and of the main-stream GEMMs
   (void)hipGetLastError();  // start from a clean error state (checked at the end)
   fill_u32_kernel<<<64, 256, 0, st>>>((unsigned*)s.xchg, s.xchg_bytes / sizeof(unsigned), 0u);
-  fill_u32_kernel<<<1, 64, 0, st>>>((unsigned*)s.err, 1, 0u);
+  fill_u32_kernel<<<1, 64, 0, st>>>((unsigned*)s.err, 1, barrier_err_init());
   fill_u32_kernel<<<1, 64, 0, st>>>((unsigned*)info, 1, 0u);
   int leaf = 0;
   bool rest_pending = false;
@@ -963,12 +1013,12 @@ int transpose_cast(int rows, int cols, const cx<Tin>* in, int64_t ldi, cx<Tout>*
 }
 
 int residual_norms(int n, int nrhs, const double2* R, double* part, RefineState* state, int mode,
-                   int crit, double tol, const int* skip, hipStream_t st) {
+                   int crit, double tol, const int* skip, hipStream_t st, int guard) {
   size_t total = (size_t)n * nrhs;
   int blocks = (int)min((size_t)NORM_BLOCKS, (total + 255) / 256);
   if (blocks < 1) blocks = 1;
   norm_partial_kernel<<<blocks, 256, 0, st>>>(n, nrhs, R, part, skip);
-  norm_final_kernel<<<1, 256, 0, st>>>(blocks, part, state, mode, crit, tol);
+  norm_final_kernel<<<1, 256, 0, st>>>(blocks, part, state, mode, crit, tol, guard);
   return 0;
 }
 
@@ -1002,7 +1052,7 @@ int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, con
                  const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
                  int max_refine, double tol, int crit, RefineState* state, double2* R, cx<TF>* RF,
                  cx<TF>* RY, double* part, const LuPlan& plan, bool g3m, hipStream_t st,
-                 const double* scale, const SplitWork& sw, bool poll) {
+                 const double* scale, const SplitWork& sw, bool poll, bool final_check) {
   const size_t cnt = (size_t)n * nrhs;
   const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
   const int* done = &state->done;
@@ -1011,29 +1061,23 @@ int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, con
   if (getrs<TF>(LU, ld, n, dinv256, RF, nrhs, nrhs, RY, nullptr, plan, g3m, st)) return -1;
   add_correction_kernel<TF><<<eg, 256, 0, st>>>(nrhs, cnt, X, RF, scale, 0, state);
   const FlagPoll fp = flag_poll(st, poll);
-  for (int it = 0; it < max_refine; ++it) {
+  // Evaluation `it` checks the iterate after `it` corrections; the last one
+  // (it == max_refine) runs only with final_check.  A residual that grows
+  // stops the sweeps and takes the last correction back
+  // (RefineState::diverged): the solve returns its best iterate.
+  for (int it = 0; it <= max_refine; ++it) {
+    if (it == max_refine && !final_check) break;
     cast_kernel<double, double><<<eg, 256, 0, st>>>(cnt, B, R, done);
     if (resid_gemm(n, nrhs, A64, lda, X, R, -1.0, 1, done, sw, st)) return -1;
-    residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
+    residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st, 1);
+    if (it > 0)
+      undo_correction_kernel<TF><<<eg, 256, 0, st>>>(nrhs, cnt, X, RF, scale, state, it + 1);
+    if (it == max_refine) break;
     if (fp.read(state, st, &RefineState::done)) break;
     permute_cast<double, TF>(n, nrhs, perm, R, RF, done, st, scale);
     if (getrs<TF>(LU, ld, n, dinv256, RF, nrhs, nrhs, RY, done, plan, g3m, st)) return -1;
     add_correction_kernel<TF><<<eg, 256, 0, st>>>(nrhs, cnt, X, RF, scale, 1, state);
   }
-  HPR_CHECK(hipGetLastError());
-  return 0;
-}
-
-// Residual check without correction (final verification of a refined X).
-int residual_check(int n, int nrhs, const double2* A64, int64_t lda, const double2* B,
-                   const double2* X, double2* R, double* part, RefineState* state, int crit,
-                   double tol, hipStream_t st, const SplitWork& sw) {
-  const size_t cnt = (size_t)n * nrhs;
-  const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
-  const int* done = &state->done;
-  cast_kernel<double, double><<<eg, 256, 0, st>>>(cnt, B, R, done);
-  if (resid_gemm(n, nrhs, A64, lda, X, R, -1.0, 1, done, sw, st)) return -1;
-  residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
   HPR_CHECK(hipGetLastError());
   return 0;
 }
@@ -1115,24 +1159,29 @@ inline dim3 copy_grid(size_t cnt, int count) {
 
 void norms_b(int n, int nrhs, const double2* R, int64_t sR, double* part, RefineState* state,
              int mode, int crit, double tol, const int* skip, int sSkip, int count,
-             hipStream_t st) {
+             hipStream_t st, int guard = 0) {
   const size_t total = (size_t)n * nrhs;
   int blocks = (int)min((size_t)BATCH_NORM_BLOCKS, (total + 255) / 256);
   if (blocks < 1) blocks = 1;
   norm_partial_kernel<<<dim3(blocks, count), 256, 0, st>>>(n, nrhs, R, part, skip, sR, sSkip);
-  norm_final_kernel<<<count, 256, 0, st>>>(blocks, part, state, mode, crit, tol);
+  norm_final_kernel<<<count, 256, 0, st>>>(blocks, part, state, mode, crit, tol, guard);
 }
 
 // per-system outcome: relative residual of the last evaluation and
-// 1 converged / 0 not converged / -info (exactly singular pivot)
+// 1 converged / 0 not converged / -info (exactly singular pivot), or
+// STATUS_TIMEOUT for every system when a panel leaf's bounded wait gave up
+// (the factors are then invalid)
 __global__ void batch_result_kernel(int count, const RefineState* __restrict__ st,
-                                    const int* __restrict__ info, double* __restrict__ res,
-                                    int* __restrict__ conv) {
+                                    const int* __restrict__ info, const int* __restrict__ err,
+                                    double* __restrict__ res, int* __restrict__ conv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const double bf = st[i].b_fro != 0.0 ? st[i].b_fro : 1.0;
   if (res) res[i] = st[i].res_fro / bf;
-  if (conv) conv[i] = info[i] != 0 ? -info[i] : st[i].done;
+  if (conv)
+    conv[i] = *err != 0     ? STATUS_TIMEOUT
+              : info[i] != 0 ? -info[i]
+                             : (st[i].done && !st[i].diverged ? 1 : 0);
 }
 }  // namespace
 
@@ -1203,7 +1252,7 @@ int getrf_batched(const BatchArgs<T>& b, hipStream_t st) {
   const int64_t ld = b.ld, sF = b.sF;
   (void)hipGetLastError();   // start from a clean error state (checked at the end)
   fill_u32_kernel<<<1024, 256, 0, st>>>((unsigned*)b.xchg, (size_t)b.sX * 2 * count, 0u);
-  fill_u32_kernel<<<1, 64, 0, st>>>((unsigned*)b.err, 1, 0u);
+  fill_u32_kernel<<<1, 64, 0, st>>>((unsigned*)b.err, 1, barrier_err_init());
   fill_u32_kernel<<<(count + 255) / 256, 256, 0, st>>>((unsigned*)b.info, (size_t)count, 0u);
   int leaf = 0;
   for (int k = 0; k < n; k += 256) {
@@ -1371,7 +1420,10 @@ int solve_batched(const BatchArgs<TF>& b, const double2* A64, int64_t lda, int64
       gemm_b<double>(n, nrhs, n, A64, lda, true, b.Xrm, nrhs, b.R, nrhs, -1.0, 1, done, sDone,
                      count, sA, sV, sV, st);
     }
-    norms_b(n, nrhs, b.R, sV, b.part, b.state, 1, crit, tol, done, sDone, count, st);
+    norms_b(n, nrhs, b.R, sV, b.part, b.state, 1, crit, tol, done, sDone, count, st, 1);
+    if (it > 0)   // a system whose residual grew returns its previous iterate
+      undo_correction_kernel<TF><<<eg, 256, 0, st>>>(nrhs, cnt, b.Xrm, b.RF, b.scale, b.state,
+                                                     it + 1, sV, b.sIv);
     if (it == max_refine) break;
     permute_cast_kernel<double, TF><<<eg, 256, 0, st>>>(n, nrhs, b.perm, b.R, b.RF, b.scale, done,
                                                         b.sIv, sV, sDone);
@@ -1382,8 +1434,8 @@ int solve_batched(const BatchArgs<TF>& b, const double2* A64, int64_t lda, int64
   transpose_cast_kernel<double, double>
       <<<dim3((nrhs + 31) / 32, (n + 31) / 32, count), 256, 0, st>>>(
           nrhs, n, b.Xrm, nrhs, dB, ldb, nullptr, nullptr, sV, sB, 0);
-  batch_result_kernel<<<(count + 255) / 256, 256, 0, st>>>(count, b.state, b.info, res_out,
-                                                           conv_out);
+  batch_result_kernel<<<(count + 255) / 256, 256, 0, st>>>(count, b.state, b.info, b.err,
+                                                           res_out, conv_out);
   HPR_CHECK(hipGetLastError());
   return 0;
 }
@@ -1399,7 +1451,7 @@ int solve_batched(const BatchArgs<TF>& b, const double2* A64, int64_t lda, int64
                                const double2*, int64_t, const double2*, double2*, int, int,    \
                                double, int, RefineState*, double2*, cx<T>*, cx<T>*, double*,   \
                                const LuPlan&, bool, hipStream_t, const double*,               \
-                               const SplitWork&, bool);                                        \
+                               const SplitWork&, bool, bool);                                  \
   template int batch_bind<T>(BatchArgs<T>&, void*, size_t, int, int, int, bool);              \
   template int getrf_batched<T>(const BatchArgs<T>&, hipStream_t);                             \
   template int getrs_batched<T>(const BatchArgs<T>&, cx<T>*, cx<T>*, int, const int*, int,     \

@@ -37,9 +37,17 @@ struct RefineState {
   double hist[32];   // ||r||_F / ||b||_F after each residual evaluation
   int gm_halt;       // FGMRES: no further Arnoldi step in this cycle (or done)
   int gm_steps;      // FGMRES: Arnoldi steps taken in this cycle
+  int diverged;      // classic refinement: the residual evaluation (1-based
+                     // count) at which ||r||_F grew; 0 = never.  `done` is
+                     // set with it, and the last correction is undone.
+  int pad2;
 };
 
 enum Criterion { CRIT_ABS_MAX = 0, CRIT_REL_FRO = 1 };
+
+// Batch status of a system whose factorization's panel leaf gave up a wait
+// (HPRMAT_STATUS_TIMEOUT of the C ABI).
+constexpr int STATUS_TIMEOUT = -0x40000000;
 
 // Host view of the refinement flags.  When the caller allows it (a solve
 // that synchronises anyway) and the stream is not being captured, the
@@ -186,26 +194,24 @@ int resid_gemm(int n, int nrhs, const double2* A64, int64_t lda, const double2* 
 // Work buffers: R (n x nrhs FP64), RF and RY (n x nrhs in factor precision),
 // part (>= 5120 doubles).  With scale non-null the factors are those of
 // diag(s) A diag(s): right-hand sides are scaled by s before the solves and
-// the solves' results by s after them.
+// the solves' results by s after them.  At most max_refine corrections, each
+// after a residual evaluation; final_check evaluates the last iterate too.
+// An evaluation whose ||r||_F exceeds the previous one stops the refinement
+// and undoes the last correction (RefineState::diverged; not converged).
 template <class TF>
 int refine_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, const int* perm,
                  const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
                  int max_refine, double tol, int crit, RefineState* state, double2* R,
                  cx<TF>* RF, cx<TF>* RY, double* part, const LuPlan& plan, bool gauss3m,
                  hipStream_t st, const double* scale, const SplitWork& sw = SplitWork(),
-                 bool poll = false);
-
-// One residual evaluation of a refined X (skipped when state->done is set):
-// R = B - A64 X, norms and the stopping rule recorded in state.
-int residual_check(int n, int nrhs, const double2* A64, int64_t lda, const double2* B,
-                   const double2* X, double2* R, double* part, RefineState* state, int crit,
-                   double tol, hipStream_t st, const SplitWork& sw = SplitWork());
+                 bool poll = false, bool final_check = false);
 
 // Norms of an n x nrhs FP64 block R (row-major): mode 0 records ||B|| and
 // resets the state, mode 1 records a residual evaluation and the stopping
-// rule (skipped when *skip is set; mode 1 also when state->done).
+// rule (skipped when *skip is set; mode 1 also when state->done).  guard:
+// stop (RefineState::diverged) when ||R||_F exceeds the previous evaluation.
 int residual_norms(int n, int nrhs, const double2* R, double* part, RefineState* state, int mode,
-                   int crit, double tol, const int* skip, hipStream_t st);
+                   int crit, double tol, const int* skip, hipStream_t st, int guard = 0);
 
 // Workspace of gmres_solve for n x nrhs blocks and cycles of m <= GMRES_MAX_M steps.
 struct GmresWork {

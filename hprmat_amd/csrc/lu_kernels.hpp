@@ -51,42 +51,44 @@ __device__ inline void put_real(u64* p, unsigned tag, double v) {
   put_word(p + 1, tag, (unsigned)(b >> 32));
 }
 // Spin (bounded, watching *err) until every word of a real carries the tag.
-template <class T> __device__ inline T get_real(const u64* p, unsigned tag, int* err);
-template <> __device__ inline float get_real<float>(const u64* p, unsigned tag, int* err) {
+// A wait that gives up clears `ok`: the value it returns is then stale and
+// must not be used as an index.
+template <class T> __device__ inline T get_real(const u64* p, unsigned tag, int* err, bool& ok);
+template <> __device__ inline float get_real<float>(const u64* p, unsigned tag, int* err, bool& ok) {
   unsigned spins = 0;
   u64 w = get_word(p);
   while (!tag_ok(w, tag)) {
     __builtin_amdgcn_s_sleep(1);
     if ((++spins & 255u) == 0) {
-      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-      if (spins > (1u << 22)) { atomicExch(err, 1); break; }
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = false; break; }
+      if (spins > (1u << 22)) { atomicExch(err, 1); ok = false; break; }
     }
     w = get_word(p);
   }
   return __uint_as_float((unsigned)w);
 }
-template <> __device__ inline double get_real<double>(const u64* p, unsigned tag, int* err) {
+template <> __device__ inline double get_real<double>(const u64* p, unsigned tag, int* err, bool& ok) {
   unsigned spins = 0;
   u64 lo = get_word(p), hi = get_word(p + 1);
   while (!tag_ok(lo, tag) || !tag_ok(hi, tag)) {
     __builtin_amdgcn_s_sleep(1);
     if ((++spins & 255u) == 0) {
-      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-      if (spins > (1u << 22)) { atomicExch(err, 1); break; }
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = false; break; }
+      if (spins > (1u << 22)) { atomicExch(err, 1); ok = false; break; }
     }
     lo = get_word(p);
     hi = get_word(p + 1);
   }
   return __longlong_as_double((long long)((lo & 0xffffffffull) | ((hi & 0xffffffffull) << 32)));
 }
-__device__ inline int get_int(const u64* p, unsigned tag, int* err) {
+__device__ inline int get_int(const u64* p, unsigned tag, int* err, bool& ok) {
   unsigned spins = 0;
   u64 w = get_word(p);
   while (!tag_ok(w, tag)) {
     __builtin_amdgcn_s_sleep(1);
     if ((++spins & 255u) == 0) {
-      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-      if (spins > (1u << 22)) { atomicExch(err, 1); break; }
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = false; break; }
+      if (spins > (1u << 22)) { atomicExch(err, 1); ok = false; break; }
     }
     w = get_word(p);
   }
@@ -302,11 +304,16 @@ leaf_getf2_kernel(cx<T>* __restrict__ P, int64_t ld, int m, int w, int row_base,
         // gather the G candidates and reduce
         T qv = T(-1);
         int qi = 0x7fffffff;
+        bool ok = true;
         for (int q = lane; q < G; q += 64) {
-          const T ov = get_real<T>(x.val + ((size_t)par * gs + q) * WPR, tag, err);
-          const int oi = get_int(x.idx + (size_t)par * gs + q, tag, err);
+          const T ov = get_real<T>(x.val + ((size_t)par * gs + q) * WPR, tag, err, ok);
+          const int oi = get_int(x.idx + (size_t)par * gs + q, tag, err, ok);
           if (ov > qv || (ov == qv && oi < qi)) { qv = ov; qi = oi; }
         }
+        // a wait that gave up (the error flag is then set and reported to the
+        // host) leaves a stale candidate: take no pivot and no update in this
+        // column, so every index used below stays in range and the kernel ends
+        if (__ballot(!ok) != 0ull) { qv = T(-1); qi = 0x7fffffff; }
         wave_argmax(qv, qi);
         const bool have = qv >= T(0);
         const int pp = have ? qi : j;
@@ -317,7 +324,7 @@ leaf_getf2_kernel(cx<T>* __restrict__ P, int64_t ld, int m, int w, int row_base,
         unsigned* fw = reinterpret_cast<unsigned*>(prow[par]);
         unsigned* uw = reinterpret_cast<unsigned*>(pupd[par]);
         for (int e = lane; e < W * RW; e += 64) {
-          const unsigned u = have ? get_u32(src + e, tag, err) : 0u;
+          const unsigned u = have ? get_u32(src + e, tag, err) : 0u;   // values only
           fw[e] = u;
           uw[e] = (e / RW < live) ? u : 0u;
         }

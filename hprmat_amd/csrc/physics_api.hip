@@ -786,4 +786,23 @@ void hprmat_solver_info(int solver, char* info, int maxlen) {
   snprintf(info, (size_t)maxlen, "%s", s);
 }
 
+// bindings/c/hprmat.h:187-197.  The reference drops the page-locked
+// registration of the caller's matrix here; this interface never page-locks
+// caller memory, so it releases what it does keep between calls: the device
+// buffers, the LU handle and the stream (recreated by the next call).
+void hprmat_gpu_finalize(void) {
+  if (g_stream) (void)hipStreamSynchronize(g_stream);
+  if (g_lu) {
+    (void)hprmat_hip_lu_destroy(g_lu);
+    g_lu = nullptr;
+  }
+  for (Slot& s : g_slots)
+    if (s.p) (void)hipFree(s.p);
+  g_slots.clear();
+  if (g_stream) {
+    (void)hipStreamDestroy(g_stream);
+    g_stream = nullptr;
+  }
+}
+
 }  // extern "C"

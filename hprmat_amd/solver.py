@@ -259,8 +259,9 @@ class Batch:
         matrices; ``b_cm``: (count, nrhs, n) column-major storages of the
         right-hand sides, overwritten with the solutions.  ``count`` (default:
         the batch size) solves only the first systems.  Per-system relative
-        residuals and statuses (1 converged, 0 not, -info singular) are left
-        in ``self.res`` / ``self.status`` on the device."""
+        residuals and statuses (1 converged, 0 not, -info singular,
+        ``_lib.STATUS_TIMEOUT`` when a panel leaf gave up a wait) are left in
+        ``self.res`` / ``self.status`` on the device."""
         _check_tensor(a_cm, 'a_cm')
         _check_tensor(b_cm, 'b_cm')
         c = self.count if count is None else int(count)

@@ -114,7 +114,9 @@ enum {
 };
 
 typedef struct hprmat_hip_stats {
-  int info;        /* 0 ok, > 0 zero pivot (1-based)                     */
+  int info;        /* 0 ok, > 0 zero pivot (1-based), -3 a panel leaf's
+                      bounded cross-workgroup wait gave up (the factors and
+                      the solution are invalid)                            */
   int sweeps;      /* corrections applied                                */
   int checks;      /* residual evaluations                               */
   int converged;   /* stopping rule met at the last evaluation           */
@@ -122,7 +124,15 @@ typedef struct hprmat_hip_stats {
   double res_rel;  /* ||B - A X||_F / ||B||_F at the last evaluation     */
   double b_fro;    /* ||B||_F                                            */
   double hist[32]; /* relative residual after each evaluation            */
+  int diverged;    /* classic refinement stopped because ||B - A X||_F
+                      grew; the last correction was undone, so X is the
+                      best iterate (res_rel / res_max are its residual)   */
+  int pad_;
 } hprmat_hip_stats;
+
+/* Batch status of every system when a panel leaf's bounded wait gave up
+   (hprmat_hip_batch_solve*; cannot collide with a zero-pivot -info). */
+enum { HPRMAT_STATUS_TIMEOUT = -0x40000000 };
 
 typedef struct hprmat_hip_lu *hprmat_hip_lu_t;
 
@@ -357,6 +367,11 @@ int hprmat_hip_propagate(int nch, int nr, int ns, const void *dX, int64_t ldx, i
                          size_t work_bytes, void *stream);
 
 /* ---------------- Part 3: host-pointer physics interface --------------- */
+/* bindings/c/hprmat.h:20-24: solver type constants of the C binding. */
+#define HPRMAT_SOLVER_DENSE     1   /* FP64 LU (the reference's LAPACK ZGESV type)   */
+#define HPRMAT_SOLVER_MIXED     2   /* FP32 LU + two FP64 sweeps (1e-10)             */
+#define HPRMAT_SOLVER_WOODBURY  3   /* the reference's CPU Woodbury: FP64 result here */
+#define HPRMAT_SOLVER_GPU       4   /* FP32 LU + FGMRES refinement to 1e-12          */
 /* The reference's C binding (bindings/c/hprmat.h:30-166, implemented in
    bindings/c/hprmat_c.F90:25-261), same names and argument meaning; complex
    arrays are double _Complex (interleaved re, im), column-major, and
@@ -419,6 +434,12 @@ int hprmat_hip_mesh(int nr, int ns, double rmax, double *tc, double *bloch, doub
 /* bindings/c/hprmat_c.F90:273-287 and :295: version and solver description. */
 void hprmat_version(char *version, int maxlen);
 void hprmat_solver_info(int solver, char *info, int maxlen);
+/* bindings/c/hprmat.h:187-197: the reference releases its cached page-locked
+   registration of the caller's matrix.  This library never page-locks
+   caller memory; the call releases the interface's cached device buffers,
+   LU handle and stream (all recreated on the next call).  Safe to call at
+   any time from the thread that calls the interface, also repeatedly. */
+void hprmat_gpu_finalize(void);
 
 #ifdef __cplusplus
 }
