@@ -94,15 +94,23 @@ def test_singular_matrix_reports_pivot_and_leaves_b():
     assert np.array_equal(bb, b)
 
 
+def wilkinson(n=100):
+    """Wilkinson's growth matrix (1 on the diagonal, -1 below, 1 in the last
+    column): partial pivoting takes no interchange and the last column grows
+    like 2^(n-1), so an FP32 LU is no usable factor (LAPACK cgetrf of the
+    equilibrated n = 100 matrix + FP64 refinement on the CPU: ||r||_max /
+    (||b||_max + 1) = 2e20, 1.3e13 after one sweep, 2e7 after two)."""
+    a = np.eye(n) - np.tril(np.ones((n, n)), -1)
+    a[:, -1] = 1.0
+    return np.asfortranarray(a.astype(complex))
+
+
 def test_multi_mixed_guard_on_hopeless_refinement():
-    # exact-condition Householder matrix, kappa = 1e12: the FP32 factor cannot
-    # be refined with 1 sweep -> info 99 and B untouched
-    rng = np.random.default_rng(0)
-    n = 200
-    v = rng.standard_normal(n)
-    h = np.eye(n) - 2 * np.outer(v, v) / (v @ v)
-    a = np.asfortranarray((h @ np.diag(np.geomspace(1, 1e-12, n)) @ h).astype(complex))
-    b = np.asfortranarray(a @ np.ones((n, 1)))
+    # one sweep on a factor without accuracy -> info 99 and B untouched.
+    # (Backward-stable LU leaves a small residual however large kappa is, so
+    # the guard needs pivot growth, not just ill-conditioning, to fire.)
+    a = wilkinson()
+    b = np.asfortranarray(a @ np.ones((a.shape[0], 1)))
     rc, info, xb = call_refined('gpu_solve_multi_mixed_', a, b, 1, 1e-12)
     assert rc == -1 and info == 99
     assert np.array_equal(xb, b)

@@ -99,9 +99,12 @@ def test_tf32_entry_converges_or_reports_99_on_ill_conditioned_scan_system():
 
 
 def test_tf32_entry_reports_99_and_leaves_b_on_hopeless_system():
-    a = householder()
+    # Wilkinson's growth matrix: the reduced-precision factors carry no
+    # accuracy (pivot growth 2^99), two FGMRES products cannot fix that
+    from test_gpu_abi import wilkinson
+    a = wilkinson()
     b = a @ np.ones((a.shape[0], 1))
-    rc, info, x = host_call('gpu_solve_tf32_', a, b, 5, 1e-12)
+    rc, info, x = host_call('gpu_solve_tf32_', a, b, 2, 1e-12)
     assert rc == -1 and info == 99
     assert np.array_equal(x, b)
 
