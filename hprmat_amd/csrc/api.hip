@@ -5,6 +5,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "hprmat_hip.h"
 #include "lu.hpp"
 
@@ -692,6 +694,70 @@ int hprmat_hip_batch_solve_mesh(int method_flags, int n, int nrhs, int count, co
   m.sq = parts->stride_q;
   return batch_solve_impl(method_flags, n, nrhs, count, dA, lda, stride_a, dB, ldb, stride_b,
                           max_refine, tol, criterion, res_rel, status, work, work_bytes, stream, &m);
+}
+
+}  // extern "C"
+
+// ---- allocation-free batched solve ------------------------------------------
+// The workspace of hprmat_hip_batch_solve_noalloc: one array per GPU, held
+// from the library's first use on that GPU until the process ends.  Host code
+// never passes `scratch` itself (its host-side name is not a device address):
+// it passes the device address hipGetSymbolAddress returns for the current
+// GPU, looked up once per device and kept in g_scratch_dev.
+static __device__ __align__(256) unsigned char scratch[HPRMAT_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+void* g_scratch_dev[kMaxScratchDevices];
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current GPU (resolved on the first call
+// per device, with that device current).
+int scratch_on_current_device(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxScratchDevices) {
+    fprintf(stderr, "hprmat_hip: no usable current device for the scratch array (%s)\n",
+            hipGetErrorString(e));
+    return -1;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "hprmat_hip: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev,
+              hipGetErrorString(e));
+      return -1;
+    }
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int hprmat_hip_batch_solve_noalloc(int method_flags, int n, int nrhs, int count, const void* dA,
+                                   int64_t lda, int64_t stride_a, void* dB, int64_t ldb,
+                                   int64_t stride_b, int max_refine, double tol, int criterion,
+                                   double* res_rel, int* status, void* stream) {
+  if (count == 0 || n == 0) return 0;
+  const size_t need = hprmat_hip_batch_work_bytes(method_flags, n, nrhs, count);
+  if (need == 0) return -1;   // unsupported method or shape
+  if (need > (size_t)HPRMAT_NOALLOC_SCRATCH_BYTES) {
+    fprintf(stderr, "hprmat_hip_batch_solve_noalloc: %d systems of order %d with %d right-hand "
+                    "sides need %zu bytes of workspace, more than the %zu-byte scratch array; "
+                    "use hprmat_hip_batch_solve with a caller workspace\n",
+            count, n, nrhs, need, (size_t)HPRMAT_NOALLOC_SCRATCH_BYTES);
+    return -2;
+  }
+  void* work = nullptr;
+  if (scratch_on_current_device(&work)) return -1;
+  return batch_solve_impl(method_flags, n, nrhs, count, dA, lda, stride_a, dB, ldb, stride_b,
+                          max_refine, tol, criterion, res_rel, status, work,
+                          (size_t)HPRMAT_NOALLOC_SCRATCH_BYTES, stream, nullptr);
 }
 
 int hprmat_hip_prof_enable(int on) { return hpr::prof_enable(on); }

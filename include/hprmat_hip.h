@@ -341,6 +341,25 @@ int hprmat_hip_batch_solve_mesh(int method_flags, int n, int nrhs, int count, co
                                 double *res_rel, int *status, void *work, size_t work_bytes,
                                 const hprmat_hip_mesh_parts *parts, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc): hprmat_hip_batch_solve with the workspace taken from a
+   scratch array of HPRMAT_NOALLOC_SCRATCH_BYTES built into the library (one
+   per GPU, held from the first call on that GPU until the process ends)
+   instead of a caller buffer.  Same arguments and results as
+   hprmat_hip_batch_solve without work / work_bytes; calls no hipMalloc,
+   hipFree or allocating routine and does not synchronise.  Returns -2 (with
+   a message) when hprmat_hip_batch_work_bytes(method, n, nrhs, count)
+   exceeds the array, e.g. more than one FP32 system of order 2000.  Every
+   call on one GPU uses the same array: calls that overlap on one GPU (on
+   different streams, or from several threads) race and give wrong results;
+   order them on one stream. */
+#define HPRMAT_NOALLOC_SCRATCH_BYTES (64u << 20)
+int hprmat_hip_batch_solve_noalloc(int method_flags, int n, int nrhs, int count, const void *dA,
+                                   int64_t lda, int64_t stride_a, void *dB, int64_t ldb,
+                                   int64_t stride_b, int max_refine, double tol, int criterion,
+                                   double *res_rel, int *status, void *stream);
+
 /* Library version string. */
 const char *hprmat_hip_version(void);
 
