@@ -29,20 +29,39 @@ Methods timed back to back (name: factorization, refinement):
             (channel blocks and couplings, hprmat_hip_lu_set_operator)
             instead of the dense matrix; reported in their own block, not
             as the headline, which keeps the dense drop-in operator.
-Each method's solution is compared with the fp64 method's (max relative
-error of X and max |dS| of the S-matrix formed from it on the device).
 ``value`` is the solves/s of the fastest non-FP64 method that reaches the
-residual gate; per-method figures, the roofline of the dominant kernel (the
-trailing-update GEMM, timed with HIP events on the launch stream) and the
-CPU baseline (LAPACK zgesv + R, reference solver type 1) are in the line.
+residual gate; per-method figures and the roofline of the dominant kernel
+(the trailing-update GEMM, timed with HIP events on the launch stream) are
+in the line.
+
+Outside the timed regions, the same seeded system is solved on the host by
+the reference's CPU path (solver type 1: LAPACK ZGESV + R + S, restated by
+the oracle): that run is ``cpu_baseline`` (one run of the full workload on
+the host cores) and the FP64 host oracle every method's X and S are compared
+with (``err_x_vs_host``, ``dS_vs_host``).  Further blocks:
+  abi_wall        gpu_solve_mixed_ / gpu_solve_double_ on host arrays (the
+                  drop-in ABI: host-to-device copy of A included);
+  memory_scale    the N = 51200 system (256 channels, 256 right-hand sides,
+                  E = 6): FP32 + FGMRES and FP64 times, and the unitarity
+                  defect of S for the real-potential variant (W0 = 0);
+  gradient        time of one loss gradient over 32 energies of the N = 6400
+                  fit system (forward and adjoint solves), checked against
+                  the FP64 CPU adjoint on two of the energies;
+  energy_scan     64 partial waves x 256 energies as uniform batches against
+                  a serial FP64 loop, with the host FP64 floor of every
+                  system left above the gate.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-scan]
+                       [--no-abi] [--no-mem] [--no-grad]
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 METRIC = "solves/s & TFLOP/s, N=25600 complex LU+solve: mixed vs FP64 at resid≤1e-12"
 # Ceiling of the trailing update in algorithmic complex TFLOP/s (8 flops per
@@ -62,6 +81,11 @@ SPECS = {'fp64': ('fp64', 10, 'ir'), 'fp32': ('fp32', 30, 'ir'), 'fp32_ir5': ('f
          'fp32_3m_gmres': ('fp32_3m', 24, 'fgmres'),
          'bf16x6': ('bf16x6', 30, 'ir'), 'bf16x6_gmres': ('bf16x6', 24, 'fgmres'),
          'fp32_3m_gmres_op': ('fp32_3m', 24, 'fgmres'), 'fp64_op': ('fp64', 10, 'ir')}
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+
+def say(msg):
+    print('# ' + msg, file=sys.stderr, flush=True)
 
 
 def flops_solve(n, nrhs):
@@ -71,8 +95,7 @@ def flops_solve(n, nrhs):
 def traffic_record(prec):
     """Counter-measured HBM bytes of one trailing-update launch (rocprofv3
     FETCH_SIZE / WRITE_SIZE runs summarised under profiles/), or None."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
-                        'trailing_gemm_traffic.json')
+    path = os.path.join(ROOT, 'profiles', 'trailing_gemm_traffic.json')
     try:
         with open(path) as f:
             return json.load(f)[prec]
@@ -80,37 +103,116 @@ def traffic_record(prec):
         return None
 
 
-def cpu_baseline(nch=64, nr=200, reps=3):
-    """Reference solver type 1 (ZGESV + R extraction) on host cores, bounded sample."""
-    import numpy as np
+def host_threads():
+    return int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+
+
+def cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def host_oracle(nch, nr, sweeps=2):
+    """The reference's CPU path, solver type 1 (src/rmatrix_hp.F90:441-477:
+    copy of the matrix, LAPACK ZGESV with nch right-hand sides, R; then S,
+    :336-368) on the host cores, on the same seeded system the GPU solves,
+    assembled by the oracle.  One run (k = 1) of the full workload after a
+    512 x 512 warm-up of the BLAS pool (examples/Ex0/benchmark_large.f90:
+    84-98); ZGESV is run as its two halves ZGETRF + ZGETRS so that the
+    factors can then refine the solution: `sweeps` FP64 sweeps
+    X += LU^-1 (B - C X), untimed, give the accurate FP64 oracle (on this
+    kappa ~ 1e9 system ZGESV's residual is 5e-12, the refined one's the FP64
+    floor).  The matrix is exactly complex-symmetric, so its row-major array
+    is handed to LAPACK as the column-major matrix (a plain copy, as the
+    reference's A_work = cmat)."""
+    from scipy.linalg import lapack
     from threadpoolctl import threadpool_limits
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    from oracle import lagrange, rmatrix as ormat, solve as osolve, synth as osynth
-    threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    sys.path.insert(0, ROOT)
+    from oracle import channels, lagrange, rmatrix as ormat, solve as osolve, synth as osynth
+    threads = host_threads()
     lval, qk, cpot, _ = osynth.system(nch, nr, 15.0, 8.0, 5.0, 0)
     mesh = lagrange.rmat_ini(nr, 1, 15.0)
-    from oracle import channels
-    _, _, _, xc1 = channels.channel_functions(lval, qk, 15.0)
+    npo, co, cop, xc1 = channels.channel_functions(lval, qk, 15.0)
+    t0 = time.perf_counter()
     c = ormat.assemble(mesh, lval, qk, cpot, xc1)
-    b = ormat.rhs(mesh['q2'][0], nch)
-    n = nch * nr
+    q2 = mesh['q2'][0]
+    b = ormat.rhs(q2, nch)
+    say('host oracle: assembled N=%d in %.1f s; LAPACK zgesv on %d threads'
+        % (c.shape[0], time.perf_counter() - t0, threads))
+
+    def r_and_s(x):
+        r = np.einsum('r,irj->ij', q2, x.reshape(nch, nr, nch))   # R = q2^T X per block
+        return r, ormat.r_to_s(r, 15.0, qk, npo, co, cop)
+
     with threadpool_limits(limits=threads):
-        osolve.zgesv(c[:512, :512], b[:512, :4])          # warm-up of the BLAS pool
-        ts = []
-        for _ in range(reps):
+        osolve.zgesv(c[:512, :512], b[:512, :4])            # warm-up of the BLAS pool
+        t0 = time.perf_counter()
+        lu, piv, info = lapack.zgetrf(np.array(c.T, order='F', copy=True), overwrite_a=True)
+        assert info == 0
+        x, info = lapack.zgetrs(lu, piv, b)                 # C^T = C
+        assert info == 0
+        _, s = r_and_s(x)
+        t = time.perf_counter() - t0
+        res = [float(np.linalg.norm(b - c @ x) / np.linalg.norm(b))]
+        x0, s0 = x, s
+        t1 = time.perf_counter()
+        for _ in range(sweeps):
+            d, info = lapack.zgetrs(lu, piv, b - c @ x)
+            x = x + d
+            res.append(float(np.linalg.norm(b - c @ x) / np.linalg.norm(b)))
+        _, s = r_and_s(x)
+        t_ref = time.perf_counter() - t1
+    del lu
+    n = nch * nr
+    say('host oracle: %.1f s (zgesv + R + S), ||B - C X||_F/||B||_F = %s; refined S moves by '
+        '%.2e' % (t, ' '.join('%.2e' % v for v in res), np.abs(s - s0).max()))
+    return dict(c=c, b=b, x=x, s=s, x_zgesv=x0, s_zgesv=s0, seconds=t, refine_s=t_ref,
+                res_rel=res, threads=threads, n=n, nrhs=nch, nopen=len(npo))
+
+
+def abi_wall(c, b, x_host):
+    """The drop-in ABI on host arrays (Fortran conventions, column-major;
+    C is complex-symmetric so its row-major array is its column-major
+    storage): wall time of gpu_solve_mixed_ and gpu_solve_double_ including
+    the host-to-device copy of A (10.5 GB) and B and the copy of X back.
+    The first call also initialises the library and grows its buffers; the
+    second is the steady state."""
+    from hprmat_amd import _lib
+    lib = _lib.load()
+    n, nrhs = c.shape[0], b.shape[1]
+    cp = c.ctypes.data_as(ctypes.c_void_p)
+    out = {'bytes_h2d': 16 * n * n + 16 * n * nrhs, 'bytes_d2h': 16 * n * nrhs}
+    for name, args in (('gpu_solve_mixed_', (30, 1e-12)), ('gpu_solve_double_', None)):
+        rec = {}
+        for call in ('first', 'steady'):
+            bb = np.asfortranarray(b.copy())
+            info = ctypes.c_int(-99)
+            pn, pr = ctypes.byref(ctypes.c_int(n)), ctypes.byref(ctypes.c_int(nrhs))
             t0 = time.perf_counter()
-            x, info = osolve.zgesv(c, b)
-            ormat.extract_r(x, mesh['q2'][0], nch)
-            ts.append(time.perf_counter() - t0)
-    ts.sort()
-    t = ts[len(ts) // 2]
-    tflops = flops_solve(n, nch) / t / 1e12
-    return {'value': tflops * 1e12 / flops_solve(25600, 128), 'unit': 'solves/s',
-            'cores': threads, 'kind': 'port',
-            'tflops': tflops, 'seconds': t,
-            'sample': ('LAPACK zgesv (scipy/OpenBLAS) + R extraction on N=%d (nch=%d x nr=%d, '
-                       'nrhs=%d) of the same recipe, median of %d; value = measured TFLOP/s '
-                       'divided by the N=25600/nrhs=128 flop count' % (n, nch, nr, nch, reps))}
+            if args is None:
+                rc = getattr(lib, name)(cp, bb.ctypes.data_as(ctypes.c_void_p), pn, pr,
+                                        ctypes.byref(info))
+            else:
+                rc = getattr(lib, name)(cp, bb.ctypes.data_as(ctypes.c_void_p), pn, pr,
+                                        ctypes.byref(ctypes.c_int(args[0])),
+                                        ctypes.byref(ctypes.c_double(args[1])), ctypes.byref(info))
+            rec[call + '_s'] = time.perf_counter() - t0
+        rec['rc'], rec['info'] = int(rc), int(info.value)
+        rec['res_rel_host'] = float(np.linalg.norm(b - c @ bb) / np.linalg.norm(b))
+        if x_host is not None:
+            rec['err_x_vs_host'] = float(np.abs(bb - x_host).max() / np.abs(x_host).max())
+        if args is not None:
+            rec['max_refine'], rec['tol_abs_max'] = args
+        say('abi %s: first %.2f s, steady %.2f s, rc %d info %d, host residual %.2e'
+            % (name, rec['first_s'], rec['steady_s'], rc, info.value, rec['res_rel_host']))
+        out[name] = rec
+    return out
 
 
 def main():
@@ -122,7 +224,11 @@ def main():
     ap.add_argument('--nr', type=int, default=200)
     ap.add_argument('--methods', default='fp64,fp32_gmres,bf16x6_gmres,fp32_3m_gmres,fp32,'
                                          'fp32_ir5,bf16x3,bf16x3_gmres,fp64_op,fp32_3m_gmres_op')
-    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-cpu', action='store_true',
+                    help='skip the host oracle / CPU baseline (LAPACK zgesv at full size)')
+    ap.add_argument('--no-abi', action='store_true', help='skip the host-array ABI wall times')
+    ap.add_argument('--no-mem', action='store_true', help='skip the N = 51200 block')
+    ap.add_argument('--no-grad', action='store_true', help='skip the gradient block')
     ap.add_argument('--copy-probe', type=int, default=0,
                     help='copy this many bytes device-to-device once before the run (a '
                          'kernel of known traffic for calibrating HBM counters)')
@@ -155,11 +261,10 @@ def main():
             dst.copy_(src)
             torch.cuda.synchronize()
             del src, dst
-        run(args, torch, lib, _lib, rmatrix, solver, synth)
+        run(args, torch, lib, rmatrix, solver, synth)
 
 
-def run(args, torch, lib, _lib, rmatrix, solver, synth):
-
+def run(args, torch, lib, rmatrix, solver, synth):
     sysm = synth.System(args.nch, args.nr, 15.0, 8.0, 5.0, 0)
     n, nrhs = sysm.n, sysm.nch
     dmesh = rmatrix.DeviceMesh(sysm.mesh)
@@ -175,7 +280,7 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
                 torch.as_tensor(xc1, dtype=torch.float64, device='cuda'),
                 torch.as_tensor(sysm.cpot, dtype=torch.complex128, device='cuda').contiguous())
     methods = {}
-    x_ref = s_ref = None
+    sols = {}          # method -> (X on the host (N, nrhs), S on the host)
     for m in args.methods.split(','):
         lu = solver.LU()
         fm, max_refine, refinement = SPECS[m]
@@ -200,7 +305,6 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
         e1.record()
         torch.cuda.synchronize()
         lib.hprmat_hip_prof_enable(0)
-        import ctypes
         gms, gfl, gcnt = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         lib.hprmat_hip_prof_read(ctypes.byref(gms), ctypes.byref(gfl), ctypes.byref(gcnt))
         ms = e0.elapsed_time(e1) / args.steps
@@ -209,27 +313,29 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
         lu.factor(a_cm, fm, equilibrate=True)
         st = lu.solve_(b, max_refine, 1e-12, 'rel_fro', True, stats=True, refinement=refinement)
         _, s_m = rmatrix.r_to_s(dmesh, b, nrhs, sysm.qk, open_idx, co, cop)
-        if m == 'fp64' or (m == 'fp64_op' and x_ref is None):
-            x_ref, s_ref = b.clone(), s_m.clone()
+        sols[m] = (b.cpu().numpy().T.copy(), s_m.cpu().numpy())
         methods[m] = {
             'ms': ms, 'solves_per_s': 1000.0 / ms, 'tflops': fl / (ms * 1e-3) / 1e12,
             'res_rel': st['res_rel'], 'sweeps': st['sweeps'], 'converged': st['converged'],
-            'info': st['info'],
+            'diverged': st['diverged'], 'info': st['info'],
             'gemm': {'ms_per_launch': gms.value / max(gcnt.value, 1),
                      'launches_per_step': gcnt.value // args.steps,
                      'tflops': gfl.value / (gms.value * 1e-3) / 1e12 if gms.value > 0 else 0.0,
                      'flops_per_launch': gfl.value / max(gcnt.value, 1)},
         }
         methods[m]['hist'] = st['hist']
-        if x_ref is not None:
-            methods[m]['err_x_vs_fp64'] = float((b - x_ref).abs().max() / x_ref.abs().max())
-            methods[m]['dS_vs_fp64'] = float((s_m - s_ref).abs().max())
-        print('# %s: %.1f ms/solve, %.2f solves/s, %.1f TFLOP/s, res_rel %.2e, sweeps %d, '
-              'trailing GEMM %.1f TFLOP/s, hist %s' % (
-                  m, ms, 1000.0 / ms, methods[m]['tflops'], st['res_rel'], st['sweeps'],
-                  methods[m]['gemm']['tflops'], ' '.join('%.1e' % h for h in st['hist'])),
-              file=sys.stderr, flush=True)
+        say('%s: %.1f ms/solve, %.2f solves/s, %.1f TFLOP/s, res_rel %.2e, sweeps %d, '
+            'trailing GEMM %.1f TFLOP/s, hist %s' % (
+                m, ms, 1000.0 / ms, methods[m]['tflops'], st['res_rel'], st['sweeps'],
+                methods[m]['gemm']['tflops'], ' '.join('%.1e' % h for h in st['hist'])))
         del lu
+    ref_m = 'fp64' if 'fp64' in sols else ('fp64_op' if 'fp64_op' in sols else None)
+    for m in methods:
+        if ref_m is not None:
+            x_ref, s_ref = sols[ref_m]
+            methods[m]['err_x_vs_fp64'] = float(np.abs(sols[m][0] - x_ref).max() /
+                                                np.abs(x_ref).max())
+            methods[m]['dS_vs_fp64'] = float(np.abs(sols[m][1] - s_ref).max())
 
     dense = {m: v for m, v in methods.items() if not m.endswith('_op')} or methods
     op_methods = {m: v for m, v in methods.items() if m.endswith('_op')}
@@ -265,17 +371,11 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
             'launches': tr['launches'], 'hbm_read_bytes': tr['hbm_read_bytes'],
             'hbm_write_bytes': tr['hbm_write_bytes'], 'alg_bytes': tr['alg_bytes'],
             'read_ratio': tr['read_ratio'], 'write_ratio': tr['write_ratio'],
+            'build': tr.get('build'),
             'source': 'profiles/trailing_gemm_counters_N25600.md (FETCH_SIZE, WRITE_SIZE of '
                       '%s over the %d trailing updates of one N = %d factorization, '
                       'copy-calibrated; mean per launch)' % (tr['kernel'], tr['launches'],
                                                              tr['N'])}
-    elif tr is not None and not head.startswith('bf16'):
-        roof['traffic'] = tr['hbm_bytes']
-        roof['traffic_launch'] = {
-            'M': tr['M'], 'N': tr['N'], 'K': tr['K'], 'hbm_read_bytes': tr['hbm_read_bytes'],
-            'hbm_write_bytes': tr['hbm_write_bytes'], 'alg_bytes': tr['alg_bytes'],
-            'source': 'profiles/trailing_gemm_counters_N25600.md (FETCH_SIZE, WRITE_SIZE, '
-                      'copy-calibrated; one GEMM of the first update\'s shape)'}
     out = {
         'metric': METRIC, 'value': hm['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': hm['ms'],
@@ -301,15 +401,211 @@ def run(args, torch, lib, _lib, rmatrix, solver, synth):
             blk['speedup_mixed_vs_fp64'] = (op_methods['fp64_op']['ms'] /
                                             op_methods['fp32_3m_gmres_op']['ms'])
         out['mesh_operator'] = blk
+    del a_cm, b0, op_parts
+    torch.cuda.empty_cache()
+
+    # the FP64 host oracle (reference solver type 1) and the CPU baseline
+    host = None
+    if not args.no_cpu:
+        host = host_oracle(args.nch, args.nr)
+        for m in methods:
+            x_m, s_m = sols[m]
+            xmax = np.abs(host['x']).max()
+            methods[m]['err_x_vs_host'] = float(np.abs(x_m - host['x']).max() / xmax)
+            methods[m]['dS_vs_host'] = float(np.abs(s_m - host['s']).max())
+            methods[m]['err_x_vs_host_zgesv'] = float(np.abs(x_m - host['x_zgesv']).max() / xmax)
+            methods[m]['dS_vs_host_zgesv'] = float(np.abs(s_m - host['s_zgesv']).max())
+        say('vs host oracle: ' + ', '.join('%s dS %.1e' % (m, methods[m]['dS_vs_host'])
+                                            for m in methods))
+        out['host_oracle'] = {
+            'res_rel_zgesv': host['res_rel'][0], 'res_rel_refined': host['res_rel'][1:],
+            'dS_refined_vs_zgesv': float(np.abs(host['s'] - host['s_zgesv']).max()),
+            'refine_s': host['refine_s'], 'nopen': host['nopen'],
+            'path': ('oracle assembly of the same seeded system; LAPACK zgetrf + zgetrs '
+                     '(= zgesv, scipy/OpenBLAS) + R + S, the reference solver type 1 '
+                     '("*_zgesv" fields), then two FP64 refinement sweeps with the same '
+                     'factors: the accurate host oracle of err_x_vs_host / dS_vs_host. The '
+                     'device-assembled matrix differs from the host one at rounding level '
+                     '(1.5e-16 of max|C|), which on this kappa ~ 1e9 system moves S by '
+                     'about 4e-11')}
+        out['cpu_baseline'] = {
+            'value': 1.0 / host['seconds'], 'unit': 'solves/s', 'cores': host['threads'],
+            'kind': 'port', 'seconds': host['seconds'],
+            'tflops': flops_solve(host['n'], host['nrhs']) / host['seconds'] / 1e12,
+            'cpu': cpu_model(),
+            'sample': ('the full workload once (k=1): LAPACK zgesv (scipy/OpenBLAS, %d threads) '
+                       'on the N=%d system with nrhs=%d, then R and S, the reference solver '
+                       'type 1, after a 512x512 warm-up' % (host['threads'], host['n'],
+                                                            host['nrhs']))}
+    if not args.no_abi:
+        try:
+            if host is not None:
+                c_host, b_host, x_host = host['c'], host['b'], host['x']
+            else:     # the device-assembled matrix (column-major storage = row-major array)
+                a_d = rmatrix.assemble(dmesh, sysm.lval, sysm.qk, xc1, sysm.cpot)
+                c_host = a_d.cpu().numpy()
+                b_host = np.ascontiguousarray(rmatrix.rhs_colmajor(dmesh, nrhs).cpu().numpy().T)
+                x_host = None
+                del a_d
+            out['abi_wall'] = abi_wall(c_host, b_host, x_host)
+            out['abi_wall']['device_resident_ms'] = {m: methods[m]['ms'] for m in ('fp64', 'fp32')
+                                                     if m in methods}
+            del c_host
+        except Exception as exc:
+            out['abi_wall'] = {'error': repr(exc)}
+            say('abi wall failed: %r' % (exc,))
+    host = None
+    if not args.no_mem:
+        try:
+            out['memory_scale'] = memory_scale(torch, rmatrix, solver, synth)
+        except Exception as exc:
+            out['memory_scale'] = {'error': repr(exc)}
+            say('memory scale failed: %r' % (exc,))
+    if not args.no_grad:
+        try:
+            out['gradient'] = gradient_block(torch)
+        except Exception as exc:
+            out['gradient'] = {'error': repr(exc)}
+            say('gradient block failed: %r' % (exc,))
     if not args.no_scan:
         try:
             out['energy_scan'] = energy_scan(torch, args)
         except Exception as exc:     # the headline line is still printed
             out['energy_scan'] = {'error': repr(exc)}
-            print('# scan failed: %r' % (exc,), file=sys.stderr, flush=True)
-    if not args.no_cpu:
-        out['cpu_baseline'] = cpu_baseline()
+            say('scan failed: %r' % (exc,))
     print(json.dumps(out))
+
+
+def memory_scale(torch, rmatrix, solver, synth):
+    """N = 51200 (nch = 256 x nr = 200, thresholds U[0, 3], E = 6: every
+    channel open), 256 right-hand sides, A resident in HBM (42 GB as
+    complex128).  FP32 LU (Gauss three-product update, equilibrated) + FGMRES,
+    and plain FP64 LU + FP64 refinement; the gate is this system's FP64
+    attainable residual, which the FP64 path also stalls at (1.35e-12).  The
+    real-potential variant (W0 = 0) is solved in FP64 and S's unitarity
+    defect max |S^H S - I| reported; S (256 x 256) is formed in FP64 on the
+    device."""
+    nch, nr = 256, 200
+    n = nch * nr
+    fl = flops_solve(n, nch)
+    out = {'N': n, 'nrhs': nch, 'flops_per_solve': fl}
+
+    def system(w0_zero):
+        s = synth.System(nch, nr, 15.0, 3.0, 6.0, 0, w0_zero=w0_zero)
+        dm = rmatrix.DeviceMesh(s.mesh)
+        o, co, cop, xc1 = rmatrix.channel_data(s.lval, s.qk, s.a)
+        a_cm = rmatrix.assemble(dm, s.lval, s.qk, xc1, s.cpot)
+        return s, dm, o, co, cop, a_cm, rmatrix.rhs_colmajor(dm, nch)
+
+    s, dm, o, co, cop, a_cm, b0 = system(False)
+    for name, method, budget, refinement, tol, reps in (
+            ('fp32_3m_gmres', 'fp32_3m', 40, 'fgmres', 2e-12, 2), ('fp64', 'fp64', 5, 'ir', 2e-12, 1)):
+        lu = solver.LU().set_plan(gmres_cycle=24)
+        b = b0.clone()
+        times = []
+        for _ in range(reps):       # the last repetition is timed (the first of two warms up)
+            b.copy_(b0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            lu.factor(a_cm, method, equilibrate=True)
+            st = lu.solve_(b, budget, tol, 'rel_fro', True, refinement=refinement)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        _, sm = rmatrix.r_to_s(dm, b, nch, s.qk, o, co, cop)
+        sm = sm.cpu().numpy()
+        out[name] = {'s': times[-1], 'solves_per_s': 1.0 / times[-1],
+                     'tflops': fl / times[-1] / 1e12, 'res_rel': st['res_rel'],
+                     'products_or_sweeps': st['sweeps'], 'converged': st['converged'],
+                     'warm': reps > 1, 'symmetry_defect_S': float(np.abs(sm - sm.T).max()),
+                     'max_abs_S': float(np.abs(sm).max())}
+        say('N=51200 %s: %.2f s, res_rel %.2e, %d products/sweeps' % (
+            name, times[-1], st['res_rel'], st['sweeps']))
+        del lu, b
+    if 'fp64' in out and 'fp32_3m_gmres' in out:
+        out['speedup_mixed_vs_fp64'] = out['fp64']['s'] / out['fp32_3m_gmres']['s']
+    del a_cm, b0
+    torch.cuda.empty_cache()
+    s, dm, o, co, cop, a_cm, b = system(True)
+    lu = solver.LU()
+    lu.factor(a_cm, 'fp64', equilibrate=True)
+    st = lu.solve_(b, 5, 1e-12, 'rel_fro', True)
+    _, sm = rmatrix.r_to_s(dm, b, nch, s.qk, o, co, cop)
+    sm = sm.cpu().numpy()
+    defect = float(np.abs(sm.conj().T @ sm - np.eye(nch)).max())
+    out['w0_zero'] = {'method': 'fp64', 'res_rel': st['res_rel'], 'unitarity_defect': defect,
+                      'symmetry_defect_S': float(np.abs(sm - sm.T).max())}
+    say('N=51200 W0=0: unitarity defect %.2e, res_rel %.2e' % (defect, st['res_rel']))
+    del lu, a_cm, b
+    torch.cuda.empty_cache()
+    return out
+
+
+def gradient_block(torch):
+    """Loss gradient over 32 energies in [1, 8] of the N = 6400 fit system
+    (nch = 32 x nr = 200, 16 Woods-Saxon parameters, evaluation point 5 %
+    off the target parameters): forward and adjoint solves on the device
+    (FP32 LU + FGMRES, the adjoint reusing the factors), S and the loss in
+    FP64.  One untimed gradient, then one timed.  Checked against the FP64
+    CPU adjoint (oracle) on the first and last energy."""
+    from hprmat_amd import adjoint
+    sys.path.insert(0, ROOT)
+    from oracle import adjoint as oadj
+    from threadpoolctl import threadpool_limits
+    energies = np.linspace(1.0, 8.0, 32)
+    opts = adjoint.SolveOptions(method='fp32', refinement='fgmres', max_steps=20)
+    fp = adjoint.FitProblem(nch=32, nr=200, energies=energies, seed=0, opts=opts)
+    fp.loss_and_grad()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss, g = fp.loss_and_grad()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    say('gradient: %.2f s for %d energies (loss %.4e)' % (t, len(energies), loss))
+    sub = (energies[0], energies[-1])
+    fs = adjoint.FitProblem(nch=32, nr=200, energies=sub, seed=0, opts=opts)
+    loss_s, g_s = fs.loss_and_grad()
+    prob = oadj.problem(32, 200, 15.0, 3.0, 0, sub)
+    th = fs.theta0.cpu().numpy()
+    with threadpool_limits(limits=host_threads()):
+        t1 = time.perf_counter()
+        loss_ref, g_ref = oadj.loss_and_grad(prob, th, [x.numpy() for x in fs.targets])
+        t_cpu = time.perf_counter() - t1
+    g_s = g_s.cpu().numpy()
+    rel = float(np.abs(g_s - g_ref).max() / np.abs(g_ref).max())
+    say('gradient vs CPU adjoint (2 energies): rel error %.2e (CPU %.1f s)' % (rel, t_cpu))
+    return {'energies': len(energies), 'N': 6400, 'parameters': 16, 's_per_gradient': t,
+            'solves_per_gradient': 2 * len(energies), 'loss': loss,
+            'grad_norm': float(g.norm()), 'solve_tol': opts.tol,
+            'check_vs_cpu_adjoint': {'energies': [float(e) for e in sub], 'rel_err': rel,
+                                     'loss_rel_err': abs(loss_s - loss_ref) / loss_ref,
+                                     'cpu_s': t_cpu}}
+
+
+def host_floor(w, k):
+    """FP64 host solve of one scan system (wave w, energy index k) by the
+    oracle: LAPACK zgesv, then two FP64 refinement sweeps with the same
+    factors, and the floor u ||(|C| |X|)||_F / ||B||_F below which no FP64
+    X can push ||B - C X||_F / ||B||_F."""
+    import scipy.linalg as sla
+    sys.path.insert(0, ROOT)
+    from oracle import channels, lagrange, rmatrix as ormat
+    mesh = lagrange.rmat_ini(w.nr, 1, w.a)
+    qk = w.qk_host[k]
+    npo, co, cop, xc1 = channels.channel_functions(w.sysm.lval, qk, w.a)
+    c = ormat.assemble(mesh, w.sysm.lval, qk, w.sysm.cpot, xc1)
+    b = ormat.rhs(mesh['q2'][0], w.nch)
+    lu = sla.lu_factor(c)
+    x = sla.lu_solve(lu, b)
+    hist = []
+    for _ in range(3):
+        r = b - c @ x
+        hist.append(float(np.linalg.norm(r) / np.linalg.norm(b)))
+        if len(hist) == 3:
+            break
+        x = x + sla.lu_solve(lu, r)
+    floor = float(np.finfo(float).eps / 2 * np.linalg.norm(np.abs(c) @ np.abs(x)) /
+                  np.linalg.norm(b))
+    return {'host_zgesv_res': hist[0], 'host_refined_res': hist[1:], 'fp64_floor': floor}
 
 
 def energy_scan(torch, args):
@@ -333,12 +629,12 @@ def energy_scan(torch, args):
         sc.run()
         sc.finish()
     t_cap = time.perf_counter() - t0
-    print('# scan: %d systems, %.3g flops, prep %.1f s, warm-up/capture %.1f s, lanes %d'
-          % (sc.n_systems, sc.flops, t_prep, t_cap, len(sc.lanes)), file=sys.stderr, flush=True)
+    say('scan: %d systems, %.3g flops, prep %.1f s, warm-up/capture %.1f s, lanes %d'
+        % (sc.n_systems, sc.flops, t_prep, t_cap, len(sc.lanes)))
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    print('# scan: timed run', file=sys.stderr, flush=True)
+    say('scan: timed run')
     t0 = time.perf_counter()
     sc.run()
     n_redo = sc.finish()
@@ -348,7 +644,7 @@ def energy_scan(torch, args):
     s_scan = {(w.index, k): w.s_of(k).clone() for w in waves for k in range(0, len(w.energies), 16)}
     pairs = [(w, k) for w in waves for k in range(0, len(w.energies), 16)]
     torch.cuda.synchronize()
-    print('# scan: serial FP64 sample', file=sys.stderr, flush=True)
+    say('scan: serial FP64 sample')
     e0.record()
     scan.serial_fp64(waves, pairs)
     e1.record()
@@ -362,6 +658,20 @@ def energy_scan(torch, args):
                  'res_rel_scan': float(ww.res[wk]), 'dS': ds}
     unconv = [{'wave': w.index, 'energy_index': int(k), 'N': w.n, 'res_rel': float(w.res[k])}
               for w in waves for k in torch.nonzero(w.status != 1).flatten().tolist()]
+    # systems still above the gate after the FP64 re-solve: the host's FP64
+    # solve of the same system and its floor (outside the timed region)
+    for u in unconv[:16]:
+        w = waves[u['wave']]
+        try:
+            u.update(host_floor(w, u['energy_index']))
+        except Exception as exc:
+            u['host_error'] = repr(exc)
+    if unconv:
+        say('scan: unconverged systems vs host FP64 floor: ' + '; '.join(
+            'w%d/e%d N=%d gpu %.2e host %.2e floor %.2e' % (
+                u['wave'], u['energy_index'], u['N'], u['res_rel'],
+                min([u.get('host_zgesv_res', 0.0)] + u.get('host_refined_res', [])),
+                u.get('fp64_floor', 0.0)) for u in unconv[:16]))
     n_sys = sc.n_systems
     out = {'systems': n_sys, 'ms': ms, 'solves_per_s': n_sys / (ms * 1e-3),
            'tflops': sc.flops / (ms * 1e-3) / 1e12, 'flops': sc.flops, 'lanes': len(sc.lanes),
@@ -373,10 +683,10 @@ def energy_scan(torch, args):
                            'sample': 'every 16th energy of every wave, one stream, eager'},
            'max_dS_vs_fp64': ds, 'worst_dS_system': worst_sys, 'unconverged_systems': unconv[:16]}
     out['speedup_vs_serial_fp64'] = out['solves_per_s'] / out['serial_fp64']['solves_per_s']
-    print('# scan: %.1f ms, %.0f solves/s, %.1f TFLOP/s; serial FP64 %.0f solves/s; x%.2f; '
-          'max|dS| %.1e; fallbacks %d; max res %.1e' % (
-              ms, out['solves_per_s'], out['tflops'], out['serial_fp64']['solves_per_s'],
-              out['speedup_vs_serial_fp64'], ds, n_redo, worst), file=sys.stderr, flush=True)
+    say('scan: %.1f ms, %.0f solves/s, %.1f TFLOP/s; serial FP64 %.0f solves/s; x%.2f; '
+        'max|dS| %.1e; fallbacks %d; max res %.1e' % (
+            ms, out['solves_per_s'], out['tflops'], out['serial_fp64']['solves_per_s'],
+            out['speedup_vs_serial_fp64'], ds, n_redo, worst))
     return out
 
 

@@ -22,6 +22,20 @@ def _oracle_s(w, k):
     return out['S']
 
 
+def _oracle_s_refined(w, k):
+    """S of the host FP64 solve refined to ||r||_F/||b||_F <= 1e-12 (the
+    reference's ZGESV result, then FP64 sweeps with the same factors), and S
+    of the plain ZGESV.  On the kappa ~ 1e9 waves ZGESV's residual is a few
+    1e-12 and its S up to 2e-10 from the refined one."""
+    from oracle import solve as osolve
+    out = ormat.rmatrix(w.sysm.lval, w.qk_host[k], w.a, w.nr, w.sysm.cpot)
+    x, _ = osolve.fp64_refined(out['C'], out['B'])
+    r = ormat.extract_r(x, out['mesh']['q2'][0], w.nch)
+    s = (ormat.r_to_s(r, w.a, w.qk_host[k], out['npo'], out['co'], out['cop'])
+         if out['nopen'] else out['S'])
+    return s, out['S']
+
+
 def test_scan_matches_oracle():
     waves = _waves()
     sc = scan.EnergyScan(waves, lanes=3, graph_max=300)   # some systems stay uncaptured
@@ -116,6 +130,40 @@ def test_batch_scan_matches_oracle_and_replays():
         assert sc.finish() == 0
         for w, e in zip(waves, eager):
             assert torch.equal(w.S, e), rep
+
+
+def test_batch_scan_bench_configuration_at_measured_sizes():
+    # the benchmark's scan configuration (Gauss three-product FP32 factors,
+    # refinement products from the Lagrange-mesh parts, 15 sweeps, two lanes,
+    # captured graphs, FP64 re-solve of stragglers) on two of its own waves:
+    # wave 0 (64 channels x 100 points, N = 6400, batched leaves of 25
+    # workgroups) and wave 21 (32 x 80, N = 2560), at four energies of its
+    # grid, among them E = 2.288 where kappa ~ 1e9; S against the host FP64
+    # solve refined to the residual gate (plain ZGESV printed beside it)
+    grid = np.linspace(0.5, 10.0, 256)
+    energies = grid[[0, 48, 128, 255]]
+    sizes = scan.draw_waves(64, 0)
+    assert sizes[0] == (64, 100) and sizes[21] == (32, 80)
+    waves = [scan.Wave(i, *sizes[i], energies, emax=3.0, seed=0) for i in (0, 21)]
+    sc = scan.BatchScan(waves, method='fp32_3m', lanes=2, max_refine=15)
+    assert sc.mesh_operator
+    print('bench-configuration scan: capture', flush=True)
+    sc.capture()
+    print('bench-configuration scan: replay', flush=True)
+    sc.run()
+    sc.finish()
+    for w in waves:
+        res = w.res.cpu().numpy()
+        for k in range(len(w.energies)):
+            ref, ref_zgesv = _oracle_s_refined(w, k)
+            got = w.s_of(k).cpu().numpy()
+            ds = np.abs(got - ref).max() if ref.size else 0.0
+            dz = np.abs(got - ref_zgesv).max() if ref.size else 0.0
+            print('wave %d N=%d E=%.3f res %.2e |dS| %.2e (vs plain zgesv %.2e)'
+                  % (w.index, w.n, w.energies[k], res[k], ds, dz), flush=True)
+            assert got.shape == ref.shape
+            assert ds <= 1e-10, (w.n, w.energies[k], ds)
+            assert res[k] <= 3e-12
 
 
 def test_batch_scan_fallback_resolves_unconverged():

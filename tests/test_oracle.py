@@ -2,7 +2,10 @@
 
 * LEGZO restatement vs numpy's Gauss-Legendre rule;
 * eta = 0 channel functions vs mpmath Coulomb / Whittaker functions and the
-  Wronskian;
+  Wronskian; the restated WHIT (asymptotic branch exact, Numerov branch
+  within its method error) and the change of S it causes;
+* rmat_ini's T + L(0) and boundary values vs brute-force Galerkin integrals
+  of the regularised Lagrange-Legendre basis;
 * the whole ns = 1 pipeline vs the golden phase shifts printed by the
   reference's examples (rmat_pierre/Ex2 Reid NN, rmat_pierre/Ex5 Yamaguchi);
 * free motion (V = 0) gives S = 1; a real potential with all channels open
@@ -30,6 +33,49 @@ def test_legzo_matches_numpy(n):
     t, ww = np.polynomial.legendre.leggauss(n)
     assert np.allclose(x, (1 + t) / 2, atol=1e-14, rtol=0)
     assert np.allclose(w, ww / 2, atol=1e-14, rtol=0)
+
+
+def _lagrange_basis(n, x):
+    """The regularised Lagrange-Legendre functions on (0, 1),
+    f_i(t) = (-1)^(n-i) sqrt(x_i (1 - x_i)) t P_n(2t - 1) / (x_i (t - x_i))
+    (i = 1..n), with first and second derivatives, as callables of t: each
+    f_i is a polynomial of degree n, and f_i(x_j) = lambda_i^-1/2 delta_ij."""
+    from numpy.polynomial import legendre as leg
+    cn = np.zeros(n + 1)
+    cn[n] = 1.0
+    d1, d2 = leg.legder(cn, 1), leg.legder(cn, 2)
+
+    def f(i, t):
+        s = 2 * t - 1
+        p, p1, p2 = leg.legval(s, cn), 2 * leg.legval(s, d1), 4 * leg.legval(s, d2)
+        c = (-1.0) ** (n - (i + 1)) * math.sqrt(x[i] * (1 - x[i])) / x[i]
+        u, u1, u2 = t * p, p + t * p1, 2 * p1 + t * p2
+        v = t - x[i]
+        return c * u / v, c * (u1 / v - u / v ** 2), c * (u2 / v - 2 * u1 / v ** 2 + 2 * u / v ** 3)
+    return f
+
+
+@pytest.mark.parametrize('n', [8, 12, 20])
+def test_rmat_ini_kinetic_bloch_matches_galerkin_integrals(n):
+    # T + L(0) of rmat_ini against the brute-force Galerkin matrix elements
+    # <f_i| -d2/dt2 |f_j> + f_i(1) f_j'(1) of the basis itself, integrated by
+    # an independent 2n-point Gauss rule (exact: the integrand has degree
+    # 2n - 2), and q2 against the basis values f_i(1)
+    m = lagrange.rmat_ini(n, 1, 1.0)
+    x = m['xle']
+    f = _lagrange_basis(n, x)
+    t, w = np.polynomial.legendre.leggauss(2 * n)
+    t, w = (t + 1) / 2, w / 2
+    one = np.array([1.0])
+    g = np.zeros((n, n))
+    for i in range(n):
+        fi = f(i, t)[0]
+        for j in range(n):
+            g[i, j] = np.sum(w * fi * -f(j, t)[2]) + f(i, one)[0][0] * f(j, one)[1][0]
+    tc = m['tc'][0]
+    assert np.abs(g - tc).max() <= 1e-12 * np.abs(tc).max()
+    q2 = np.array([f(i, one)[0][0] for i in range(n)])
+    assert np.allclose(m['q2'][0], q2, rtol=1e-12, atol=0)
 
 
 def test_mesh_symmetry_and_rank_one_bloch():
@@ -61,6 +107,67 @@ def test_closed_channel_log_derivative_matches_whittaker(l, rho):
     ld = float(mpmath.diff(wm, rho) / wm(rho)) * rho
     assert abs(xc1 - ld) <= 1e-10 * max(1.0, abs(ld))
     assert abs(w / float(wm(rho)) - 1.0) < 1e-10
+
+
+def _whit_exact(l, rho):
+    wm = lambda z: mpmath.whitw(0, l + 0.5, 2 * z)
+    return float(wm(rho)), float(mpmath.diff(wm, rho))
+
+
+@pytest.mark.parametrize('rho', [10.0, 12.5, 25.0])
+def test_whit_asymptotic_branch_is_the_closed_form(rho):
+    # rho >= 10 (eta + 1): WHIT sums its series at rho; at eta = 0 every term
+    # after the first vanishes and W = exp(-rho), exact; l > 0 by recurrence
+    f, fd, ie = channels.whit(0.0, rho, 1.0, 3, 0)
+    assert ie == 0
+    for l in range(4):
+        w, wd = _whit_exact(l, rho)
+        assert abs(f[l] / w - 1.0) <= 1e-13 and abs(fd[l] / wd - 1.0) <= 1e-13
+
+
+@pytest.mark.parametrize('rho', [0.3, 1.0, 4.64, 7.5, 9.99])
+def test_whit_numerov_branch_has_the_method_error(rho):
+    # rho < 10: Numerov from two series values beyond rho, the 7-point
+    # difference for W', the l-recurrence.  The restatement lands within its
+    # method's error of the exact function (about 1e-8: h ~ 1/30 and the
+    # single-precision literals of the Fortran source), not at rounding level
+    # and nowhere near O(1), which a wrong sign, index or step would give
+    f, fd, _ = channels.whit(0.0, rho, 1.0, 3, 0)
+    errs = []
+    for l in range(4):
+        w, wd = _whit_exact(l, rho)
+        errs += [abs(f[l] / w - 1.0), abs(fd[l] / wd - 1.0)]
+    assert max(errs) <= 1e-7
+    assert max(errs) >= 1e-10
+
+
+@pytest.mark.parametrize('nch,seed', [(8, 0), (8, 6), (8, 9)])
+def test_whit_closed_channels_change_of_s(nch, seed):
+    # the reference's closed-channel data (WHIT) against the closed form on
+    # N = 480 systems of the configuration recipe (thresholds in [0, 3],
+    # E = 2) with closed channels at kappa a = 2.3 .. 9.95: the
+    # log-derivatives differ by up to about 1e-8 relative, and S by the
+    # printed amount (measured 1.7e-14 .. 1.1e-12, below the 1e-10 S
+    # tolerance; the product follows the closed form)
+    lval, qk, cpot, _ = synth.system(nch, 60, 15.0, 3.0, 2.0, seed)
+    mesh = lagrange.rmat_ini(60, 1, 15.0)
+    npo, co, cop, xc1 = channels.channel_functions(lval, qk, 15.0)
+    xw = xc1.copy()
+    closed = [c for c in range(nch) if qk[c] < 0]
+    assert closed and min(-qk[c] * 15.0 for c in closed) < 10.0
+    for c in closed:
+        xw[c] = channels.closed_channel_whit(int(lval[c]), -qk[c], 15.0)[0]
+        assert abs(xw[c] / xc1[c] - 1.0) <= 1e-7
+    b = rmatrix.rhs(mesh['q2'][0], nch)
+    s = []
+    for x1 in (xc1, xw):
+        c = rmatrix.assemble(mesh, lval, qk, cpot, x1)
+        x, info = solve.zgesv(c, b)
+        r = rmatrix.extract_r(x, mesh['q2'][0], nch)
+        s.append(rmatrix.r_to_s(r, 15.0, qk, npo, co, cop))
+    ds = np.abs(s[0] - s[1]).max()
+    print('N=%d seed %d: |S(WHIT) - S(closed form)| = %.2e' % (nch * 60, seed, ds))
+    assert 0.0 < ds <= 1e-10
 
 
 def _reid(j, nr, a):
