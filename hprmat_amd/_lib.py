@@ -22,6 +22,7 @@ CRIT_REL_FRO = 1
 PLAN_TRAIL_TILE = 1      # hprmat_hip_lu_set_plan keys
 PLAN_LOOKAHEAD = 2
 PLAN_GMRES_CYCLE = 3
+PLAN_SOLVE_LANES = 4
 FGMRES = 0x10            # flag OR-ed into the criterion: flexible GMRES refinement
 STATUS_TIMEOUT = -0x40000000   # batch status: a panel leaf's bounded wait gave up
 

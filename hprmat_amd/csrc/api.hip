@@ -58,6 +58,8 @@ struct hprmat_hip_lu {
   hpr::MeshOperator mop;      // the factored matrix by its Lagrange-mesh parts (nch = 0: none)
   int lookahead = 1;
   int gm_cycle = 0;           // FGMRES cycle length (0: GMRES_DEFAULT_CYCLE)
+  int solve_lanes = 0;        // FGMRES lanes: 0 automatic, 1 one lane, 2 two lanes
+  hpr::SubstGraph sgraph[hpr::GMRES_MAX_LANES];   // replayed substitutions per lane
 };
 
 namespace {
@@ -86,7 +88,9 @@ int ensure_side(hprmat_hip_lu* h) {
     return -1;
   }
   if ((e = hipEventCreateWithFlags(&h->side.ready, hipEventDisableTiming)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&h->side.rest, hipEventDisableTiming)) != hipSuccess) {
+      (e = hipEventCreateWithFlags(&h->side.rest, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&h->side.lane, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&h->side.join, hipEventDisableTiming)) != hipSuccess) {
     fprintf(stderr, "hprmat_hip: event creation failed: %s\n", hipGetErrorString(e));
     return -1;
   }
@@ -94,10 +98,13 @@ int ensure_side(hprmat_hip_lu* h) {
 }
 
 void free_lu(hprmat_hip_lu* h) {
+  for (hpr::SubstGraph& g : h->sgraph) hpr::subst_graph_release(g);   // before its streams go
   if (h->side.stream) {
     (void)hipStreamSynchronize(h->side.stream);
     (void)hipEventDestroy(h->side.ready);
     (void)hipEventDestroy(h->side.rest);
+    (void)hipEventDestroy(h->side.lane);
+    (void)hipEventDestroy(h->side.join);
     (void)hipStreamDestroy(h->side.stream);
   }
   void* ps[] = {h->fac, h->ipiv, h->dinfo, h->dinv64, h->dinv256, h->ustage, h->ftmp, h->split, h->gm,
@@ -126,8 +133,11 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
     if (hipMalloc((void**)&h->dinfo, 256) != hipSuccess) return -2;
   }
   if (!h->state) {
-    if (hipMalloc((void**)&h->state, sizeof(RefineState)) != hipSuccess) return -2;
-    if (hipMalloc((void**)&h->part, 8192 * sizeof(double)) != hipSuccess) return -2;
+    // one refinement state and norm workspace per FGMRES lane
+    if (hipMalloc((void**)&h->state, hpr::GMRES_MAX_LANES * sizeof(RefineState)) != hipSuccess)
+      return -2;
+    if (hipMalloc((void**)&h->part, hpr::GMRES_MAX_LANES * 8192 * sizeof(double)) != hipSuccess)
+      return -2;
   }
   if (grow(&h->scr, &h->scr_cap, hpr::scratch_bytes())) return -2;
   hpr::scratch_bind(h->sc, h->scr);
@@ -191,6 +201,65 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
   return rc;
 }
 
+// FGMRES over nl lanes (nl <= 2): lane l takes the columns [c0[l], c0[l] +
+// nc[l]) of B, its own row-major blocks carved out of the handle's n x nrhs
+// buffers, refinement state l, norm workspace l and its own FGMRES and
+// split-K workspaces; lane 0 runs on st, lane 1 on the handle's side stream,
+// forked from st and joined back to it.
+template <class TF>
+int fgmres_lanes(hprmat_hip_lu* h, int nl, const int* c0, const int* nc, void* dB, int64_t ldb,
+                 int max_refine, int m, double tol, int crit, const hpr::SplitWork& sw0,
+                 bool poll, hipStream_t st) {
+  const int n = h->n;
+  hpr::GmresLane<TF> lanes[hpr::GMRES_MAX_LANES];
+  size_t off = 0, gmoff = 0, koff = 0;
+  for (int l = 0; l < nl; ++l) {
+    hpr::GmresLane<TF>& L = lanes[l];
+    const size_t cnt = (size_t)n * nc[l];
+    L.nrhs = nc[l];
+    L.st = l == 0 ? st : h->side.stream;
+    L.B = h->Brm + off;
+    L.X = h->Xrm + off;
+    L.R = h->R + off;
+    L.RF = (cx<TF>*)h->RF + off;
+    L.RY = (cx<TF>*)h->RY + off;
+    L.state = h->state + l;
+    L.part = h->part + (size_t)l * 8192;
+    L.graph = &h->sgraph[l];
+    hpr::gmres_work_bind(L.gw, (char*)h->gm + gmoff, n, nc[l], m);
+    gmoff += hpr::gmres_work_bytes(n, nc[l], m);
+    L.sw.mesh = sw0.mesh;
+    L.sw.splits = sw0.mesh ? 1 : hpr::resid_splits(n, nc[l]);
+    if (L.sw.splits > 1) {
+      L.sw.part = (double2*)h->kpart + koff;
+      koff += (size_t)hpr::RESID_MAX_SPLITS * cnt;
+    }
+    off += cnt;
+  }
+  if (nl > 1) {   // fork: the second lane starts after everything queued on st
+    HPR_CHECK(hipEventRecord(h->side.join, st));
+    HPR_CHECK(hipStreamWaitEvent(h->side.stream, h->side.join, 0));
+  }
+  for (int l = 0; l < nl; ++l)
+    if (hpr::transpose_cast<double, double>(n, nc[l], (const double2*)dB + (size_t)c0[l] * ldb, ldb,
+                                            (double2*)lanes[l].B, nc[l], lanes[l].st))
+      return -1;
+  const int rc = hpr::gmres_solve_lanes<TF>(
+      (const cx<TF>*)h->fac, h->ld, n, (const cx<TF>*)h->dinv256, h->perm, h->A64, h->lda,
+      max_refine, m, tol, crit, h->plan, false, h->equil ? h->scale : nullptr, nl, lanes, poll,
+      nl > 1 ? h->side.lane : nullptr);
+  if (rc) return rc;
+  for (int l = 0; l < nl; ++l)
+    if (hpr::transpose_cast<double, double>(nc[l], n, lanes[l].X, nc[l],
+                                            (double2*)dB + (size_t)c0[l] * ldb, ldb, lanes[l].st))
+      return -1;
+  if (nl > 1) {   // join
+    HPR_CHECK(hipEventRecord(h->side.join, h->side.stream));
+    HPR_CHECK(hipStreamWaitEvent(st, h->side.join, 0));
+  }
+  return 0;
+}
+
 int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, double tol,
              int crit, int final_check, hipStream_t st, hprmat_hip_stats* stats) {
   if (h->method < 0) return -1;
@@ -219,9 +288,9 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
     if (grow(&h->kpart, &h->kpart_cap, (size_t)hpr::RESID_MAX_SPLITS * cnt * 16)) return -2;
     sw.part = (double2*)h->kpart;
   }
-  if (hpr::transpose_cast<double, double>(n, nrhs, (const double2*)dB, ldb, h->Brm, nrhs, st))
-    return -1;
   int rc;
+  int nl = 1;                              // FGMRES lanes
+  int c0[hpr::GMRES_MAX_LANES] = {0}, nc[hpr::GMRES_MAX_LANES] = {nrhs};
   // The substitutions always use the four-product kernels, whatever the
   // factorization's arithmetic: they are launch-latency-bound chains (one
   // two-problem launch per 256-row block forward), which the Gauss form would
@@ -240,23 +309,43 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
     if (m > max_refine) m = max_refine;
     if (m < 1) m = 1;
     if (nrhs > 1024) return -1;
-    if (grow(&h->gm, &h->gm_cap, hpr::gmres_work_bytes(n, nrhs, m))) return -2;
-    hpr::GmresWork gw;
-    hpr::gmres_work_bind(gw, h->gm, n, nrhs, m);
+    // Two lanes of columns on two streams, staggered by one substitution, so
+    // that one lane's latency-bound substitutions run beside the other's
+    // MFMA-bound FP64 products (N = 25600, 128 columns: a substitution of 64
+    // columns takes 6.7 ms alone and 12 ms beside a 64-column FP64 product,
+    // which it slows by 4 %).  Automatic for large systems with at least 64
+    // columns; not under stream capture (a graph spans the capturing stream).
+    nl = h->solve_lanes > 0 ? h->solve_lanes : (n >= 8192 && nrhs >= 64 ? 2 : 1);
+    if (nl > 1) {
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone ||
+          !h->side.stream || nrhs < 2) {
+        (void)hipGetLastError();
+        nl = 1;
+      }
+    }
+    if (nl > 1) {
+      nc[0] = (((nrhs + 1) / 2 + 31) / 32) * 32;
+      if (nc[0] >= nrhs) nc[0] = nrhs / 2;
+      nc[1] = nrhs - nc[0];
+      c0[1] = nc[0];
+    }
+    size_t gm_bytes = 0;
+    for (int l = 0; l < nl; ++l) gm_bytes += hpr::gmres_work_bytes(n, nc[l], m);
+    if (grow(&h->gm, &h->gm_cap, gm_bytes)) return -2;
+    if (!sw.mesh && nl > 1 && sw.splits <= 1 &&
+        grow(&h->kpart, &h->kpart_cap, (size_t)hpr::RESID_MAX_SPLITS * cnt * 16))
+      return -2;
     if (h->method == HPRMAT_PLAIN_FP64)
-      rc = hpr::gmres_solve<double>((const double2*)h->fac, h->ld, n, (const double2*)h->dinv256,
-                                    h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, m,
-                                    tol, crit, h->state, h->R, (double2*)h->RF, (double2*)h->RY,
-                                    h->part, gw, h->plan, false, st, h->equil ? h->scale : nullptr,
-                                    sw, poll);
+      rc = fgmres_lanes<double>(h, nl, c0, nc, dB, ldb, max_refine, m, tol, crit, sw, poll, st);
     else
-      rc = hpr::gmres_solve<float>((const float2*)h->fac, h->ld, n, (const float2*)h->dinv256,
-                                   h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, m,
-                                   tol, crit, h->state, h->R, (float2*)h->RF, (float2*)h->RY,
-                                   h->part, gw, h->plan, g3m, st, h->equil ? h->scale : nullptr,
-                                   sw, poll);
+      rc = fgmres_lanes<float>(h, nl, c0, nc, dB, ldb, max_refine, m, tol, crit, sw, poll, st);
+    if (rc) return rc;
     final_check = 0;   // the FGMRES solve ends with a true residual evaluation
-  } else if (h->method == HPRMAT_PLAIN_FP64)
+  } else {
+    if (hpr::transpose_cast<double, double>(n, nrhs, (const double2*)dB, ldb, h->Brm, nrhs, st))
+      return -1;
+    if (h->method == HPRMAT_PLAIN_FP64)
     rc = hpr::refine_solve<double>((const double2*)h->fac, h->ld, n, (const double2*)h->dinv256,
                                    h->perm, h->A64, h->lda, h->Brm, h->Xrm, nrhs, max_refine, tol,
                                    crit, h->state, h->R, (double2*)h->RF, (double2*)h->RY,
@@ -268,15 +357,47 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
                                   crit, h->state, h->R, (float2*)h->RF, (float2*)h->RY, h->part,
                                   h->plan, g3m, st, h->equil ? h->scale : nullptr, sw, poll,
                                   final_check != 0);
-  if (rc) return rc;
-  if (hpr::transpose_cast<double, double>(nrhs, n, h->Xrm, nrhs, (double2*)dB, ldb, st)) return -1;
+    if (rc) return rc;
+    if (hpr::transpose_cast<double, double>(nrhs, n, h->Xrm, nrhs, (double2*)dB, ldb, st))
+      return -1;
+  }
   if (stats) {
-    RefineState s;
+    RefineState s, ls[hpr::GMRES_MAX_LANES];
     int info = 0, err = 0;
-    HPR_CHECK(hipMemcpyAsync(&s, h->state, sizeof(s), hipMemcpyDeviceToHost, st));
+    HPR_CHECK(hipMemcpyAsync(ls, h->state, nl * sizeof(RefineState), hipMemcpyDeviceToHost, st));
     HPR_CHECK(hipMemcpyAsync(&info, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, st));
     HPR_CHECK(hipMemcpyAsync(&err, h->sc.err, sizeof(int), hipMemcpyDeviceToHost, st));
     HPR_CHECK(hipStreamSynchronize(st));
+    s = ls[0];
+    if (nl > 1) {
+      // the lanes as one solve: norms of the union of columns, the larger
+      // step counts, converged when every lane is; kept in state 0 for the
+      // reference ABI's convergence guard
+      double rf = 0.0, bf = 0.0;
+      for (int l = 0; l < nl; ++l) {
+        rf += ls[l].res_fro * ls[l].res_fro;
+        bf += ls[l].b_fro * ls[l].b_fro;
+        s.done = s.done && ls[l].done;
+        s.diverged = s.diverged || ls[l].diverged;
+        s.sweeps = ls[l].sweeps > s.sweeps ? ls[l].sweeps : s.sweeps;
+        s.checks = ls[l].checks > s.checks ? ls[l].checks : s.checks;
+        s.res_max = fmax(s.res_max, ls[l].res_max);
+        s.res_absmax = fmax(s.res_absmax, ls[l].res_absmax);
+        s.b_max = fmax(s.b_max, ls[l].b_max);
+      }
+      s.res_fro = sqrt(rf);
+      s.b_fro = sqrt(bf);
+      for (int i = 0; i < 32 && i < s.checks; ++i) {
+        double acc = 0.0;
+        for (int l = 0; l < nl; ++l) {
+          const int k = i < ls[l].checks ? i : ls[l].checks - 1;
+          const double v = k >= 0 ? ls[l].hist[k] * ls[l].b_fro : 0.0;
+          acc += v * v;
+        }
+        s.hist[i] = s.b_fro != 0.0 ? sqrt(acc) / s.b_fro : 0.0;
+      }
+      HPR_CHECK(hipMemcpy(h->state, &s, sizeof(s), hipMemcpyHostToDevice));
+    }
     // a panel leaf that gave up a wait leaves invalid factors: report it
     // before anything else
     stats->info = err ? -3 : info;
@@ -533,6 +654,10 @@ int hprmat_hip_lu_set_plan(hprmat_hip_lu_t h, int key, int value) {
     case HPRMAT_PLAN_GMRES_CYCLE:
       if (value < 0 || value > hpr::GMRES_MAX_M) return -1;
       h->gm_cycle = value;
+      return 0;
+    case HPRMAT_PLAN_SOLVE_LANES:
+      if (value < 0 || value > 2) return -1;
+      h->solve_lanes = value;
       return 0;
     default:
       return -1;

@@ -296,73 +296,192 @@ void gmres_work_bind(GmresWork& w, void* base, int n, int nrhs, int m) {
   w.part = (double2*)take((size_t)GMRES_DOT_BLOCKS * (m + 1) * nrhs * sizeof(double2));
 }
 
+namespace {
+
+// Host-side progress of one lane through the algorithm above.  advance()
+// enqueues work up to the next point where the host may read a device flag
+// (after a cycle's true residual: `done`; after an Arnoldi step that is not
+// the cycle's last: `gm_halt`) and then returns, so that several lanes can
+// be driven round-robin: while the host waits for one lane's flag, the work
+// of the others is already queued on their streams.
+template <class TF> struct LaneRun {
+  enum Stage { CYCLE, AFTER_CHECK, STEP, AFTER_STEP, FINAL, DONE };
+  GmresLane<TF>* L = nullptr;
+  int left = 0, mc = 0, j = 0;
+  Stage stage = CYCLE;
+  bool converged = false;
+  int RefineState::*pending = nullptr;   // flag to read before the next advance
+  FlagPoll fp;
+};
+
+template <class TF>
+int lane_advance(LaneRun<TF>& r, const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256,
+                 const int* perm, const double2* A64, int64_t lda, int m, double tol, int crit,
+                 const LuPlan& plan, bool g3m, const double* scale) {
+  GmresLane<TF>& L = *r.L;
+  const int nrhs = L.nrhs;
+  const size_t cnt = (size_t)n * nrhs;
+  const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
+  const hipStream_t st = L.st;
+  const GmresWork& gw = L.gw;
+  RefineState* state = L.state;
+  const int* done = &state->done;
+  const int* halt = &state->gm_halt;
+  r.pending = nullptr;
+  for (;;) {
+    switch (r.stage) {
+      case LaneRun<TF>::CYCLE:
+        if (r.left <= 0) { r.stage = LaneRun<TF>::FINAL; break; }
+        r.mc = min(m, r.left);
+        r.left -= r.mc;
+        // true residual and the stopping rule
+        copy_kernel<<<eg, 256, 0, st>>>(cnt, L.B, L.R, done);
+        if (resid_gemm(n, nrhs, A64, lda, L.X, L.R, -1.0, 1, done, L.sw, st)) return -1;
+        residual_norms(n, nrhs, L.R, L.part, state, 1, crit, tol, done, st);
+        r.stage = LaneRun<TF>::AFTER_CHECK;
+        r.pending = &RefineState::done;
+        return 0;
+      case LaneRun<TF>::AFTER_CHECK:   // not converged: start a cycle
+        dots(n, nrhs, 1, L.R, 0, L.R, gw.h, 0, gw, done, st);
+        cycle_begin_kernel<<<1, 256, 0, st>>>(nrhs, r.mc, gw.h, gw.g, gw.inv, state);
+        colscale_kernel<<<eg, 256, 0, st>>>(cnt, nrhs, L.R, gw.inv, gw.V, halt);
+        r.j = 0;
+        r.stage = LaneRun<TF>::STEP;
+        break;
+      case LaneRun<TF>::STEP: {
+        const int j = r.j, mc = r.mc;
+        double2* Vj = gw.V + (size_t)j * cnt;
+        double2* Zj = gw.Z + (size_t)j * cnt;
+        // Z_j = M^-1 V_j
+        permute_cast<double, TF>(n, nrhs, perm, Vj, L.RF, halt, st, scale);
+        if (getrs_replay<TF>(L.graph, LU, ld, n, dinv256, L.RF, nrhs, nrhs, L.RY, halt, plan,
+                             g3m, st))
+          return -1;
+        widen_kernel<TF><<<eg, 256, 0, st>>>(cnt, nrhs, L.RF, scale, Zj, halt);
+        // W = A Z_j, two CGS passes, ||W||^2
+        if (resid_gemm(n, nrhs, A64, lda, Zj, gw.W, 1.0, 0, halt, L.sw, st)) return -1;
+        double2* h1 = gw.h;
+        double2* h2 = gw.h + (size_t)(mc + 1) * nrhs;
+        dots(n, nrhs, j + 1, gw.V, cnt, gw.W, h1, 0, gw, halt, st);
+        axpy_kernel<<<eg, 256, 0, st>>>(cnt, nrhs, j + 1, gw.V, cnt, h1, gw.W, halt);
+        dots(n, nrhs, j + 1, gw.V, cnt, gw.W, h2, 0, gw, halt, st);
+        axpy_kernel<<<eg, 256, 0, st>>>(cnt, nrhs, j + 1, gw.V, cnt, h2, gw.W, halt);
+        dots(n, nrhs, 1, gw.W, 0, gw.W, gw.y, 0, gw, halt, st);   // ||W||^2 into y (scratch)
+        arnoldi_step_kernel<<<1, 256, 0, st>>>(nrhs, mc, j, gw.h, gw.y, gw.H, gw.cs, gw.sn, gw.g,
+                                                gw.inv, state, tol);
+        if (j + 1 < mc) {
+          r.stage = LaneRun<TF>::AFTER_STEP;
+          r.pending = &RefineState::gm_halt;
+          return 0;
+        }
+        solve_y_kernel<<<(nrhs + 255) / 256, 256, 0, st>>>(nrhs, mc, gw.H, gw.g, gw.y, state);
+        update_x_kernel<<<eg, 256, 0, st>>>(cnt, nrhs, gw.Z, cnt, gw.y, L.X, state);
+        r.stage = LaneRun<TF>::CYCLE;
+        break;
+      }
+      case LaneRun<TF>::AFTER_STEP:   // the cycle goes on
+        colscale_kernel<<<eg, 256, 0, st>>>(cnt, nrhs, gw.W, gw.inv,
+                                            gw.V + (size_t)(r.j + 1) * cnt, halt);
+        r.j += 1;
+        r.stage = LaneRun<TF>::STEP;
+        break;
+      case LaneRun<TF>::FINAL:   // final true residual
+        if (!r.converged) {
+          copy_kernel<<<eg, 256, 0, st>>>(cnt, L.B, L.R, done);
+          if (resid_gemm(n, nrhs, A64, lda, L.X, L.R, -1.0, 1, done, L.sw, st)) return -1;
+          residual_norms(n, nrhs, L.R, L.part, state, 1, crit, tol, done, st);
+        }
+        r.stage = LaneRun<TF>::DONE;
+        return 0;
+      case LaneRun<TF>::DONE:
+        return 0;
+    }
+  }
+}
+
+}  // namespace
+
+template <class TF>
+int gmres_solve_lanes(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, const int* perm,
+                      const double2* A64, int64_t lda, int max_steps, int m, double tol, int crit,
+                      const LuPlan& plan, bool g3m, const double* scale, int nl,
+                      GmresLane<TF>* lanes, bool poll, hipEvent_t stagger) {
+  if (m < 1 || m > GMRES_MAX_M || nl < 1 || nl > GMRES_MAX_LANES) return -1;
+  LaneRun<TF> run[GMRES_MAX_LANES];
+  // initial solutions X = M^-1 B; each lane after the first starts once the
+  // previous lane's substitution is done, so that the lanes' latency-bound
+  // substitutions run beside the others' FP64 products
+  for (int l = 0; l < nl; ++l) {
+    GmresLane<TF>& L = lanes[l];
+    if (L.nrhs < 1 || L.nrhs > 1024) return -1;
+    const size_t cnt = (size_t)n * L.nrhs;
+    const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
+    if (l > 0 && stagger) HPR_CHECK(hipStreamWaitEvent(L.st, stagger, 0));
+    residual_norms(n, L.nrhs, L.B, L.part, L.state, 0, crit, tol, nullptr, L.st);
+    permute_cast<double, TF>(n, L.nrhs, perm, L.B, L.RF, nullptr, L.st, scale);
+    if (getrs<TF>(LU, ld, n, dinv256, L.RF, L.nrhs, L.nrhs, L.RY, nullptr, plan, g3m, L.st))
+      return -1;
+    if (l + 1 < nl && stagger) HPR_CHECK(hipEventRecord(stagger, L.st));
+    widen_kernel<TF><<<eg, 256, 0, L.st>>>(cnt, L.nrhs, L.RF, scale, L.X, &L.state->gm_halt);
+    run[l].L = &L;
+    run[l].left = max_steps;
+    run[l].fp = flag_poll(L.st, poll);
+  }
+  // round-robin: read one lane's flag, enqueue its next segment, go to the next lane
+  for (int l = 0; l < nl; ++l)
+    if (lane_advance(run[l], LU, ld, n, dinv256, perm, A64, lda, m, tol, crit, plan, g3m, scale))
+      return -1;
+  for (;;) {
+    bool active = false;
+    for (int l = 0; l < nl; ++l) {
+      LaneRun<TF>& r = run[l];
+      if (r.stage == LaneRun<TF>::DONE) continue;
+      active = true;
+      if (r.pending && r.fp.read(r.L->state, r.L->st, r.pending)) {
+        if (r.stage == LaneRun<TF>::AFTER_CHECK) {   // converged
+          r.converged = true;
+          r.stage = LaneRun<TF>::DONE;
+          continue;
+        }
+        // the cycle stopped early (its estimate met the rule): update X
+        const GmresWork& gw = r.L->gw;
+        const size_t cnt = (size_t)n * r.L->nrhs;
+        const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
+        solve_y_kernel<<<(r.L->nrhs + 255) / 256, 256, 0, r.L->st>>>(r.L->nrhs, r.mc, gw.H,
+                                                                      gw.g, gw.y, r.L->state);
+        update_x_kernel<<<eg, 256, 0, r.L->st>>>(cnt, r.L->nrhs, gw.Z, cnt, gw.y, r.L->X,
+                                                 r.L->state);
+        r.stage = LaneRun<TF>::CYCLE;
+      }
+      if (lane_advance(r, LU, ld, n, dinv256, perm, A64, lda, m, tol, crit, plan, g3m, scale))
+        return -1;
+    }
+    if (!active) break;
+  }
+  HPR_CHECK(hipGetLastError());
+  return 0;
+}
+
 template <class TF>
 int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, const int* perm,
                 const double2* A64, int64_t lda, const double2* B, double2* X, int nrhs,
                 int max_steps, int m, double tol, int crit, RefineState* state, double2* R,
                 cx<TF>* RF, cx<TF>* RY, double* part, const GmresWork& gw, const LuPlan& plan,
                 bool g3m, hipStream_t st, const double* scale, const SplitWork& sw, bool poll) {
-  if (m < 1 || m > GMRES_MAX_M || nrhs > 1024) return -1;
-  const size_t cnt = (size_t)n * nrhs;
-  const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
-  const int* done = &state->done;
-  const int* halt = &state->gm_halt;
-  // initial solution X = M^-1 B
-  residual_norms(n, nrhs, B, part, state, 0, crit, tol, nullptr, st);
-  permute_cast<double, TF>(n, nrhs, perm, B, RF, nullptr, st, scale);
-  if (getrs<TF>(LU, ld, n, dinv256, RF, nrhs, nrhs, RY, nullptr, plan, g3m, st)) return -1;
-  widen_kernel<TF><<<eg, 256, 0, st>>>(cnt, nrhs, RF, scale, X, halt);
-  const FlagPoll fp = flag_poll(st, poll);
-  bool converged = false;
-  int left = max_steps;
-  while (left > 0) {
-    const int mc = min(m, left);
-    left -= mc;
-    // true residual, stopping rule, start of a cycle
-    copy_kernel<<<eg, 256, 0, st>>>(cnt, B, R, done);
-    if (resid_gemm(n, nrhs, A64, lda, X, R, -1.0, 1, done, sw, st)) return -1;
-    residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
-    if (fp.read(state, st, &RefineState::done)) {
-      converged = true;
-      break;
-    }
-    dots(n, nrhs, 1, R, 0, R, gw.h, 0, gw, done, st);
-    cycle_begin_kernel<<<1, 256, 0, st>>>(nrhs, mc, gw.h, gw.g, gw.inv, state);
-    colscale_kernel<<<eg, 256, 0, st>>>(cnt, nrhs, R, gw.inv, gw.V, halt);
-    for (int j = 0; j < mc; ++j) {
-      double2* Vj = gw.V + (size_t)j * cnt;
-      double2* Zj = gw.Z + (size_t)j * cnt;
-      // Z_j = M^-1 V_j
-      permute_cast<double, TF>(n, nrhs, perm, Vj, RF, halt, st, scale);
-      if (getrs<TF>(LU, ld, n, dinv256, RF, nrhs, nrhs, RY, halt, plan, g3m, st)) return -1;
-      widen_kernel<TF><<<eg, 256, 0, st>>>(cnt, nrhs, RF, scale, Zj, halt);
-      // W = A Z_j, two CGS passes, ||W||^2
-      if (resid_gemm(n, nrhs, A64, lda, Zj, gw.W, 1.0, 0, halt, sw, st)) return -1;
-      double2* h1 = gw.h;
-      double2* h2 = gw.h + (size_t)(mc + 1) * nrhs;
-      dots(n, nrhs, j + 1, gw.V, cnt, gw.W, h1, 0, gw, halt, st);
-      axpy_kernel<<<eg, 256, 0, st>>>(cnt, nrhs, j + 1, gw.V, cnt, h1, gw.W, halt);
-      dots(n, nrhs, j + 1, gw.V, cnt, gw.W, h2, 0, gw, halt, st);
-      axpy_kernel<<<eg, 256, 0, st>>>(cnt, nrhs, j + 1, gw.V, cnt, h2, gw.W, halt);
-      dots(n, nrhs, 1, gw.W, 0, gw.W, gw.y, 0, gw, halt, st);   // ||W||^2 into y (scratch)
-      arnoldi_step_kernel<<<1, 256, 0, st>>>(nrhs, mc, j, gw.h, gw.y, gw.H, gw.cs, gw.sn, gw.g,
-                                              gw.inv, state, tol);
-      if (j + 1 < mc && fp.read(state, st, &RefineState::gm_halt)) break;
-      if (j + 1 < mc)
-        colscale_kernel<<<eg, 256, 0, st>>>(cnt, nrhs, gw.W, gw.inv, gw.V + (size_t)(j + 1) * cnt,
-                                            halt);
-    }
-    solve_y_kernel<<<(nrhs + 255) / 256, 256, 0, st>>>(nrhs, mc, gw.H, gw.g, gw.y, state);
-    update_x_kernel<<<eg, 256, 0, st>>>(cnt, nrhs, gw.Z, cnt, gw.y, X, state);
-  }
-  // final true residual
-  if (!converged) {
-    copy_kernel<<<eg, 256, 0, st>>>(cnt, B, R, done);
-    if (resid_gemm(n, nrhs, A64, lda, X, R, -1.0, 1, done, sw, st)) return -1;
-    residual_norms(n, nrhs, R, part, state, 1, crit, tol, done, st);
-  }
-  HPR_CHECK(hipGetLastError());
-  return 0;
+  GmresLane<TF> lane;
+  lane.B = B;
+  lane.X = X;
+  lane.nrhs = nrhs;
+  lane.state = state;
+  lane.R = R;
+  lane.RF = RF;
+  lane.RY = RY;
+  lane.part = part;
+  lane.gw = gw;
+  lane.sw = sw;
+  lane.st = st;
+  return gmres_solve_lanes<TF>(LU, ld, n, dinv256, perm, A64, lda, max_steps, m, tol, crit, plan,
+                               g3m, scale, 1, &lane, poll, nullptr);
 }
 
 template int gmres_solve<double>(const cx<double>*, int64_t, int, const cx<double>*, const int*,
@@ -375,5 +494,13 @@ template int gmres_solve<float>(const cx<float>*, int64_t, int, const cx<float>*
                                 double, int, RefineState*, double2*, cx<float>*, cx<float>*,
                                 double*, const GmresWork&, const LuPlan&, bool, hipStream_t,
                                 const double*, const SplitWork&, bool);
+template int gmres_solve_lanes<double>(const cx<double>*, int64_t, int, const cx<double>*,
+                                       const int*, const double2*, int64_t, int, int, double, int,
+                                       const LuPlan&, bool, const double*, int,
+                                       GmresLane<double>*, bool, hipEvent_t);
+template int gmres_solve_lanes<float>(const cx<float>*, int64_t, int, const cx<float>*,
+                                      const int*, const double2*, int64_t, int, int, double, int,
+                                      const LuPlan&, bool, const double*, int, GmresLane<float>*,
+                                      bool, hipEvent_t);
 
 }  // namespace hpr

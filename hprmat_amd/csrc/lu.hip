@@ -18,6 +18,7 @@
 // 256-row block rows: one MFMA block-solve kernel per block row using the
 // stored 64 x 64 inverses, and one MFMA GEMM update of the remaining rows
 // (cusolverDn?getrs, src/cusolver_interface.cu:393, 787).
+#include <mutex>
 #include <cmath>
 #include <vector>
 
@@ -985,6 +986,91 @@ int getrs(const cx<T>* LU, int64_t ld, int n, const cx<T>* dinv256, cx<T>* B, in
   return 0;
 }
 
+void subst_graph_release(SubstGraph& g) {
+  if (g.exec) {
+    (void)hipStreamSynchronize(g.last);
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGetLastError();
+  }
+  g = SubstGraph();
+}
+
+// One non-blocking stream per device, used only to record substitution
+// graphs (nothing is ever executed on it).  Created on first use, kept until
+// the process ends.
+static hipStream_t capture_stream() {
+  static std::mutex mu;
+  static hipStream_t streams[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!streams[dev] && hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    streams[dev] = nullptr;
+  }
+  return streams[dev];
+}
+
+template <class T>
+int getrs_replay(SubstGraph* g, const cx<T>* LU, int64_t ld, int n, const cx<T>* dinv256,
+                 cx<T>* B, int64_t ldb, int nrhs, cx<T>* Y, const int* skip, const LuPlan& plan,
+                 bool g3m, hipStream_t st) {
+  const char* env = getenv("HPRMAT_HIP_SUBST_GRAPH");   // read per call (tests toggle it)
+  const bool enabled = !(env && env[0] == '0');
+  if (!g || !enabled || n <= 0 || nrhs <= 0)
+    return getrs<T>(LU, ld, n, dinv256, B, ldb, nrhs, Y, skip, plan, g3m, st);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (st && (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)) {
+    (void)hipGetLastError();   // the caller is capturing: its graph takes the launches
+    return getrs<T>(LU, ld, n, dinv256, B, ldb, nrhs, Y, skip, plan, g3m, st);
+  }
+  if (!g->exec || g->LU != LU || g->dinv != dinv256 || g->B != B || g->Y != Y ||
+      g->skip != skip || g->n != n || g->nrhs != nrhs || g->ld != ld || g->ldb != ldb ||
+      g->g3m != g3m || g->elem != (int)sizeof(cx<T>)) {
+    subst_graph_release(*g);
+    // The launches are recorded on a private stream of this device (the
+    // caller's stream may be the null stream, which cannot be captured); the
+    // instantiated graph is then launched on the caller's stream.
+    hipStream_t cap = capture_stream();
+    hipGraph_t graph = nullptr;
+    if (!cap || hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipGetLastError();
+      return getrs<T>(LU, ld, n, dinv256, B, ldb, nrhs, Y, skip, plan, g3m, st);
+    }
+    const int rc = getrs<T>(LU, ld, n, dinv256, B, ldb, nrhs, Y, skip, plan, g3m, cap);
+    const hipError_t e = hipStreamEndCapture(cap, &graph);
+    if (rc || e != hipSuccess || !graph) {
+      if (graph) (void)hipGraphDestroy(graph);
+      (void)hipGetLastError();
+      return getrs<T>(LU, ld, n, dinv256, B, ldb, nrhs, Y, skip, plan, g3m, st);
+    }
+    const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {   // launch the kernels directly instead
+      g->exec = nullptr;
+      (void)hipGetLastError();
+      return getrs<T>(LU, ld, n, dinv256, B, ldb, nrhs, Y, skip, plan, g3m, st);
+    }
+    g->LU = LU;
+    g->dinv = dinv256;
+    g->B = B;
+    g->Y = Y;
+    g->skip = skip;
+    g->n = n;
+    g->nrhs = nrhs;
+    g->ld = ld;
+    g->ldb = ldb;
+    g->g3m = g3m;
+    g->elem = (int)sizeof(cx<T>);
+  }
+  HPR_CHECK(hipGraphLaunch(g->exec, st));
+  g->last = st;
+  return 0;
+}
+
 template <class Tin, class Tout>
 int permute_cast(int n, int nrhs, const int* perm, const cx<Tin>* in, cx<Tout>* out,
                  const int* skip, hipStream_t st, const double* scale) {
@@ -1445,6 +1531,9 @@ int solve_batched(const BatchArgs<TF>& b, const double2* A64, int64_t lda, int64
                         const LuPlan&, int, hipStream_t, const SideStream*);                   \
   template int getrs<T>(const cx<T>*, int64_t, int, const cx<T>*, cx<T>*, int64_t, int, cx<T>*, \
                         const int*, const LuPlan&, bool, hipStream_t);                         \
+  template int getrs_replay<T>(SubstGraph*, const cx<T>*, int64_t, int, const cx<T>*, cx<T>*,  \
+                               int64_t, int, cx<T>*, const int*, const LuPlan&, bool,          \
+                               hipStream_t);                                                   \
   template int gemm<T>(int, int, int, const cx<T>*, int64_t, bool, const cx<T>*, int64_t,      \
                        cx<T>*, int64_t, T, int, bool, const int*, hipStream_t, size_t, int);   \
   template int refine_solve<T>(const cx<T>*, int64_t, int, const cx<T>*, const int*,          \

@@ -98,6 +98,8 @@ struct SideStream {
   hipStream_t stream = nullptr;
   hipEvent_t ready = nullptr;   // main -> side: U12 staged, next panel updated
   hipEvent_t rest = nullptr;    // side -> main: rest of the trailing update done
+  hipEvent_t lane = nullptr;    // FGMRES lanes: the first lane's first substitution done
+  hipEvent_t join = nullptr;    // FGMRES lanes: the second lane's work done
 };
 
 // Device factorization P A = L U (row-major A, in place).  Also produces the
@@ -118,6 +120,29 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
 template <class T>
 int getrs(const cx<T>* LU, int64_t ld, int n, const cx<T>* dinv256, cx<T>* B, int64_t ldb,
           int nrhs, cx<T>* Y, const int* skip, const LuPlan& plan, bool gauss3m, hipStream_t st);
+
+// A substitution chain (getrs) captured once into a HIP graph and then
+// replayed: one graph launch instead of about 3 n / 256 kernel launches,
+// which at n = 25600 cost the host about 6 ms per substitution -- more than
+// the GPU needs for it when two FGMRES lanes share the host thread.  The
+// graph bakes in the factors, the right-hand-side and work buffers and the
+// skip flag; a call with any of them (or n, nrhs, ld, ldb) different
+// captures anew.  Under stream capture, or with HPRMAT_HIP_SUBST_GRAPH=0,
+// the kernels are launched directly.
+struct SubstGraph {
+  hipGraphExec_t exec = nullptr;
+  hipStream_t last = nullptr;   // stream of the latest launch (drained before release)
+  const void *LU = nullptr, *dinv = nullptr, *B = nullptr, *Y = nullptr;
+  const int* skip = nullptr;
+  int n = 0, nrhs = 0, elem = 0;
+  int64_t ld = 0, ldb = 0;
+  bool g3m = false;
+};
+void subst_graph_release(SubstGraph& g);
+template <class T>
+int getrs_replay(SubstGraph* g, const cx<T>* LU, int64_t ld, int n, const cx<T>* dinv256,
+                 cx<T>* B, int64_t ldb, int nrhs, cx<T>* Y, const int* skip, const LuPlan& plan,
+                 bool gauss3m, hipStream_t st);
 
 // out[i][:] = (Tout) (scale[perm[i]] in[perm[i]][:]) (row-major n x nrhs;
 // scale = null: no scaling).
@@ -247,6 +272,37 @@ int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, cons
                 cx<TF>* RF, cx<TF>* RY, double* part, const GmresWork& gw, const LuPlan& plan,
                 bool gauss3m, hipStream_t st, const double* scale,
                 const SplitWork& sw = SplitWork(), bool poll = false);
+
+// One lane of a multi-lane FGMRES solve: a subset of the right-hand sides
+// (row-major n x nrhs blocks of their own) with its own refinement state,
+// work buffers and stream.
+template <class TF> struct GmresLane {
+  const double2* B = nullptr;
+  double2* X = nullptr;
+  int nrhs = 0;
+  RefineState* state = nullptr;
+  double2* R = nullptr;
+  cx<TF>* RF = nullptr;
+  cx<TF>* RY = nullptr;
+  double* part = nullptr;
+  GmresWork gw;
+  SplitWork sw;
+  hipStream_t st = nullptr;
+  SubstGraph* graph = nullptr;   // replayed substitution of the Arnoldi steps (null: launches)
+};
+constexpr int GMRES_MAX_LANES = 4;
+// gmres_solve for nl lanes at once, each lane on its own stream with its
+// own stopping rule (a lane that meets the rule stops; the union meets it
+// too).  Lane l + 1 starts after lane l's first substitution (`stagger`, an
+// event; null: no stagger) and, with `poll`, the host reads the lanes' flags
+// round-robin, so that the lanes stay staggered: the latency-bound
+// substitutions of one lane run beside the MFMA-bound FP64 products of
+// another.  The caller joins the lanes' streams.
+template <class TF>
+int gmres_solve_lanes(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, const int* perm,
+                      const double2* A64, int64_t lda, int max_steps, int m, double tol, int crit,
+                      const LuPlan& plan, bool gauss3m, const double* scale, int nl,
+                      GmresLane<TF>* lanes, bool poll, hipEvent_t stagger);
 
 // ---- uniform batches ------------------------------------------------------
 // `count` independent systems of one order n, each at a fixed element

@@ -59,12 +59,15 @@ class LU:
             self._lib.hprmat_hip_lu_destroy(h)
             self._h = None
 
-    def set_plan(self, trail_tile=None, lookahead=None, gmres_cycle=None):
+    def set_plan(self, trail_tile=None, lookahead=None, gmres_cycle=None, solve_lanes=None):
         """Factorization plan: trailing-update GEMM tile (0 64x64x16,
         1 64x128x16, 2 128x64x16 on 16x16x4 MFMA; FP32 factors only, on
         32x32x2 MFMA: 3 64x64x16, 4 128x128x16, 5 128x64x16, 6 64x128x16),
-        look-ahead on/off, and the FGMRES cycle length (1..24 steps; 0 =
-        default 16)."""
+        look-ahead on/off, the FGMRES cycle length (1..24 steps; 0 =
+        default 16) and the FGMRES lanes (0 automatic, 1, 2)."""
+        if solve_lanes is not None:
+            _lib.check(self._lib.hprmat_hip_lu_set_plan(self._h, _lib.PLAN_SOLVE_LANES,
+                                                         int(solve_lanes)), 'set_plan')
         if gmres_cycle is not None:
             _lib.check(self._lib.hprmat_hip_lu_set_plan(self._h, _lib.PLAN_GMRES_CYCLE,
                                                          int(gmres_cycle)), 'set_plan')

@@ -150,8 +150,16 @@ int hprmat_hip_lu_destroy(hprmat_hip_lu_t h);
                              update on a second stream (default), 0 = off
      HPRMAT_PLAN_GMRES_CYCLE steps per FGMRES cycle, 1..24 (0 = default 16);
                              the solve's step budget spans several cycles
+     HPRMAT_PLAN_SOLVE_LANES FGMRES solves: 1 = all columns in one lane,
+                             2 = two lanes of columns on two streams, the
+                             second staggered by one substitution (one
+                             lane's substitutions overlap the other's FP64
+                             products; each lane stops on its own rule), 0 =
+                             automatic (two for n >= 8192 and >= 64 columns,
+                             one under stream capture)
    Returns -1 for an unknown key or value. */
-enum { HPRMAT_PLAN_TRAIL_TILE = 1, HPRMAT_PLAN_LOOKAHEAD = 2, HPRMAT_PLAN_GMRES_CYCLE = 3 };
+enum { HPRMAT_PLAN_TRAIL_TILE = 1, HPRMAT_PLAN_LOOKAHEAD = 2, HPRMAT_PLAN_GMRES_CYCLE = 3,
+       HPRMAT_PLAN_SOLVE_LANES = 4 };
 int hprmat_hip_lu_set_plan(hprmat_hip_lu_t h, int key, int value);
 
 /* No reference counterpart: describe the factored matrix by its

@@ -188,6 +188,42 @@ def test_fgmres_zero_rhs_and_exact_start():
     assert np.all(x == 0) and st['sweeps'] == 0, st
 
 
+@pytest.mark.parametrize('method,nrhs', [('fp32', 128), ('fp32_3m', 100), ('fp64', 70)])
+def test_fgmres_two_lanes_match_one_lane(method, nrhs):
+    # FGMRES with the columns in two staggered lanes on two streams: every
+    # lane reaches the rule on its own, so the union does; the solution
+    # agrees with the one-lane solve (same products per column up to where
+    # each lane stops) and with LAPACK; the combined statistics are the
+    # norms of the union (ragged split: 64 + 36 and 64 + 6 columns)
+    from hprmat_amd import synth as hsynth
+    sysm = hsynth.System(16, 160, 15.0, 8.0, 5.0, 3)
+    dm = rmatrix.DeviceMesh(sysm.mesh)
+    _, _, _, xc1 = rmatrix.channel_data(sysm.lval, sysm.qk, sysm.a)
+    a_cm = rmatrix.assemble(dm, sysm.lval, sysm.qk, xc1, sysm.cpot)
+    n = a_cm.shape[0]
+    # random combinations of the physical right-hand sides (q2 in one channel
+    # block each), whose solutions the FP64 floor lets reach 1e-12
+    rng = np.random.default_rng(nrhs)
+    b = rmatrix.rhs_colmajor(dm, 16).cpu().numpy().T @ (
+        rng.standard_normal((16, nrhs)) + 1j * rng.standard_normal((16, nrhs)))
+    out = {}
+    for lanes in (1, 2):
+        print('fgmres lanes %d method %s n %d nrhs %d' % (lanes, method, n, nrhs), flush=True)
+        lu = solver.LU().set_plan(solve_lanes=lanes).factor(a_cm, method, equilibrate=True)
+        b_cm = torch.from_numpy(np.ascontiguousarray(b.T)).cuda()
+        st = lu.solve_(b_cm, 24, 1e-12, 'rel_fro', True, refinement='fgmres')
+        assert st['info'] == 0 and st['converged'] and st['res_rel'] <= 1e-12, (lanes, st)
+        out[lanes] = (b_cm.cpu().numpy().T, st)
+    a = a_cm.cpu().numpy()          # A^T = A: the storage is the matrix
+    x_ref, info = osolve.zgesv(a, b)
+    for lanes, (x, st) in out.items():
+        rel = np.linalg.norm(b - a @ x) / np.linalg.norm(b)
+        assert rel <= 1.5e-12 and abs(rel - st['res_rel']) <= 0.2 * rel, (lanes, rel, st['res_rel'])
+        assert np.abs(x - x_ref).max() / np.abs(x_ref).max() <= 1e-8
+    x1, x2 = out[1][0], out[2][0]
+    assert np.abs(x1 - x2).max() / np.abs(x1).max() <= 1e-9
+
+
 def test_full_size_mixed_solve_gate():
     # N = 25600 complex, 128 right-hand sides: the benchmark system.  Checked by
     # properties that do not need a CPU solve of that size: the FP64 relative
@@ -292,3 +328,32 @@ def test_substitution_tiles_wide_rhs(method):
     x, st = gpu_solve(a, b, method, max_refine=0 if method == 'fp64' else 12)
     assert st['info'] == 0
     assert np.abs(x - x_ref).max() / np.abs(x_ref).max() <= 1e-12, st
+
+
+def test_replayed_substitution_graph_matches_direct_launches(monkeypatch):
+    # FGMRES replays each lane's substitution chain from a captured HIP graph;
+    # with HPRMAT_HIP_SUBST_GRAPH=0 the same kernels are launched directly.
+    # Three solves through the graphs, with other device work and host
+    # synchronisation between them, give the direct launches' solution and
+    # statistics bit for bit
+    s = synth.System(8, 200, 15.0, 8.0, 5.0, 1)
+    dm = rmatrix.DeviceMesh(s.mesh)
+    _, _, _, xc1 = rmatrix.channel_data(s.lval, s.qk, s.a)
+    a_cm = rmatrix.assemble(dm, s.lval, s.qk, xc1, s.cpot)
+    b0 = rmatrix.rhs_colmajor(dm, s.nch)
+    lu = solver.LU().set_plan(solve_lanes=2).factor(a_cm, 'fp32', equilibrate=True)
+
+    def run():
+        b = b0.clone()
+        st = lu.solve_(b, 24, 1e-12, 'rel_fro', True, refinement='fgmres')
+        return b, st
+    monkeypatch.setenv('HPRMAT_HIP_SUBST_GRAPH', '0')
+    x_direct, st_direct = run()
+    assert st_direct['converged'], st_direct
+    monkeypatch.setenv('HPRMAT_HIP_SUBST_GRAPH', '1')
+    for rep in range(3):
+        torch.randn(1000, 1000, device='cuda').sum().item()   # other work in between
+        torch.cuda.synchronize()
+        x, st = run()
+        assert torch.equal(x, x_direct), rep
+        assert st == st_direct, rep
