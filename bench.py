@@ -240,6 +240,11 @@ def main():
     ap.add_argument('--scan-refine', type=int, default=15)
     ap.add_argument('--scan-method', default='fp32_3m', choices=['fp32', 'fp32_3m'])
     ap.add_argument('--scan-graph', type=int, default=1)
+    ap.add_argument('--scan-autotune', type=int, default=1,
+                    help='per-size plans (arithmetic, tiles, sweeps) and lanes from '
+                         'scan.autotune_batch, measured before the timed run; 0: the fixed '
+                         '--scan-method / --scan-refine / --scan-lanes plan')
+    ap.add_argument('--scan-tune-sample', type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path '
@@ -619,8 +624,18 @@ def energy_scan(torch, args):
     t0 = time.perf_counter()
     waves = scan.make_waves(args.scan_waves, args.scan_energies, seed=0)
     t_prep = time.perf_counter() - t0
-    sc = scan.BatchScan(waves, method=args.scan_method, lanes=args.scan_lanes,
-                        max_refine=args.scan_refine,
+    plans, lanes, tune = None, args.scan_lanes, None
+    if args.scan_autotune:
+        say('scan: autotune')
+        t0 = time.perf_counter()
+        plans, lanes = scan.autotune_batch(waves, sample=args.scan_tune_sample, log=sys.stderr)
+        tune = {'seconds': time.perf_counter() - t0, 'sample_energies': args.scan_tune_sample,
+                'lanes': lanes,
+                'plans': {'%dx%d' % k: {kk: v[kk] for kk in ('method', 'trail', 'subst',
+                                                              'max_refine')}
+                          for k, v in sorted(plans.items())}}
+    sc = scan.BatchScan(waves, method=args.scan_method, lanes=lanes,
+                        max_refine=args.scan_refine, plans=plans,
                         log=sys.stderr)
     t0 = time.perf_counter()
     if args.scan_graph:
@@ -675,8 +690,10 @@ def energy_scan(torch, args):
     n_sys = sc.n_systems
     out = {'systems': n_sys, 'ms': ms, 'solves_per_s': n_sys / (ms * 1e-3),
            'tflops': sc.flops / (ms * 1e-3) / 1e12, 'flops': sc.flops, 'lanes': len(sc.lanes),
-           'graphs': bool(args.scan_graph), 'max_refine': args.scan_refine,
-           'method': args.scan_method,
+           'graphs': bool(args.scan_graph),
+           'plan': 'autotuned' if plans else {'method': args.scan_method,
+                                              'max_refine': args.scan_refine},
+           'autotune': tune,
            'fp64_fallbacks': n_redo, 'unconverged': bad, 'max_res_rel': worst,
            'serial_fp64': {'systems_timed': len(pairs), 'ms': ms_ser,
                            'solves_per_s': len(pairs) / (ms_ser * 1e-3),

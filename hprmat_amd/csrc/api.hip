@@ -754,10 +754,16 @@ int batch_solve_impl(int method_flags, int n, int nrhs, int count, const void* d
                      size_t work_bytes, void* stream, const hpr::MeshOperator* mesh) {
   const int method = method_flags & 0xff;
   const bool equil = (method_flags & HPRMAT_EQUILIBRATE) != 0;
+  const int trail = (method_flags >> 16) & 0xf, subst = (method_flags >> 20) & 0xf;
   if (method != HPRMAT_PLAIN_FP64 && method != HPRMAT_MIXED_FP32 &&
       method != HPRMAT_MIXED_FP32_3M) {
     fprintf(stderr, "hprmat_hip: batched solve supports HPRMAT_PLAIN_FP64, HPRMAT_MIXED_FP32 "
                     "and HPRMAT_MIXED_FP32_3M\n");
+    return -1;
+  }
+  if (trail > 2 || subst > 1 || (method_flags & ~0xff01ff) != 0) {
+    fprintf(stderr, "hprmat_hip: batched solve: unknown plan bits in method_flags 0x%x\n",
+            method_flags);
     return -1;
   }
   if (criterion & HPRMAT_FGMRES) {
@@ -773,6 +779,8 @@ int batch_solve_impl(int method_flags, int n, int nrhs, int count, const void* d
     hpr::BatchArgs<double> b;
     const int rc = hpr::batch_bind<double>(b, work, work_bytes, n, nrhs, count, equil);
     if (rc) return rc;
+    b.trail_tile = trail;
+    b.subst_tile = subst;
     return hpr::solve_batched<double>(b, (const double2*)dA, lda, stride_a, (double2*)dB, ldb,
                                       stride_b, nrhs, max_refine, tol, criterion, res_rel, status,
                                       st, mesh);
@@ -781,6 +789,8 @@ int batch_solve_impl(int method_flags, int n, int nrhs, int count, const void* d
   const int rc = hpr::batch_bind<float>(b, work, work_bytes, n, nrhs, count, equil);
   if (rc) return rc;
   b.g3m = method == HPRMAT_MIXED_FP32_3M;
+  b.trail_tile = trail;
+  b.subst_tile = subst;
   return hpr::solve_batched<float>(b, (const double2*)dA, lda, stride_a, (double2*)dB, ldb,
                                    stride_b, nrhs, max_refine, tol, criterion, res_rel, status, st,
                                    mesh);

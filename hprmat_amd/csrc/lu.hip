@@ -1197,11 +1197,43 @@ inline int64_t batch_xchg_words(int n) {
   return up32(4 * g + 2 * g + 2 * g * LEAF_W * 4 + 2 * LEAF_W * 4);
 }
 
+// Tiles of a batched product (row-major A): 0 = 64 x 64 (waves of 32 x 32),
+// 1 = 32 x 32 (waves of 16 x 16: four times the workgroups, for small
+// systems), 2 = 128 x 64 (waves of 64 x 32).
+template <class T, bool G3M>
+void gemm_b_tiled(int tile, int M, int N, int K, const cx<T>* A, int64_t lda, const cx<T>* B,
+                  int64_t ldb, cx<T>* C, int64_t ldc, T alpha, int beta, const int* skip,
+                  int sSkip, int count, int64_t sA, int64_t sB, int64_t sC, hipStream_t st) {
+  if (tile == 1) {
+    const dim3 g(((M + 31) / 32) * ((N + 31) / 32), count);
+    zgemm_kernel<T, 32, 32, 16, 16, 16, false, G3M><<<g, 256, 0, st>>>(
+        M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, sA, sB, sC, sSkip);
+  } else if (tile == 2) {
+    const dim3 g(((M + 127) / 128) * ((N + 63) / 64), count);
+    zgemm_kernel<T, 128, 64, 16, 64, 32, false, G3M><<<g, 256, 0, st>>>(
+        M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, sA, sB, sC, sSkip);
+  } else {
+    const dim3 g(((M + 63) / 64) * ((N + 63) / 64), count);
+    zgemm_kernel<T, 64, 64, 16, 32, 32, false, G3M><<<g, 256, 0, st>>>(
+        M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, sA, sB, sC, sSkip);
+  }
+}
+
 template <class T>
 void gemm_b(int M, int N, int K, const cx<T>* A, int64_t lda, bool a_col, const cx<T>* B,
             int64_t ldb, cx<T>* C, int64_t ldc, T alpha, int beta, const int* skip, int sSkip,
-            int count, int64_t sA, int64_t sB, int64_t sC, hipStream_t st, bool g3m = false) {
+            int count, int64_t sA, int64_t sB, int64_t sC, hipStream_t st, bool g3m = false,
+            int tile = 0) {
   if (M <= 0 || N <= 0 || K <= 0 || count <= 0) return;
+  if (tile != 0 && !a_col) {
+    if (g3m)
+      gemm_b_tiled<T, true>(tile, M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, sSkip,
+                            count, sA, sB, sC, st);
+    else
+      gemm_b_tiled<T, false>(tile, M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, skip, sSkip,
+                             count, sA, sB, sC, st);
+    return;
+  }
   constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32;
   if (N <= 32 && !g3m) {
     // narrow products (a batch's right-hand sides): 64 x 32 tiles, no idle
@@ -1382,10 +1414,11 @@ int getrf_batched(const BatchArgs<T>& b, hipStream_t st) {
       laswp_b<T>(b, k + kb, n, k, kb, st);
       // U12 = inv(L11) A12 (staged, row stride nc), A22 -= L21 U12, U12 back
       gemm_b<T>(kb, nc, kb, b.dinv256 + (size_t)(k / 256) * 2 * 65536, 256, false, Akk + kb, ld,
-                b.ustage, nc, T(1), 0, nullptr, 0, count, b.s256, sF, b.sU, st);
+                b.ustage, nc, T(1), 0, nullptr, 0, count, b.s256, sF, b.sU, st, false,
+                b.trail_tile);
       gemm_b<T>(nc, nc, kb, Akk + (int64_t)kb * ld, ld, false, b.ustage, nc,
                 Akk + (int64_t)kb * ld + kb, ld, T(-1), 1, nullptr, 0, count, sF, b.sU, sF, st,
-                b.g3m);
+                b.g3m, b.trail_tile);
       copy2d_kernel<T><<<copy_grid((size_t)kb * nc, count), 256, 0, st>>>(
           kb, nc, b.ustage, nc, Akk + kb, ld, b.sU, sF);
     }
@@ -1435,7 +1468,11 @@ int getrs_batched(const BatchArgs<T>& b, cx<T>* B, cx<T>* Y, int nrhs, const int
     // the product with inv(L_bb) on 32 x 32 tiles over its nonzero triangle
     // (tri_dual_kernel), the update on 64 x 32 tiles for narrow batches
     const int t1 = ((bs + 31) / 32) * ((nrhs + 31) / 32);
-    if (nrhs <= 32) {
+    if (b.subst_tile == 1) {   // 32 x 32 update tiles
+      const int t2 = p2.M > 0 ? ((p2.M + 31) / 32) * ((nrhs + 31) / 32) : 0;
+      tri_dual_kernel<T, true, 32, 32, 16, 16><<<dim3(t1 + t2, count), 256, 0, st>>>(
+          p1, p2, t1, skip, sSkip);
+    } else if (nrhs <= 32) {
       const int t2 = p2.M > 0 ? ((p2.M + 63) / 64) * ((nrhs + 31) / 32) : 0;
       tri_dual_kernel<T, true, 64, 32, 32, 16><<<dim3(t1 + t2, count), 256, 0, st>>>(
           p1, p2, t1, skip, sSkip);
@@ -1457,7 +1494,7 @@ int getrs_batched(const BatchArgs<T>& b, cx<T>* B, cx<T>* Y, int nrhs, const int
                                                                                sSkip);
     if (r > 0)
       gemm_b<T>(r, nrhs, bs, b.F + r, ld, false, B + (int64_t)r * nrhs, nrhs, Y, nrhs, T(-1), 1,
-                skip, sSkip, count, b.sF, sV, sV, st);
+                skip, sSkip, count, b.sF, sV, sV, st, false, b.subst_tile == 1 ? 1 : 0);
   }
   HPR_CHECK(hipGetLastError());
   return 0;

@@ -330,6 +330,8 @@ template <class T> struct BatchArgs {
   cx<T>* RY = nullptr;
   int64_t sV = 0;                                   // stride of the n x nrhs blocks
   bool g3m = false;                                 // Gauss 3-product trailing update
+  int trail_tile = 0;    // tiles of U12 and the trailing update (gemm_b_tiled: 0, 1, 2)
+  int subst_tile = 0;    // substitution update tiles: 0 by width, 1 32 x 32
 };
 constexpr int BATCH_NORM_BLOCKS = 64;
 

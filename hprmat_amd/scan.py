@@ -307,29 +307,42 @@ class BatchScan:
     ``mesh_operator`` the refinement's FP64 products are formed from the
     Lagrange-mesh parts of the matrices (hprmat_hip_batch_solve_mesh).
 
+    Plans: ``plans`` maps a size class (N, nch) to its factorization
+    arithmetic ('fp32' / 'fp32_3m'), tile plan ('trail', 'subst': the
+    HPRMAT_BATCH_TRAIL / _SUBST tiles) and launched sweep count
+    ('max_refine'), as ``autotune_batch`` measures them; classes without an
+    entry use ``method`` / ``max_refine`` and the default tiles.
+
     Waves are spread over ``lanes`` streams longest-processing-time first, so
     the launch-latency-bound chains of small waves overlap the MFMA-bound
     chains of large ones.  ``capture()`` records each lane's chunks into HIP
-    graphs of at most NODE_BUDGET kernels.  Factorization 'fp32' (FP64
-    refinement, at most ``max_refine`` sweeps; converged systems skip the
-    rest) or 'fp64'.  ``finish()`` synchronises and re-solves any system left
-    unconverged with the plain FP64 batch (the reference's fallback to its
-    FP64 path when the fast one fails, src/rmat_solvers.F90:576-588).
+    graphs of at most NODE_BUDGET kernels.  Refinement: FP64 classic sweeps
+    (converged systems skip the rest).  ``finish()`` synchronises and
+    re-solves any system left unconverged with the plain FP64 batch (the
+    reference's fallback to its FP64 path when the fast one fails,
+    src/rmat_solvers.F90:576-588).
     """
 
     def __init__(self, waves, method='fp32', lanes=2, mem_bytes=48 << 30, max_refine=20,
-                 tol=1e-12, log=None, mesh_operator=True):
+                 tol=1e-12, log=None, mesh_operator=True, plans=None):
         self.waves, self.method, self.max_refine, self.tol = waves, method, max_refine, tol
         self.mesh_operator = mesh_operator
-        self.code = solver.batch_code(method)
         self.log = log
         self.captured = False
         self.fallbacks = 0
+        self.plans = {}
+        for w in waves:
+            p = dict((plans or {}).get((w.n, w.nch), {}))
+            p.setdefault('method', method)
+            p.setdefault('trail', 0)
+            p.setdefault('subst', 0)
+            p.setdefault('max_refine', max_refine)
+            self.plans[id(w)] = p
         budget = mem_bytes // max(1, lanes)
         self.chunk = {}
         for w in waves:
             per = (16 * w.n * w.n + 16 * w.n * w.nch
-                   + solver.batch_work_bytes(method, w.n, w.nch, 1))
+                   + solver.batch_work_bytes(self.plans[id(w)]['method'], w.n, w.nch, 1))
             self.chunk[id(w)] = int(max(1, min(len(w.energies), budget // per)))
         items = [(self._cost(w), w) for w in waves if len(w.energies)]
         self.lanes = []
@@ -342,11 +355,16 @@ class BatchScan:
                           for w in lane.waves for k0 in range(0, len(w.energies), self.chunk[id(w)])]
             ma = max(c * w.n * w.n for w, _, c in lane.calls)
             mx = max(c * w.n * w.nch for w, _, c in lane.calls)
-            mw = max(solver.batch_work_bytes(method, w.n, w.nch, c) for w, _, c in lane.calls)
+            mw = max(solver.batch_work_bytes(self.plans[id(w)]['method'], w.n, w.nch, c)
+                     for w, _, c in lane.calls)
             lane.a = torch.empty(ma, dtype=torch.complex128, device='cuda')
             lane.x = torch.empty(mx, dtype=torch.complex128, device='cuda')
             lane.work = torch.empty(mw, dtype=torch.uint8, device='cuda')
             self.lanes.append(lane)
+
+    def plan_of(self, w):
+        """The plan the scan runs wave w with."""
+        return dict(self.plans[id(w)])
 
     def _cost(self, w):
         """Estimated seconds of a wave: MFMA work plus the launch chain."""
@@ -359,7 +377,7 @@ class BatchScan:
         nb = (w.n + 255) // 256
         g = (w.n + 255) // 256
         leaf = 8 * ((count * g + 479) // 480)
-        return 20 + nb * (45 + leaf) + (self.max_refine + 1) * (10 + 3 * nb)
+        return 20 + nb * (45 + leaf) + (self.plans[id(w)]['max_refine'] + 1) * (10 + 3 * nb)
 
     def _say(self, msg):
         if self.log:
@@ -374,7 +392,9 @@ class BatchScan:
         x.copy_(w.b.unsqueeze(0).expand(c, -1, -1))
         mesh = (solver.mesh_parts(w.dmesh, w.lv, w.qk[e], w.xc1[e], w.cpot)
                 if self.mesh_operator else None)
-        solver.batch_solve(self.code, n, nch, c, a.data_ptr(), x.data_ptr(), self.max_refine,
+        p = self.plans[id(w)]
+        code = solver.batch_code(p['method'], trail=p['trail'], subst=p['subst'])
+        solver.batch_solve(code, n, nch, c, a.data_ptr(), x.data_ptr(), p['max_refine'],
                            self.tol, 'rel_fro', w.res[k0].data_ptr(), w.status[k0].data_ptr(),
                            lane.work, mesh=mesh)
         rmatrix.r_to_s_batch(w.dmesh, x, w.qk[e], w.nopen_dev[e], w.open[e], w.co[e], w.cop[e],
@@ -563,6 +583,133 @@ def autotune(waves, sample=3, cache=None, lane_options=(2, 4, 8), log=None):
     return plans, best_l[1]
 
 
+BATCH_REFINE_LADDER = (3, 4, 6, 8, 10, 12, 15, 20)
+
+
+def _batch_trial(w, ks, plan, bufs, tol, mesh_operator, timed):
+    """Solve the energies ``ks`` of wave ``w`` as one uniform batch with
+    ``plan`` (eager, on the current stream).  Returns (every system
+    converged, milliseconds of a second identical run when ``timed``)."""
+    n, nch, c = w.n, w.nch, len(ks)
+    a = bufs['a'][:c * n * n].view(c, n, n)
+    x = bufs['x'][:c * nch * n].view(c, nch, n)
+    idx = torch.as_tensor(np.asarray(ks), device='cuda')
+    qk, xc1 = w.qk[idx].contiguous(), w.xc1[idx].contiguous()
+    res = torch.zeros(c, dtype=torch.float64, device='cuda')
+    status = torch.zeros(c, dtype=torch.int32, device='cuda')
+    code = solver.batch_code(plan['method'], trail=plan['trail'], subst=plan['subst'])
+
+    def once():
+        rmatrix.assemble_batch(w.dmesh, w.lv, qk, xc1, w.cpot, a)
+        x.copy_(w.b.unsqueeze(0).expand(c, -1, -1))
+        mesh = solver.mesh_parts(w.dmesh, w.lv, qk, xc1, w.cpot) if mesh_operator else None
+        solver.batch_solve(code, n, nch, c, a.data_ptr(), x.data_ptr(), plan['max_refine'], tol,
+                           'rel_fro', res.data_ptr(), status.data_ptr(), bufs['work'], mesh=mesh)
+
+    once()
+    torch.cuda.synchronize()
+    ok = bool((status == 1).all())
+    if not timed:
+        return ok, None
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    once()
+    e1.record()
+    torch.cuda.synchronize()
+    return ok, e0.elapsed_time(e1)
+
+
+def autotune_batch(waves, sample=16, mem_bytes=48 << 30, lane_options=(2, 3, 4), tol=1e-12,
+                   mesh_operator=True, margin=2, cache=None, log=None):
+    """Plans of a BatchScan, measured on this GPU.  Per size class (N, nch),
+    on ``sample`` energies spread over one wave of the class, solved as one
+    batch: for each FP32 arithmetic ('fp32', 'fp32_3m') the fewest launched
+    sweeps (BATCH_REFINE_LADDER) at which every sampled system meets ``tol``
+    (bisection), timed; the faster arithmetic is kept and every tile plan
+    (HPRMAT_BATCH_TRAIL x HPRMAT_BATCH_SUBST) is timed with it; the plan keeps
+    the fastest tiles and the sweep count plus ``margin`` (energies outside
+    the sample may need more; a system still unconverged falls back to FP64
+    in BatchScan.finish).  Then the number of lanes, by timing a scan of
+    every 16th energy of every wave with those plans.  Returns (plans,
+    lanes); cached in ``cache`` (JSON) when given."""
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            d = json.load(f)
+        plans = {tuple(map(int, key.split('x'))): v for key, v in d['plans'].items()}
+        return plans, int(d['lanes'])
+    classes = {}
+    for w in waves:
+        if len(w.energies):
+            classes.setdefault((w.n, w.nch), w)
+    plans = {}
+    for key, w in sorted(classes.items()):
+        ne = len(w.energies)
+        ks = sorted(set(np.linspace(0, ne - 1, min(sample, ne)).round().astype(int).tolist()))
+        c = len(ks)
+        bufs = {'a': torch.empty(c * w.n * w.n, dtype=torch.complex128, device='cuda'),
+                'x': torch.empty(c * w.n * w.nch, dtype=torch.complex128, device='cuda'),
+                'work': torch.empty(solver.batch_work_bytes('fp32', w.n, w.nch, c),
+                                    dtype=torch.uint8, device='cuda')}
+        base = {'trail': 0, 'subst': 0}
+        timed = []
+        for method in ('fp32_3m', 'fp32'):
+            p = dict(base, method=method, max_refine=BATCH_REFINE_LADDER[-1])
+            if not _batch_trial(w, ks, p, bufs, tol, mesh_operator, False)[0]:
+                continue
+            lo, hi = 0, len(BATCH_REFINE_LADDER) - 1     # hi converges
+            while lo < hi:
+                mid = (lo + hi) // 2
+                p['max_refine'] = BATCH_REFINE_LADDER[mid]
+                if _batch_trial(w, ks, p, bufs, tol, mesh_operator, False)[0]:
+                    hi = mid
+                else:
+                    lo = mid + 1
+            p['max_refine'] = BATCH_REFINE_LADDER[hi]
+            ok, ms = _batch_trial(w, ks, p, bufs, tol, mesh_operator, True)
+            if ok:
+                timed.append((ms, p))
+        if not timed:
+            plans[key] = dict(base, method='fp32', max_refine=BATCH_REFINE_LADDER[-1])
+            continue
+        ms_best, best = min(timed, key=lambda t: t[0])
+        for trail in solver.BATCH_TRAIL_TILES:
+            for subst in solver.BATCH_SUBST_TILES:
+                if (trail, subst) == (0, 0):
+                    continue
+                p = dict(best, trail=trail, subst=subst)
+                ok, ms = _batch_trial(w, ks, p, bufs, tol, mesh_operator, True)
+                if ok and ms < ms_best:
+                    ms_best, best = ms, p
+        best = dict(best, max_refine=min(BATCH_REFINE_LADDER[-1], best['max_refine'] + margin),
+                    ms_per_system=ms_best / c)
+        plans[key] = best
+        if log:
+            print('# batch autotune: N=%d nch=%d: %s' % (key[0], key[1], best), file=log,
+                  flush=True)
+        del bufs
+    best_l = None
+    for nl in lane_options:
+        sc = BatchScan(_slice_waves(waves, 16), lanes=nl, mem_bytes=mem_bytes, tol=tol,
+                       mesh_operator=mesh_operator, plans=plans)
+        sc.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sc.run()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if log:
+            print('# batch autotune: %d lanes: %.1f ms' % (nl, dt * 1e3), file=log, flush=True)
+        if best_l is None or dt < best_l[0]:
+            best_l = (dt, nl)
+        del sc
+    if cache:
+        os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+        with open(cache, 'w') as f:
+            json.dump({'plans': {'%dx%d' % k: v for k, v in plans.items()}, 'lanes': best_l[1]}, f)
+    return plans, best_l[1]
+
+
 class _WaveSlice:
     """View of a wave restricted to every step-th energy (shares device data)."""
 
@@ -577,8 +724,12 @@ class _WaveSlice:
         self.open = w.open[self._idx]
         self.co = w.co[self._idx]
         self.cop = w.cop[self._idx]
+        self.nopen_dev = w.nopen_dev[self._idx]
+        self.qk_host = w.qk_host[self._idx]
         self.R = torch.zeros((len(self._idx), w.nch, w.nch), dtype=torch.complex128, device='cuda')
         self.S = torch.zeros_like(self.R)
+        self.res = torch.zeros(len(self._idx), dtype=torch.float64, device='cuda')
+        self.status = torch.zeros(len(self._idx), dtype=torch.int32, device='cuda')
 
     cost = Wave.cost
     s_of = Wave.s_of

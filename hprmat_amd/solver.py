@@ -184,12 +184,21 @@ def solve(a, b, method='fp64', max_refine=0, tol=1e-12, criterion='rel_fro', fin
     return b_cm.transpose(0, 1), st
 
 
-def batch_code(method, equilibrate=True):
+BATCH_TRAIL_TILES = (0, 1, 2)   # 64 x 64, 32 x 32, 128 x 64 (HPRMAT_BATCH_TRAIL)
+BATCH_SUBST_TILES = (0, 1)      # by width, 32 x 32 (HPRMAT_BATCH_SUBST)
+
+
+def batch_code(method, equilibrate=True, trail=0, subst=0):
     """Method flags of hprmat_hip_batch_solve ('fp64', 'fp32' or 'fp32_3m': FP32
-    factors whose trailing updates use Gauss's three-product complex GEMM)."""
+    factors whose trailing updates use Gauss's three-product complex GEMM),
+    with the tile plan: ``trail`` the U12 / trailing-update tiles, ``subst``
+    the substitution-update tiles (HPRMAT_BATCH_TRAIL / HPRMAT_BATCH_SUBST)."""
     if method not in ('fp64', 'fp32', 'fp32_3m'):
         raise ValueError('batched method must be fp64, fp32 or fp32_3m')
-    return METHODS[method] | (_lib.EQUILIBRATE if equilibrate else 0)
+    if trail not in BATCH_TRAIL_TILES or subst not in BATCH_SUBST_TILES:
+        raise ValueError('unknown batched tile plan (%r, %r)' % (trail, subst))
+    return (METHODS[method] | (_lib.EQUILIBRATE if equilibrate else 0)
+            | (int(trail) << 16) | (int(subst) << 20))
 
 
 def batch_work_bytes(method, n, nrhs, count, equilibrate=True):

@@ -310,7 +310,8 @@ int hprmat_hip_zgemm(int precision, int M, int N, int K, const void *A, int64_t 
    elements apart); dB: count column-major n x nrhs right-hand sides (ldb,
    stride_b apart), overwritten with the solutions.  method_flags:
    HPRMAT_PLAIN_FP64, HPRMAT_MIXED_FP32 or HPRMAT_MIXED_FP32_3M (Gauss's
-   three-product trailing update), optionally | HPRMAT_EQUILIBRATE;
+   three-product trailing update), optionally | HPRMAT_EQUILIBRATE and a
+   tile plan | HPRMAT_BATCH_TRAIL(t) | HPRMAT_BATCH_SUBST(s) (see below);
    criterion: HPRMAT_CRIT_* (no FGMRES).  max_refine correction sweeps at
    most, each system stopping once it meets tol; res_rel[i] receives the
    final ||B - A X||_F / ||B||_F and status[i] 1 (converged), 0 (not
@@ -322,6 +323,14 @@ int hprmat_hip_zgemm(int precision, int M, int N, int K, const void *A, int64_t 
    until the call completes on `stream`.  Batches running concurrently on
    several streams need streams * ceil(n / 256) <= 512: the panel leaf of
    one system of each batch keeps ceil(n / 256) workgroups resident. */
+/* Tile plan of a batched solve (OR-ed into method_flags; the scan's
+   autotuner picks it per system size): HPRMAT_BATCH_TRAIL(t) sets the tiles
+   of U12 and of the trailing update (0: 64 x 64, 1: 32 x 32 -- four times
+   the workgroups, for small systems --, 2: 128 x 64), HPRMAT_BATCH_SUBST(s)
+   those of the substitutions' block updates (0: 64 x 64, or 64 x 32 for at
+   most 32 right-hand sides; 1: 32 x 32).  Any other bit is an error. */
+#define HPRMAT_BATCH_TRAIL(t) (((t) & 0xf) << 16)
+#define HPRMAT_BATCH_SUBST(s) (((s) & 0xf) << 20)
 size_t hprmat_hip_batch_work_bytes(int method, int n, int nrhs, int count);
 int hprmat_hip_batch_solve(int method_flags, int n, int nrhs, int count, const void *dA,
                            int64_t lda, int64_t stride_a, void *dB, int64_t ldb,

@@ -180,3 +180,49 @@ def test_batch_scan_fallback_resolves_unconverged():
             ref = _oracle_s(w, k)
             if ref.size:
                 assert np.abs(w.s_of(k).cpu().numpy() - ref).max() <= 1e-10
+
+
+@pytest.mark.parametrize('method', ['fp32', 'fp32_3m'])
+@pytest.mark.parametrize('trail,subst', [(0, 0), (1, 0), (2, 0), (0, 1), (1, 1), (2, 1)])
+def test_batch_tile_plans_match_oracle(method, trail, subst):
+    # every tile plan the batched autotuner chooses from gives the oracle's S
+    # on waves of one block (N = 80, 240), several blocks (N = 400, 960) and a
+    # ragged last block (N = 640)
+    sizes = [(2, 40), (4, 60), (4, 100), (16, 60), (8, 80)]
+    waves = [scan.Wave(i, nc, nr, np.asarray([0.9, 4.7, 8.8])) for i, (nc, nr) in
+             enumerate(sizes)]
+    plans = {(w.n, w.nch): {'method': method, 'trail': trail, 'subst': subst, 'max_refine': 20}
+             for w in waves}
+    sc = scan.BatchScan(waves, lanes=2, plans=plans)
+    assert all(sc.plan_of(w)['trail'] == trail for w in waves)
+    sc.run()
+    assert sc.finish() == 0
+    for w in waves:
+        assert (w.status.cpu().numpy() == 1).all()
+        for k in range(len(w.energies)):
+            ref = _oracle_s(w, k)
+            if ref.size:
+                assert np.abs(w.s_of(k).cpu().numpy() - ref).max() <= 1e-10, (w.n, k)
+
+
+def test_batch_autotune_plans_and_cache(tmp_path):
+    waves = [scan.Wave(i, nc, nr, np.linspace(0.5, 10, 6)) for i, (nc, nr) in
+             enumerate([(2, 40), (8, 80), (16, 40)])]
+    cache = str(tmp_path / 'batch_tune.json')
+    plans, lanes = scan.autotune_batch(waves, sample=3, cache=cache, lane_options=(1, 2))
+    assert set(plans) == {(80, 2), (640, 8), (640, 16)} and lanes in (1, 2)
+    for p in plans.values():
+        assert p['method'] in ('fp32', 'fp32_3m')
+        assert p['trail'] in (0, 1, 2) and p['subst'] in (0, 1)
+        assert p['max_refine'] in range(3, 21)
+    plans2, lanes2 = scan.autotune_batch(waves, sample=3, cache=cache)
+    assert plans2 == plans and lanes2 == lanes
+    sc = scan.BatchScan(waves, lanes=lanes, plans=plans)
+    sc.capture()
+    sc.run()
+    sc.finish()
+    for w in waves:
+        for k in range(len(w.energies)):
+            ref = _oracle_s(w, k)
+            if ref.size:
+                assert np.abs(w.s_of(k).cpu().numpy() - ref).max() <= 1e-10

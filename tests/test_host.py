@@ -73,6 +73,19 @@ assert lib.hprmat_hip_batch_solve(1, 0, 1, 5, nul, 1, 0, nul, 1, 0, 0, 0.0, 1, n
                                   0, nul) == 0             # empty systems: nothing to do
 assert lib.hprmat_hip_r_to_s_batch(65, 10, 1, nul, 650, 650 * 65, nul, 1.0, 1.0, nul, nul, nul,
                                    nul, nul, nul, nul, nul) == -1   # nch > 64
+for bad in (1 | (3 << 16), 1 | (2 << 20), 1 | (1 << 12), 1 | (1 << 24)):   # unknown plan bits
+    assert lib.hprmat_hip_batch_solve(bad, 10, 1, 1, nul, 10, 100, nul, 10, 10, 0, 0.0, 1, nul,
+                                      nul, nul, 0, nul) == -1, hex(bad)
+from hprmat_amd import solver
+assert solver.batch_code('fp32_3m', trail=2, subst=1) == 2 | 0x100 | (2 << 16) | (1 << 20)
+assert solver.batch_code('fp64', equilibrate=False) == 0
+for kw in ({'trail': 3}, {'subst': 2}):
+    try:
+        solver.batch_code('fp32', **kw)
+    except ValueError:
+        pass
+    else:
+        raise AssertionError(kw)
 print('ok')
 ''' % ROOT
     p = subprocess.run([sys.executable, '-c', code], capture_output=True, text=True, timeout=120)
