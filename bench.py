@@ -573,7 +573,7 @@ def gradient_block(torch):
     th = fs.theta0.cpu().numpy()
     with threadpool_limits(limits=host_threads()):
         t1 = time.perf_counter()
-        loss_ref, g_ref = oadj.loss_and_grad(prob, th, [x.numpy() for x in fs.targets])
+        loss_ref, g_ref = oadj.loss_and_grad(prob, th, [x.cpu().numpy() for x in fs.targets])
         t_cpu = time.perf_counter() - t1
     g_s = g_s.cpu().numpy()
     rel = float(np.abs(g_s - g_ref).max() / np.abs(g_ref).max())

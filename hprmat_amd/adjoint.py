@@ -10,7 +10,8 @@ the loss  L = sum_E sum_cc' |S_cc'(E) - S_target_cc'(E)|^2.
 Forward, per energy: the matrix C(E) = A0(E) + P(V) (P puts V_cc'(r_i) on
 the (c, i), (c', i) entries) is assembled on the device and X = C^-1 B is
 solved by libhprmat_hip (FP32 LU + FGMRES refinement to 1e-12 by default);
-R = q2^T X and S follow with differentiable torch operations in FP64.
+R = q2^T X and S follow with differentiable torch operations in FP64 on the
+device.
 
 Backward (``_Solve.backward``): with G the torch gradient of X (the
 conjugate-Wirtinger convention, G = 2 dL/dconj(X)), dL = Re <G, dX> and
@@ -50,14 +51,36 @@ def coupling_potential(theta, r, nch):
 
 @dataclass
 class SolveOptions:
-    """Forward/adjoint solve settings.  tol is ||B - C X||_F / ||B||_F; the
-    default 1e-11 sits above the FP64 attainable residual of the N = 6400
-    fit system at its lowest energies (about 2e-12) and keeps S, and so the
-    gradient, accurate far below the 1e-8 the checks ask for."""
+    """Forward/adjoint solve settings.  tol is the gate on ||B - C X||_F /
+    ||B||_F.  A solve that stops above it is still accepted when its residual
+    is within ``floor_factor`` of the FP64 floor u ||(|C| |X|)||_F / ||B||_F
+    (u = 2^-53): no FP64 X can do better (the N = 6400 fit system reaches
+    that floor, about 2e-12, at its lowest energies)."""
     method: str = 'fp32'          # factorization: fp64, fp32, fp32_3m, bf16x3
     refinement: str = 'fgmres'    # ir or fgmres
     max_steps: int = 24
-    tol: float = 1e-11
+    tol: float = 1e-12
+    floor_factor: float = 4.0
+
+
+def fp64_floor(a_cm, x_cm, b_cm):
+    """u ||(|C| |X|)||_F / ||B||_F for column-major storages a_cm (n x n) and
+    x_cm, b_cm (nrhs x n) on the device."""
+    ax = x_cm.abs() @ a_cm.abs()          # row k: (|C| |x_k|)^T, column-major storages
+    return float(2.0 ** -53 * torch.linalg.vector_norm(ax) / torch.linalg.vector_norm(b_cm))
+
+
+def _accept(st, a_cm, x_cm, b_cm, opts, what, energy):
+    """Stats of a converged solve, or of one that ended at the FP64 floor
+    (then st['at_floor'] is set); raises otherwise."""
+    if st['info'] == 0 and st['converged']:
+        return st
+    if st['info'] == 0 and not st['diverged']:
+        fl = fp64_floor(a_cm, x_cm, b_cm)
+        if st['res_rel'] <= opts.floor_factor * fl:
+            st['at_floor'] = fl
+            return st
+    raise RuntimeError('%s solve at E = %g did not converge: %r' % (what, energy, st))
 
 
 class EnergyPoint:
@@ -82,11 +105,11 @@ class _Solve(torch.autograd.Function):
     def forward(ctx, cpot, pt, opts):
         a_cm = rmatrix.assemble(pt.dmesh, pt.lval, pt.qk, pt.xc1, cpot.detach())
         x = rmatrix.rhs_colmajor(pt.dmesh, pt.nch)
+        b = x.clone()
         lu = solver.LU().factor(a_cm, opts.method, equilibrate=True)
         st = lu.solve_(x, opts.max_steps, opts.tol, 'rel_fro', True, refinement=opts.refinement)
-        if st['info'] != 0 or not st['converged']:
-            raise RuntimeError('forward solve at E = %g did not converge: %r' % (pt.energy, st))
-        ctx.lu, ctx.pt, ctx.opts = lu, pt, opts
+        _accept(st, a_cm, x, b, opts, 'forward', pt.energy)
+        ctx.lu, ctx.pt, ctx.opts, ctx.a_cm = lu, pt, opts, a_cm
         ctx.save_for_backward(x)
         return x
 
@@ -97,11 +120,11 @@ class _Solve(torch.autograd.Function):
         # conj_physical: a lazy conj() view would hand unconjugated memory to
         # the library through data_ptr()
         m = torch.conj_physical(gx).contiguous()
+        b = m.clone()
         st = lu.solve_(m, opts.max_steps, opts.tol, 'rel_fro', True, refinement=opts.refinement)
-        if st['info'] != 0 or not st['converged']:
-            raise RuntimeError('adjoint solve at E = %g did not converge: %r' % (pt.energy, st))
+        _accept(st, ctx.a_cm, m, b, opts, 'adjoint', pt.energy)
         k = adjoint_contract(m, x, pt.nch, pt.nr)
-        ctx.lu = None
+        ctx.lu = ctx.a_cm = None
         return -k.conj(), None, None
 
 
@@ -143,8 +166,9 @@ def s_matrix(rm, a, qk, open_idx, co, cop):
 
 
 def s_of_energy(cpot, pt, opts, q2):
+    """S(E) on the device (R and the nopen x nopen solve in FP64)."""
     x = _Solve.apply(cpot, pt, opts)
-    rm = r_matrix(x, q2, pt.nch, pt.nr).cpu()
+    rm = r_matrix(x, q2, pt.nch, pt.nr)
     return s_matrix(rm, pt.a, pt.qk, pt.open_idx, pt.co, pt.cop)
 
 
@@ -152,7 +176,7 @@ def scan_loss(theta, points, targets, r, q2, opts=None):
     """L(theta) = sum_E sum |S(E; theta) - S_target(E)|^2 (theta on the device)."""
     opts = opts or SolveOptions()
     cpot = coupling_potential(theta, r, points[0].nch)
-    loss = torch.zeros((), dtype=torch.float64)
+    loss = torch.zeros((), dtype=torch.float64, device=theta.device)
     for pt, tgt in zip(points, targets):
         s = s_of_energy(cpot, pt, opts, q2)
         loss = loss + (s - tgt).abs().pow(2).sum()
