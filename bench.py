@@ -369,6 +369,7 @@ def run(args, torch, lib, rmatrix, solver, synth):
                              'algorithmic flops of a complex multiply-add; executed rate = 0.75 x '
                              'achieved = %.1f TFLOP/s (%.2f of the FP32 matrix peak)' % (
                                  0.75 * g['tflops'], 0.75 * g['tflops'] / PEAK[head]))
+    from hprmat_amd import tracesum
     if tr is not None and not head.startswith('bf16') and 'launches' in tr:
         # mean over the trailing updates of one factorization, like 'achieved'
         roof['traffic'] = tr['hbm_bytes']
@@ -377,6 +378,7 @@ def run(args, torch, lib, rmatrix, solver, synth):
             'hbm_write_bytes': tr['hbm_write_bytes'], 'alg_bytes': tr['alg_bytes'],
             'read_ratio': tr['read_ratio'], 'write_ratio': tr['write_ratio'],
             'build': tr.get('build'),
+            'build_matches_this_source': tr.get('build') == tracesum.kernel_source_hash(),
             'source': 'profiles/trailing_gemm_counters_N25600.md (FETCH_SIZE, WRITE_SIZE of '
                       '%s over the %d trailing updates of one N = %d factorization, '
                       'copy-calibrated; mean per launch)' % (tr['kernel'], tr['launches'],

@@ -242,6 +242,17 @@ def print_traffic(rec):
              f['write_ratio']))
 
 
+def kernel_source_hash():
+    """First 16 hex digits of the SHA-256 of csrc/gemm.hpp, the source of the
+    trailing-update GEMM tile: stamped into traffic records so that a reader
+    can tell whether a record was measured on the kernel it is used for."""
+    import hashlib
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'csrc', 'gemm.hpp')
+    with open(path, 'rb') as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('trace', nargs='+')
@@ -258,6 +269,7 @@ def main(argv=None):
     if args.traffic:
         rec = factor_traffic(args.trace, args.copy_bytes, args.kernel, args.traffic, args.nb,
                              args.elem_bytes)
+        rec['build'] = kernel_source_hash()
         print_traffic(rec)
         if args.json:
             with open(args.json, 'w') as f:
