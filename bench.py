@@ -520,7 +520,9 @@ def memory_scale(torch, rmatrix, solver, synth):
             times.append(time.perf_counter() - t0)
         _, sm = rmatrix.r_to_s(dm, b, nch, s.qk, o, co, cop)
         sm = sm.cpu().numpy()
-        out[name] = {'s': times[-1], 'solves_per_s': 1.0 / times[-1],
+        from hprmat_amd import adjoint
+        floor = adjoint.fp64_floor(a_cm, b, b0)   # u ||(|C| |X|)||_F / ||B||_F
+        out[name] = {'s': times[-1], 'solves_per_s': 1.0 / times[-1], 'fp64_floor': floor,
                      'tflops': fl / times[-1] / 1e12, 'res_rel': st['res_rel'],
                      'products_or_sweeps': st['sweeps'], 'converged': st['converged'],
                      'warm': reps > 1, 'symmetry_defect_S': float(np.abs(sm - sm.T).max()),
@@ -634,7 +636,8 @@ def energy_scan(torch, args):
         tune = {'seconds': time.perf_counter() - t0, 'sample_energies': args.scan_tune_sample,
                 'lanes': lanes,
                 'plans': {'%dx%d' % k: {kk: v[kk] for kk in ('method', 'trail', 'subst',
-                                                              'max_refine')}
+                                                              'max_refine', 'sampled_refine')
+                                        if kk in v}
                           for k, v in sorted(plans.items())}}
     sc = scan.BatchScan(waves, method=args.scan_method, lanes=lanes,
                         max_refine=args.scan_refine, plans=plans,

@@ -621,18 +621,20 @@ def _batch_trial(w, ks, plan, bufs, tol, mesh_operator, timed):
 
 
 def autotune_batch(waves, sample=16, mem_bytes=48 << 30, lane_options=(2, 3, 4), tol=1e-12,
-                   mesh_operator=True, margin=2, cache=None, log=None):
+                   mesh_operator=True, margin=4, min_refine=15, cache=None, log=None):
     """Plans of a BatchScan, measured on this GPU.  Per size class (N, nch),
     on ``sample`` energies spread over one wave of the class, solved as one
     batch: for each FP32 arithmetic ('fp32', 'fp32_3m') the fewest launched
     sweeps (BATCH_REFINE_LADDER) at which every sampled system meets ``tol``
     (bisection), timed; the faster arithmetic is kept and every tile plan
     (HPRMAT_BATCH_TRAIL x HPRMAT_BATCH_SUBST) is timed with it; the plan keeps
-    the fastest tiles and the sweep count plus ``margin`` (energies outside
-    the sample may need more; a system still unconverged falls back to FP64
-    in BatchScan.finish).  Then the number of lanes, by timing a scan of
-    every 16th energy of every wave with those plans.  Returns (plans,
-    lanes); cached in ``cache`` (JSON) when given."""
+    the fastest tiles; its launched sweep count is the sampled one plus
+    ``margin``, and at least ``min_refine``: a converged system skips the
+    remaining sweeps (their kernels exit at once), while one that runs out of
+    sweeps is re-solved alone in FP64 by BatchScan.finish, which costs far
+    more than a few idle launches.  Then the number of lanes, by timing a
+    scan of every 16th energy of every wave with those plans.  Returns
+    (plans, lanes); cached in ``cache`` (JSON) when given."""
     if cache and os.path.exists(cache):
         with open(cache) as f:
             d = json.load(f)
@@ -681,7 +683,9 @@ def autotune_batch(waves, sample=16, mem_bytes=48 << 30, lane_options=(2, 3, 4),
                 ok, ms = _batch_trial(w, ks, p, bufs, tol, mesh_operator, True)
                 if ok and ms < ms_best:
                     ms_best, best = ms, p
-        best = dict(best, max_refine=min(BATCH_REFINE_LADDER[-1], best['max_refine'] + margin),
+        best = dict(best, sampled_refine=best['max_refine'],
+                    max_refine=min(BATCH_REFINE_LADDER[-1],
+                                   max(min_refine, best['max_refine'] + margin)),
                     ms_per_system=ms_best / c)
         plans[key] = best
         if log:

@@ -264,12 +264,28 @@ def test_memory_scale_mixed_solve():
     # refinement stalls there too), so the gate is 2e-12 (8 products with the
     # row-2-norm equilibration); S (256 x 256, FP64 on the device) is
     # symmetric and, with absorption, |S_ij| <= 1.
+    # The gate's floor is shown, not assumed: the plain FP64 LU with FP64
+    # refinement on the same matrix stops above 1e-12 as well, and the mixed
+    # solve lands within 20 % of it; u ||(|C| |X|)||_F / ||B||_F is printed.
+    from hprmat_amd import adjoint
     s, dm, o, co, cop, a_cm, b = _memory_system(False)
+    b0 = b.clone()
     lu = solver.LU().set_plan(gmres_cycle=24).factor(a_cm, 'fp32', True)
     st = lu.solve_(b, 40, 2e-12, 'rel_fro', True, refinement='fgmres')
-    print('memory-scale mixed: res_rel %.2e products %d hist %s'
-          % (st['res_rel'], st['sweeps'], ' '.join('%.1e' % h for h in st['hist'])))
+    floor = adjoint.fp64_floor(a_cm, b, b0)
+    print('memory-scale mixed: res_rel %.2e products %d hist %s; u|C||X| floor %.2e'
+          % (st['res_rel'], st['sweeps'], ' '.join('%.1e' % h for h in st['hist']), floor))
     assert st['converged'] and st['res_rel'] <= 2e-12, st
+    del lu
+    b64 = b0.clone()
+    lu64 = solver.LU().factor(a_cm, 'fp64', True)
+    st64 = lu64.solve_(b64, 4, 1e-13, 'rel_fro', True)
+    print('memory-scale FP64 LU + FP64 sweeps: res_rel %.2e after %d sweeps, hist %s'
+          % (st64['res_rel'], st64['sweeps'], ' '.join('%.1e' % h for h in st64['hist'])))
+    assert st64['res_rel'] > 1e-12              # FP64 cannot reach the nominal gate either
+    assert st['res_rel'] <= 1.2 * st64['res_rel']
+    assert st['res_rel'] <= 4.0 * floor
+    del lu64, b64
     _, sm = rmatrix.r_to_s(dm, b, s.nch, s.qk, o, co, cop)
     sm = sm.cpu().numpy()
     assert np.abs(sm - sm.T).max() <= 1e-10
