@@ -315,8 +315,8 @@ class BatchScan:
 
     Waves are spread over ``lanes`` streams longest-processing-time first, so
     the launch-latency-bound chains of small waves overlap the MFMA-bound
-    chains of large ones.  ``capture()`` records each lane's chunks into HIP
-    graphs of at most NODE_BUDGET kernels.  Refinement: FP64 classic sweeps
+    chains of large ones.  ``capture()`` records the whole scan as one HIP
+    graph with the lanes as parallel branches (one graph launch per scan).  Refinement: FP64 classic sweeps
     (converged systems skip the rest).  ``finish()`` synchronises and
     re-solves any system left unconverged with the plain FP64 batch (the
     reference's fallback to its FP64 path when the fast one fails,
@@ -329,6 +329,7 @@ class BatchScan:
         self.mesh_operator = mesh_operator
         self.log = log
         self.captured = False
+        self.graph = None          # the whole scan as one HIP graph, once captured
         self.fallbacks = 0
         self.plans = {}
         for w in waves:
@@ -405,12 +406,32 @@ class BatchScan:
             for w, k0, c in calls:
                 self._chunk_call(lane, w, k0, c)
 
-    def capture(self):
-        """Eager warm-up (first-use kernel attributes), then each lane's chunks
-        captured into chain graphs of at most NODE_BUDGET kernels."""
+    def capture(self, single=True):
+        """Eager warm-up (first-use kernel attributes), then the graphs.
+        ``single`` (and at most NODE_BUDGET kernels in the whole scan): the
+        scan as one HIP graph -- a capture stream forks to every lane's
+        stream, each lane records its chunks, and the lanes join back, so the
+        lanes become parallel branches of one graph and one graph launch
+        replays the scan.  Otherwise (or if that capture fails) each lane's
+        chunks go into chain graphs of at most NODE_BUDGET kernels replayed on
+        the lane's stream.  (A capture of the 64-wave x 256-energy scan as one graph,
+        about 2.4e5 kernels in three dependency chains of ~8e4, ended the
+        process with a segmentation fault on the host, while chains of up to
+        NODE_BUDGET kernels are captured and replayed routinely.)"""
         self._say('# batch scan: warm-up')
         self.run()
         torch.cuda.synchronize()
+        self.graph = None
+        nodes = sum(self._nodes(w, c) for lane in self.lanes for w, _, c in lane.calls)
+        if single and nodes <= NODE_BUDGET:
+            try:
+                self._capture_single()
+                self.captured = True
+                return
+            except RuntimeError as exc:
+                self._say('# batch scan: single-graph capture failed (%s); lane graphs' % exc)
+                self.graph = None
+                torch.cuda.synchronize()
         total = 0
         for li, lane in enumerate(self.lanes):
             groups, cur, nodes = [], [], 0
@@ -437,10 +458,31 @@ class BatchScan:
         self.captured = True
         self._say('# batch scan: %d graphs captured' % total)
 
+    def _capture_single(self):
+        root = torch.cuda.Stream()
+        g = torch.cuda.CUDAGraph()
+        t0 = time.perf_counter()
+        with torch.cuda.graph(g, stream=root, capture_error_mode='relaxed'):
+            for lane in self.lanes:
+                lane.stream.wait_stream(root)            # fork
+                self._lane_calls(lane, lane.calls)
+                root.wait_stream(lane.stream)            # join
+        torch.cuda.synchronize()
+        self.graph, self.graph_stream = g, root
+        self._say('# batch scan: one graph of %d lanes, %d chunks, captured in %.1f s'
+                  % (len(self.lanes), sum(len(l.calls) for l in self.lanes),
+                     time.perf_counter() - t0))
+
     def run(self):
-        """Enqueue one full scan on the lanes (graph replays once captured);
-        returns without synchronising."""
+        """Enqueue one full scan (one graph launch, or the lanes' graph replays
+        once captured; eager launches before); returns without synchronising."""
         root = torch.cuda.current_stream()
+        if self.captured and getattr(self, 'graph', None) is not None:
+            self.graph_stream.wait_stream(root)
+            with torch.cuda.stream(self.graph_stream):
+                self.graph.replay()
+            root.wait_stream(self.graph_stream)
+            return
         for lane in self.lanes:
             lane.stream.wait_stream(root)
             if self.captured:

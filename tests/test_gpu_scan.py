@@ -121,6 +121,7 @@ def test_batch_scan_matches_oracle_and_replays():
     eager = [w.S.clone() for w in waves]
     print('batch scan: capture', flush=True)
     sc.capture()
+    assert sc.graph is not None          # the whole scan is one graph
     for rep in range(3):
         for w in waves:
             w.S.zero_()
@@ -164,6 +165,26 @@ def test_batch_scan_bench_configuration_at_measured_sizes():
             assert got.shape == ref.shape
             assert ds <= 1e-10, (w.n, w.energies[k], ds)
             assert res[k] <= 3e-12
+
+
+def test_batch_scan_lane_graphs_match_single_graph():
+    # the per-lane chain graphs (capture(single=False)) give the single
+    # graph's S bit for bit
+    waves = _waves((1.1, 5.3, 9.1))
+    sc1 = scan.BatchScan(waves, method='fp32_3m', lanes=3)
+    sc1.capture()
+    sc1.run()
+    assert sc1.finish() == 0
+    one = [w.S.clone() for w in waves]
+    sc2 = scan.BatchScan(waves, method='fp32_3m', lanes=3)
+    sc2.capture(single=False)
+    assert sc2.graph is None and any(l.graphs for l in sc2.lanes)
+    for w in waves:
+        w.S.zero_()
+    sc2.run()
+    assert sc2.finish() == 0
+    for w, e in zip(waves, one):
+        assert torch.equal(w.S, e)
 
 
 def test_batch_scan_fallback_resolves_unconverged():
