@@ -24,6 +24,10 @@ Methods timed back to back (name: factorization, refinement):
   bf16x6    FP32 LU whose trailing update uses three-piece split-BF16 (six
             products per real product, FP32 accuracy), classic FP64 IR;
   bf16x6_gmres  the bf16x6 factorization with flexible-GMRES refinement;
+  f16x3     FP32 LU whose trailing update uses two FP16 pieces per FP32
+            operand with power-of-two row / column scales (three FP16
+            products per real product, about 2^-22 relative), classic IR;
+  f16x3_gmres  the f16x3 factorization with flexible-GMRES refinement;
   fp32_3m_gmres_op, fp64_op  fp32_3m_gmres and fp64 with the refinement's
             FP64 products formed from the Lagrange-mesh parts of the matrix
             (channel blocks and couplings, hprmat_hip_lu_set_operator)
@@ -66,13 +70,15 @@ import numpy as np
 METRIC = "solves/s & TFLOP/s, N=25600 complex LU+solve: mixed vs FP64 at resid≤1e-12"
 # Ceiling of the trailing update in algorithmic complex TFLOP/s (8 flops per
 # complex multiply-add) on MI355X's dense matrix peaks: FP64 78.6, FP32 157.3;
-# the split-BF16 kernels execute 3 (bf16x3) or 6 (bf16x6) BF16 products per
-# real product, so their ceiling is the BF16 peak 2516.6 divided by that.
+# the split kernels execute 3 (bf16x3, f16x3) or 6 (bf16x6) 16-bit products
+# per real product, so their ceiling is the BF16 / FP16 peak 2516.6 divided
+# by that.
 BF16_PEAK = 2516.6
 PEAK = {'fp64': 78.6, 'fp32': 157.3, 'fp32_ir5': 157.3, 'fp32_gmres': 157.3, 'fp32_3m': 157.3,
         'fp32_3m_gmres': 157.3,
         'bf16x3': BF16_PEAK / 3, 'bf16x3_gmres': BF16_PEAK / 3,
         'bf16x6': BF16_PEAK / 6, 'bf16x6_gmres': BF16_PEAK / 6,
+        'f16x3': BF16_PEAK / 3, 'f16x3_gmres': BF16_PEAK / 3,
         'fp32_3m_gmres_op': 157.3, 'fp64_op': 78.6}
 # name -> (factorization method, max refinement sweeps)
 SPECS = {'fp64': ('fp64', 10, 'ir'), 'fp32': ('fp32', 30, 'ir'), 'fp32_ir5': ('fp32', 5, 'ir'),
@@ -80,6 +86,7 @@ SPECS = {'fp64': ('fp64', 10, 'ir'), 'fp32': ('fp32', 30, 'ir'), 'fp32_ir5': ('f
          'bf16x3_gmres': ('bf16x3', 32, 'fgmres'), 'fp32_3m': ('fp32_3m', 30, 'ir'),
          'fp32_3m_gmres': ('fp32_3m', 24, 'fgmres'),
          'bf16x6': ('bf16x6', 30, 'ir'), 'bf16x6_gmres': ('bf16x6', 24, 'fgmres'),
+         'f16x3': ('f16x3', 30, 'ir'), 'f16x3_gmres': ('f16x3', 24, 'fgmres'),
          'fp32_3m_gmres_op': ('fp32_3m', 24, 'fgmres'), 'fp64_op': ('fp64', 10, 'ir')}
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
@@ -353,24 +360,30 @@ def run(args, torch, lib, rmatrix, solver, synth):
     roof = {'bound': 'mfma', 'achieved': g['tflops'], 'peak': PEAK[head],
             'unit': 'TFLOP/s', 'frac': g['tflops'] / PEAK[head], 'traffic': None,
             'kernel': '%s (trailing update A22 -= L21 U12, %s)' % (
-                'zgemm_bf16x3_kernel<64, 64, 32, 32>' if head.startswith('bf16x3') else
-                'zgemm_bf16x6_kernel<128, 64, 64, 32>' if head.startswith('bf16x6') else
+                'zgemm_split_kernel<SPLIT_BF16X2, 128, 128, 64, 64, 32, 2, 1>'
+                if head.startswith('bf16x3') else
+                'zgemm_split_kernel<SPLIT_BF16X3, 128, 64, 64, 32, 16, 1, 2>'
+                if head.startswith('bf16x6') else
+                'zgemm_split_kernel<SPLIT_F16X2, 128, 128, 64, 64, 32, 2, 1>'
+                if head.startswith('f16x3') else
                 'trailing_update_kernel<%s, 64, 64, 16, 32, 32, %s>' % (
                     'double' if head == 'fp64' else 'float',
                     'true' if head.startswith('fp32_3m') else 'false'), head),
             'flops_per_launch': g['flops_per_launch'], 'ms_per_launch': g['ms_per_launch']}
     tr = traffic_record('fp64_factor' if head == 'fp64' else
                         'fp32_3m_factor' if head.startswith('fp32_3m') else 'fp32')
-    if head.startswith('bf16'):
-        roof['peak_note'] = 'BF16 dense matrix peak %.1f TFLOP/s / %d BF16 products per real product' % (
-            BF16_PEAK, 3 if head.startswith('bf16x3') else 6)
+    split = head.startswith('bf16') or head.startswith('f16')
+    if split:
+        roof['peak_note'] = ('%s dense matrix peak %.1f TFLOP/s / %d 16-bit products per real '
+                             'product' % ('FP16' if head.startswith('f16') else 'BF16', BF16_PEAK,
+                                          6 if head.startswith('bf16x6') else 3))
     if head.startswith('fp32_3m'):
         roof['peak_note'] = ('Gauss three-product complex GEMM: the matrix cores execute 6 of the 8 '
                              'algorithmic flops of a complex multiply-add; executed rate = 0.75 x '
                              'achieved = %.1f TFLOP/s (%.2f of the FP32 matrix peak)' % (
                                  0.75 * g['tflops'], 0.75 * g['tflops'] / PEAK[head]))
     from hprmat_amd import tracesum
-    if tr is not None and not head.startswith('bf16') and 'launches' in tr:
+    if tr is not None and not split and 'launches' in tr:
         # mean over the trailing updates of one factorization, like 'achieved'
         roof['traffic'] = tr['hbm_bytes']
         roof['traffic_launch'] = {
@@ -389,6 +402,8 @@ def run(args, torch, lib, rmatrix, solver, synth):
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': ('c128' if head == 'fp64' else
                   'c64 LU (split-bf16 update) + c128 refinement' if head.startswith('bf16')
+                  else 'c64 LU (scaled split-fp16 update) + c128 refinement'
+                  if head.startswith('f16')
                   else 'c64 LU + c128 refinement'),
         'data': 'synthetic',
         'config': {'workload': 'single Lagrange-mesh solve N=%d (nch=%d x nr=%d), nrhs=%d, '

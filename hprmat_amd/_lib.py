@@ -16,6 +16,7 @@ MIXED_FP32 = 1
 MIXED_FP32_3M = 2
 SPLIT_BF16 = 3
 SPLIT_BF16X6 = 4
+SPLIT_F16 = 5
 EQUILIBRATE = 0x100
 CRIT_ABS_MAX = 0
 CRIT_REL_FRO = 1

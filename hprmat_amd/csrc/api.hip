@@ -29,7 +29,7 @@ struct hprmat_hip_lu {
   size_t ustage_cap = 0;
   void* ftmp = nullptr;       // staging of the look-ahead stream's folds (256 x n)
   size_t ftmp_cap = 0;
-  void* split = nullptr;      // split-BF16 operand planes (HPRMAT_SPLIT_BF16, _BF16X6)
+  void* split = nullptr;      // split operands (HPRMAT_SPLIT_BF16, _BF16X6, _F16)
   size_t split_cap = 0;
   void* gm = nullptr;         // FGMRES workspace (HPRMAT_FGMRES solves)
   size_t gm_cap = 0;
@@ -120,7 +120,7 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
   if (n < 0 || lda < n) return -1;
   const int method = method_flags & 0xff;
   const int equil = (method_flags & HPRMAT_EQUILIBRATE) ? 1 : 0;
-  if (method < HPRMAT_PLAIN_FP64 || method > HPRMAT_SPLIT_BF16X6) {
+  if (method < HPRMAT_PLAIN_FP64 || method > HPRMAT_SPLIT_F16) {
     fprintf(stderr, "hprmat_hip: factorization method %d is not available\n", method);
     return -1;
   }
@@ -148,7 +148,8 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
   if (h->lookahead && grow(&h->ftmp, &h->ftmp_cap, hpr::ustage_elems(n) * eb)) return -2;
   if (grow((void**)&h->perm, &h->perm_cap, (size_t)(n + 1) * sizeof(int))) return -2;
   if (grow((void**)&h->scale, &h->scale_cap, (size_t)(n + 1) * sizeof(double))) return -2;
-  if ((method == HPRMAT_SPLIT_BF16 || method == HPRMAT_SPLIT_BF16X6) &&
+  if ((method == HPRMAT_SPLIT_BF16 || method == HPRMAT_SPLIT_BF16X6 ||
+       method == HPRMAT_SPLIT_F16) &&
       grow(&h->split, &h->split_cap, hpr::split_elems(n) * sizeof(unsigned short)))
     return -2;
   h->equil = equil;
@@ -194,6 +195,7 @@ int lu_factor(hprmat_hip_lu* h, int method_flags, const void* dA, int64_t lda, i
     const int gmode = method == HPRMAT_MIXED_FP32_3M ? hpr::GEMM_3M
                       : method == HPRMAT_SPLIT_BF16   ? hpr::GEMM_BF16X3
                       : method == HPRMAT_SPLIT_BF16X6 ? hpr::GEMM_BF16X6
+                      : method == HPRMAT_SPLIT_F16    ? hpr::GEMM_F16X3
                                                       : hpr::GEMM_4M;
     if (!rc)
       rc = hpr::getrf<float>(F, ld, n, h->ipiv, h->dinfo, w, h->sc, h->plan, gmode, st, side);

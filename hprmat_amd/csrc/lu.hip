@@ -25,7 +25,7 @@
 #include <type_traits>
 
 #include "lu.hpp"
-#include "bf16x3.hpp"
+#include "split_gemm.hpp"
 #include "gemm.hpp"
 #include "lu_kernels.hpp"
 
@@ -534,33 +534,84 @@ int resid_gemm(int n, int nrhs, const double2* A64, int64_t lda, const double2* 
   return 0;
 }
 
-// Split-BF16 planes of L21 (nc x kb at L, row stride ld) and of U12^T (U12:
-// kb x nc at U, row stride ldu) into `split` (see bf16x3.hpp).
-// With split_l false only U12's columns [c0, c0 + cols) are split (the
-// planes keep the layout of all nc columns).
+// Split operands of one trailing update in `split` (split_gemm.hpp): L21
+// (nc x kb at L, row stride ld) as packed rows, U12^T (U12: kb x nc at U, row
+// stride ldu) as packed rows after them, then the scale exponents of both
+// (FP16 pieces).
+struct SplitView {
+  half_bits* a;
+  half_bits* b;
+  int* ka;            // scale exponents of the rows of a / b (FP16 pieces)
+  int* kb;
+  unsigned* colmax;   // K scales: column maxima of L21 (256, FP16 pieces)
+};
+inline int split_row_halves_of(int kind) {
+  return kind == SPLIT_BF16X3 ? split_row_halves<SPLIT_BF16X3>() : split_row_halves<SPLIT_F16X2>();
+}
+inline SplitView split_view(unsigned short* split, int nc, int kind) {
+  const size_t rh = split_row_halves_of(kind);
+  SplitView v;
+  v.a = split;
+  v.b = split + (size_t)nc * rh;
+  v.ka = reinterpret_cast<int*>(split + 2 * (size_t)nc * rh);   // 4 KB aligned
+  v.kb = v.ka + nc;
+  v.colmax = reinterpret_cast<unsigned*>(v.kb + nc);
+  return v;
+}
+
+template <int KIND>
+void launch_split(int nc, int kb, const cx<float>* L, int64_t ld, const cx<float>* U, int64_t ldu,
+                  const SplitView& v, hipStream_t st, bool split_l, int c0, int cols) {
+  if (split_l) {
+    if constexpr (SplitTraits<KIND>::SCALED) {
+      // the K scales: column maxima of L21 (read by every split of this step)
+      fill_u32_kernel<<<1, 256, 0, st>>>(v.colmax, 256, 0u);
+      split_colmax_kernel<<<(nc + 255) / 256, 256, 0, st>>>(nc, kb, L, ld, v.colmax);
+    }
+    split_rows_kernel<KIND><<<(nc + 3) / 4, 256, 0, st>>>(nc, kb, L, ld, v.a, v.ka, v.colmax);
+  }
+  if (cols > 0)
+    split_cols_kernel<KIND><<<(cols + 31) / 32, 256, 0, st>>>(
+        cols, kb, U + c0, ldu, v.b + (size_t)c0 * split_row_halves<KIND>(), v.kb + c0, v.colmax);
+}
+
+// With split_l false only U12's columns [c0, c0 + cols) are split (the packed
+// rows keep the positions of all nc columns).
 template <class T>
 void split_operands(int nc, int kb, const cx<T>* L, int64_t ld, const cx<T>* U, int64_t ldu,
-                    unsigned short* split, hipStream_t st, bool split_l = true, int c0 = 0,
-                    int cols = -1, int pieces = 2) {
+                    unsigned short* split, hipStream_t st, bool split_l, int c0, int cols,
+                    int kind) {
   if constexpr (std::is_same<T, float>::value) {
-    const size_t plane = (size_t)nc * 256;
-    const size_t uoff = (size_t)2 * pieces * plane;   // U12^T planes follow L21's
     if (cols < 0) cols = nc - c0;
-    const dim3 gl(256 / 32, (nc + 31) / 32), gu(256 / 32, (cols + 31) / 32);
-    if (pieces == 3) {
-      if (split_l)
-        split_bf16_kernel<false, 3><<<gl, 256, 0, st>>>(nc, kb, L, ld, split, plane, 256);
-      if (cols > 0)
-        split_bf16_kernel<true, 3><<<gu, 256, 0, st>>>(cols, kb, U + c0, ldu,
-                                                       split + uoff + (size_t)c0 * 256, plane, 256);
-    } else {
-      if (split_l)
-        split_bf16_kernel<false, 2><<<gl, 256, 0, st>>>(nc, kb, L, ld, split, plane, 256);
-      if (cols > 0)
-        split_bf16_kernel<true, 2><<<gu, 256, 0, st>>>(cols, kb, U + c0, ldu,
-                                                       split + uoff + (size_t)c0 * 256, plane, 256);
-    }
+    const SplitView v = split_view(split, nc, kind);
+    if (kind == SPLIT_F16X2)
+      launch_split<SPLIT_F16X2>(nc, kb, L, ld, U, ldu, v, st, split_l, c0, cols);
+    else if (kind == SPLIT_BF16X3)
+      launch_split<SPLIT_BF16X3>(nc, kb, L, ld, U, ldu, v, st, split_l, c0, cols);
+    else
+      launch_split<SPLIT_BF16X2>(nc, kb, L, ld, U, ldu, v, st, split_l, c0, cols);
   }
+}
+
+// C[:, 0:N] -= L21 U12[:, c0:c0+N] over the split operands (M = nc rows).
+// Tiles (measured at M = N = 25344): 128 x 128 with 32-K slices and two
+// register sets for the two-piece splits; 128 x 64 with 16-K slices and two
+// workgroups per CU for the three-piece split (its six planes).
+template <int KIND>
+void launch_split_gemm(int M, int N, const SplitView& v, int c0, cx<float>* C, int64_t ldc,
+                       hipStream_t st) {
+  constexpr bool three = KIND == SPLIT_BF16X3;
+  constexpr int BM = 128, BN = three ? 64 : 128, WM = 64, WN = three ? 32 : 64;
+  constexpr int BK = three ? 16 : 32, NSET = three ? 1 : 2, MINB = three ? 2 : 1;
+  static bool attr = false;
+  if (!attr) {
+    allow_big_lds(zgemm_split_kernel<KIND, BM, BN, WM, WN, BK, NSET, MINB>);
+    attr = true;
+  }
+  const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+  zgemm_split_kernel<KIND, BM, BN, WM, WN, BK, NSET, MINB>
+      <<<tiles, 256, zgemm_split_lds_bytes<KIND, BM, BN, BK>(), st>>>(
+          M, N, v.a, v.ka, v.b + (size_t)c0 * split_row_halves<KIND>(), v.kb + c0, C, ldc, -1.f);
 }
 
 // Trailing update C[:, 0:ncols] -= L21 U12[:, c0:c0+ncols] (C is the block
@@ -594,7 +645,7 @@ template <class T>
 void trailing(bool bfx, int nc, int ncols, int c0, int kb, const cx<T>* L, int64_t ld,
               const cx<T>* U, int64_t ldu, cx<T>* C, int64_t ldc, bool g3m,
               const unsigned short* split, hipStream_t st, size_t lds_extra, int variant,
-              int prio = 0, bool timed = false, int pieces = 2) {
+              int prio = 0, bool timed = false, int skind = SPLIT_BF16X2) {
   if (nc <= 0 || ncols <= 0) return;
   if constexpr (std::is_same<T, float>::value) {
     if (!g3m && !bfx && variant >= 3) {
@@ -617,26 +668,14 @@ void trailing(bool bfx, int nc, int ncols, int c0, int kb, const cx<T>* L, int64
     return;
   }
   if constexpr (std::is_same<T, float>::value) {
-    if (bfx && pieces == 3) {
-      constexpr int BM = 128, BN = 64;
-      const size_t plane = (size_t)nc * 256, lds = zgemm_bf16x6_lds_bytes<BM, BN>();
-      static bool attr = false;
-      if (!attr) { allow_big_lds(zgemm_bf16x6_kernel<BM, BN, 64, 32>); attr = true; }
-      const int tiles = ((nc + BM - 1) / BM) * ((ncols + BN - 1) / BN);
-      zgemm_bf16x6_kernel<BM, BN, 64, 32><<<tiles, 256, lds, st>>>(
-          nc, ncols, 256, split, plane, split + 6 * plane + (size_t)c0 * 256, plane, C, ldc,
-          -1.f);
-      return;
-    }
     if (bfx) {
-      constexpr int BM = 64, BN = 64;
-      const size_t plane = (size_t)nc * 256, lds = zgemm_bf16x3_lds_bytes<BM, BN>();
-      static bool attr = false;
-      if (!attr) { allow_big_lds(zgemm_bf16x3_kernel<BM, BN, 32, 32>); attr = true; }
-      const int tiles = ((nc + BM - 1) / BM) * ((ncols + BN - 1) / BN);
-      zgemm_bf16x3_kernel<BM, BN, 32, 32><<<tiles, 256, lds, st>>>(
-          nc, ncols, 256, split, plane, split + 4 * plane + (size_t)c0 * 256, plane, C, ldc,
-          -1.f, 1);
+      const SplitView v = split_view(const_cast<unsigned short*>(split), nc, skind);
+      if (skind == SPLIT_F16X2)
+        launch_split_gemm<SPLIT_F16X2>(nc, ncols, v, c0, C, ldc, st);
+      else if (skind == SPLIT_BF16X3)
+        launch_split_gemm<SPLIT_BF16X3>(nc, ncols, v, c0, C, ldc, st);
+      else
+        launch_split_gemm<SPLIT_BF16X2>(nc, ncols, v, c0, C, ldc, st);
       return;
     }
   }
@@ -766,8 +805,11 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
   if (n <= 0) return 0;
   const int nb = plan.nb;
   const bool g3m = gmode == GEMM_3M;
-  const bool bfx = (gmode == GEMM_BF16X3 || gmode == GEMM_BF16X6) && std::is_same<T, float>::value;
-  const int pieces = gmode == GEMM_BF16X6 ? 3 : 2;   // BF16 pieces per FP32 operand
+  const bool bfx = (gmode == GEMM_BF16X3 || gmode == GEMM_BF16X6 || gmode == GEMM_F16X3) &&
+                   std::is_same<T, float>::value;
+  const int skind = gmode == GEMM_BF16X6  ? SPLIT_BF16X3
+                    : gmode == GEMM_F16X3 ? SPLIT_F16X2
+                                          : SPLIT_BF16X2;   // split_gemm.hpp kind
   if (bfx && !w.split) return -1;
   if (nb != 256) return -1;
   const bool la = side && side->stream && n > 2 * nb;  // look-ahead
@@ -841,12 +883,12 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       u12(0, nb, st, pg);
       if (bfx) {
         split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, st, true, 0,
-                          nb, pieces);
+                          nb, skind);
         HPR_CHECK(hipEventRecord(side->ready, st));
       }
       trailing<T>(bfx, nc, nb, 0, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
                   Akk + (int64_t)kb * ld + kb, ld, g3m, w.split, st, 0, plan.trail, pg, false,
-                  pieces);
+                  skind);
       copy_back(0, nb, st);
       hipStream_t cs = side->stream;
       HPR_CHECK(hipStreamWaitEvent(cs, side->ready, 0));
@@ -854,12 +896,12 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       u12(nb, nc - nb, cs, 0);
       if (bfx)
         split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, cs, false,
-                          nb, nc - nb, pieces);
+                          nb, nc - nb, skind);
       ProfRec* pr2 = g_prof_on ? prof_slot() : nullptr;
       if (pr2) (void)hipEventRecord(pr2->a, cs);
       trailing<T>(bfx, nc, nc - nb, nb, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
                   Akk + (int64_t)kb * ld + kb + nb, ld, g3m, w.split, cs,
-                  trail_lds_extra(sizeof(T) == 4), plan.trail, 0, true, pieces);
+                  trail_lds_extra(sizeof(T) == 4), plan.trail, 0, true, skind);
       if (pr2) {
         (void)hipEventRecord(pr2->b, cs);
         pr2->flops = 8.0 * (double)nc * (double)(nc - nb) * (double)kb;
@@ -894,11 +936,11 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
         ProfRec* pr = g_prof_on ? prof_slot() : nullptr;
         if (bfx)
           split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, st, true,
-                            0, -1, pieces);
+                            0, -1, skind);
         if (pr) (void)hipEventRecord(pr->a, st);
         trailing<T>(bfx, nc, nc, 0, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc,
                     Akk + (int64_t)kb * ld + kb, ld, g3m, w.split, st, 0, plan.trail, 0, true,
-                    pieces);
+                    skind);
         if (pr) {
           (void)hipEventRecord(pr->b, st);
           pr->flops = 8.0 * (double)nc * (double)nc * (double)kb;

@@ -72,13 +72,20 @@ inline size_t dinv64_elems(int n) { return (size_t)((n + 63) / 64) * 2 * 4096; }
 inline size_t dinv256_elems(int n) { return (size_t)((n + 255) / 256) * 2 * 65536; }
 inline size_t ustage_elems(int n) { return (size_t)256 * (n > 0 ? n : 1); }
 
-// Split-BF16 operand planes of one trailing update: L21 and U12^T, four
-// BF16 planes each (n rows of 256 elements).
-// (sized for the three-piece split: six planes per operand)
-inline size_t split_elems(int n) { return (size_t)2 * 6 * 256 * (n > 0 ? n : 1); }
+// Split operands of one trailing update (16-bit elements): packed L21 and
+// U12^T (n rows of 256 K x up to six planes each, split_gemm.hpp) followed
+// by their int32 scale exponents (n each) and the 256 column maxima of L21
+// (FP16 pieces only).
+inline size_t split_elems(int n) {
+  const size_t r = n > 0 ? n : 1;
+  return (size_t)2 * 6 * 256 * r + 4 * r + 1024;
+}
 
-// Arithmetic of the FP32 trailing update A22 -= L21 U12.
-enum GemmMode { GEMM_4M = 0, GEMM_3M = 1, GEMM_BF16X3 = 2, GEMM_BF16X6 = 3 };
+// Arithmetic of the FP32 trailing update A22 -= L21 U12: FP32 products
+// (four or Gauss's three per complex product) or split 16-bit pieces on
+// the 32x32x16 matrix cores (split_gemm.hpp): two BF16 pieces (3 products),
+// three BF16 pieces (6 products), two scaled FP16 pieces (3 products).
+enum GemmMode { GEMM_4M = 0, GEMM_3M = 1, GEMM_BF16X3 = 2, GEMM_BF16X6 = 3, GEMM_F16X3 = 4 };
 
 template <class T> struct FactorWork {
   cx<T>* dinv64 = nullptr;

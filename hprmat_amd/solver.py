@@ -16,7 +16,7 @@ from . import _lib
 
 METHODS = {'fp64': _lib.PLAIN_FP64, 'fp32': _lib.MIXED_FP32,
            'fp32_3m': _lib.MIXED_FP32_3M, 'bf16x3': _lib.SPLIT_BF16,
-           'bf16x6': _lib.SPLIT_BF16X6}
+           'bf16x6': _lib.SPLIT_BF16X6, 'f16x3': _lib.SPLIT_F16}
 CRITERIA = {'abs_max': _lib.CRIT_ABS_MAX, 'rel_fro': _lib.CRIT_REL_FRO}
 
 

@@ -97,6 +97,9 @@ enum {
   HPRMAT_SPLIT_BF16 = 3,     /* FP32-stored LU, split-BF16 3-term products  */
   HPRMAT_SPLIT_BF16X6 = 4,   /* FP32-stored LU, three-piece split-BF16 with
                                 6 products per real product: FP32 accuracy  */
+  HPRMAT_SPLIT_F16 = 5,      /* FP32-stored LU, two-piece FP16 split with
+                                power-of-two row / column scales, 3 products
+                                per real product (about 2^-22 relative)     */
   /* flag OR-ed into the method: factor diag(s) A diag(s), s_i = ||row i of
      A||_2^-1/2 (symmetric equilibration; keeps A^T = A), refinement
      residuals still on A */

@@ -25,7 +25,7 @@ def gpu_solve(a, b, method, max_refine=0, tol=1e-13, equilibrate=False):
 
 
 @pytest.mark.parametrize('n', [1, 7, 32, 33, 64, 255, 256, 257, 500, 1100])
-@pytest.mark.parametrize('method', ['fp64', 'fp32', 'fp32_3m', 'bf16x3', 'bf16x6'])
+@pytest.mark.parametrize('method', ['fp64', 'fp32', 'fp32_3m', 'bf16x3', 'bf16x6', 'f16x3'])
 def test_solve_matches_lapack(n, method):
     a, b = rand_system(n, 5, n)
     x_ref, info = osolve.zgesv(a, b)
@@ -84,6 +84,64 @@ def test_split_bf16x6_trailing_update_precision(n):
     assert errs['bf16x6'] < 4 * errs['fp32'], errs
     assert errs['bf16x6'] * 3 < errs['bf16x3'], errs
     assert not np.array_equal(xs['bf16x6'], xs['fp32'])
+
+
+@pytest.mark.parametrize('n', [700, 1500, 3000])
+def test_split_f16_trailing_update_precision(n):
+    # Two FP16 pieces carry 22 significant bits; with every row of L21 and
+    # every column of U12 scaled by a power of two into FP16's range, the
+    # three kept products (hh, hl, lh) drop only terms below about 2^-22
+    # relative.  The unrefined f16x3 factorization is within a small factor
+    # of the FP32 one and far more accurate than bf16x3 (about 2^-16); its
+    # result differs from FP32's, which shows its kernel ran.
+    a, b = rand_system(n, 4, 83 + n, dd=0.0)
+    x_ref, _ = osolve.zgesv(a, b)
+    errs, xs = {}, {}
+    for method in ('fp32', 'bf16x3', 'f16x3'):
+        x, st = gpu_solve(a, b, method, max_refine=0)
+        assert st['info'] == 0
+        xs[method] = x
+        errs[method] = np.abs(x - x_ref).max() / np.abs(x_ref).max()
+    print('n %d errors %s' % (n, errs), flush=True)
+    assert errs['f16x3'] < 8 * errs['fp32'], errs
+    assert errs['f16x3'] * 4 < errs['bf16x3'], errs
+    assert not np.array_equal(xs['f16x3'], xs['fp32'])
+
+
+@pytest.mark.parametrize('n', [600, 1300])
+def test_split_f16_scaled_rows_and_columns(n):
+    # Rows and columns of A scaled by powers of two spanning 2^-60 .. 2^60:
+    # far outside FP16's range (6e-5 .. 65504), so the factorization only
+    # works through the power-of-two scales of the split.  The row scaling
+    # D A puts D_k into column k of L (inversely) and row k of U, so without
+    # the K-index scales of L21's columns / U12's rows this fails (errors of
+    # order 1e2); with them the unrefined error stays within a small factor
+    # of the FP32 factorization's, and the refined one too.
+    rng = np.random.default_rng(n)
+    a, b = rand_system(n, 3, 7 + n, dd=0.0)
+    dr = np.exp2(rng.integers(-60, 61, n).astype(float))
+    dc = np.exp2(rng.integers(-60, 61, n).astype(float))
+    a = dr[:, None] * a * dc[None, :]
+    b = dr[:, None] * b
+    x_ref, info = osolve.zgesv(a, b)
+    assert info == 0
+    errs = {}
+    for method in ('fp32', 'f16x3'):
+        x, st = gpu_solve(a, b, method, max_refine=0)
+        assert st['info'] == 0
+        # error of the unscaled solution dc * x
+        errs[method] = np.abs((x - x_ref) * dc[:, None]).max() / np.abs(x_ref * dc[:, None]).max()
+    print('n %d scaled errors %s' % (n, errs), flush=True)
+    assert errs['f16x3'] < 8 * errs['fp32'], errs
+    # refined: as close to LAPACK as the refined FP32 factorization gets
+    # (this system's scaling leaves a floor of a few 1e-12 for both)
+    ref = {}
+    for method in ('fp32', 'f16x3'):
+        x, st = gpu_solve(a, b, method, max_refine=12, tol=1e-13)
+        assert st['info'] == 0
+        ref[method] = np.abs((x - x_ref) * dc[:, None]).max() / np.abs(x_ref * dc[:, None]).max()
+    print('n %d refined errors %s' % (n, ref), flush=True)
+    assert ref['f16x3'] <= max(4 * ref['fp32'], 1e-12), ref
 
 
 @pytest.mark.parametrize('n', [16500, 20000])
@@ -159,7 +217,7 @@ def test_refinement_history_is_monotone_until_floor():
 
 
 @pytest.mark.parametrize('n', [300, 1100, 3000])
-@pytest.mark.parametrize('method', ['fp64', 'fp32', 'bf16x3'])
+@pytest.mark.parametrize('method', ['fp64', 'fp32', 'bf16x3', 'f16x3'])
 def test_fgmres_refinement_matches_lapack(n, method):
     a, b = rand_system(n, 6, 500 + n, dd=0.002)
     x_ref, info = osolve.zgesv(a, b)
