@@ -24,6 +24,7 @@ PLAN_TRAIL_TILE = 1      # hprmat_hip_lu_set_plan keys
 PLAN_LOOKAHEAD = 2
 PLAN_GMRES_CYCLE = 3
 PLAN_SOLVE_LANES = 4
+PLAN_PIVOT = 5
 FGMRES = 0x10            # flag OR-ed into the criterion: flexible GMRES refinement
 STATUS_TIMEOUT = -0x40000000   # batch status: a panel leaf's bounded wait gave up
 

@@ -661,6 +661,10 @@ int hprmat_hip_lu_set_plan(hprmat_hip_lu_t h, int key, int value) {
       if (value < 0 || value > 2) return -1;
       h->solve_lanes = value;
       return 0;
+    case HPRMAT_PLAN_PIVOT:
+      if (value < 0 || value > 2) return -1;
+      h->plan.pivot = value;
+      return 0;
     default:
       return -1;
   }

@@ -12,7 +12,8 @@
 //           V_0..V_j give column j of the Hessenberg matrix; V_j+1 = W / h;
 //           complex Givens rotations keep it triangular and update g, whose
 //           last entry is the residual of the correction for that column;
-//           the cycle ends when sqrt(sum_c |g_j+1|^2) / ||B||_F <= tol / 2 or
+//           the cycle ends when sqrt(sum_c |g_j+1|^2) / ||B||_F <= margin tol
+//           (margin 1/2 unless HPRMAT_HIP_GMRES_MARGIN says otherwise) or
 //           after m steps.
 //   update: X += Z y with R y = g (per column).
 // Because the Z_j are kept (flexible variant), A Z = V H holds in FP64 and
@@ -150,7 +151,7 @@ __global__ void __launch_bounds__(256)
 arnoldi_step_kernel(int nrhs, int m, int j, const double2* __restrict__ h,
                     const double2* __restrict__ ww, double2* __restrict__ H,
                     double* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
-                    double* __restrict__ inv, RefineState* __restrict__ st, double tol) {
+                    double* __restrict__ inv, RefineState* __restrict__ st, double thresh) {
   __shared__ double red[256];
   const bool halt = st->gm_halt != 0;
   __syncthreads();
@@ -209,7 +210,7 @@ arnoldi_step_kernel(int nrhs, int m, int j, const double2* __restrict__ h,
     st->sweeps += 1;
     st->gm_steps = j + 1;
     const double bf = st->b_fro != 0.0 ? st->b_fro : 1.0;
-    if (sqrt(red[0]) / bf <= 0.5 * tol || j + 1 == m) st->gm_halt = 1;
+    if (sqrt(red[0]) / bf <= thresh || j + 1 == m) st->gm_halt = 1;
   }
 }
 
@@ -298,6 +299,14 @@ void gmres_work_bind(GmresWork& w, void* base, int n, int nrhs, int m) {
 
 namespace {
 
+// Fraction of the tolerance the Arnoldi residual estimate must reach to end a
+// cycle (the true residual is then checked against the tolerance itself).
+double gmres_margin() {
+  const char* e = getenv("HPRMAT_HIP_GMRES_MARGIN");
+  const double v = e ? atof(e) : 0.5;
+  return v > 0.0 && v <= 1.0 ? v : 0.5;
+}
+
 // Host-side progress of one lane through the algorithm above.  advance()
 // enqueues work up to the next point where the host may read a device flag
 // (after a cycle's true residual: `done`; after an Arnoldi step that is not
@@ -368,7 +377,7 @@ int lane_advance(LaneRun<TF>& r, const cx<TF>* LU, int64_t ld, int n, const cx<T
         axpy_kernel<<<eg, 256, 0, st>>>(cnt, nrhs, j + 1, gw.V, cnt, h2, gw.W, halt);
         dots(n, nrhs, 1, gw.W, 0, gw.W, gw.y, 0, gw, halt, st);   // ||W||^2 into y (scratch)
         arnoldi_step_kernel<<<1, 256, 0, st>>>(nrhs, mc, j, gw.h, gw.y, gw.H, gw.cs, gw.sn, gw.g,
-                                                gw.inv, state, tol);
+                                                gw.inv, state, gmres_margin() * tol);
         if (j + 1 < mc) {
           r.stage = LaneRun<TF>::AFTER_STEP;
           r.pending = &RefineState::gm_halt;
@@ -443,7 +452,9 @@ int gmres_solve_lanes(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256
           r.stage = LaneRun<TF>::DONE;
           continue;
         }
-        // the cycle stopped early (its estimate met the rule): update X
+        // the cycle stopped early (its estimate met the rule): update X; the
+        // steps it did not take return to the budget
+        r.left += r.mc - (r.j + 1);
         const GmresWork& gw = r.L->gw;
         const size_t cnt = (size_t)n * r.L->nrhs;
         const int eg = (int)min((size_t)4096, (cnt + 255) / 256);

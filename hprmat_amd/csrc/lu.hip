@@ -299,7 +299,7 @@ constexpr size_t XW_TOTAL = XW_VAL + XW_IDX + XW_ROW + XW_ROWJ;
 
 template <class T> int launch_leaf(cx<T>* P, int64_t ld, int m, int w, int row_base, int* ipiv,
                                    int* info, const Scratch& s, int leaf, hipStream_t st,
-                                   int prio = 0) {
+                                   int prio = 0, int pivot = 1) {
   constexpr int NT = 256;   // one row per thread
   const int G = (m + NT - 1) / NT;
   if (G < 1 || G > MAX_LEAF_G) return -1;   // all leaf workgroups must be co-resident
@@ -310,6 +310,21 @@ template <class T> int launch_leaf(cx<T>* P, int64_t ld, int m, int w, int row_b
   x.row = x.idx + XW_IDX;
   x.rowj = x.row + XW_ROW;
   x.gstride = MAX_LEAF_G;
+  if constexpr (std::is_same<T, float>::value) {
+    if (pivot == 0) {
+      // tournament pivoting; its exchange words alias the candidate rows
+      static_assert((size_t)(CALU_LEVELS * MAX_LEAF_G + 4 * LEAF_W) * LEAF_W <= XW_ROW,
+                    "CALU exchange fits the leaf exchange area");
+      CaluXchg cx_;
+      cx_.sets = x.row;
+      cx_.lu = cx_.sets + (size_t)CALU_LEVELS * MAX_LEAF_G * LEAF_W;
+      cx_.disp = cx_.lu + (size_t)LEAF_W * LEAF_W * 2;
+      cx_.gstride = MAX_LEAF_G;
+      leaf_calu_kernel<LEAF_W><<<G, NT, 0, st>>>(P, ld, m, w, row_base, ipiv, info, cx_,
+                                                 (unsigned)leaf * LEAF_W, s.err, prio);
+      return 0;
+    }
+  }
   leaf_getf2_kernel<T, LEAF_W><<<G, NT, 0, st>>>(P, ld, m, w, row_base, ipiv, info, x,
                                                   (unsigned)leaf * LEAF_W, s.err, 0, 0, 0, prio);
   return 0;
@@ -564,9 +579,14 @@ void launch_split(int nc, int kb, const cx<float>* L, int64_t ld, const cx<float
                   const SplitView& v, hipStream_t st, bool split_l, int c0, int cols) {
   if (split_l) {
     if constexpr (SplitTraits<KIND>::SCALED) {
-      // the K scales: column maxima of L21 (read by every split of this step)
-      fill_u32_kernel<<<1, 256, 0, st>>>(v.colmax, 256, 0u);
-      split_colmax_kernel<<<(nc + 255) / 256, 256, 0, st>>>(nc, kb, L, ld, v.colmax);
+      // the K scales: column maxima of L21 (read by every split of this
+      // step); the partial maxima are staged in the B operand's rows, which
+      // are written only after them (ceil(nc / 64) x 256 floats: 2 KB per 64
+      // rows of a 2 KB+ packed row each)
+      const int nparts = (nc + 63) / 64;
+      float* part = reinterpret_cast<float*>(v.b);
+      split_colmax_part_kernel<<<nparts, 256, 0, st>>>(nc, kb, L, ld, part);
+      split_colmax_final_kernel<<<1, 256, 0, st>>>(nparts, part, v.colmax);
     }
     split_rows_kernel<KIND><<<(nc + 3) / 4, 256, 0, st>>>(nc, kb, L, ld, v.a, v.ka, v.colmax);
   }
@@ -812,6 +832,9 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
                                           : SPLIT_BF16X2;   // split_gemm.hpp kind
   if (bfx && !w.split) return -1;
   if (nb != 256) return -1;
+  // pivot rule of the panel leaves (LuPlan::pivot)
+  const bool tournament =
+      std::is_same<T, float>::value && (plan.pivot == 2 || (plan.pivot == 0 && !g3m));
   const bool la = side && side->stream && n > 2 * nb;  // look-ahead
   const int pk = la && panel_prio_level() >= 1;   // priority of the panel kernels
   const int pg = la && panel_prio_level() >= 2;   // ... and of the main-stream GEMMs
@@ -830,7 +853,8 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
     for (int t = 0; t < kb; t += LEAF_W) {
       const int wt = min(LEAF_W, kb - t);
       cx<T>* P = A + (int64_t)(k + t) * ld + (k + t);
-      if (launch_leaf<T>(P, ld, m - t, wt, k + t, ipiv + k + t, info, s, leaf++, st, pk))
+      if (launch_leaf<T>(P, ld, m - t, wt, k + t, ipiv + k + t, info, s, leaf++, st, pk,
+                         tournament ? 0 : 1))
         return -1;
       if (t > 0) launch_laswp<T>(A, ld, k, k + t, k + t, wt, ipiv, nullptr, st, pk);
       if (t + wt < kb) {

@@ -11,6 +11,10 @@ struct LuPlan {
   // FP32 only, 32x32x2 MFMA: 3 64x64x16, 4 128x128x16, 5 128x64x16, 6 64x128x16
   // (FP64 uses variant 0 for 3..6)
   int trail = 0;
+  // panel leaves of the FP32 factors: 1 partial pivoting (leaf_getf2_kernel,
+  // LAPACK's pivots), 2 tournament pivoting (leaf_calu_kernel), 0 automatic:
+  // tournament except for GEMM_3M; FP64 always partial
+  int pivot = 0;
 };
 
 // Device scratch used by getrf: exchange area of the cooperative panel

@@ -384,6 +384,356 @@ leaf_getf2_kernel(cx<T>* __restrict__ P, int64_t ld, int m, int w, int row_base,
 }
 
 // ---------------------------------------------------------------------------
+// Communication-avoiding leaf with tournament pivoting (CALU: Grigori, Demmel
+// & Xiang, SIAM J. Matrix Anal. Appl. 32 (2011) 1317) of an m x w FP32 leaf
+// at P (row-major, ld), w <= W.  G workgroups of 256 threads own 256
+// consecutive rows each, one row per thread, as in leaf_getf2_kernel; the
+// pivots are chosen with a handful of exchanges instead of one per column:
+//  1. every workgroup runs GEPP on its own rows (registers and LDS only) and
+//     publishes the rows it took as pivots, in order: its candidate set;
+//  2. tournament: at level l, workgroup q < ceil(G_l / 8) gathers the
+//     candidate sets 8q .. 8q+7 of level l-1 (up to 256 rows, read in their
+//     original values from P), runs GEPP on them and publishes its pivots;
+//     the single set of the last level is the leaf's pivot sequence, and the
+//     GEPP rows of that workgroup (always workgroup 0) are the factors
+//     L11 \ U11 of the pivoted w x w block, which it publishes;
+//  3. every workgroup turns the sequence into LAPACK's interchanges (row i
+//     <-> the current row of pivot i), then writes the rows it owns: rows
+//     i < w get the published LU rows, every other row its multipliers
+//     l = a U11^-1 from its original values a (original rows i < w that the
+//     interchanges displace are republished by workgroup 0, their owner).
+// Pivots differ from GEPP's (each is the largest entry among its candidates,
+// not among all rows) and are for the FP32 factors only; the FP64 path keeps
+// leaf_getf2_kernel and LAPACK's pivots.  Exchanges use the tagged words of
+// leaf_getf2_kernel with tags seq_base + 1 + stage.  P is written only in
+// step 3, after every read of original rows by the tournament (each read's
+// value feeds a publication that step 3 waits for).
+// ---------------------------------------------------------------------------
+struct CaluXchg {
+  u64* sets;   // [CALU_LEVELS][gstride][W]  candidate rows (panel-local), 0xffffffff = none
+  u64* lu;     // [W][W][2]                  LU rows of the pivoted block
+  u64* disp;   // [W][W][2]                  original rows 0 .. w-1 (republished)
+  int gstride;
+};
+constexpr int CALU_LEVELS = 4;   // G <= 8^3 = 512 workgroups
+constexpr unsigned CALU_TAG_LU = 8, CALU_TAG_DISP = 9;
+
+template <int W> struct CaluShared {
+  cx<float> cand[2][4][W];   // the waves' candidate pivot rows, by step parity
+  cx<float> crcp[2][4];      // 1 / (their entry in the step's column)
+  float red_v[2][4];
+  int red_i[2][4];
+  int sel[W];                // pivot thread of each step (-1: none)
+  float selv[W];             // its |Re| + |Im|
+  int rowid[256];            // panel-local row held by each thread (-1: none)
+  cx<float> u[W][W];         // L11 \ U11 of the pivoted block
+  cx<float> rdiag[W];        // 1 / U11[c][c] (0 for a zero pivot)
+  int piv[W];                // pivot sequence (panel-local rows)
+  int dpos[W], drow[W];      // displaced original rows: position dpos holds row drow
+  int fpos[W];               // final position of original row r < W
+  int ndisp;
+};
+
+// a - b c for FP32 complex values as two fused multiply-adds per component.
+__device__ inline cx<float> cfms_fused(cx<float> a, cx<float> b, cx<float> c) {
+  return mkc<float>(fmaf(-b.x, c.x, fmaf(b.y, c.y, a.x)), fmaf(-b.x, c.y, fmaf(-b.y, c.x, a.y)));
+}
+
+// GEPP over the rows of the 256 threads (one row each, `live` marks rows that
+// take part), w <= W steps, in registers.  The step loop is unrolled, so that
+// step j updates only the W - 1 - j columns right of j and every register
+// index is a constant; the winning lane of each wave stages its row and the
+// reciprocal of its pivot entry.  On return (after a barrier) sh.sel[j] is
+// the thread holding the pivot of step j (-1 once no row is left) and
+// sh.selv[j] its |Re| + |Im|.  The pivot thread of step j stops updating at
+// step j, so its registers hold its LU row: multipliers in columns < j, U in
+// columns >= j.  Returns the step at which this thread's row was taken (-1:
+// never).  Ties go to the lowest thread, as the i?amax rule.
+template <int W>
+__device__ __forceinline__ int calu_gepp(cx<float> (&a)[W], bool live, int w,
+                                         CaluShared<W>& sh) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int mystep = -1;
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    if (j < w) {   // uniform
+      const int par = j & 1;
+      float v = live ? cabs1<float>(a[j]) : -1.f;
+      int vi = tid;
+      wave_argmax(v, vi);
+      if (tid == vi && v >= 0.f) {
+#pragma unroll
+        for (int c = j; c < W; ++c) sh.cand[par][wave][c] = a[c];
+        sh.crcp[par][wave] = v > 0.f ? crecip<float>(a[j]) : mkc<float>(0.f, 0.f);
+      }
+      if (lane == 0) { sh.red_v[par][wave] = v; sh.red_i[par][wave] = vi; }
+      __syncthreads();
+      float bv = sh.red_v[par][0];
+      int bi = sh.red_i[par][0], bw = 0;
+#pragma unroll
+      for (int q = 1; q < 4; ++q) {
+        const float ov = sh.red_v[par][q];
+        if (ov > bv) { bv = ov; bi = sh.red_i[par][q]; bw = q; }   // waves ascend in threads
+      }
+      if (tid == 0) { sh.sel[j] = bv >= 0.f ? bi : -1; sh.selv[j] = bv; }
+      if (bv >= 0.f) {   // uniform: a row is left
+        if (tid == bi) {
+          live = false;
+          mystep = j;
+        } else if (live) {
+          const cx<float>* F = sh.cand[par][bw];
+          const cx<float> l = cmul<float>(a[j], sh.crcp[par][bw]);   // 0 for a zero pivot
+          a[j] = l;
+#pragma unroll
+          for (int c = j + 1; c < W; ++c) a[c] = cfms_fused(a[c], l, F[c]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  return mystep;
+}
+
+// Loads row `r` (panel-local, r < 0: zeros) of P, w <= W columns, into a,
+// 16 bytes at a time when `vec` (even ld, 16-byte aligned P).
+template <int W>
+__device__ __forceinline__ void calu_load_row(cx<float> (&a)[W], const cx<float>* P, int64_t ld,
+                                              int r, int w, bool vec) {
+  if (vec) {
+#pragma unroll
+    for (int c = 0; c < W; c += 2) {
+      if (r >= 0 && c + 1 < w) {
+        const float4 q = *reinterpret_cast<const float4*>(P + (int64_t)r * ld + c);
+        a[c] = mkc<float>(q.x, q.y);
+        a[c + 1] = mkc<float>(q.z, q.w);
+      } else {
+        a[c] = (r >= 0 && c < w) ? P[(int64_t)r * ld + c] : mkc<float>(0.f, 0.f);
+        a[c + 1] = mkc<float>(0.f, 0.f);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < W; ++c)
+      a[c] = (r >= 0 && c < w) ? P[(int64_t)r * ld + c] : mkc<float>(0.f, 0.f);
+  }
+}
+
+template <int W>
+__global__ void __launch_bounds__(256)
+leaf_calu_kernel(cx<float>* __restrict__ P, int64_t ld, int m, int w, int row_base,
+                 int* __restrict__ ipiv, int* __restrict__ info, CaluXchg x, unsigned seq_base,
+                 int* __restrict__ err, int prio = 0) {
+  raise_wave_priority(prio);
+  constexpr int NT = 256;
+  static_assert(W % 2 == 0 && 8 * W <= NT, "eight candidate sets fill a workgroup");
+  __shared__ CaluShared<W> sh;
+  const int tid = threadIdx.x;
+  const int G = gridDim.x, g = blockIdx.x;
+  const int lo = g * NT, hi = min(m, lo + NT);
+  const int row = lo + tid;
+  const bool mine = row < hi;
+  const int gs = x.gstride;
+  const bool vec = (ld % 2 == 0) && ((reinterpret_cast<uintptr_t>(P) & 15) == 0);
+
+  // levels: G_0 = G sets, G_l = ceil(G_{l-1} / 8), until one set is left
+  int nlev = 0;
+  for (int gl = G; gl > 1; gl = (gl + 7) / 8) ++nlev;
+
+  cx<float> a[W];
+  int mystep = -1;
+  bool final_here = false;   // this workgroup holds the last level's GEPP
+  // level 0: local GEPP on the workgroup's own rows (with G = 1 already the
+  // leaf's GEPP); level l >= 1: the tournament's reduction of 8 sets.  One
+  // call site of calu_gepp, so that it is inlined and `a` stays in registers.
+  for (int l = 0, gl = G; l <= nlev; ++l) {
+    const int gn = l == 0 ? G : (gl + 7) / 8;
+    if (g < gn) {
+      int r = -1;
+      if (l == 0) {
+        r = mine ? row : -1;
+      } else {
+        const int set = 8 * g + tid / W, slot = tid % W;
+        if (set < gl && slot < w) {
+          const unsigned u = get_u32(x.sets + ((size_t)(l - 1) * gs + set) * W + slot,
+                                     seq_base + (unsigned)l, err);
+          r = (u < (unsigned)m) ? (int)u : -1;   // none, or a wait that gave up
+        }
+        __syncthreads();   // sh.rowid / sh.sel of the previous GEPP are read no more
+      }
+      sh.rowid[tid] = r;
+      calu_load_row<W>(a, P, ld, r, w, vec);
+      __syncthreads();
+      mystep = calu_gepp<W>(a, r >= 0, w, sh);
+      if (l < nlev && tid < w) {
+        const int s = sh.sel[tid];
+        put_word(x.sets + ((size_t)l * gs + g) * W + tid, seq_base + 1u + (unsigned)l,
+                 s >= 0 ? (unsigned)sh.rowid[s] : 0xffffffffu);
+      }
+      final_here = l == nlev;
+    }
+    if (l > 0) gl = gn;
+  }
+  // -- the last level (workgroup 0): pivot sequence, LU rows, ipiv, info
+  if (final_here) {
+    if (tid < W) sh.piv[tid] = (tid < w && sh.sel[tid] >= 0) ? sh.rowid[sh.sel[tid]] : -1;
+    for (int e = tid; e < W * W; e += NT) sh.u[e / W][e % W] = mkc<float>(0.f, 0.f);
+    __syncthreads();
+    if (mystep >= 0) {   // LU row `mystep`
+#pragma unroll
+      for (int c = 0; c < W; ++c) sh.u[mystep][c] = a[c];
+    }
+    __syncthreads();
+    if (G > 1) {
+      for (int e = tid; e < W * W; e += NT) {
+        const int i = e / W, c = e % W;
+        const cx<float> v = sh.u[i][c];
+        put_word(x.lu + (size_t)e * 2, seq_base + 1u + CALU_TAG_LU, __float_as_uint(v.x));
+        put_word(x.lu + (size_t)e * 2 + 1, seq_base + 1u + CALU_TAG_LU, __float_as_uint(v.y));
+      }
+      if (tid < W) {
+        const int pv = tid < w ? sh.piv[tid] : -1;
+        put_word(x.sets + ((size_t)nlev * gs) * W + tid, seq_base + 1u + (unsigned)nlev,
+                 pv >= 0 ? (unsigned)pv : 0xffffffffu);
+      }
+    }
+  } else {
+    // pivot sequence and LU rows from the last level
+    if (tid < W) {
+      int pv = -1;
+      if (tid < w) {
+        const unsigned u = get_u32(x.sets + ((size_t)nlev * gs) * W + tid,
+                                   seq_base + 1u + (unsigned)nlev, err);
+        pv = u < (unsigned)m ? (int)u : -1;
+      }
+      sh.piv[tid] = pv;
+    }
+    for (int e = tid; e < W * W; e += NT) {
+      const int i = e / W, c = e % W;
+      const float re = __uint_as_float(get_u32(x.lu + (size_t)e * 2, seq_base + 1u + CALU_TAG_LU, err));
+      const float im = __uint_as_float(get_u32(x.lu + (size_t)e * 2 + 1, seq_base + 1u + CALU_TAG_LU, err));
+      sh.u[i][c] = mkc<float>(re, im);
+    }
+  }
+  __syncthreads();
+  // -- 3. interchanges: wave 0 replays them with lane r < W holding the
+  // position of original row r and the original row at position r (rows
+  // >= w are still in place when chosen: only rows < w move before their
+  // turn).  The row a displaced row lands on is recorded (position dpos
+  // holds original row drow; a later record for the same position wins).
+  if (tid < 64) {
+    const int lane = tid;
+    const int pivl = lane < W ? sh.piv[lane] : -1;   // lane i: pivot i
+    int cur = lane, at = lane, nd = 0;
+    for (int i = 0; i < w; ++i) {
+      int r = __builtin_amdgcn_readlane(pivl, i);
+      const int at_i = __builtin_amdgcn_readlane(at, i);
+      if (r < 0) r = at_i;   // a failed wait: keep the row in place
+      const int p = r < w ? __builtin_amdgcn_readlane(cur, r) : r;
+      const int xr = at_i;   // the small row at position i moves to p
+      if (p < w) {
+        if (lane == p) at = xr;
+      } else {
+        if (lane == 0) { sh.dpos[nd] = p; sh.drow[nd] = xr; }
+        ++nd;
+      }
+      if (lane == xr) cur = p;
+      if (lane == i) at = r;
+      if (r < w && lane == r) cur = i;
+      if (g == 0 && lane == 0) {
+        ipiv[i] = row_base + p;
+        if (sh.selv[i] == 0.f && *info == 0 && final_here) *info = row_base + i + 1;
+      }
+    }
+    if (lane == 0) sh.ndisp = nd;
+    if (lane < W) sh.fpos[lane] = cur;
+  }
+  if (tid < W) {
+    const cx<float> d = tid < w ? sh.u[tid][tid] : mkc<float>(1.f, 0.f);
+    sh.rdiag[tid] = (d.x == 0.f && d.y == 0.f) ? mkc<float>(0.f, 0.f) : crecip<float>(d);
+  }
+  __syncthreads();
+  // workgroup 0 republishes the displaced original rows (it owns rows < w and
+  // has not written them yet)
+  if (g == 0 && G > 1) {
+    for (int e = tid; e < W * W; e += NT) {
+      const int i = e / W, c = e % W;
+      const cx<float> v = (i < w && c < w && i < m) ? P[(int64_t)i * ld + c] : mkc<float>(0.f, 0.f);
+      put_word(x.disp + (size_t)e * 2, seq_base + 1u + CALU_TAG_DISP, __float_as_uint(v.x));
+      put_word(x.disp + (size_t)e * 2 + 1, seq_base + 1u + CALU_TAG_DISP, __float_as_uint(v.y));
+    }
+  }
+  if (G == 1) {
+    // one workgroup: its GEPP ran on all rows, so every register row already
+    // is a row of the factors (pivot rows their LU row, the others their
+    // multipliers); each goes to its final position
+    if (mine) {
+      const int dest = mystep >= 0 ? mystep : (row < w ? sh.fpos[row] : row);
+      if (vec) {
+#pragma unroll
+        for (int c = 0; c < W; c += 2) {
+          if (c + 1 < w)
+            *reinterpret_cast<float4*>(P + (int64_t)dest * ld + c) =
+                make_float4(a[c].x, a[c].y, a[c + 1].x, a[c + 1].y);
+          else if (c < w)
+            P[(int64_t)dest * ld + c] = a[c];
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < W; ++c)
+          if (c < w) P[(int64_t)dest * ld + c] = a[c];
+      }
+    }
+    return;
+  }
+  // the original values of the row that ends at this thread's position
+  int src = row;
+  for (int k = 0; k < sh.ndisp; ++k)
+    if (sh.dpos[k] == row) src = sh.drow[k];
+  if (mine && row >= w) {
+    if (src == row || g == 0 || G == 1) {
+      // (workgroup 0, the owner of rows < w, reads them before writing)
+      calu_load_row<W>(a, P, ld, src, w, vec);
+    } else {
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        const size_t e = (size_t)src * W + c;
+        const float re = __uint_as_float(get_u32(x.disp + e * 2, seq_base + 1u + CALU_TAG_DISP, err));
+        const float im = __uint_as_float(get_u32(x.disp + e * 2 + 1, seq_base + 1u + CALU_TAG_DISP, err));
+        a[c] = mkc<float>(re, im);
+      }
+    }
+    // multipliers: l U11 = a, row by row of U11 (right-looking)
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+      const cx<float> lc = cmul<float>(a[c], sh.rdiag[c]);
+      a[c] = lc;
+#pragma unroll
+      for (int c2 = c + 1; c2 < W; ++c2) a[c2] = cfms_fused(a[c2], lc, sh.u[c][c2]);
+    }
+  } else if (mine) {
+#pragma unroll
+    for (int c = 0; c < W; ++c) a[c] = sh.u[row][c];
+  }
+  __syncthreads();   // workgroup 0: every read of rows < w is done
+  if (mine) {
+    if (vec) {
+#pragma unroll
+      for (int c = 0; c < W; c += 2) {
+        if (c + 1 < w)
+          *reinterpret_cast<float4*>(P + (int64_t)row * ld + c) =
+              make_float4(a[c].x, a[c].y, a[c + 1].x, a[c + 1].y);
+        else if (c < w)
+          P[(int64_t)row * ld + c] = a[c];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < W; ++c)
+        if (c < w) P[(int64_t)row * ld + c] = a[c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Row interchanges k1 .. k1+nk-1 (global 0-based ipiv, ipiv[k] >= k) applied
 // to the columns [c0, c1) of a row-major matrix.  Each workgroup takes a
 // 128-byte column strip, stages every row the sequence touches in LDS (the nk

@@ -41,7 +41,7 @@
 //   half index ((r * 8 + kb) * 2P + plane) * 32 + kk.
 // Scales (SPLIT_F16X2 only): kexp[r] = k of row r (int32); the K scales
 // come from colmax[k] = bits of max_i max(|Re L_ik|, |Im L_ik|)
-// (split_colmax_kernel).
+// (split_colmax_part_kernel, split_colmax_final_kernel).
 #pragma once
 #include "common.hpp"
 
@@ -143,24 +143,37 @@ __device__ inline int split_kexp(const unsigned* colmax, int k) {
   return e;
 }
 
-// colmax[k] = max(colmax[k], bits of max_r max(|Re S_rk|, |Im S_rk|)) over
-// the rows r < rows of a row-major FP32 complex block (element (r, k) at
-// S[r * ld + k], k < K <= 256); colmax must start at 0.  Thread k of a
-// workgroup scans 256 rows (the rows of a workgroup are read as 2 KB
-// coalesced row segments); grid = ceil(rows / 256), block 256.
+// Column maxima of a row-major FP32 complex block (element (r, k) at
+// S[r * ld + k], r < rows, k < K <= 256) in two launches without atomics.
+// Stage 1: workgroup g writes part[g][k] = max over its 64 rows of
+// max(|Re S_rk|, |Im S_rk|) (thread k, its 64 loads issued together; the
+// workgroup reads 64 coalesced 2 KB row segments); grid = ceil(rows / 64).
 __global__ void __launch_bounds__(256)
-split_colmax_kernel(int rows, int K, const cx<float>* __restrict__ S, int64_t ld,
-                    unsigned* __restrict__ colmax) {
+split_colmax_part_kernel(int rows, int K, const cx<float>* __restrict__ S, int64_t ld,
+                         float* __restrict__ part) {
   const int k = threadIdx.x;
-  if (k >= K) return;
-  const int r0 = blockIdx.x * 256, r1 = min(rows, r0 + 256);
+  const int r0 = blockIdx.x * 64;
   float m = 0.f;
-  for (int r = r0; r < r1; ++r) {
-    const cx<float> v = S[(int64_t)r * ld + k];
-    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+  if (k < K) {
+#pragma unroll 16
+    for (int r = 0; r < 64; ++r) {
+      if (r0 + r < rows) {
+        const cx<float> v = S[(int64_t)(r0 + r) * ld + k];
+        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+      }
+    }
   }
-  // non-negative floats order like their bit patterns
-  if (m > 0.f) atomicMax(colmax + k, __float_as_uint(m));
+  part[(size_t)blockIdx.x * 256 + k] = m;
+}
+// Stage 2 (one workgroup): colmax[k] = bits of max over the nparts partial
+// maxima (non-negative floats order like their bit patterns).
+__global__ void __launch_bounds__(256)
+split_colmax_final_kernel(int nparts, const float* __restrict__ part,
+                          unsigned* __restrict__ colmax) {
+  const int k = threadIdx.x;
+  float m = 0.f;
+  for (int g = 0; g < nparts; ++g) m = fmaxf(m, part[(size_t)g * 256 + k]);
+  colmax[k] = __float_as_uint(m);
 }
 
 // Split `rows` rows of a row-major FP32 complex block S (element (r, k) at

@@ -59,12 +59,19 @@ class LU:
             self._lib.hprmat_hip_lu_destroy(h)
             self._h = None
 
-    def set_plan(self, trail_tile=None, lookahead=None, gmres_cycle=None, solve_lanes=None):
+    def set_plan(self, trail_tile=None, lookahead=None, gmres_cycle=None, solve_lanes=None,
+                 pivot=None):
         """Factorization plan: trailing-update GEMM tile (0 64x64x16,
         1 64x128x16, 2 128x64x16 on 16x16x4 MFMA; FP32 factors only, on
         32x32x2 MFMA: 3 64x64x16, 4 128x128x16, 5 128x64x16, 6 64x128x16),
         look-ahead on/off, the FGMRES cycle length (1..24 steps; 0 =
-        default 16) and the FGMRES lanes (0 automatic, 1, 2)."""
+        default 16), the FGMRES lanes (0 automatic, 1, 2) and the pivot rule
+        of the FP32 factors ('auto', the default: tournament except for
+        fp32_3m; 'tournament'; 'partial': LAPACK's pivots)."""
+        if pivot is not None:
+            code = {'auto': 0, 'partial': 1, 'tournament': 2}[pivot]
+            _lib.check(self._lib.hprmat_hip_lu_set_plan(self._h, _lib.PLAN_PIVOT, code),
+                       'set_plan')
         if solve_lanes is not None:
             _lib.check(self._lib.hprmat_hip_lu_set_plan(self._h, _lib.PLAN_SOLVE_LANES,
                                                          int(solve_lanes)), 'set_plan')
@@ -169,15 +176,19 @@ class LU:
 
 
 def solve(a, b, method='fp64', max_refine=0, tol=1e-12, criterion='rel_fro', final_check=True,
-          symmetric=False, equilibrate=False, refinement='ir'):
+          symmetric=False, equilibrate=False, refinement='ir', pivot=None):
     """X = A^-1 B for complex128 CUDA tensors a (n x n) and b (n x nrhs).
 
-    Returns (X, stats) with X an (n x nrhs) tensor.
+    ``pivot``: pivot rule of FP32 factors ('tournament' or 'partial'; None:
+    the library default).  Returns (X, stats) with X an (n x nrhs) tensor.
     """
     _check_tensor(a, 'a')
     a_cm = colmajor(a, symmetric)
     b_cm = b.transpose(0, 1).contiguous()
-    lu = LU().factor(a_cm, method, equilibrate)
+    lu = LU()
+    if pivot is not None:
+        lu.set_plan(pivot=pivot)
+    lu.factor(a_cm, method, equilibrate)
     st = lu.solve_(b_cm, max_refine, tol, criterion, final_check, refinement=refinement)
     if st['info'] != 0:
         raise RuntimeError('zero pivot at column %d' % st['info'])

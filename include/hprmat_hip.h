@@ -160,9 +160,17 @@ int hprmat_hip_lu_destroy(hprmat_hip_lu_t h);
                              products; each lane stops on its own rule), 0 =
                              automatic (two for n >= 8192 and >= 64 columns,
                              one under stream capture)
+     HPRMAT_PLAN_PIVOT       pivot rule of the FP32 factors: 1 = partial
+                             pivoting (LAPACK's pivots, one exchange per
+                             column), 2 = tournament pivoting (CALU: a few
+                             exchanges per 32 columns), 0 = automatic
+                             (default): tournament, except partial for
+                             HPRMAT_MIXED_FP32_3M, whose Gauss form amplifies
+                             the larger multipliers tournament pivots allow;
+                             the FP64 factors always use partial pivoting
    Returns -1 for an unknown key or value. */
 enum { HPRMAT_PLAN_TRAIL_TILE = 1, HPRMAT_PLAN_LOOKAHEAD = 2, HPRMAT_PLAN_GMRES_CYCLE = 3,
-       HPRMAT_PLAN_SOLVE_LANES = 4 };
+       HPRMAT_PLAN_SOLVE_LANES = 4, HPRMAT_PLAN_PIVOT = 5 };
 int hprmat_hip_lu_set_plan(hprmat_hip_lu_t h, int key, int value);
 
 /* No reference counterpart: describe the factored matrix by its
