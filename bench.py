@@ -28,11 +28,14 @@ Methods timed back to back (name: factorization, refinement):
             operand with power-of-two row / column scales (three FP16
             products per real product, about 2^-22 relative), classic IR;
   f16x3_gmres  the f16x3 factorization with flexible-GMRES refinement;
-  fp32_3m_gmres_op, fp64_op  fp32_3m_gmres and fp64 with the refinement's
-            FP64 products formed from the Lagrange-mesh parts of the matrix
-            (channel blocks and couplings, hprmat_hip_lu_set_operator)
-            instead of the dense matrix; reported in their own block, not
-            as the headline, which keeps the dense drop-in operator.
+  f16x3_gmres_op, fp32_3m_gmres_op, fp64_op  f16x3_gmres, fp32_3m_gmres
+            and fp64 with the refinement's FP64 products formed from the
+            Lagrange-mesh parts of the matrix (channel blocks and couplings,
+            hprmat_hip_lu_set_operator) instead of the dense matrix; reported
+            in their own block, not as the headline, which keeps the dense
+            drop-in operator.
+The FP32 factorizations choose pivots by tournament (CALU) in their panel
+leaves, except fp32_3m (partial pivoting); fp64 keeps LAPACK's pivots.
 ``value`` is the solves/s of the fastest non-FP64 method that reaches the
 residual gate; per-method figures and the roofline of the dominant kernel
 (the trailing-update GEMM, timed with HIP events on the launch stream) are
@@ -79,7 +82,7 @@ PEAK = {'fp64': 78.6, 'fp32': 157.3, 'fp32_ir5': 157.3, 'fp32_gmres': 157.3, 'fp
         'bf16x3': BF16_PEAK / 3, 'bf16x3_gmres': BF16_PEAK / 3,
         'bf16x6': BF16_PEAK / 6, 'bf16x6_gmres': BF16_PEAK / 6,
         'f16x3': BF16_PEAK / 3, 'f16x3_gmres': BF16_PEAK / 3,
-        'fp32_3m_gmres_op': 157.3, 'fp64_op': 78.6}
+        'fp32_3m_gmres_op': 157.3, 'fp64_op': 78.6, 'f16x3_gmres_op': BF16_PEAK / 3}
 # name -> (factorization method, max refinement sweeps)
 SPECS = {'fp64': ('fp64', 10, 'ir'), 'fp32': ('fp32', 30, 'ir'), 'fp32_ir5': ('fp32', 5, 'ir'),
          'fp32_gmres': ('fp32', 24, 'fgmres'), 'bf16x3': ('bf16x3', 30, 'ir'),
@@ -87,7 +90,8 @@ SPECS = {'fp64': ('fp64', 10, 'ir'), 'fp32': ('fp32', 30, 'ir'), 'fp32_ir5': ('f
          'fp32_3m_gmres': ('fp32_3m', 24, 'fgmres'),
          'bf16x6': ('bf16x6', 30, 'ir'), 'bf16x6_gmres': ('bf16x6', 24, 'fgmres'),
          'f16x3': ('f16x3', 30, 'ir'), 'f16x3_gmres': ('f16x3', 24, 'fgmres'),
-         'fp32_3m_gmres_op': ('fp32_3m', 24, 'fgmres'), 'fp64_op': ('fp64', 10, 'ir')}
+         'fp32_3m_gmres_op': ('fp32_3m', 24, 'fgmres'), 'fp64_op': ('fp64', 10, 'ir'),
+         'f16x3_gmres_op': ('f16x3', 24, 'fgmres')}
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -229,8 +233,9 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--nch', type=int, default=128)
     ap.add_argument('--nr', type=int, default=200)
-    ap.add_argument('--methods', default='fp64,fp32_gmres,bf16x6_gmres,fp32_3m_gmres,fp32,'
-                                         'fp32_ir5,bf16x3,bf16x3_gmres,fp64_op,fp32_3m_gmres_op')
+    ap.add_argument('--methods', default='fp64,f16x3_gmres,fp32_3m_gmres,fp32_gmres,bf16x6_gmres,'
+                                         'fp32,fp32_ir5,bf16x3,bf16x3_gmres,fp64_op,'
+                                         'f16x3_gmres_op')
     ap.add_argument('--no-cpu', action='store_true',
                     help='skip the host oracle / CPU baseline (LAPACK zgesv at full size)')
     ap.add_argument('--no-abi', action='store_true', help='skip the host-array ABI wall times')
@@ -360,18 +365,20 @@ def run(args, torch, lib, rmatrix, solver, synth):
     roof = {'bound': 'mfma', 'achieved': g['tflops'], 'peak': PEAK[head],
             'unit': 'TFLOP/s', 'frac': g['tflops'] / PEAK[head], 'traffic': None,
             'kernel': '%s (trailing update A22 -= L21 U12, %s)' % (
-                'zgemm_split_kernel<SPLIT_BF16X2, 128, 128, 64, 64, 32, 2, 1>'
+                'zgemm_split_kernel<SPLIT_BF16X2, 128, 128, 32, 64, 32, 2, 1, 512>'
                 if head.startswith('bf16x3') else
-                'zgemm_split_kernel<SPLIT_BF16X3, 128, 64, 64, 32, 16, 1, 2>'
+                'zgemm_split_kernel<SPLIT_BF16X3, 128, 64, 64, 32, 16, 1, 2, 256>'
                 if head.startswith('bf16x6') else
-                'zgemm_split_kernel<SPLIT_F16X2, 128, 128, 64, 64, 32, 2, 1>'
+                'zgemm_split_kernel<SPLIT_F16X2, 128, 128, 32, 64, 32, 2, 1, 512>'
                 if head.startswith('f16x3') else
                 'trailing_update_kernel<%s, 64, 64, 16, 32, 32, %s>' % (
                     'double' if head == 'fp64' else 'float',
                     'true' if head.startswith('fp32_3m') else 'false'), head),
             'flops_per_launch': g['flops_per_launch'], 'ms_per_launch': g['ms_per_launch']}
     tr = traffic_record('fp64_factor' if head == 'fp64' else
-                        'fp32_3m_factor' if head.startswith('fp32_3m') else 'fp32')
+                        'fp32_3m_factor' if head.startswith('fp32_3m') else
+                        'f16x3_factor' if head.startswith('f16x3') else
+                        'bf16x6_factor' if head.startswith('bf16x6') else 'fp32')
     split = head.startswith('bf16') or head.startswith('f16')
     if split:
         roof['peak_note'] = ('%s dense matrix peak %.1f TFLOP/s / %d 16-bit products per real '
@@ -383,7 +390,7 @@ def run(args, torch, lib, rmatrix, solver, synth):
                              'achieved = %.1f TFLOP/s (%.2f of the FP32 matrix peak)' % (
                                  0.75 * g['tflops'], 0.75 * g['tflops'] / PEAK[head]))
     from hprmat_amd import tracesum
-    if tr is not None and not split and 'launches' in tr:
+    if tr is not None and 'launches' in tr:
         # mean over the trailing updates of one factorization, like 'achieved'
         roof['traffic'] = tr['hbm_bytes']
         roof['traffic_launch'] = {
@@ -391,11 +398,13 @@ def run(args, torch, lib, rmatrix, solver, synth):
             'hbm_write_bytes': tr['hbm_write_bytes'], 'alg_bytes': tr['alg_bytes'],
             'read_ratio': tr['read_ratio'], 'write_ratio': tr['write_ratio'],
             'build': tr.get('build'),
-            'build_matches_this_source': tr.get('build') == tracesum.kernel_source_hash(),
-            'source': 'profiles/trailing_gemm_counters_N25600.md (FETCH_SIZE, WRITE_SIZE of '
-                      '%s over the %d trailing updates of one N = %d factorization, '
-                      'copy-calibrated; mean per launch)' % (tr['kernel'], tr['launches'],
-                                                             tr['N'])}
+            'build_matches_this_source':
+                tr.get('build') == tracesum.kernel_source_hash(tr['kernel']),
+            'source': '%s (FETCH_SIZE, WRITE_SIZE of %s over the %d trailing updates of one '
+                      'N = %d factorization, copy-calibrated; mean per launch)' % (
+                          'profiles/split_f16_update_N25600.md' if 'split' in tr['kernel']
+                          else 'profiles/trailing_gemm_counters_N25600.md',
+                          tr['kernel'], tr['launches'], tr['N'])}
     out = {
         'metric': METRIC, 'value': hm['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': hm['ms'],
@@ -419,9 +428,11 @@ def run(args, torch, lib, rmatrix, solver, synth):
         # Lagrange-mesh parts of the matrix (not the headline)
         blk = {m: {'ms': v['ms'], 'solves_per_s': v['solves_per_s'], 'res_rel': v['res_rel'],
                    'sweeps': v['sweeps']} for m, v in op_methods.items()}
-        if 'fp32_3m_gmres_op' in op_methods and 'fp64_op' in op_methods:
-            blk['speedup_mixed_vs_fp64'] = (op_methods['fp64_op']['ms'] /
-                                            op_methods['fp32_3m_gmres_op']['ms'])
+        mixed_op = [m for m in op_methods if m != 'fp64_op' and op_methods[m]['res_rel'] <= 1e-12]
+        if mixed_op and 'fp64_op' in op_methods:
+            best = min(mixed_op, key=lambda m: op_methods[m]['ms'])
+            blk['speedup_mixed_vs_fp64'] = op_methods['fp64_op']['ms'] / op_methods[best]['ms']
+            blk['mixed_method'] = best
         out['mesh_operator'] = blk
     del a_cm, b0, op_parts
     torch.cuda.empty_cache()

@@ -581,12 +581,12 @@ void launch_split(int nc, int kb, const cx<float>* L, int64_t ld, const cx<float
     if constexpr (SplitTraits<KIND>::SCALED) {
       // the K scales: column maxima of L21 (read by every split of this
       // step); the partial maxima are staged in the B operand's rows, which
-      // are written only after them (ceil(nc / 64) x 256 floats: 2 KB per 64
-      // rows of a 2 KB+ packed row each)
-      const int nparts = (nc + 63) / 64;
+      // are written only after them (ceil(nc / 256) x 256 floats: 1 KB per
+      // 256 rows of 2 KB+ packed rows)
+      const int nparts = (nc + 255) / 256;
       float* part = reinterpret_cast<float*>(v.b);
-      split_colmax_part_kernel<<<nparts, 256, 0, st>>>(nc, kb, L, ld, part);
-      split_colmax_final_kernel<<<1, 256, 0, st>>>(nparts, part, v.colmax);
+      split_colmax_part_kernel<<<nparts, 1024, 0, st>>>(nc, kb, L, ld, part);
+      split_colmax_final_kernel<<<1, 1024, 0, st>>>(nparts, part, v.colmax);
     }
     split_rows_kernel<KIND><<<(nc + 3) / 4, 256, 0, st>>>(nc, kb, L, ld, v.a, v.ka, v.colmax);
   }
@@ -614,23 +614,26 @@ void split_operands(int nc, int kb, const cx<T>* L, int64_t ld, const cx<T>* U, 
 }
 
 // C[:, 0:N] -= L21 U12[:, c0:c0+N] over the split operands (M = nc rows).
-// Tiles (measured at M = N = 25344): 128 x 128 with 32-K slices and two
-// register sets for the two-piece splits; 128 x 64 with 16-K slices and two
-// workgroups per CU for the three-piece split (its six planes).
+// Tiles (measured at M = N = 25344): 128 x 128 with 32-K slices, two
+// register sets and eight waves of 32 x 64 (two per SIMD) for the two-piece
+// splits (4.9 ms against 5.2 ms with four waves of 64 x 64); 128 x 64 with
+// 16-K slices and two workgroups per CU for the three-piece split (its six
+// planes).
 template <int KIND>
 void launch_split_gemm(int M, int N, const SplitView& v, int c0, cx<float>* C, int64_t ldc,
                        hipStream_t st) {
   constexpr bool three = KIND == SPLIT_BF16X3;
-  constexpr int BM = 128, BN = three ? 64 : 128, WM = 64, WN = three ? 32 : 64;
+  constexpr int BM = 128, BN = three ? 64 : 128, WM = three ? 64 : 32, WN = three ? 32 : 64;
   constexpr int BK = three ? 16 : 32, NSET = three ? 1 : 2, MINB = three ? 2 : 1;
+  constexpr int NTH = three ? 256 : 512;
   static bool attr = false;
   if (!attr) {
-    allow_big_lds(zgemm_split_kernel<KIND, BM, BN, WM, WN, BK, NSET, MINB>);
+    allow_big_lds(zgemm_split_kernel<KIND, BM, BN, WM, WN, BK, NSET, MINB, NTH>);
     attr = true;
   }
   const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
-  zgemm_split_kernel<KIND, BM, BN, WM, WN, BK, NSET, MINB>
-      <<<tiles, 256, zgemm_split_lds_bytes<KIND, BM, BN, BK>(), st>>>(
+  zgemm_split_kernel<KIND, BM, BN, WM, WN, BK, NSET, MINB, NTH>
+      <<<tiles, NTH, zgemm_split_lds_bytes<KIND, BM, BN, BK>(), st>>>(
           M, N, v.a, v.ka, v.b + (size_t)c0 * split_row_halves<KIND>(), v.kb + c0, C, ldc, -1.f);
 }
 

@@ -145,35 +145,48 @@ __device__ inline int split_kexp(const unsigned* colmax, int k) {
 
 // Column maxima of a row-major FP32 complex block (element (r, k) at
 // S[r * ld + k], r < rows, k < K <= 256) in two launches without atomics.
-// Stage 1: workgroup g writes part[g][k] = max over its 64 rows of
-// max(|Re S_rk|, |Im S_rk|) (thread k, its 64 loads issued together; the
-// workgroup reads 64 coalesced 2 KB row segments); grid = ceil(rows / 64).
-__global__ void __launch_bounds__(256)
+// Stage 1: workgroup g (1024 threads: column k = t % 256, row phase
+// q = t / 256) writes part[g][k] = max over its 256 rows of
+// max(|Re S_rk|, |Im S_rk|); thread (k, q) takes rows q, q + 4, ... of the
+// workgroup's block, 16 loads in flight, and the four phases combine in
+// LDS; the workgroup reads 256 coalesced 2 KB row segments.
+// grid = ceil(rows / 256), block 1024.
+__global__ void __launch_bounds__(1024)
 split_colmax_part_kernel(int rows, int K, const cx<float>* __restrict__ S, int64_t ld,
                          float* __restrict__ part) {
-  const int k = threadIdx.x;
-  const int r0 = blockIdx.x * 64;
+  __shared__ float red[4][256];
+  const int k = threadIdx.x & 255, q = threadIdx.x >> 8;
+  const int r0 = blockIdx.x * 256;
   float m = 0.f;
   if (k < K) {
 #pragma unroll 16
-    for (int r = 0; r < 64; ++r) {
-      if (r0 + r < rows) {
-        const cx<float> v = S[(int64_t)(r0 + r) * ld + k];
+    for (int i = q; i < 256; i += 4) {
+      if (r0 + i < rows) {
+        const cx<float> v = S[(int64_t)(r0 + i) * ld + k];
         m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
       }
     }
   }
-  part[(size_t)blockIdx.x * 256 + k] = m;
+  red[q][k] = m;
+  __syncthreads();
+  if (q == 0)
+    part[(size_t)blockIdx.x * 256 + k] = fmaxf(fmaxf(red[0][k], red[1][k]), fmaxf(red[2][k], red[3][k]));
 }
-// Stage 2 (one workgroup): colmax[k] = bits of max over the nparts partial
-// maxima (non-negative floats order like their bit patterns).
-__global__ void __launch_bounds__(256)
+// Stage 2 (one workgroup of 1024 threads, four phases as above): colmax[k]
+// = bits of max over the nparts partial maxima (non-negative floats order
+// like their bit patterns).
+__global__ void __launch_bounds__(1024)
 split_colmax_final_kernel(int nparts, const float* __restrict__ part,
                           unsigned* __restrict__ colmax) {
-  const int k = threadIdx.x;
+  __shared__ float red[4][256];
+  const int k = threadIdx.x & 255, q = threadIdx.x >> 8;
   float m = 0.f;
-  for (int g = 0; g < nparts; ++g) m = fmaxf(m, part[(size_t)g * 256 + k]);
-  colmax[k] = __float_as_uint(m);
+#pragma unroll 8
+  for (int g = q; g < nparts; g += 4) m = fmaxf(m, part[(size_t)g * 256 + k]);
+  red[q][k] = m;
+  __syncthreads();
+  if (q == 0)
+    colmax[k] = __float_as_uint(fmaxf(fmaxf(red[0][k], red[1][k]), fmaxf(red[2][k], red[3][k])));
 }
 
 // Split `rows` rows of a row-major FP32 complex block S (element (r, k) at
@@ -290,7 +303,7 @@ constexpr size_t zgemm_split_lds_bytes() {
 // right factor), C row-major FP32 complex with leading dimension ldc.
 // ka / kb are the scale exponents (SPLIT_F16X2; ignored otherwise).
 //
-// Workgroup: 256 threads (four waves) computing BM x BN of C; wave (wr, wc)
+// Workgroup: NTH threads (NTH / 64 waves) computing BM x BN of C; wave (wr, wc)
 // owns WM x WN, as (WM / 32) x (WN / 32) blocks of 32 x 32 with separate real
 // and imaginary FP32 accumulators.  K runs in slices of BK (16 or 32): the
 // slice NSET ahead is loaded from global memory into registers while the
@@ -307,14 +320,14 @@ constexpr size_t zgemm_split_lds_bytes() {
 // Tile order: XCD-aware when the tile count is a multiple of 8 (consecutive
 // workgroups go to the 8 XCDs in turn, so each XCD receives a contiguous
 // range), then grouped by 8 tile rows for L2 reuse of the B rows.
-template <int KIND, int BM, int BN, int WM, int WN, int BK, int NSET, int MINB>
-__global__ void __launch_bounds__(256, MINB)
+template <int KIND, int BM, int BN, int WM, int WN, int BK, int NSET, int MINB, int NTH = 256>
+__global__ void __launch_bounds__(NTH, MINB)
 zgemm_split_kernel(int M, int N, const half_bits* __restrict__ A, const int* __restrict__ ka,
                    const half_bits* __restrict__ B, const int* __restrict__ kb_,
                    cx<float>* __restrict__ C, int64_t ldc, float alpha) {
   constexpr int P = SplitTraits<KIND>::P, NP = 2 * P;
   constexpr int RH = split_row_halves<KIND>();           // halves per packed row
-  constexpr int NT = 256, FM = WM / 32, FN = WN / 32;
+  constexpr int NT = NTH, FM = WM / 32, FN = WN / 32;
   constexpr int ROWS = BM + BN;
   constexpr int CHK = BK / 8;                             // 16-byte chunks per LDS row
   constexpr int RSW = 16 / CHK;                           // rows per swizzle step
@@ -322,7 +335,7 @@ zgemm_split_kernel(int M, int N, const half_bits* __restrict__ A, const int* __r
   constexpr int RPP = NT / CHK;                           // rows per load pass
   constexpr int LE = (ROWS + RPP - 1) / RPP;              // load passes per plane
   constexpr int KS = 256 / BK;                            // slices
-  static_assert((BM / WM) * (BN / WN) == 4, "four waves per workgroup");
+  static_assert((BM / WM) * (BN / WN) == NT / 64, "one wave per WM x WN block");
   static_assert(WM % 32 == 0 && WN % 32 == 0, "32 x 32 blocks");
   static_assert(BK == 16 || BK == 32, "slice of one or two 16-K steps");
   static_assert(NSET == 1 || NSET == 2, "register sets");

@@ -21,14 +21,18 @@ With --traffic N the counter files (a FETCH_SIZE pass and a WRITE_SIZE pass)
 hold the trailing-update launches of one blocked factorization of order N
 (block --nb): the j-th launch of --kernel updates the nc x ncols block with
 nc = N - (j + 1) nb and ncols = nc, or nc - nb on a look-ahead step (its grid
-of 64 x 64 tiles tells which), and its algorithmic bytes are one read of
-L21 (nc x nb), U12 (nb x ncols) and C (nc x ncols) and one write of C.  The
+of --tile x --tile tiles tells which; 64 for the FP32/FP64 kernels, 128 for
+the split-precision ones), and its algorithmic bytes are one read of L21
+(nc x nb), U12 (nb x ncols) and C (nc x ncols) and one write of C, at
+--elem-bytes per complex element (8: FP32 complex, or the four 16-bit planes
+of the two-piece splits).  The
 summary compares the copy-calibrated HBM bytes with those, per launch and in
 total (--json writes the same figures).
 
 Usage: python -m hprmat_amd.tracesum TRACE.csv [--kernel zgemm_kernel<float] [--min-wg 1000]
        python -m hprmat_amd.tracesum --counters COUNTERS.csv [MORE.csv ...] [--copy-bytes N]
-       python -m hprmat_amd.tracesum --traffic 25600 FETCH.csv WRITE.csv --kernel K [--json OUT]
+       python -m hprmat_amd.tracesum --traffic 25600 FETCH.csv WRITE.csv --kernel K [--tile 64]
+                                     [--json OUT]
 """
 import argparse
 import collections
@@ -55,8 +59,8 @@ def stats(us):
     return len(us), sum(us), sum(us) / len(us), min(us), max(us)
 
 
-PEAK_TFLOPS = {'F32': 157.3, 'F64': 78.6, 'BF16': 2516.6}   # MI355X dense matrix peaks
-MOPS = ('F32', 'F64', 'BF16')
+PEAK_TFLOPS = {'F32': 157.3, 'F64': 78.6, 'BF16': 2516.6, 'F16': 2516.6}   # MI355X dense peaks
+MOPS = ('F32', 'F64', 'BF16', 'F16')
 
 
 def counters(path, copy_bytes):
@@ -162,31 +166,49 @@ def dispatches(path):
 
 def factor_traffic(paths, copy_bytes, kernel, n, nb, elem, tile=64):
     """Copy-calibrated HBM bytes against algorithmic bytes for the trailing
-    updates of one factorization (one counter per file)."""
+    updates of one factorization (one counter per file).
+
+    One launch per step is the update of every column right of the panel
+    (nc x nc, or nc x (nc - nb) under look-ahead).  A kernel that also
+    updates the next panel's nb columns on the main stream shows a strip
+    launch (nc x nb) before the bulk one of the same step: the strips are
+    left out, so that the record covers the launches the bench times."""
     cdiv = lambda a, b: (a + b - 1) // b
     per = None
+    keep = None
     scales = {}
     for path in paths:
         ds = dispatches(path)
         copies = [d for d in ds if 'copyBuffer' in d['name']]
         sel = [d for d in ds if d['name'].startswith(kernel)]
         if per is None:
-            per = []
-            for j, d in enumerate(sel):
+            per, keep = [], []
+            j, i = 0, 0
+            while i < len(sel):
+                d = sel[i]
                 nc = n - (j + 1) * nb
                 if nc <= 0:
                     raise ValueError('more launches of %s than steps of N=%d' % (kernel, n))
-                shapes = [c for c in (nc, nc - nb)
-                          if c > 0 and cdiv(nc, tile) * cdiv(c, tile) == d['wgs']]
+                grid = lambda c: cdiv(nc, tile) * cdiv(c, tile)
+                if (nc > nb and d['wgs'] == grid(nb) and i + 1 < len(sel)
+                        and sel[i + 1]['wgs'] == grid(nc - nb)):
+                    i += 1          # strip of the next panel's columns
+                    continue
+                shapes = [c for c in (nc, nc - nb) if c > 0 and grid(c) == d['wgs']]
                 if not shapes:
                     raise ValueError('launch %d: %d workgroups fit no update of %d rows'
-                                     % (j, d['wgs'], nc))
+                                     % (i, d['wgs'], nc))
                 ncols = shapes[0]
                 per.append({'M': nc, 'N': ncols, 'K': nb, 'ns': d['ns'],
                             'alg_read': elem * (nc * nb + nb * ncols + nc * ncols),
                             'alg_write': elem * nc * ncols})
-        elif len(sel) != len(per):
-            raise ValueError('%s: %d launches, %d in the first file' % (path, len(sel), len(per)))
+                keep.append(i)
+                i += 1
+                j += 1
+            nsel = len(sel)
+        elif len(sel) != nsel:
+            raise ValueError('%s: %d launches, %d in the first file' % (path, len(sel), nsel))
+        sel = [sel[i] for i in keep]
         for c in ('FETCH_SIZE', 'WRITE_SIZE'):
             vals = [d[c] for d in copies if c in d]
             if not vals or not any(c in d for d in sel):
@@ -242,13 +264,15 @@ def print_traffic(rec):
              f['write_ratio']))
 
 
-def kernel_source_hash():
-    """First 16 hex digits of the SHA-256 of csrc/gemm.hpp, the source of the
-    trailing-update GEMM tile: stamped into traffic records so that a reader
-    can tell whether a record was measured on the kernel it is used for."""
+def kernel_source_hash(kernel=''):
+    """First 16 hex digits of the SHA-256 of the source of a trailing-update
+    GEMM (csrc/split_gemm.hpp for the split-precision kernels, csrc/gemm.hpp
+    otherwise): stamped into traffic records so that a reader can tell
+    whether a record was measured on the kernel it is used for."""
     import hashlib
     import os
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'csrc', 'gemm.hpp')
+    src = 'split_gemm.hpp' if 'split' in kernel else 'gemm.hpp'
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'csrc', src)
     with open(path, 'rb') as f:
         return hashlib.sha256(f.read()).hexdigest()[:16]
 
@@ -264,12 +288,13 @@ def main(argv=None):
     ap.add_argument('--traffic', type=int, default=0, metavar='N')
     ap.add_argument('--nb', type=int, default=256)
     ap.add_argument('--elem-bytes', type=int, default=8)
+    ap.add_argument('--tile', type=int, default=64)
     ap.add_argument('--json', default='')
     args = ap.parse_args(argv)
     if args.traffic:
         rec = factor_traffic(args.trace, args.copy_bytes, args.kernel, args.traffic, args.nb,
-                             args.elem_bytes)
-        rec['build'] = kernel_source_hash()
+                             args.elem_bytes, args.tile)
+        rec['build'] = kernel_source_hash(args.kernel)
         print_traffic(rec)
         if args.json:
             with open(args.json, 'w') as f:
