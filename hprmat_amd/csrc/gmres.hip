@@ -21,6 +21,8 @@
 // Device flags in RefineState replace host synchronisation: `done` (the
 // true residual met the rule) and `gm_halt` (the current cycle stopped)
 // turn later kernels into no-ops.
+#include <thread>
+
 #include "lu.hpp"
 
 namespace hpr {
@@ -310,9 +312,8 @@ double gmres_margin() {
 // Host-side progress of one lane through the algorithm above.  advance()
 // enqueues work up to the next point where the host may read a device flag
 // (after a cycle's true residual: `done`; after an Arnoldi step that is not
-// the cycle's last: `gm_halt`) and then returns, so that several lanes can
-// be driven round-robin: while the host waits for one lane's flag, the work
-// of the others is already queued on their streams.
+// the cycle's last: `gm_halt`) and then returns; lane_drive() alternates
+// advance() and the flag reads of one lane.
 template <class TF> struct LaneRun {
   enum Stage { CYCLE, AFTER_CHECK, STEP, AFTER_STEP, FINAL, DONE };
   GmresLane<TF>* L = nullptr;
@@ -408,6 +409,39 @@ int lane_advance(LaneRun<TF>& r, const cx<TF>* LU, int64_t ld, int n, const cx<T
   }
 }
 
+// One lane from its first cycle to the end: enqueue up to the next flag,
+// read the flag (in poll mode), go on.
+template <class TF>
+int lane_drive(LaneRun<TF>& r, const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256,
+               const int* perm, const double2* A64, int64_t lda, int m, double tol, int crit,
+               const LuPlan& plan, bool g3m, const double* scale) {
+  if (lane_advance(r, LU, ld, n, dinv256, perm, A64, lda, m, tol, crit, plan, g3m, scale))
+    return -1;
+  while (r.stage != LaneRun<TF>::DONE) {
+    if (r.pending && r.fp.read(r.L->state, r.L->st, r.pending)) {
+      if (r.stage == LaneRun<TF>::AFTER_CHECK) {   // converged
+        r.converged = true;
+        r.stage = LaneRun<TF>::DONE;
+        break;
+      }
+      // the cycle stopped early (its estimate met the rule): update X; the
+      // steps it did not take return to the budget
+      r.left += r.mc - (r.j + 1);
+      const GmresWork& gw = r.L->gw;
+      const size_t cnt = (size_t)n * r.L->nrhs;
+      const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
+      solve_y_kernel<<<(r.L->nrhs + 255) / 256, 256, 0, r.L->st>>>(r.L->nrhs, r.mc, gw.H, gw.g,
+                                                                    gw.y, r.L->state);
+      update_x_kernel<<<eg, 256, 0, r.L->st>>>(cnt, r.L->nrhs, gw.Z, cnt, gw.y, r.L->X,
+                                               r.L->state);
+      r.stage = LaneRun<TF>::CYCLE;
+    }
+    if (lane_advance(r, LU, ld, n, dinv256, perm, A64, lda, m, tol, crit, plan, g3m, scale))
+      return -1;
+  }
+  return 0;
+}
+
 }  // namespace
 
 template <class TF>
@@ -436,39 +470,39 @@ int gmres_solve_lanes(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256
     run[l].left = max_steps;
     run[l].fp = flag_poll(L.st, poll);
   }
-  // round-robin: read one lane's flag, enqueue its next segment, go to the next lane
-  for (int l = 0; l < nl; ++l)
-    if (lane_advance(run[l], LU, ld, n, dinv256, perm, A64, lda, m, tol, crit, plan, g3m, scale))
-      return -1;
-  for (;;) {
-    bool active = false;
-    for (int l = 0; l < nl; ++l) {
-      LaneRun<TF>& r = run[l];
-      if (r.stage == LaneRun<TF>::DONE) continue;
-      active = true;
-      if (r.pending && r.fp.read(r.L->state, r.L->st, r.pending)) {
-        if (r.stage == LaneRun<TF>::AFTER_CHECK) {   // converged
-          r.converged = true;
-          r.stage = LaneRun<TF>::DONE;
-          continue;
-        }
-        // the cycle stopped early (its estimate met the rule): update X; the
-        // steps it did not take return to the budget
-        r.left += r.mc - (r.j + 1);
-        const GmresWork& gw = r.L->gw;
-        const size_t cnt = (size_t)n * r.L->nrhs;
-        const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
-        solve_y_kernel<<<(r.L->nrhs + 255) / 256, 256, 0, r.L->st>>>(r.L->nrhs, r.mc, gw.H,
-                                                                      gw.g, gw.y, r.L->state);
-        update_x_kernel<<<eg, 256, 0, r.L->st>>>(cnt, r.L->nrhs, gw.Z, cnt, gw.y, r.L->X,
-                                                 r.L->state);
-        r.stage = LaneRun<TF>::CYCLE;
-      }
-      if (lane_advance(r, LU, ld, n, dinv256, perm, A64, lda, m, tol, crit, plan, g3m, scale))
-        return -1;
+  // Each lane after the first is driven by a host thread of its own: the
+  // host blocks in a lane's flag reads, and a single host thread serving the
+  // lanes in turn would hold a lane that is ready behind the wait for
+  // another (under a kernel tracer, where a graph launch also blocks until
+  // the graph has run, 8 ms per lane and step at N = 25600 with two lanes
+  // of 64 columns).
+  int dev = 0;
+  HPR_CHECK(hipGetDevice(&dev));
+  int rc[GMRES_MAX_LANES] = {};
+  std::thread th[GMRES_MAX_LANES];
+  bool threaded[GMRES_MAX_LANES] = {};
+  for (int l = 1; l < nl; ++l) {
+    try {
+      th[l] = std::thread([&, l] {
+        rc[l] = hipSetDevice(dev) == hipSuccess
+                    ? lane_drive(run[l], LU, ld, n, dinv256, perm, A64, lda, m, tol, crit, plan,
+                                 g3m, scale)
+                    : -1;
+      });
+      threaded[l] = true;
+    } catch (...) {   // no thread: the lane runs after lane 0 on this one
     }
-    if (!active) break;
   }
+  rc[0] = lane_drive(run[0], LU, ld, n, dinv256, perm, A64, lda, m, tol, crit, plan, g3m, scale);
+  for (int l = 1; l < nl; ++l) {
+    if (threaded[l])
+      th[l].join();
+    else
+      rc[l] = lane_drive(run[l], LU, ld, n, dinv256, perm, A64, lda, m, tol, crit, plan, g3m,
+                         scale);
+  }
+  for (int l = 0; l < nl; ++l)
+    if (rc[l]) return rc[l];
   HPR_CHECK(hipGetLastError());
   return 0;
 }

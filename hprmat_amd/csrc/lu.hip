@@ -876,7 +876,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
     // those are formed for every block in one pass after the last step
     // (U11 is final once its panel is factored: later interchanges only
     // touch rows below it), off the update chain.
-    diag_inv_kernel<T><<<dim3((kb + 63) / 64, 1), 256, 0, st>>>(A, ld, n, k / 64, w.dinv64, 0, 0,
+    diag_inv_kernel<T><<<dim3((kb + 63) / 64, 1), 64, 0, st>>>(A, ld, n, k / 64, w.dinv64, 0, 0,
                                                                 pk, 0);
     blk_inv_kernel<T><<<dim3(1, 1, 4), 256, 0, st>>>(A, ld, n, k / 256, w.dinv64, w.dinv256, 0, 0,
                                                      0, pk, 0);
@@ -995,7 +995,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
   if (!attr) { allow_big_lds(perm_from_ipiv_kernel); attr = true; }
   perm_from_ipiv_kernel<<<1, 256, in_lds ? perm_lds : 0, ps>>>(ipiv, n, w.perm, in_lds);
   // inv(U_bb) of every diagonal block: 64 x 64 blocks, then 256 x 256
-  diag_inv_kernel<T><<<dim3((n + 63) / 64, 1), 256, 0, st>>>(A, ld, n, 0, w.dinv64, 0, 0, 0, 1);
+  diag_inv_kernel<T><<<dim3((n + 63) / 64, 1), 64, 0, st>>>(A, ld, n, 0, w.dinv64, 0, 0, 0, 1);
   blk_inv_kernel<T><<<dim3((n + 255) / 256, 1, 4), 256, 0, st>>>(A, ld, n, 0, w.dinv64, w.dinv256,
                                                                   0, 0, 0, 0, 1);
   fold_inverses<T>(A, ld, n, w.dinv256, w.ustage, st, folded);
@@ -1102,7 +1102,10 @@ int getrs_replay(SubstGraph* g, const cx<T>* LU, int64_t ld, int n, const cx<T>*
     subst_graph_release(*g);
     // The launches are recorded on a private stream of this device (the
     // caller's stream may be the null stream, which cannot be captured); the
-    // instantiated graph is then launched on the caller's stream.
+    // instantiated graph is then launched on the caller's stream.  Host
+    // threads (the FGMRES lanes) take turns at the recording.
+    static std::mutex cap_mu;
+    std::lock_guard<std::mutex> cap_lock(cap_mu);
     hipStream_t cap = capture_stream();
     hipGraph_t graph = nullptr;
     if (!cap || hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
@@ -1472,7 +1475,7 @@ int getrf_batched(const BatchArgs<T>& b, hipStream_t st) {
                   P + (int64_t)wt * ld + wt, ld, T(-1), 1, nullptr, 0, count, sF, sF, sF, st);
       }
     }
-    diag_inv_kernel<T><<<dim3((kb + 63) / 64, 2, count), 256, 0, st>>>(b.F, ld, n, k / 64,
+    diag_inv_kernel<T><<<dim3((kb + 63) / 64, 2, count), 64, 0, st>>>(b.F, ld, n, k / 64,
                                                                        b.dinv64, sF, b.s64);
     blk_inv_kernel<T><<<dim3(1, 2, 4 * count), 256, 0, st>>>(b.F, ld, n, k / 256, b.dinv64,
                                                              b.dinv256, sF, b.s64, b.s256);

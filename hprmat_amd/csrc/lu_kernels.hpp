@@ -921,65 +921,67 @@ permute_cast_kernel(int n, int nrhs, const int* __restrict__ perm, const cx<Tin>
 }
 
 // ---------------------------------------------------------------------------
-// Inverses of the 64 x 64 diagonal blocks of a row-major LU factor, in place
-// in LDS (LAPACK ztrti2 order): blockIdx.y = 0 -> inverse of the unit lower
+// Inverses of the 64 x 64 diagonal blocks of a row-major LU factor, one block
+// per 64-thread workgroup: blockIdx.y = 0 -> inverse of the unit lower
 // triangle, 1 -> inverse of the upper triangle.  Output: dinv[(b*2+y)*4096 +
-// i*64 + k], zero outside the triangle (and unit diagonal for L).
+// i*64 + k], zero outside the triangle and beyond a partial block (unit
+// diagonal for L).
+//
+// The inverse X is formed from X T = I (LAPACK ztrti2's equations, whose
+// small left residual makes X T_12 behave like a triangular solve): row i of
+// X depends only on row i and on T, so lane i forms row i on its own, with
+// no barrier between the 64 steps.  Upper: X_ii = 1 / U_ii, then for j > i
+// X_ij = -X_jj sum_{i <= k < j} X_ik U_kj.  Unit lower: for j < i, from the
+// right, X_ij = -(L_ij + sum_{j < k < i} X_ik L_kj).  The sums run in
+// ztrti2's order; the lanes run the same loops, the zeros of row i outside
+// its triangle adding exact zeros.  The triangle is staged in LDS and read
+// as broadcasts; the rows of X are kept transposed in LDS (xt[k][i] = X_ik,
+// lane i's column), so every lane's reads are conflict-free.
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64)
 diag_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int blk0,
                 cx<T>* __restrict__ dinv, int64_t sA = 0, int64_t sD = 0, int prio = 0,
                 int part0 = 0) {
   raise_wave_priority(prio);
   LU += blockIdx.z * sA;   // system of a batch
   dinv += blockIdx.z * sD;
-  __shared__ cx<T> a[64][65];
-  __shared__ cx<T> tmp[64];
+  __shared__ cx<T> tr[64][65];   // the triangle, zero elsewhere
+  __shared__ cx<T> xt[64][64];   // xt[k][i] = X_ik
   // part0 + blockIdx.y: 0 the unit-lower factor, 1 the upper factor
-  const int bi = blk0 + blockIdx.x, upper = part0 + blockIdx.y, tid = threadIdx.x;
+  const int bi = blk0 + blockIdx.x, upper = part0 + blockIdx.y, i = threadIdx.x;
   const int r0 = bi * 64, bs = min(64, n - r0);
-  for (int e = tid; e < 64 * 64; e += 256) {
-    int i = e / 64, k = e % 64;
-    bool in = i < bs && k < bs && (upper ? (k >= i) : (k < i));
-    a[i][k] = in ? LU[(int64_t)(r0 + i) * ld + r0 + k] : mkc<T>(0, 0);
+  for (int e = i; e < 64 * 64; e += 64) {
+    const int r = e / 64, k = e % 64;
+    const bool in = r < bs && k < bs && (upper ? (k >= r) : (k < r));
+    tr[r][k] = in ? LU[(int64_t)(r0 + r) * ld + r0 + k] : mkc<T>(0, 0);
   }
+  for (int k = 0; k < 64; ++k) xt[k][i] = mkc<T>(0, 0);
   __syncthreads();
+  auto fma_c = [](cx<T> s, cx<T> a, cx<T> b) {   // s + a b
+    return mkc<T>(s.x + a.x * b.x - a.y * b.y, s.y + a.x * b.y + a.y * b.x);
+  };
   if (upper) {
     for (int j = 0; j < bs; ++j) {
-      if (tid == 0) a[j][j] = crecip<T>(a[j][j]);
-      if (tid < j) tmp[tid] = a[tid][j];
-      __syncthreads();
+      const cx<T> xjj = crecip<T>(tr[j][j]);
+      if (i == j) xt[j][i] = xjj;
       cx<T> s = mkc<T>(0, 0);
-      if (tid < j)
-        for (int k = tid; k < j; ++k) s = mkc<T>(s.x + a[tid][k].x * tmp[k].x - a[tid][k].y * tmp[k].y,
-                                                 s.y + a[tid][k].x * tmp[k].y + a[tid][k].y * tmp[k].x);
-      __syncthreads();
-      if (tid < j) {
-        cx<T> ajj = a[j][j];
-        a[tid][j] = mkc<T>(-(ajj.x * s.x - ajj.y * s.y), -(ajj.x * s.y + ajj.y * s.x));
-      }
-      __syncthreads();
+      for (int k = 0; k < j; ++k) s = fma_c(s, xt[k][i], tr[k][j]);
+      if (i < j) xt[j][i] = mkc<T>(-(xjj.x * s.x - xjj.y * s.y), -(xjj.x * s.y + xjj.y * s.x));
     }
   } else {
-    for (int j = bs - 1; j >= 0; --j) {
-      if (tid > j && tid < bs) tmp[tid] = a[tid][j];
-      __syncthreads();
-      cx<T> s = mkc<T>(0, 0);
-      if (tid > j && tid < bs) {
-        s = tmp[tid];
-        for (int k = j + 1; k < tid; ++k) s = mkc<T>(s.x + a[tid][k].x * tmp[k].x - a[tid][k].y * tmp[k].y,
-                                                     s.y + a[tid][k].x * tmp[k].y + a[tid][k].y * tmp[k].x);
-      }
-      __syncthreads();
-      if (tid > j && tid < bs) a[tid][j] = mkc<T>(-s.x, -s.y);
-      __syncthreads();
+    // the diagonal of X stays zero until the end, so that the sums stop at k < i
+    for (int j = bs - 2; j >= 0; --j) {
+      cx<T> s = tr[i][j];
+      for (int k = j + 1; k < bs; ++k) s = fma_c(s, xt[k][i], tr[k][j]);
+      if (i > j && i < bs) xt[j][i] = mkc<T>(-s.x, -s.y);
     }
-    if (tid < bs) a[tid][tid] = mkc<T>(1, 0);
-    __syncthreads();
+    if (i < bs) xt[i][i] = mkc<T>(1, 0);
   }
+  __syncthreads();
+  // out[r][k] = X_rk = xt[k][r]: lane i writes column i (coalesced rows)
   cx<T>* out = dinv + ((int64_t)bi * 2 + upper) * 4096;
-  for (int e = tid; e < 64 * 64; e += 256) out[e] = a[e / 64][e % 64];
+  for (int r = 0; r < 64; ++r) out[r * 64 + i] = xt[i][r];
 }
 
 // ---------------------------------------------------------------------------
