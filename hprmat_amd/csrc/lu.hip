@@ -580,13 +580,9 @@ void launch_split(int nc, int kb, const cx<float>* L, int64_t ld, const cx<float
   if (split_l) {
     if constexpr (SplitTraits<KIND>::SCALED) {
       // the K scales: column maxima of L21 (read by every split of this
-      // step); the partial maxima are staged in the B operand's rows, which
-      // are written only after them (ceil(nc / 256) x 256 floats: 1 KB per
-      // 256 rows of 2 KB+ packed rows)
-      const int nparts = (nc + 255) / 256;
-      float* part = reinterpret_cast<float*>(v.b);
-      split_colmax_part_kernel<<<nparts, 1024, 0, st>>>(nc, kb, L, ld, part);
-      split_colmax_final_kernel<<<1, 1024, 0, st>>>(nparts, part, v.colmax);
+      // step), atomic maxima over a zeroed array
+      fill_u32_kernel<<<1, 256, 0, st>>>(v.colmax, 256, 0u);
+      split_colmax_kernel<<<(nc + 63) / 64, 1024, 0, st>>>(nc, kb, L, ld, v.colmax);
     }
     split_rows_kernel<KIND><<<(nc + 3) / 4, 256, 0, st>>>(nc, kb, L, ld, v.a, v.ka, v.colmax);
   }
@@ -878,7 +874,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
     // touch rows below it), off the update chain.
     diag_inv_kernel<T><<<dim3((kb + 63) / 64, 1), 64, 0, st>>>(A, ld, n, k / 64, w.dinv64, 0, 0,
                                                                 pk, 0);
-    blk_inv_kernel<T><<<dim3(1, 1, 4), 256, 0, st>>>(A, ld, n, k / 256, w.dinv64, w.dinv256, 0, 0,
+    blk_inv_kernel<T><<<dim3(1, 1, 16), 256, 0, st>>>(A, ld, n, k / 256, w.dinv64, w.dinv256, 0, 0,
                                                      0, pk, 0);
     // U12 = inv(L11) A12 for the trailing columns [c0, c0 + cols) of the
     // step, staged in ustage (row stride nc), then C -= L21 U12 over them
@@ -996,7 +992,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
   perm_from_ipiv_kernel<<<1, 256, in_lds ? perm_lds : 0, ps>>>(ipiv, n, w.perm, in_lds);
   // inv(U_bb) of every diagonal block: 64 x 64 blocks, then 256 x 256
   diag_inv_kernel<T><<<dim3((n + 63) / 64, 1), 64, 0, st>>>(A, ld, n, 0, w.dinv64, 0, 0, 0, 1);
-  blk_inv_kernel<T><<<dim3((n + 255) / 256, 1, 4), 256, 0, st>>>(A, ld, n, 0, w.dinv64, w.dinv256,
+  blk_inv_kernel<T><<<dim3((n + 255) / 256, 1, 16), 256, 0, st>>>(A, ld, n, 0, w.dinv64, w.dinv256,
                                                                   0, 0, 0, 0, 1);
   fold_inverses<T>(A, ld, n, w.dinv256, w.ustage, st, folded);
   if (side) {
@@ -1477,8 +1473,13 @@ int getrf_batched(const BatchArgs<T>& b, hipStream_t st) {
     }
     diag_inv_kernel<T><<<dim3((kb + 63) / 64, 2, count), 64, 0, st>>>(b.F, ld, n, k / 64,
                                                                        b.dinv64, sF, b.s64);
-    blk_inv_kernel<T><<<dim3(1, 2, 4 * count), 256, 0, st>>>(b.F, ld, n, k / 256, b.dinv64,
-                                                             b.dinv256, sF, b.s64, b.s256);
+    // 16 workgroups per system in the grid's z extent (<= 65535)
+    for (int c0 = 0; c0 < count; c0 += 4095) {
+      const int cn = min(4095, count - c0);
+      blk_inv_kernel<T><<<dim3(1, 2, 16 * cn), 256, 0, st>>>(
+          b.F + c0 * sF, ld, n, k / 256, b.dinv64 + c0 * b.s64, b.dinv256 + c0 * b.s256, sF,
+          b.s64, b.s256);
+    }
     if (k > 0) laswp_b<T>(b, 0, k, k, kb, st);
     if (k + kb < n) {
       const int nc = n - k - kb;

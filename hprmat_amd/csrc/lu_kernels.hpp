@@ -947,14 +947,22 @@ diag_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int blk0,
   LU += blockIdx.z * sA;   // system of a batch
   dinv += blockIdx.z * sD;
   __shared__ cx<T> tr[64][65];   // the triangle, zero elsewhere
-  __shared__ cx<T> xt[64][64];   // xt[k][i] = X_ik
+  __shared__ cx<T> xt[64][65];   // xt[k][i] = X_ik (padded: the output pass reads rows)
   // part0 + blockIdx.y: 0 the unit-lower factor, 1 the upper factor
   const int bi = blk0 + blockIdx.x, upper = part0 + blockIdx.y, i = threadIdx.x;
   const int r0 = bi * 64, bs = min(64, n - r0);
-  for (int e = i; e < 64 * 64; e += 64) {
-    const int r = e / 64, k = e % 64;
-    const bool in = r < bs && k < bs && (upper ? (k >= r) : (k < r));
-    tr[r][k] = in ? LU[(int64_t)(r0 + r) * ld + r0 + k] : mkc<T>(0, 0);
+  // 16 rows in flight per pass (one wave: the loads would otherwise wait
+  // for one another)
+  for (int r1 = 0; r1 < 64; r1 += 16) {
+    cx<T> v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int r = r1 + u;
+      const bool in = r < bs && i < bs && (upper ? (i >= r) : (i < r));
+      v[u] = in ? LU[(int64_t)(r0 + r) * ld + r0 + i] : mkc<T>(0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) tr[r1 + u][i] = v[u];
   }
   for (int k = 0; k < 64; ++k) xt[k][i] = mkc<T>(0, 0);
   __syncthreads();
@@ -966,6 +974,7 @@ diag_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int blk0,
       const cx<T> xjj = crecip<T>(tr[j][j]);
       if (i == j) xt[j][i] = xjj;
       cx<T> s = mkc<T>(0, 0);
+#pragma unroll 8
       for (int k = 0; k < j; ++k) s = fma_c(s, xt[k][i], tr[k][j]);
       if (i < j) xt[j][i] = mkc<T>(-(xjj.x * s.x - xjj.y * s.y), -(xjj.x * s.y + xjj.y * s.x));
     }
@@ -973,6 +982,7 @@ diag_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int blk0,
     // the diagonal of X stays zero until the end, so that the sums stop at k < i
     for (int j = bs - 2; j >= 0; --j) {
       cx<T> s = tr[i][j];
+#pragma unroll 8
       for (int k = j + 1; k < bs; ++k) s = fma_c(s, xt[k][i], tr[k][j]);
       if (i > j && i < bs) xt[j][i] = mkc<T>(-s.x, -s.y);
     }
@@ -984,74 +994,54 @@ diag_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int blk0,
   for (int r = 0; r < 64; ++r) out[r * 64 + i] = xt[i][r];
 }
 
-// ---------------------------------------------------------------------------
-// 64 x 64 x 64 complex tile product on MFMA for one workgroup of 4 waves:
-// acc[t] += A(rows 16w .. 16w+15, :) x B(:, cols 16t .. 16t+15), with A and B
-// held in LDS as real/imaginary planes of row stride 65 (conflict-free
-// operand reads).  Accumulator element r of lane l is row mfma_row(l/16, r),
-// column l % 16 of its 16 x 16 tile.
-// ---------------------------------------------------------------------------
+// C[16 w + i][j] += sum_k A[16 w + i][k] B[k][j] (j < 16) for one 64 x 64 x 16
+// complex block product on MFMA (wave w: one 16 x 16 tile), operands read
+// from global memory (small, L2-resident blocks); A is zero beyond arows x
+// acols.  Accumulator element r of lane l is row mfma_row(l/16, r), column
+// l % 16.
 template <class T>
-__device__ inline void prod64(const T (*ar)[65], const T (*ai)[65], const T (*br)[65],
-                              const T (*bi)[65], typename acc_t<T>::type* re,
-                              typename acc_t<T>::type* im) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
-#pragma unroll 4
-  for (int k0 = 0; k0 < 64; k0 += 4) {
-    const T a_r = ar[16 * w + li][k0 + lk], a_i = ai[16 * w + li][k0 + lk];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const T b_r = br[k0 + lk][16 * t + li], b_i = bi[k0 + lk][16 * t + li];
-      re[t] = mfma16x4(a_r, b_r, re[t]);
-      re[t] = mfma16x4(-a_i, b_i, re[t]);
-      im[t] = mfma16x4(a_r, b_i, im[t]);
-      im[t] = mfma16x4(a_i, b_r, im[t]);
-    }
-  }
-}
-
-// C[16 w + i][16 t + j] += sum_k A[16 w + i][k] B[k][16 t + j] for one
-// 64 x 64 x 64 complex block product on MFMA, operands read from global
-// memory (small, L2-resident blocks); A is zero beyond arows x acols.
-template <class T>
-__device__ inline void prod64_g(const cx<T>* __restrict__ A, int64_t lda, int arows, int acols,
-                                const cx<T>* __restrict__ B, int64_t ldb,
-                                typename acc_t<T>::type* re, typename acc_t<T>::type* im) {
+__device__ inline void prod64x16_g(const cx<T>* __restrict__ A, int64_t lda, int arows,
+                                   int acols, const cx<T>* __restrict__ B, int64_t ldb,
+                                   typename acc_t<T>::type& re, typename acc_t<T>::type& im) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
   const int i = 16 * w + li;
-#pragma unroll 4
-  for (int k0 = 0; k0 < 64; k0 += 4) {
-    const int k = k0 + lk;
-    const cx<T> a = (i < arows && k < acols) ? A[(int64_t)i * lda + k] : mkc<T>(0, 0);
-    cx<T> b[4];
+  // all 32 loads in flight at once: the chain of products in
+  // blk_inv_kernel is bound by their latency
+  cx<T> av[16], bv[16];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) b[t] = B[(int64_t)k * ldb + 16 * t + li];
+  for (int s = 0; s < 16; ++s) {
+    const int k = 4 * s + lk;
+    av[s] = (i < arows && k < acols) ? A[(int64_t)i * lda + k] : mkc<T>(0, 0);
+    bv[s] = B[(int64_t)k * ldb + li];
+  }
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      re[t] = mfma16x4(a.x, b[t].x, re[t]);
-      re[t] = mfma16x4(-a.y, b[t].y, re[t]);
-      im[t] = mfma16x4(a.x, b[t].y, im[t]);
-      im[t] = mfma16x4(a.y, b[t].x, im[t]);
-    }
+  for (int s = 0; s < 16; ++s) {
+    const cx<T> a = av[s], b = bv[s];
+    re = mfma16x4(a.x, b.x, re);
+    re = mfma16x4(-a.y, b.y, re);
+    im = mfma16x4(a.x, b.y, im);
+    im = mfma16x4(a.y, b.x, im);
   }
 }
 
 // ---------------------------------------------------------------------------
-// Inverse of one 256 x 256 diagonal block of a row-major LU factor, one 64-
-// column block column per workgroup (grid: blocks x {L, U} x 4), from the
+// Inverse of one 256 x 256 diagonal block of a row-major LU factor from the
 // 64 x 64 inverses of its diagonal sub-blocks (block forward / backward
-// substitution on MFMA):
+// substitution on MFMA), one 16-column slice of a 64-column block column per
+// workgroup (grid: blocks x {L, U} x 16: z = 4 jb + q, slice q of block
+// column jb; the columns of X are independent):
 //   L: X_jj = inv(L_jj),  X_ij = -inv(L_ii) sum_{k=j}^{i-1} L_ik X_kj  (i > j)
 //   U: X_jj = inv(U_jj),  X_ij = -inv(U_ii) sum_{k=i+1}^{j} U_ik X_kj  (i < j)
 // Output dinv256[(b*2+u)*65536 + r*256 + c], zero outside the triangle and
 // outside a trailing partial block.
 //
 // The kernel runs on the panel's critical path while the previous trailing
-// update occupies the CUs, so it uses no LDS and a GEMM workgroup's worth of
-// registers at most: it can start wherever one GEMM workgroup retires.  The
-// intermediate sum is parked in the output block X_ij itself.  A workgroup
-// only reads X blocks of its own block column, written by itself: barriers
-// (workgroup-scope release/acquire; the waves share the CU's L1) order them.
+// update occupies the CUs, so it uses no LDS and few registers: it can start
+// wherever one GEMM workgroup retires.  The intermediate sum is parked in
+// the output block X_ij itself.  A workgroup only reads X entries of its own
+// slice, written by itself: barriers (workgroup-scope release/acquire; the
+// waves share the CU's L1) order them.  The longest chain (j = 0 of L) is
+// nine dependent products of 64 x 64 x 16.
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(256)
@@ -1061,51 +1051,53 @@ blk_inv_kernel(const cx<T>* __restrict__ LU, int64_t ld, int n, int bb0,
   raise_wave_priority(prio);
   typedef typename acc_t<T>::type acc;
   // part0 + blockIdx.y: 0 the unit-lower factor, 1 the upper factor
-  const int bb = bb0 + blockIdx.x, upper = part0 + blockIdx.y, jb = blockIdx.z & 3;
+  const int bb = bb0 + blockIdx.x, upper = part0 + blockIdx.y;
+  const int jb = (blockIdx.z >> 2) & 3, q = blockIdx.z & 3;
   {
-    const int b = blockIdx.z >> 2;   // system of a batch (z = 4 b + jb)
+    const int b = blockIdx.z >> 4;   // system of a batch (z = 16 b + 4 jb + q)
     LU += b * sA;
     dinv64 += b * sD64;
     dinv256 += b * sD256;
   }
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
   const int r0 = bb * 256, kb = min(256, n - r0), nsb = (kb + 63) / 64;
+  const int c0 = jb * 64 + q * 16;   // first column of the slice
   cx<T>* X = dinv256 + ((int64_t)bb * 2 + upper) * 65536;
   for (int ib = 0; ib < 4; ++ib) {
     const bool zero = jb >= nsb || ib >= nsb || (upper ? ib > jb : ib < jb);
     if (!zero) continue;
-    for (int e = tid; e < 4096; e += 256) X[(ib * 64 + e / 64) * 256 + jb * 64 + e % 64] = mkc<T>(0, 0);
+    for (int e = tid; e < 1024; e += 256) X[(ib * 64 + e / 16) * 256 + c0 + e % 16] = mkc<T>(0, 0);
   }
   if (jb >= nsb) return;
   auto inv64 = [&](int sb) { return dinv64 + ((int64_t)(bb * 4 + sb) * 2 + upper) * 4096; };
   {
-    const cx<T>* src = inv64(jb);
-    for (int e = tid; e < 4096; e += 256) X[(jb * 64 + e / 64) * 256 + jb * 64 + e % 64] = src[e];
+    const cx<T>* src = inv64(jb) + q * 16;
+    for (int e = tid; e < 1024; e += 256)
+      X[(jb * 64 + e / 16) * 256 + c0 + e % 16] = src[(e / 16) * 64 + e % 16];
   }
   __syncthreads();
   const int step = upper ? -1 : 1;
   for (int ib = jb + step; ib >= 0 && ib < nsb; ib += step) {
-    acc re[4], im[4];
-    for (int t = 0; t < 4; ++t) { re[t] = acc{}; im[t] = acc{}; }
+    acc re = acc{}, im = acc{};
     const int k_lo = upper ? ib + 1 : jb, k_hi = upper ? jb : ib - 1;
     for (int kk = k_lo; kk <= k_hi; ++kk)
-      prod64_g<T>(LU + (int64_t)(r0 + 64 * ib) * ld + r0 + 64 * kk, ld, min(64, kb - 64 * ib),
-                  min(64, kb - 64 * kk), X + (int64_t)(64 * kk) * 256 + 64 * jb, 256, re, im);
-    cx<T>* Xij = X + (int64_t)(64 * ib) * 256 + 64 * jb;
-    for (int t = 0; t < 4; ++t)
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * w + mfma_row<T>(lk, r), col = 16 * t + li;
-        Xij[(int64_t)row * 256 + col] = mkc<T>(re[t][r], im[t][r]);
-      }
+      prod64x16_g<T>(LU + (int64_t)(r0 + 64 * ib) * ld + r0 + 64 * kk, ld,
+                     min(64, kb - 64 * ib), min(64, kb - 64 * kk),
+                     X + (int64_t)(64 * kk) * 256 + c0, 256, re, im);
+    cx<T>* Xij = X + (int64_t)(64 * ib) * 256 + c0;
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * w + mfma_row<T>(lk, r);
+      Xij[(int64_t)row * 256 + li] = mkc<T>(re[r], im[r]);
+    }
     __syncthreads();
-    for (int t = 0; t < 4; ++t) { re[t] = acc{}; im[t] = acc{}; }
-    prod64_g<T>(inv64(ib), 64, 64, 64, Xij, 256, re, im);
+    re = acc{};
+    im = acc{};
+    prod64x16_g<T>(inv64(ib), 64, 64, 64, Xij, 256, re, im);
     __syncthreads();   // every wave has read the parked sum before it is overwritten
-    for (int t = 0; t < 4; ++t)
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * w + mfma_row<T>(lk, r), col = 16 * t + li;
-        Xij[(int64_t)row * 256 + col] = mkc<T>(-re[t][r], -im[t][r]);
-      }
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * w + mfma_row<T>(lk, r);
+      Xij[(int64_t)row * 256 + li] = mkc<T>(-re[r], -im[r]);
+    }
     __syncthreads();
   }
 }

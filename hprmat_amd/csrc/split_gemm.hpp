@@ -41,7 +41,7 @@
 //   half index ((r * 8 + kb) * 2P + plane) * 32 + kk.
 // Scales (SPLIT_F16X2 only): kexp[r] = k of row r (int32); the K scales
 // come from colmax[k] = bits of max_i max(|Re L_ik|, |Im L_ik|)
-// (split_colmax_part_kernel, split_colmax_final_kernel).
+// (split_colmax_kernel).
 #pragma once
 #include "common.hpp"
 
@@ -144,23 +144,24 @@ __device__ inline int split_kexp(const unsigned* colmax, int k) {
 }
 
 // Column maxima of a row-major FP32 complex block (element (r, k) at
-// S[r * ld + k], r < rows, k < K <= 256) in two launches without atomics.
-// Stage 1: workgroup g (1024 threads: column k = t % 256, row phase
-// q = t / 256) writes part[g][k] = max over its 256 rows of
-// max(|Re S_rk|, |Im S_rk|); thread (k, q) takes rows q, q + 4, ... of the
-// workgroup's block, 16 loads in flight, and the four phases combine in
-// LDS; the workgroup reads 256 coalesced 2 KB row segments.
-// grid = ceil(rows / 256), block 1024.
+// S[r * ld + k], r < rows, k < K <= 256): colmax[k] = bits of the max over
+// the rows of max(|Re S_rk|, |Im S_rk|) (non-negative floats order like
+// their bit patterns), by atomic maxima into colmax, which the caller
+// zeroes first.  Workgroup g (1024 threads: column k = t % 256, row phase
+// q = t / 256) takes 64 rows, thread (k, q) rows q, q + 4, ... of them, 16
+// loads in flight; the four phases combine in LDS, then one atomic per
+// column.  The max is independent of the order of the atomics.
+// grid = ceil(rows / 64), block 1024.
 __global__ void __launch_bounds__(1024)
-split_colmax_part_kernel(int rows, int K, const cx<float>* __restrict__ S, int64_t ld,
-                         float* __restrict__ part) {
+split_colmax_kernel(int rows, int K, const cx<float>* __restrict__ S, int64_t ld,
+                    unsigned* __restrict__ colmax) {
   __shared__ float red[4][256];
   const int k = threadIdx.x & 255, q = threadIdx.x >> 8;
-  const int r0 = blockIdx.x * 256;
+  const int r0 = blockIdx.x * 64;
   float m = 0.f;
   if (k < K) {
-#pragma unroll 16
-    for (int i = q; i < 256; i += 4) {
+#pragma unroll
+    for (int i = q; i < 64; i += 4) {
       if (r0 + i < rows) {
         const cx<float> v = S[(int64_t)(r0 + i) * ld + k];
         m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
@@ -169,24 +170,10 @@ split_colmax_part_kernel(int rows, int K, const cx<float>* __restrict__ S, int64
   }
   red[q][k] = m;
   __syncthreads();
-  if (q == 0)
-    part[(size_t)blockIdx.x * 256 + k] = fmaxf(fmaxf(red[0][k], red[1][k]), fmaxf(red[2][k], red[3][k]));
-}
-// Stage 2 (one workgroup of 1024 threads, four phases as above): colmax[k]
-// = bits of max over the nparts partial maxima (non-negative floats order
-// like their bit patterns).
-__global__ void __launch_bounds__(1024)
-split_colmax_final_kernel(int nparts, const float* __restrict__ part,
-                          unsigned* __restrict__ colmax) {
-  __shared__ float red[4][256];
-  const int k = threadIdx.x & 255, q = threadIdx.x >> 8;
-  float m = 0.f;
-#pragma unroll 8
-  for (int g = q; g < nparts; g += 4) m = fmaxf(m, part[(size_t)g * 256 + k]);
-  red[q][k] = m;
-  __syncthreads();
-  if (q == 0)
-    colmax[k] = __float_as_uint(fmaxf(fmaxf(red[0][k], red[1][k]), fmaxf(red[2][k], red[3][k])));
+  if (q == 0 && k < K) {
+    const float mk = fmaxf(fmaxf(red[0][k], red[1][k]), fmaxf(red[2][k], red[3][k]));
+    atomicMax(colmax + k, __float_as_uint(mk));
+  }
 }
 
 // Split `rows` rows of a row-major FP32 complex block S (element (r, k) at
@@ -397,6 +384,25 @@ zgemm_split_kernel(int M, int N, const half_bits* __restrict__ A, const int* __r
       if (ROWS % RPP == 0 || e + 1 < LE || lr + RPP * e < ROWS)                        \
         *reinterpret_cast<u32x4_v*>(lds_st + ((BUF) * NP + p) * PLANE + RPP * e * BK) = ST[p][e];
 
+  // With two staging sets, C is read during the last two slices into the
+  // registers the sets free once their last slice is in LDS (set 0 one slice
+  // before set 1: the first CPRE column blocks then), so that the epilogue
+  // does not wait for it; with one set, at the epilogue.  Every C element of
+  // the wave is read before any store (stores to C would otherwise order
+  // each read after the previous write).
+  constexpr bool CEARLY = NSET == 2 && KS >= 3;
+  constexpr int CPRE = FN / 2;
+  cx<float> cv[FN][FM][16];
+#define HPR_SPLIT_CLOAD(B0, B1)                                                          \
+  _Pragma("unroll") for (int b = (B0); b < (B1); ++b) {                                  \
+    const int gj = col0 + wc * WN + b * 32 + l32;                                        \
+    _Pragma("unroll") for (int a = 0; a < FM; ++a)                                       \
+    _Pragma("unroll") for (int r = 0; r < 16; ++r) {                                     \
+      const int gi = row0 + wr * WM + a * 32 + 8 * (r / 4) + 4 * lh + (r % 4);           \
+      cv[b][a][r] = (gi < M && gj < N) ? C[(int64_t)gi * ldc + gj] : mkc<float>(0.f, 0.f); \
+    }                                                                                    \
+  }
+
   HPR_SPLIT_LOAD(stg0, 0)
   if constexpr (NSET == 2) { HPR_SPLIT_LOAD(stg1, 1) }
   HPR_SPLIT_STORE(stg0, 0)
@@ -412,6 +418,8 @@ zgemm_split_kernel(int M, int N, const half_bits* __restrict__ A, const int* __r
     } else {
       if (kb + 1 < KS) { HPR_SPLIT_LOAD(stg0, kb + 1) }
     }
+    if (CEARLY && kb == KS - 2) { HPR_SPLIT_CLOAD(0, CPRE) }
+    if (CEARLY && kb == KS - 1) { HPR_SPLIT_CLOAD(CPRE, FN) }
     // keep the loads here, ahead of the products (the scheduler would
     // otherwise sink them to the LDS stores)
     __builtin_amdgcn_sched_barrier(0);
@@ -473,22 +481,13 @@ zgemm_split_kernel(int M, int N, const half_bits* __restrict__ A, const int* __r
     }
   }
 
-  // epilogue: every C element of the wave is read first (stores to C would
-  // otherwise order each read after the previous write), then updated
-  cx<float> cv[FN][FM][16];
+  // epilogue: C += the scaled product
+  if (!CEARLY) { HPR_SPLIT_CLOAD(0, FN) }
   int sa[FM][16], sb[FN];   // scale exponents of the rows / columns
 #pragma unroll
   for (int b = 0; b < FN; ++b) {
-    const int gj = col0 + wc * WN + b * 32 + l32;
     sb[b] = 0;
-    if constexpr (SplitTraits<KIND>::SCALED) sb[b] = kb_[min(gj, N - 1)];
-#pragma unroll
-    for (int a = 0; a < FM; ++a)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int gi = row0 + wr * WM + a * 32 + 8 * (r / 4) + 4 * lh + (r % 4);
-        cv[b][a][r] = (gi < M && gj < N) ? C[(int64_t)gi * ldc + gj] : mkc<float>(0.f, 0.f);
-      }
+    if constexpr (SplitTraits<KIND>::SCALED) sb[b] = kb_[min(col0 + wc * WN + b * 32 + l32, N - 1)];
   }
 #pragma unroll
   for (int a = 0; a < FM; ++a)
@@ -517,6 +516,7 @@ zgemm_split_kernel(int M, int N, const half_bits* __restrict__ A, const int* __r
         }
       }
   }
+#undef HPR_SPLIT_CLOAD
 #undef HPR_SPLIT_KOFF
 #undef HPR_SPLIT_LOAD
 #undef HPR_SPLIT_STORE
