@@ -468,6 +468,24 @@ def test_memory_scale_mixed_solve():
     assert np.abs(sm).max() <= 1.0 + 1e-10     # absorption: |S_ij| <= 1
 
 
+def test_memory_scale_split_f16_solve():
+    # The same N = 51200 system with the scaled split-FP16 trailing update
+    # and tournament-pivoted leaves (the benchmark head): FGMRES to the
+    # system's FP64 floor (1.35e-12, shown by the test above), and S
+    # symmetric with |S_ij| <= 1.
+    s, dm, o, co, cop, a_cm, b = _memory_system(False)
+    lu = solver.LU().set_plan(gmres_cycle=24).factor(a_cm, 'f16x3', True)
+    st = lu.solve_(b, 40, 2e-12, 'rel_fro', True, refinement='fgmres')
+    print('memory-scale f16x3: res_rel %.2e products %d hist %s'
+          % (st['res_rel'], st['sweeps'], ' '.join('%.1e' % h for h in st['hist'])))
+    assert st['converged'] and st['res_rel'] <= 2e-12, st
+    del lu
+    _, sm = rmatrix.r_to_s(dm, b, s.nch, s.qk, o, co, cop)
+    sm = sm.cpu().numpy()
+    assert np.abs(sm - sm.T).max() <= 1e-10
+    assert np.abs(sm).max() <= 1.0 + 1e-10
+
+
 def test_memory_scale_real_potential_unitarity():
     # W0 = 0 (real potential): C is real symmetric indefinite with E inside
     # its spectrum, far too ill-conditioned for FP32 factors to precondition;
