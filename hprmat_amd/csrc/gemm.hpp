@@ -188,8 +188,11 @@ zgemm_kernel(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
 // The LU's trailing update A22 -= L21 U12 (the launches timed for the
 // roofline record): zgemm_kernel's tile under its own kernel name, so that
 // kernel traces list these launches apart from the library's other GEMMs.
+// FP64 four-product tiles are built for three workgroups per CU (150
+// registers, accumulators out of AGPRs); the launch reserves LDS so that two
+// run per CU beside a panel leaf (trail_lds_extra in lu.hip).
 template <class T, int BM, int BN, int BK, int WM, int WN, bool GAUSS3M>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, (sizeof(T) == 8 && !GAUSS3M) ? 3 : 1)
 trailing_update_kernel(int M, int N, int K, const cx<T>* __restrict__ A, int64_t lda,
                        const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
                        int64_t ldc, T alpha) {

@@ -452,14 +452,18 @@ unsigned barrier_err_init() {
 }
 
 // Extra LDS of the look-ahead trailing update (bytes, HPRMAT_HIP_TRAIL_LDS;
-// default: enough to hold FP32 updates at three workgroups per CU).
+// default: FP32 updates at three workgroups per CU; FP64 updates, built for
+// three (150 registers), held at two, so that every CU keeps the registers
+// and LDS of a panel-leaf workgroup free (176 registers, 17 KB) while the
+// update runs: N = 25600 FP64 solve 965 -> 902 ms, the update at 60 instead
+// of 55 TFLOP/s; three per CU starve the panel chain, 1002 ms).
 size_t trail_lds_extra(bool fp32) {
   static long v = [] {
     const char* e = getenv("HPRMAT_HIP_TRAIL_LDS");
     return e ? atol(e) : -1L;
   }();
   if (v >= 0) return (size_t)v;
-  return fp32 ? 32768 : 0;
+  return fp32 ? 32768 : 24576;
 }
 
 // ---- split-K FP64 refinement product ---------------------------------------
@@ -467,7 +471,12 @@ namespace {
 constexpr int SK_BM = 64, SK_BN = 64, SK_BK = 16;
 
 // Slice s = blockIdx.y of K: P_s = A[:, k0:k0+kn] X[k0:k0+kn, :] (A column-major).
-__global__ void __launch_bounds__(256)
+// Built for three workgroups per CU (150 registers instead of 226 with the
+// accumulators in AGPRs): when two FGMRES lanes share the GPU, the other
+// lane's substitution launches then find registers free beside these
+// long-running workgroups instead of waiting for one to retire (N = 25600,
+// two lanes of 64 columns: f16x3_gmres 458 -> 449 ms, same iterates).
+__global__ void __launch_bounds__(256, 3)
 zgemm_splitk_kernel(int M, int N, int K, int Kc, const double2* __restrict__ A, int64_t lda,
                     const double2* __restrict__ B, int64_t ldb, double2* __restrict__ P,
                     int64_t sP, const int* __restrict__ skip) {
@@ -499,16 +508,20 @@ splitk_reduce_kernel(size_t cnt, int S, const double2* __restrict__ P, size_t sP
 }
 }  // namespace
 
+// The slice count fills rounds of two workgroups per CU.  It also fixes the
+// summation order of the products, which moves the refinement on
+// ill-conditioned systems: kept at the rule it was tuned with (rounds of
+// three per CU, the kernel's residency, would take the N = 25600 benchmark
+// system's Gauss-factor FGMRES from 6 to 8 products).
 int resid_splits(int n, int nrhs) {
   static int slots = 0;
   if (!slots) {
-    int dev = 0, cus = 0, per = 0;
+    int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, zgemm_splitk_kernel, 256, 0) !=
-            hipSuccess || cus <= 0 || per <= 0)
+        cus <= 0)
       return 1;
-    slots = cus * per;
+    slots = cus * 2;
   }
   if (n <= 0 || nrhs <= 0) return 1;
   const double tiles = (double)((n + SK_BM - 1) / SK_BM) * ((nrhs + SK_BN - 1) / SK_BN);
