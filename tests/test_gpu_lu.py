@@ -293,6 +293,19 @@ def test_zero_pivot_info_matches_lapack(col):
     assert lu.info() == info_ref == col + 1
 
 
+@pytest.mark.parametrize('method,col', [('fp32', 5), ('fp32', 300), ('f16x3', 300)])
+def test_zero_pivot_info_tournament(method, col):
+    # FP32 factors pivot by tournament (three leaf workgroups at n = 700):
+    # an exactly zero column stays zero under every elimination, so the
+    # first zero pivot is still at that column, as LAPACK reports it
+    n = 700
+    a, _ = rand_system(n, 1, 17, dd=0.0)
+    a[:, col] = 0
+    lu = solver.LU().factor(solver.colmajor(torch.from_numpy(a).cuda()), method)
+    _, _, info_ref = lapack.zgetrf(a)
+    assert lu.info() == info_ref == col + 1
+
+
 @pytest.mark.parametrize('method', ['fp64', 'fp32'])
 def test_equilibrated_factor_same_solution(method):
     s = synth.config('small')
