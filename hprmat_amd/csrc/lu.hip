@@ -342,6 +342,18 @@ template <class T> void launch_laswp(cx<T>* A, int64_t ld, int c0, int c1, int k
   laswp_kernel<T><<<grid, 256, lds, st>>>(A, ld, c0, c1, k1, nk, ipiv, skip, 0, 0, 0, prio);
 }
 
+// The panel's interchanges (k1, nk <= 256) as row moves, then their
+// application to column ranges (laswp_map_kernel, laswp_apply_kernel).
+void launch_laswp_map(const int* ipiv, int k1, int nk, int* map, hipStream_t st) {
+  if (nk <= 0) return;
+  laswp_map_kernel<<<1, 256, 0, st>>>(ipiv, k1, nk, map);
+}
+template <class T> void launch_laswp_apply(cx<T>* A, int64_t ld, int c0, int c1, const int* map,
+                                           int nk, hipStream_t st, int prio = 0) {
+  if (c1 <= c0 || nk <= 0) return;
+  laswp_apply_kernel<T><<<(c1 - c0 + 15) / 16, 256, 0, st>>>(A, ld, c0, c1, map, 2 * nk, prio);
+}
+
 // ---- optional HIP-event timing of the trailing-update GEMM ------------------
 // When enabled, each trailing update of getrf is bracketed by two events on
 // the launch stream; prof_read sums the elapsed times and the algorithmic
@@ -389,12 +401,13 @@ int prof_read(double* ms, double* flops, int* count) {
   return 0;
 }
 
-size_t scratch_bytes() { return xchg_bytes() + 256; }
+size_t scratch_bytes() { return xchg_bytes() + 256 + 3 * 1024 * sizeof(int); }
 
 void scratch_bind(Scratch& s, void* base) {
   s.xchg = base;
   s.xchg_bytes = xchg_bytes();
   s.err = (int*)((char*)base + s.xchg_bytes);
+  s.lmap = (int*)((char*)base + s.xchg_bytes + 256);
 }
 
 template <class T>
@@ -880,6 +893,13 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
     const int nc = n - k - kb;
     cx<T>* Akk = A + (int64_t)k * ld + k;
     const cx<T>* iL = w.dinv256 + (size_t)(k / 256) * 2 * 65536;
+    // the panel's interchanges as row moves, for every column range outside
+    // the panel.  Three maps in turn: the last reader of a map, this step's
+    // left interchanges on the look-ahead stream, precedes that stream's next
+    // bulk update, whose `rest` event the main stream waits for before it
+    // writes the map again two steps later.
+    int* lmap = s.lmap + (size_t)((k / nb) % 3) * 1024;
+    launch_laswp_map(ipiv, k, kb, lmap, st);
     // Inverses of the panel's diagonal blocks (64 x 64, then 256 x 256):
     // inv(L11) now, it is needed for U12; inv(U11) only by the solves, so
     // those are formed for every block in one pass after the last step
@@ -915,7 +935,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
         HPR_CHECK(hipStreamWaitEvent(st, side->rest, 0));
         rest_pending = false;
       }
-      launch_laswp<T>(A, ld, k + kb, k + 2 * kb, k, kb, ipiv, nullptr, st, pk);
+      launch_laswp_apply<T>(A, ld, k + kb, k + 2 * kb, lmap, kb, st, pk);
       u12(0, nb, st, pg);
       if (bfx) {
         split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, st, true, 0,
@@ -928,7 +948,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       copy_back(0, nb, st);
       hipStream_t cs = side->stream;
       HPR_CHECK(hipStreamWaitEvent(cs, side->ready, 0));
-      launch_laswp<T>(A, ld, k + 2 * kb, n, k, kb, ipiv, nullptr, cs);
+      launch_laswp_apply<T>(A, ld, k + 2 * kb, n, lmap, kb, cs);
       u12(nb, nc - nb, cs, 0);
       if (bfx)
         split_operands<T>(nc, kb, Akk + (int64_t)kb * ld, ld, w.ustage, nc, w.split, cs, false,
@@ -949,7 +969,7 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       // the solves: their interchanges follow the update on the side stream,
       // after the event the main stream waits for (run concurrently with the
       // update they would be starved by its workgroups)
-      if (k > 0) launch_laswp<T>(A, ld, 0, k, k, kb, ipiv, nullptr, cs);
+      if (k > 0) launch_laswp_apply<T>(A, ld, 0, k, lmap, kb, cs);
       // in the second half, where the panel chain outlasts the update, this
       // stream has slack: fold inv(L_bb) into up to two finished block
       // columns b0 < k per step (the current panel's column is still read
@@ -965,9 +985,9 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
         HPR_CHECK(hipStreamWaitEvent(st, side->rest, 0));
         rest_pending = false;
       }
-      if (k > 0) launch_laswp<T>(A, ld, 0, k, k, kb, ipiv, nullptr, st, pk);
+      if (k > 0) launch_laswp_apply<T>(A, ld, 0, k, lmap, kb, st, pk);
       if (nc > 0) {
-        launch_laswp<T>(A, ld, k + kb, n, k, kb, ipiv, nullptr, st, pk);
+        launch_laswp_apply<T>(A, ld, k + kb, n, lmap, kb, st, pk);
         u12(0, nc, st, pg);
         ProfRec* pr = g_prof_on ? prof_slot() : nullptr;
         if (bfx)

@@ -24,6 +24,7 @@ struct Scratch {
   void* xchg = nullptr;
   size_t xchg_bytes = 0;
   int* err = nullptr;
+  int* lmap = nullptr;   // interchange maps of the last three panels (3 x 1024 ints)
 };
 
 // Refinement bookkeeping kept on the device so no host round trip is needed

@@ -804,6 +804,88 @@ laswp_kernel(cx<T>* __restrict__ A, int64_t ld, int c0, int c1, int k1, int nk,
 }
 
 // ---------------------------------------------------------------------------
+// The net effect of the interchanges k1 .. k1+nk-1 (nk <= 256) as row moves:
+// map[2 r] = destination row of staging position r, map[2 r + 1] = its source
+// row (-1 / -1: nothing moves), r < 2 nk.  One workgroup: the staging slots
+// of laswp_kernel, then one thread replays the swaps on slot indices instead
+// of every workgroup of every interchange launch replaying them on data
+// (256 dependent LDS swaps per column strip).  The moves then apply with
+// laswp_apply_kernel to any column range.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+laswp_map_kernel(const int* __restrict__ ipiv, int k1, int nk, int* __restrict__ map) {
+  __shared__ int piv[256], slot[256], srcrow[512], idx[512];
+  const int tid = threadIdx.x;
+  for (int t = tid; t < nk; t += 256) piv[t] = ipiv[k1 + t];
+  for (int r = tid; r < 2 * nk; r += 256) idx[r] = r;
+  __syncthreads();
+  for (int t = tid; t < nk; t += 256) {
+    const int p = piv[t];
+    int sl;
+    if (p < k1 + nk) {
+      sl = p - k1;
+    } else {
+      int first = t;
+      for (int q = 0; q < t; ++q)
+        if (piv[q] == p) { first = q; break; }
+      sl = nk + first;
+    }
+    slot[t] = sl;
+    srcrow[t] = k1 + t;
+    srcrow[nk + t] = (sl == nk + t) ? p : -1;
+  }
+  __syncthreads();
+  if (tid == 0)
+    for (int t = 0; t < nk; ++t) {
+      const int sl = slot[t];
+      if (sl != t) {
+        const int a = idx[t];
+        idx[t] = idx[sl];
+        idx[sl] = a;
+      }
+    }
+  __syncthreads();
+  for (int r = tid; r < 2 * nk; r += 256) {
+    const int dst = srcrow[r];
+    const int src = dst >= 0 ? srcrow[idx[r]] : -1;
+    const bool moves = dst >= 0 && src != dst;
+    map[2 * r] = moves ? dst : -1;
+    map[2 * r + 1] = moves ? src : -1;
+  }
+}
+
+// Row moves of a laswp_map_kernel map (npos positions) over the columns
+// [c0, c1): each workgroup takes 16 columns, sixteen lanes per row; every
+// read lands in registers before the barrier and every write follows it (a
+// row can be both a source and a destination).  No LDS staging of rows: many
+// workgroups per CU, one memory round trip each way.
+template <class T>
+__global__ void __launch_bounds__(256)
+laswp_apply_kernel(cx<T>* __restrict__ A, int64_t ld, int c0, int c1,
+                   const int* __restrict__ map, int npos, int prio = 0) {
+  raise_wave_priority(prio);
+  constexpr int CW = 16, GR = 256 / CW, MAXP = 512 / GR;
+  __shared__ int m[1024];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < 2 * npos; e += 256) m[e] = map[e];
+  __syncthreads();
+  const int q = tid / CW, col = c0 + blockIdx.x * CW + tid % CW;
+  const bool ok = col < c1;
+  cx<T> v[MAXP];
+#pragma unroll
+  for (int i = 0; i < MAXP; ++i) {
+    const int r = q + i * GR;
+    if (ok && r < npos && m[2 * r] >= 0) v[i] = A[(int64_t)m[2 * r + 1] * ld + col];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < MAXP; ++i) {
+    const int r = q + i * GR;
+    if (ok && r < npos && m[2 * r] >= 0) A[(int64_t)m[2 * r] * ld + col] = v[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // In-place triangular solve with one diagonal block D (nb x nb, nb <= 64) of a
 // row-major factor: lower unit (forward) or upper non-unit (backward), on the
 // nb x ncols block row S of a row-major right-hand side.  Each workgroup
