@@ -330,16 +330,21 @@ template <class T> int launch_leaf(cx<T>* P, int64_t ld, int m, int w, int row_b
   return 0;
 }
 
+// Interchanges k1 .. k1+nk-1 over the columns [c0, c1) and [c2, c3) (either
+// may be empty) in one launch.
 template <class T> void launch_laswp(cx<T>* A, int64_t ld, int c0, int c1, int k1, int nk,
                                      const int* ipiv, const int* skip, hipStream_t st,
-                                     int prio = 0) {
-  if (c1 <= c0 || nk <= 0) return;
+                                     int prio = 0, int c2 = 0, int c3 = 0) {
+  if (c1 < c0) c1 = c0;
+  if (c3 < c2) c3 = c2;
+  if ((c1 == c0 && c3 == c2) || nk <= 0) return;
   constexpr int CW = 128 / sizeof(cx<T>);
   size_t lds = (size_t)2 * nk * 128 + (size_t)4 * nk * sizeof(int);
   static bool attr = false;
   if (!attr) { allow_big_lds(laswp_kernel<T>); attr = true; }
-  int grid = (c1 - c0 + CW - 1) / CW;
-  laswp_kernel<T><<<grid, 256, lds, st>>>(A, ld, c0, c1, k1, nk, ipiv, skip, 0, 0, 0, prio);
+  const int grid = (c1 - c0 + CW - 1) / CW + (c3 - c2 + CW - 1) / CW;
+  laswp_kernel<T><<<grid, 256, lds, st>>>(A, ld, c0, c1, k1, nk, ipiv, skip, 0, 0, 0, prio, c2,
+                                          c3);
 }
 
 // The panel's interchanges (k1, nk <= 256) as row moves, then their
@@ -881,9 +886,9 @@ int getrf(cx<T>* A, int64_t ld, int n, int* ipiv, int* info, const FactorWork<T>
       if (launch_leaf<T>(P, ld, m - t, wt, k + t, ipiv + k + t, info, s, leaf++, st, pk,
                          tournament ? 0 : 1))
         return -1;
-      if (t > 0) launch_laswp<T>(A, ld, k, k + t, k + t, wt, ipiv, nullptr, st, pk);
+      // the leaf's interchanges on the panel's columns left and right of it
+      launch_laswp<T>(A, ld, k, k + t, k + t, wt, ipiv, nullptr, st, pk, k + t + wt, k + kb);
       if (t + wt < kb) {
-        launch_laswp<T>(A, ld, k + t + wt, k + kb, k + t, wt, ipiv, nullptr, st, pk);
         trsm_diag_kernel<T, false><<<(kb - t - wt + 15) / 16, 256, 0, st>>>(
             P, ld, wt, P + wt, ld, kb - t - wt, nullptr, 0, pk);
         gemm<T>(m - t - wt, kb - t - wt, wt, P + (int64_t)wt * ld, ld, false, P + wt, ld,

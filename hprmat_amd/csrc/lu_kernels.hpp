@@ -735,7 +735,7 @@ leaf_calu_kernel(cx<float>* __restrict__ P, int64_t ld, int m, int w, int row_ba
 
 // ---------------------------------------------------------------------------
 // Row interchanges k1 .. k1+nk-1 (global 0-based ipiv, ipiv[k] >= k) applied
-// to the columns [c0, c1) of a row-major matrix.  Each workgroup takes a
+// to the columns [c0, c1) (and [c2, c3)) of a row-major matrix.  Each workgroup takes a
 // 128-byte column strip, stages every row the sequence touches in LDS (the nk
 // rows k1.. and each distinct pivot row below them), replays the swaps in
 // order there and writes the rows back: two coalesced passes instead of a
@@ -745,7 +745,7 @@ template <class T>
 __global__ void __launch_bounds__(256)
 laswp_kernel(cx<T>* __restrict__ A, int64_t ld, int c0, int c1, int k1, int nk,
              const int* __restrict__ ipiv, const int* __restrict__ skip, int64_t sA = 0,
-             int sIv = 0, int sSkip = 0, int prio = 0) {
+             int sIv = 0, int sSkip = 0, int prio = 0, int c2 = 0, int c3 = 0) {
   raise_wave_priority(prio);
   if (skip && skip[(int64_t)blockIdx.y * sSkip]) return;
   A += blockIdx.y * sA;
@@ -757,8 +757,10 @@ laswp_kernel(cx<T>* __restrict__ A, int64_t ld, int c0, int c1, int k1, int nk,
   int* srcrow = slot + nk;                                 // [2 * nk]
   int* piv = srcrow + 2 * nk;                              // [nk]
   const int tid = threadIdx.x;
-  const int cb = c0 + blockIdx.x * CW;
-  const int ncol = min(CW, c1 - cb);
+  // strips of [c0, c1), then of the optional second range [c2, c3)
+  const int g1 = (c1 - c0 + CW - 1) / CW;
+  const int cb = (int)blockIdx.x < g1 ? c0 + blockIdx.x * CW : c2 + (blockIdx.x - g1) * CW;
+  const int ncol = min(CW, ((int)blockIdx.x < g1 ? c1 : c3) - cb);
   if (ncol <= 0) return;
 
   for (int t = tid; t < nk; t += 256) piv[t] = ipiv[k1 + t];
