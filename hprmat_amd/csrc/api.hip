@@ -89,7 +89,6 @@ int ensure_side(hprmat_hip_lu* h) {
   }
   if ((e = hipEventCreateWithFlags(&h->side.ready, hipEventDisableTiming)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&h->side.rest, hipEventDisableTiming)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&h->side.lane, hipEventDisableTiming)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&h->side.join, hipEventDisableTiming)) != hipSuccess) {
     fprintf(stderr, "hprmat_hip: event creation failed: %s\n", hipGetErrorString(e));
     return -1;
@@ -103,7 +102,6 @@ void free_lu(hprmat_hip_lu* h) {
     (void)hipStreamSynchronize(h->side.stream);
     (void)hipEventDestroy(h->side.ready);
     (void)hipEventDestroy(h->side.rest);
-    (void)hipEventDestroy(h->side.lane);
     (void)hipEventDestroy(h->side.join);
     (void)hipStreamDestroy(h->side.stream);
   }
@@ -248,8 +246,7 @@ int fgmres_lanes(hprmat_hip_lu* h, int nl, const int* c0, const int* nc, void* d
       return -1;
   const int rc = hpr::gmres_solve_lanes<TF>(
       (const cx<TF>*)h->fac, h->ld, n, (const cx<TF>*)h->dinv256, h->perm, h->A64, h->lda,
-      max_refine, m, tol, crit, h->plan, false, h->equil ? h->scale : nullptr, nl, lanes, poll,
-      nl > 1 ? h->side.lane : nullptr);
+      max_refine, m, tol, crit, h->plan, false, h->equil ? h->scale : nullptr, nl, lanes, poll);
   if (rc) return rc;
   for (int l = 0; l < nl; ++l)
     if (hpr::transpose_cast<double, double>(nc[l], n, lanes[l].X, nc[l],
@@ -311,12 +308,11 @@ int lu_solve(hprmat_hip_lu* h, void* dB, int64_t ldb, int nrhs, int max_refine, 
     if (m > max_refine) m = max_refine;
     if (m < 1) m = 1;
     if (nrhs > 1024) return -1;
-    // Two lanes of columns on two streams, staggered by one substitution, so
-    // that one lane's latency-bound substitutions run beside the other's
-    // MFMA-bound FP64 products (N = 25600, 128 columns: a substitution of 64
-    // columns takes 6.7 ms alone and 12 ms beside a 64-column FP64 product,
-    // which it slows by 4 %).  Automatic for large systems with at least 64
-    // columns; not under stream capture (a graph spans the capturing stream).
+    // Two lanes of columns on two streams, each an FGMRES of its own: the
+    // two latency-bound substitution chains run side by side (N = 25600,
+    // 128 columns: 180 ms per solve phase against 206 ms in one lane).
+    // Automatic for large systems with at least 64 columns; not under stream
+    // capture (a graph spans the capturing stream).
     nl = h->solve_lanes > 0 ? h->solve_lanes : (n >= 8192 && nrhs >= 64 ? 2 : 1);
     if (nl > 1) {
       hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;

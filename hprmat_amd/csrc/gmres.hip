@@ -448,23 +448,19 @@ template <class TF>
 int gmres_solve_lanes(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, const int* perm,
                       const double2* A64, int64_t lda, int max_steps, int m, double tol, int crit,
                       const LuPlan& plan, bool g3m, const double* scale, int nl,
-                      GmresLane<TF>* lanes, bool poll, hipEvent_t stagger) {
+                      GmresLane<TF>* lanes, bool poll) {
   if (m < 1 || m > GMRES_MAX_M || nl < 1 || nl > GMRES_MAX_LANES) return -1;
   LaneRun<TF> run[GMRES_MAX_LANES];
-  // initial solutions X = M^-1 B; each lane after the first starts once the
-  // previous lane's substitution is done, so that the lanes' latency-bound
-  // substitutions run beside the others' FP64 products
+  // initial solutions X = M^-1 B
   for (int l = 0; l < nl; ++l) {
     GmresLane<TF>& L = lanes[l];
     if (L.nrhs < 1 || L.nrhs > 1024) return -1;
     const size_t cnt = (size_t)n * L.nrhs;
     const int eg = (int)min((size_t)4096, (cnt + 255) / 256);
-    if (l > 0 && stagger) HPR_CHECK(hipStreamWaitEvent(L.st, stagger, 0));
     residual_norms(n, L.nrhs, L.B, L.part, L.state, 0, crit, tol, nullptr, L.st);
     permute_cast<double, TF>(n, L.nrhs, perm, L.B, L.RF, nullptr, L.st, scale);
     if (getrs<TF>(LU, ld, n, dinv256, L.RF, L.nrhs, L.nrhs, L.RY, nullptr, plan, g3m, L.st))
       return -1;
-    if (l + 1 < nl && stagger) HPR_CHECK(hipEventRecord(stagger, L.st));
     widen_kernel<TF><<<eg, 256, 0, L.st>>>(cnt, L.nrhs, L.RF, scale, L.X, &L.state->gm_halt);
     run[l].L = &L;
     run[l].left = max_steps;
@@ -526,7 +522,7 @@ int gmres_solve(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, cons
   lane.sw = sw;
   lane.st = st;
   return gmres_solve_lanes<TF>(LU, ld, n, dinv256, perm, A64, lda, max_steps, m, tol, crit, plan,
-                               g3m, scale, 1, &lane, poll, nullptr);
+                               g3m, scale, 1, &lane, poll);
 }
 
 template int gmres_solve<double>(const cx<double>*, int64_t, int, const cx<double>*, const int*,
@@ -542,10 +538,10 @@ template int gmres_solve<float>(const cx<float>*, int64_t, int, const cx<float>*
 template int gmres_solve_lanes<double>(const cx<double>*, int64_t, int, const cx<double>*,
                                        const int*, const double2*, int64_t, int, int, double, int,
                                        const LuPlan&, bool, const double*, int,
-                                       GmresLane<double>*, bool, hipEvent_t);
+                                       GmresLane<double>*, bool);
 template int gmres_solve_lanes<float>(const cx<float>*, int64_t, int, const cx<float>*,
                                       const int*, const double2*, int64_t, int, int, double, int,
                                       const LuPlan&, bool, const double*, int, GmresLane<float>*,
-                                      bool, hipEvent_t);
+                                      bool);
 
 }  // namespace hpr

@@ -110,7 +110,6 @@ struct SideStream {
   hipStream_t stream = nullptr;
   hipEvent_t ready = nullptr;   // main -> side: U12 staged, next panel updated
   hipEvent_t rest = nullptr;    // side -> main: rest of the trailing update done
-  hipEvent_t lane = nullptr;    // FGMRES lanes: the first lane's first substitution done
   hipEvent_t join = nullptr;    // FGMRES lanes: the second lane's work done
 };
 
@@ -305,16 +304,17 @@ template <class TF> struct GmresLane {
 constexpr int GMRES_MAX_LANES = 4;
 // gmres_solve for nl lanes at once, each lane on its own stream with its
 // own stopping rule (a lane that meets the rule stops; the union meets it
-// too).  Lane l + 1 starts after lane l's first substitution (`stagger`, an
-// event; null: no stagger) and, with `poll`, the host reads the lanes' flags
-// round-robin, so that the lanes stay staggered: the latency-bound
-// substitutions of one lane run beside the MFMA-bound FP64 products of
-// another.  The caller joins the lanes' streams.
+// too).  The lanes start together and, doing the same work, keep in step:
+// their latency-bound substitutions run beside each other, and so do their
+// FP64 products (N = 25600, two lanes of 64 columns: 180 ms per solve phase
+// against 185 ms with the second lane started one substitution later, whose
+// substitutions then ran beside the other lane's products and took twice as
+// long).  The caller joins the lanes' streams.
 template <class TF>
 int gmres_solve_lanes(const cx<TF>* LU, int64_t ld, int n, const cx<TF>* dinv256, const int* perm,
                       const double2* A64, int64_t lda, int max_steps, int m, double tol, int crit,
                       const LuPlan& plan, bool gauss3m, const double* scale, int nl,
-                      GmresLane<TF>* lanes, bool poll, hipEvent_t stagger);
+                      GmresLane<TF>* lanes, bool poll);
 
 // ---- uniform batches ------------------------------------------------------
 // `count` independent systems of one order n, each at a fixed element
