@@ -419,7 +419,7 @@ constexpr int CALU_LEVELS = 4;   // G <= 8^3 = 512 workgroups
 constexpr unsigned CALU_TAG_LU = 8, CALU_TAG_DISP = 9;
 
 template <int W> struct CaluShared {
-  alignas(16) cx<float> cand[2][4][W];   // the waves' candidate pivot rows, by step parity
+  cx<float> cand[2][4][W];   // the waves' candidate pivot rows, by step parity
   cx<float> crcp[2][4];      // 1 / (their entry in the step's column)
   float red_v[2][4];
   int red_i[2][4];
@@ -437,20 +437,6 @@ template <int W> struct CaluShared {
 // a - b c for FP32 complex values as two fused multiply-adds per component.
 __device__ inline cx<float> cfms_fused(cx<float> a, cx<float> b, cx<float> c) {
   return mkc<float>(fmaf(-b.x, c.x, fmaf(b.y, c.y, a.x)), fmaf(-b.x, c.y, fmaf(-b.y, c.x, a.y)));
-}
-
-// 1 / z by Smith's method with its two branches turned into operand
-// selects: p is the larger part, q the other, r = q / p, d = p + q r, and
-// the parts of the result are 1 / d and r / d (signs and places by the
-// branch).  These are the same three correctly rounded quotients as crecip's,
-// so the result is bit-identical, but the compiler no longer evaluates both
-// branches (six IEEE division sequences on the pivot's critical path).
-__device__ inline cx<float> crecip_sel(cx<float> z) {
-  const bool xr = fabsf(z.x) >= fabsf(z.y);
-  const float p = xr ? z.x : z.y, q = xr ? z.y : z.x;
-  const float r = q / p, d = p + q * r;
-  const float inv = 1.f / d, rd = r / d;
-  return xr ? mkc<float>(inv, -rd) : mkc<float>(rd, -inv);
 }
 
 // GEPP over the rows of the 256 threads (one row each, `live` marks rows that
@@ -472,18 +458,13 @@ __device__ __forceinline__ int calu_gepp(cx<float> (&a)[W], bool live, int w,
   for (int j = 0; j < W; ++j) {
     if (j < w) {   // uniform
       const int par = j & 1;
-      const float mine = live ? cabs1<float>(a[j]) : -1.f;
-      // the wave's largest value and the lowest thread holding it (threads
-      // ascend with the lane): the first lane of the ballot
-      const float v = wave_max(mine);
-      const unsigned long long hold = __ballot(mine == v);
-      const int vi = hold ? (tid & ~63) + __builtin_ctzll(hold) : 0x7fffffff;
+      float v = live ? cabs1<float>(a[j]) : -1.f;
+      int vi = tid;
+      wave_argmax(v, vi);
       if (tid == vi && v >= 0.f) {
-        // the whole row in 16-byte stores (readers use columns > j only)
-        float4* cr = reinterpret_cast<float4*>(&sh.cand[par][wave][0]);
 #pragma unroll
-        for (int c = 0; c < W; c += 2) cr[c / 2] = make_float4(a[c].x, a[c].y, a[c + 1].x, a[c + 1].y);
-        sh.crcp[par][wave] = v > 0.f ? crecip_sel(a[j]) : mkc<float>(0.f, 0.f);
+        for (int c = j; c < W; ++c) sh.cand[par][wave][c] = a[c];
+        sh.crcp[par][wave] = v > 0.f ? crecip<float>(a[j]) : mkc<float>(0.f, 0.f);
       }
       if (lane == 0) { sh.red_v[par][wave] = v; sh.red_i[par][wave] = vi; }
       __syncthreads();
@@ -500,19 +481,11 @@ __device__ __forceinline__ int calu_gepp(cx<float> (&a)[W], bool live, int w,
           live = false;
           mystep = j;
         } else if (live) {
-          // the pivot row's columns > j in 16-byte reads issued together
-          // (one LDS round trip per step, not one per pair of columns)
-          const float4* F4 = reinterpret_cast<const float4*>(sh.cand[par][bw]);
-          float4 f[W / 2];
-#pragma unroll
-          for (int c2 = (j + 1) / 2; c2 < W / 2; ++c2) f[c2] = F4[c2];
+          const cx<float>* F = sh.cand[par][bw];
           const cx<float> l = cmul<float>(a[j], sh.crcp[par][bw]);   // 0 for a zero pivot
           a[j] = l;
 #pragma unroll
-          for (int c = j + 1; c < W; ++c) {
-            const float4 q = f[c / 2];
-            a[c] = cfms_fused(a[c], l, (c & 1) ? mkc<float>(q.z, q.w) : mkc<float>(q.x, q.y));
-          }
+          for (int c = j + 1; c < W; ++c) a[c] = cfms_fused(a[c], l, F[c]);
         }
       }
     }
