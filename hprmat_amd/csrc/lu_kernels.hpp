@@ -816,7 +816,9 @@ laswp_kernel(cx<T>* __restrict__ A, int64_t ld, int c0, int c1, int k1, int nk,
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 laswp_map_kernel(const int* __restrict__ ipiv, int k1, int nk, int* __restrict__ map) {
-  __shared__ int piv[256], slot[256], srcrow[512], idx[512];
+  __shared__ __align__(16) int piv[256];
+  __shared__ __align__(16) int slot[256];
+  __shared__ int srcrow[512], idx[512];
   const int tid = threadIdx.x;
   for (int t = tid; t < nk; t += 256) piv[t] = ipiv[k1 + t];
   for (int r = tid; r < 2 * nk; r += 256) idx[r] = r;
@@ -827,9 +829,16 @@ laswp_map_kernel(const int* __restrict__ ipiv, int k1, int nk, int* __restrict__
     if (p < k1 + nk) {
       sl = p - k1;
     } else {
+      // the first earlier pivot to the same row, four entries per LDS read
+      // (entries at or beyond t are not compared)
       int first = t;
-      for (int q = 0; q < t; ++q)
-        if (piv[q] == p) { first = q; break; }
+      for (int q0 = 0; q0 < t && first == t; q0 += 4) {
+        const int4 v = *reinterpret_cast<const int4*>(piv + q0);
+        if (v.x == p) first = q0;
+        else if (v.y == p && q0 + 1 < t) first = q0 + 1;
+        else if (v.z == p && q0 + 2 < t) first = q0 + 2;
+        else if (v.w == p && q0 + 3 < t) first = q0 + 3;
+      }
       sl = nk + first;
     }
     slot[t] = sl;
@@ -837,13 +846,18 @@ laswp_map_kernel(const int* __restrict__ ipiv, int k1, int nk, int* __restrict__
     srcrow[nk + t] = (sl == nk + t) ? p : -1;
   }
   __syncthreads();
-  if (tid == 0)
-    for (int t = 0; t < nk; ++t) {
-      const int sl = slot[t];
-      if (sl != t) {
-        const int a = idx[t];
-        idx[t] = idx[sl];
-        idx[sl] = a;
+  if (tid == 0)   // the slots of four swaps per LDS read, so each swap waits once
+    for (int t0 = 0; t0 < nk; t0 += 4) {
+      const int4 s4 = *reinterpret_cast<const int4*>(slot + t0);
+      const int sv[4] = {s4.x, s4.y, s4.z, s4.w};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = t0 + u, sl = sv[u];
+        if (t < nk && sl != t) {
+          const int a = idx[t];
+          idx[t] = idx[sl];
+          idx[sl] = a;
+        }
       }
     }
   __syncthreads();
@@ -959,7 +973,7 @@ perm_from_ipiv_kernel(const int* __restrict__ ipiv, int n, int* __restrict__ per
   ipiv += (int64_t)blockIdx.x * sIv;   // one workgroup per system
   perm += (int64_t)blockIdx.x * sIv;
   extern __shared__ int sh_perm[];
-  __shared__ int chunk[1024];
+  __shared__ __align__(16) int chunk[1024];
   int* p = in_lds ? sh_perm : perm;
   for (int i = threadIdx.x; i < n; i += 256) p[i] = i;
   __syncthreads();
@@ -967,10 +981,15 @@ perm_from_ipiv_kernel(const int* __restrict__ ipiv, int n, int* __restrict__ per
     const int cn = min(1024, n - c0);
     for (int i = threadIdx.x; i < cn; i += 256) chunk[i] = ipiv[c0 + i];
     __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int i = 0; i < cn; ++i) {
-        int k = chunk[i], a = c0 + i;
-        if (k != a) { int t = p[a]; p[a] = p[k]; p[k] = t; }
+    if (threadIdx.x == 0) {   // four interchange rows per LDS read: one wait per swap
+      for (int i0 = 0; i0 < cn; i0 += 4) {
+        const int4 k4 = *reinterpret_cast<const int4*>(chunk + i0);
+        const int kv[4] = {k4.x, k4.y, k4.z, k4.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = i0 + u, k = kv[u], a = c0 + i;
+          if (i < cn && k != a) { int t = p[a]; p[a] = p[k]; p[k] = t; }
+        }
       }
     }
     __syncthreads();
